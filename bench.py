@@ -1,0 +1,238 @@
+"""Benchmark: batched cost evaluations of the 8-qubit / 40-layer ansatz on one MI355X.
+
+A step is one call of optimization_problem_batched on 512 parameter vectors
+(8 qubits, U3 layer + 40 seeded-random adjacent CNOT layers: 128 gates, 264
+angles; 256 x 256 complex128 Haar target), inputs resident in HBM.  The JSON line
+also carries: the dominant kernel's roofline (live HIP-event timing inside the
+library), the unfused streaming per-gate path's HBM bandwidth on the same batch
+(512 MiB of matrices), the complex64 leg with its error against the FP64 oracle,
+the 8-qubit single-vector cost+gradient latency, and the CPU oracle timed on the
+host cores on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "sequential-quantum-gate-decomposer_amd"))
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gate_work(gates, d, m, esz):
+    """Algorithmic (flops, streamed bytes) of one pass of the gate list over a d x m
+    matrix: 1-qubit parametric gate 14*d*m flop / 2*e*d*m B; CRY half of that;
+    CNOT 0 flop / e*d*m B; CZ 0 flop / e*d*m/2 B."""
+    from squander_hip import _native as nat
+    fl, by = 0.0, 0.0
+    for t, *_ in gates:
+        t = int(t)
+        if t in (nat.GATE_U3, nat.GATE_RX, nat.GATE_RY, nat.GATE_RZ):
+            fl += 14.0 * d * m
+            by += 2.0 * esz * d * m
+        elif t == nat.GATE_CRY:
+            fl += 7.0 * d * m
+            by += esz * d * m
+        elif t == nat.GATE_CNOT:
+            by += esz * d * m
+        elif t == nat.GATE_CZ:
+            by += esz * d * m / 2
+    return fl, by
+
+
+def cpu_baseline(gates, U, n, P, seconds=10.0):
+    """The CPU oracle's batched cost, threaded over batch elements on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as orc
+    from squander_hip import workloads as W
+
+    o = orc.Oracle()
+    threads = min(os.cpu_count() or 1, 64)
+    ps = W.random_params(P, 99, batch=64)
+    done, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        o.optimization_problem_batched(gates, U, n, ps, nthreads=threads)
+        done += ps.shape[0]
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "cost evals/s", "cores": threads, "kind": "port",
+            "sample": f"{done} cost evals of the 8q/40-layer ansatz (batches of 64, OpenMP over the batch, "
+                      f"{dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--only-main", action="store_true", help="time the headline step only (for profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from squander_hip import PREC_F32, PREC_F64, Context
+    from squander_hip import workloads as W
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+
+    dev = torch.device("cuda:0")
+    n, d = 8, 256
+    gates = W.ansatz_8q_40(2026)
+    P = W.param_count(gates)
+    B = args.batch
+    U = W.haar_unitary(d, 2026).conj().T.copy()
+
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    L = ctx._L
+    params = torch.from_numpy(W.random_params(P, 7, batch=B)).to(dev)
+    costs = torch.empty(B, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def step(prec=PREC_F64):
+        ctx.check(L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, prec, costs.data_ptr()))
+
+    log(f"warmup {args.warmup} steps")
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    ctx.kernel_timing()
+    ctx.set_kernel_timing(True)
+    log(f"timing {args.steps} steps of B={B} cost evaluations (FP64)")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kms, nl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    wall = t1 - t0
+    value = B * args.steps / wall
+    k_avg_s = kms / max(nl, 1) / 1e3
+    flops_eval, _ = gate_work(gates, d, d, 16)
+    achieved = flops_eval * B / k_avg_s / 1e12
+
+    # error of the GPU costs against the FP64 oracle on a sample
+    o = orc.Oracle()
+    pc = params[:8].cpu().numpy()
+    ref = o.optimization_problem_batched(gates, U, n, pc, nthreads=8)
+    err64 = float(np.abs(costs[:8].cpu().numpy() - ref).max())
+
+    out = {
+        "metric": "cost evals/s @8q batch 512 (FP64, complex128)",
+        "value": value,
+        "unit": "cost evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Haar target, seeded uniform [0,2pi) angles)",
+        "config": {"workload": "8-qubit U3 layer + 40 random adjacent CNOT layers (128 gates, 264 angles), "
+                               "256x256 complex128 Haar target, batch of parameter vectors",
+                   "qubits": n, "batch": B, "gates": int(len(gates)), "params": P},
+        "err_vs_fp64_oracle": err64,
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_tile_forward<8,double,1>", "kernel_avg_ms": k_avg_s * 1e3,
+                     "algorithmic_flops_per_launch": flops_eval * B},
+    }
+    if args.only_main:
+        print(json.dumps(out), flush=True)
+        return
+
+    # complex64 leg
+    for _ in range(2):
+        step(PREC_F32)
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(PREC_F32)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    kms32, nl32 = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    err32 = float(np.abs(costs[:8].cpu().numpy() - ref).max())
+    out["c64"] = {"value": B * args.steps / (t1 - t0), "unit": "cost evals/s", "err_vs_fp64_oracle": err32,
+                  "kernel_avg_ms": kms32 / max(nl32, 1)}
+    log(f"c64: {out['c64']['value']:.0f} evals/s, err {err32:.2e}")
+
+    # unfused streaming path: one HBM pass per gate over B matrices of 1 MiB
+    mats = torch.from_numpy(np.broadcast_to(U, (B, d, d)).copy()).to(dev)
+    torch.cuda.synchronize()
+
+    def stream_step():
+        ctx.check(L.sqh_apply_to_stream_dev(ctx.h, params.data_ptr(), P, mats.data_ptr(), d, d, d, d * d, B, 0))
+
+    stream_step()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        stream_step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    skms, snl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    _, bytes_pass = gate_work(gates, d, d, 16)
+    gbs = bytes_pass * B * reps / (skms / 1e3) / 1e9
+    out["unfused_stream"] = {"achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
+                             "evals_per_s": B * reps / (t1 - t0), "kernel": "k_stream_gate<double>",
+                             "launches": snl, "bytes_per_pass": bytes_pass * B,
+                             "note": "algorithmic bytes over summed per-gate kernel time"}
+    log(f"unfused stream: {gbs:.0f} GB/s")
+    del mats
+
+    # single-vector cost + gradient latency (8 qubits, 264 angles)
+    f0 = torch.empty(1, dtype=torch.float64, device=dev)
+    g0 = torch.empty(P, dtype=torch.float64, device=dev)
+    p1 = params[:1].contiguous()
+
+    def grad_step():
+        ctx.check(L.sqh_optimization_problem_combined_batch_dev(ctx.h, p1.data_ptr(), 1, PREC_F64, f0.data_ptr(),
+                                                                 g0.data_ptr()))
+
+    for _ in range(3):
+        grad_step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(20):
+        grad_step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    fr, gr = o.optimization_problem_combined(gates, U, n, p1.cpu().numpy()[0])
+    out["grad_8q_single"] = {"ms": (t1 - t0) / 20 * 1e3, "err_grad_vs_oracle": float(np.abs(g0.cpu().numpy() - gr).max()),
+                             "err_cost_vs_oracle": float(abs(f0.item() - fr))}
+
+    if not args.skip_cpu:
+        log("cpu baseline (oracle) ...")
+        out["cpu_baseline"] = cpu_baseline(gates, U, n, P)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

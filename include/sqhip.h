@@ -1,0 +1,152 @@
+/*
+ * sqhip.h -- C ABI of the MI355X (gfx950) gate-application / cost / gradient library.
+ *
+ * The library replaces the in-process calls the SQUANDER reference makes on its hot
+ * path (paths relative to the reference checkout):
+ *
+ *   Gates_block::apply_to                 squander/src-cpp/gates/include/Gates_block.h:130
+ *   Gates_block::apply_derivate_to        squander/src-cpp/gates/include/Gates_block.h:170-178
+ *   Gates_block::apply_to_combined        squander/src-cpp/gates/include/Gates_block.h:182-196
+ *   Optimization_Interface::optimization_problem          .../decomposition/include/Optimization_Interface.h:333
+ *   Optimization_Interface::optimization_problem_batched  .../decomposition/include/Optimization_Interface.h:385
+ *   Optimization_Interface::optimization_problem_grad     .../decomposition/include/Optimization_Interface.h:422
+ *   Optimization_Interface::optimization_problem_combined .../decomposition/include/Optimization_Interface.h:434
+ *   Optimization_Interface::optimization_problem_combined_unitary (Optimization_Interface.cpp:1525-1561)
+ *
+ * and follows the reference's own accelerator precedent (the dlopen'ed DFE C ABI,
+ * squander/src-cpp/common/include/common_DFE.h:43-124): plain pointers and sizes,
+ * integer status codes, one context per device with its own stream.
+ *
+ * Conventions (the reference's):
+ *   - complex matrices are row-major, interleaved {re, im}; complex128 with float64
+ *     parameters, or complex64 with float32 parameters where a call says so;
+ *   - qubit q is bit q of the row index; gates run in list order, gate 0 first;
+ *   - raw parameters: half-angles for RX/RY/RZ/CRY and for U3's theta;
+ *   - gate types and cost variants use the reference's enum values.
+ *
+ * Every call returns SQH_OK (0) or a negative status; sqh_last_error() gives the
+ * message (the reference throws std::string with the same information).
+ * Calls on one context are serialised by a mutex; distinct contexts are independent.
+ * "_dev" entry points take device pointers and are asynchronous on the context's
+ * stream (sqh_get_stream); all others take host pointers and are synchronous.
+ */
+#ifndef SQHIP_H
+#define SQHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SQH_OK 0
+#define SQH_ERR_ARG (-1)      /* bad argument: sizes, dtypes, parameter count */
+#define SQH_ERR_GATE (-2)     /* unsupported gate or qubit index */
+#define SQH_ERR_STATE (-3)    /* circuit or target not set */
+#define SQH_ERR_DEVICE (-4)   /* HIP runtime failure */
+#define SQH_ERR_NOMEM (-5)    /* device allocation failed */
+
+/* Reference gate_type values (squander/src-cpp/gates/include/Gate.h:39-51). */
+#define SQH_GATE_CZ 4
+#define SQH_GATE_CNOT 5
+#define SQH_GATE_U3 7
+#define SQH_GATE_RY 8
+#define SQH_GATE_RX 9
+#define SQH_GATE_RZ 10
+#define SQH_GATE_CRY 14
+
+/* Reference cost_function_type values (Optimization_Interface.h:43-45);
+ * SQH_COST_ABS_TRACE = 1 - |Tr|/m has no reference counterpart. */
+#define SQH_COST_FROBENIUS 0
+#define SQH_COST_HILBERT_SCHMIDT 3
+#define SQH_COST_INFIDELITY 9
+#define SQH_COST_ABS_TRACE 100
+
+/* Precision of a cost / gradient evaluation. */
+#define SQH_PREC_F64 0   /* complex128 arithmetic */
+#define SQH_PREC_F32 1   /* complex64 arithmetic, FP32 accumulation, FP64 trace sum */
+
+typedef struct sqh_ctx sqh_ctx;
+
+/* One gate: the reference's gate object reduced to what the path reads
+ * (Gate.h: type, target_qbit, control_qbit, parameter_start_idx). control = -1 when
+ * the gate has none. */
+typedef struct sqh_gate {
+    int32_t type;
+    int32_t target;
+    int32_t control;
+    int32_t param_start;
+} sqh_gate;
+
+/* ---- context -------------------------------------------------------------- */
+int sqh_create(int device, sqh_ctx** out);
+void sqh_destroy(sqh_ctx* ctx);
+const char* sqh_last_error(const sqh_ctx* ctx);
+void* sqh_get_stream(sqh_ctx* ctx);            /* the context's hipStream_t */
+int sqh_synchronize(sqh_ctx* ctx);
+const char* sqh_version(void);
+
+/* ---- circuit, target and cost configuration ------------------------------- */
+/* Gates_block gate list (Gates_block.cpp:2500-2525 assigns param_start in order). */
+int sqh_set_circuit(sqh_ctx* ctx, int qbit_num, const sqh_gate* gates, int n_gates);
+int sqh_get_parameter_num(const sqh_ctx* ctx);
+/* Decomposition target Umtx (rows = 2^qbit_num, cols <= rows), complex128 host
+ * buffer with a row stride in elements (Decomposition_Base: Umtx). */
+int sqh_set_target(sqh_ctx* ctx, const double* U, int rows, int cols, int stride);
+int sqh_set_trace_offset(sqh_ctx* ctx, int trace_offset);       /* set_Trace_Offset */
+int sqh_set_cost_function(sqh_ctx* ctx, int variant);           /* set_Cost_Function_Variant */
+
+/* ---- Gates_block::apply_to -------------------------------------------------- */
+/* In place on a host matrix. is_f32: complex64 matrix + float32 parameters. */
+int sqh_apply_to(sqh_ctx* ctx, const void* params, void* inout, int rows, int cols, int stride, int is_f32);
+/* Device variant: B matrices at inout + b*bstride elements, parameters shared
+ * (pbstride = 0) or per matrix (pbstride = P). */
+int sqh_apply_to_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
+                     int stride, long long bstride, int B, int is_f32);
+/* The same result computed by one streaming HBM pass per gate (the unfused path). */
+int sqh_apply_to_stream_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
+                            int stride, long long bstride, int B, int is_f32);
+
+/* ---- Gates_block::apply_derivate_to / apply_to_combined ---------------------- */
+/* out receives P dense rows x cols complex128 matrices: out[k] = (dC/dp_k) . in. */
+int sqh_apply_derivate_to(sqh_ctx* ctx, const double* params, const double* in, int rows, int cols, int stride,
+                          double* out);
+int sqh_apply_derivate_to_dev(sqh_ctx* ctx, const double* d_params, const double* d_in, int rows, int cols,
+                              int stride, double* d_out);
+
+/* ---- Optimization_Interface: cost / gradient on the target ------------------ */
+/* costs[b] for B parameter vectors (params is B x P row-major). */
+int sqh_optimization_problem_batch(sqh_ctx* ctx, const double* params, int B, int precision, double* costs);
+int sqh_optimization_problem_batch_dev(sqh_ctx* ctx, const double* d_params, int B, int precision,
+                                       double* d_costs);
+/* f0[b] and grad[b*P + k] (adjoint evaluation of d cost / d p_k). */
+int sqh_optimization_problem_combined_batch(sqh_ctx* ctx, const double* params, int B, int precision,
+                                            double* f0, double* grad);
+int sqh_optimization_problem_combined_batch_dev(sqh_ctx* ctx, const double* d_params, int B, int precision,
+                                                double* d_f0, double* d_grad);
+/* optimization_problem_combined_unitary: C.Umtx (rows x cols dense) and the P
+ * derivative matrices (P x rows x cols). */
+int sqh_optimization_problem_combined_unitary(sqh_ctx* ctx, const double* params, double* out_unitary,
+                                              double* out_derivs);
+
+/* ---- kernel timers --------------------------------------------------------- */
+/* When enabled, every call brackets its dominant kernel (the register-tile kernel,
+ * or the derivative kernel) with HIP events on the context's stream.
+ * sqh_get_kernel_timing synchronises, returns the summed kernel time and the number
+ * of timed launches since the last query, and resets the timers.  Counterpart of
+ * the reference's circuit_simulation_time accumulator (Optimization_Interface.h:114-117). */
+int sqh_set_kernel_timing(sqh_ctx* ctx, int enable);
+int sqh_get_kernel_timing(sqh_ctx* ctx, double* total_ms, int* launches);
+
+/* ---- introspection (host only, no device needed) ---------------------------- */
+/* Compiles a gate list into the register-tile program the kernels execute and
+ * copies out its op records (8 bytes each) and layout records (48 bytes each);
+ * n_ops / n_layouts receive the counts.  Buffers may be NULL to query the counts.
+ * No counterpart in the reference: it exists for tooling and the CPU test suite. */
+int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* ops_out, int ops_cap,
+                        void* layouts_out, int layouts_cap, int* n_ops, int* n_layouts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SQHIP_H */

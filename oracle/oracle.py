@@ -1,0 +1,317 @@
+"""CPU oracle for the gate-application / cost / gradient path.
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product
+library never imports it.
+
+Two independent restatements live here:
+
+* ``Oracle`` -- ctypes binding of ``liboracle.so`` (``sq_oracle.c``), the
+  line-by-line restatement of the reference's pair-row kernels, gate loop,
+  derivative matrices, cost variants and BFGS driver (each C function cites
+  its reference file:line).
+* ``brute_*`` -- a dense numpy model built from textbook definitions only:
+  every gate is the Kronecker embedding of its 2x2 (or controlled 4x4) matrix,
+  rotations come from ``scipy.linalg.expm`` of Pauli generators and U3 from the
+  ZYZ identity U3(t,p,l) = e^{i(p+l)/2} RZ(p) RY(t) RZ(l) (Qiskit convention,
+  the reference checks against it in tests/gates/test_gates.py:162-209).  It
+  shares no formula with ``sq_oracle.c`` and pins it.
+
+Conventions follow the reference: row-major complex128, qubit q = bit q of the
+row index, gates applied in list order, raw parameters are half-angles for the
+RX/RY/RZ/CRY angle and for U3's theta (U3.cpp:49-51, RX.cpp:24).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+import scipy.linalg
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+# Reference gate_type values (Gate.h:39-51).
+CZ, CNOT, U3, RY, RX, RZ, CRY = 4, 5, 7, 8, 9, 10, 14
+PARAM_NUM = {U3: 3, RX: 1, RY: 1, RZ: 1, CRY: 1, CNOT: 0, CZ: 0}
+
+# Reference cost_function_type values (Optimization_Interface.h:43-45) + ABS_TRACE.
+COST_FROBENIUS, COST_HS, COST_INFIDELITY, COST_ABS_TRACE = 0, 3, 9, 100
+
+
+class _Gate(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("target", ctypes.c_int),
+                ("control", ctypes.c_int), ("param_start", ctypes.c_int)]
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [("gates", ctypes.POINTER(_Gate)), ("ng", ctypes.c_int), ("P", ctypes.c_int),
+                ("qbit_num", ctypes.c_int), ("target", ctypes.c_void_p),
+                ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("stride", ctypes.c_int),
+                ("variant", ctypes.c_int), ("trace_offset", ctypes.c_int)]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the Makefile beside this file."""
+    so = os.path.join(HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    return so
+
+
+def gates_array(gates) -> np.ndarray:
+    """(G,4) int32 array of (type, target, control, param_start)."""
+    return np.ascontiguousarray(np.asarray(gates, dtype=np.int32).reshape(-1, 4))
+
+
+def param_count(gates) -> int:
+    g = gates_array(gates)
+    return int(sum(PARAM_NUM[int(t)] for t in g[:, 0]))
+
+
+class Oracle:
+    """ctypes binding of sq_oracle.c."""
+
+    def __init__(self):
+        self.lib = ctypes.CDLL(build())
+        L = self.lib
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.or_apply_circuit.argtypes = [ctypes.POINTER(_Gate), ctypes.c_int, dp, ctypes.c_void_p,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_apply_derivate.argtypes = [ctypes.POINTER(_Gate), ctypes.c_int, dp, ctypes.c_void_p,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+        L.or_gate_kernel.argtypes = [ctypes.c_int, dp, ctypes.c_void_p]
+        L.or_deriv_kernel.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_void_p]
+        L.or_cost.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.or_cost.restype = ctypes.c_double
+        pp = ctypes.POINTER(_Problem)
+        L.or_optimization_problem.argtypes = [pp, dp]
+        L.or_optimization_problem.restype = ctypes.c_double
+        L.or_optimization_problem_combined.argtypes = [pp, dp, dp, dp]
+        L.or_optimization_problem_batched.argtypes = [pp, dp, ctypes.c_int, ctypes.c_int, dp]
+        L.or_optimization_problem_combined_batched.argtypes = [pp, dp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.or_bfgs_powell.argtypes = [pp, dp, ctypes.c_long, ctypes.POINTER(ctypes.c_long),
+                                     ctypes.POINTER(ctypes.c_int)]
+        L.or_bfgs_powell.restype = ctypes.c_double
+        L.or_decompose_custom.argtypes = [pp, ctypes.c_uint, ctypes.c_int, ctypes.c_long, ctypes.c_long,
+                                          ctypes.c_long, ctypes.c_double, ctypes.c_double, dp,
+                                          ctypes.POINTER(ctypes.c_long)]
+        L.or_decompose_custom.restype = ctypes.c_double
+        L.or_mt_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint]
+        L.or_mt_next.argtypes = [ctypes.c_void_p]
+        L.or_mt_next.restype = ctypes.c_uint
+        L.or_mt_uniform_2pi.argtypes = [ctypes.c_void_p]
+        L.or_mt_uniform_2pi.restype = ctypes.c_double
+
+    # -- helpers -----------------------------------------------------------
+    @staticmethod
+    def _gates(gates):
+        g = gates_array(gates)
+        arr = (_Gate * max(len(g), 1))()
+        for i, (t, tg, c, ps) in enumerate(g):
+            arr[i].type, arr[i].target, arr[i].control, arr[i].param_start = int(t), int(tg), int(c), int(ps)
+        return arr, len(g)
+
+    @staticmethod
+    def _dp(a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+    def _problem(self, gates, target, qbit_num, variant, trace_offset):
+        garr, ng = self._gates(gates)
+        target = np.ascontiguousarray(target, dtype=np.complex128)
+        pr = _Problem(garr, ng, param_count(gates), qbit_num, target.ctypes.data,
+                      target.shape[0], target.shape[1], target.shape[1], variant, trace_offset)
+        return pr, (garr, target)
+
+    # -- gate level --------------------------------------------------------
+    def gate_kernel(self, gtype, params):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        out = np.zeros(4, dtype=np.complex128)
+        self.lib.or_gate_kernel(gtype, self._dp(p), out.ctypes.data)
+        return out.reshape(2, 2)
+
+    def deriv_kernel(self, gtype, params, pidx):
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        out = np.zeros(4, dtype=np.complex128)
+        self.lib.or_deriv_kernel(gtype, self._dp(p), pidx, out.ctypes.data)
+        return out.reshape(2, 2)
+
+    # -- circuit level -----------------------------------------------------
+    def apply_to(self, gates, params, M):
+        """Gates_block::apply_to on a copy of M (rows x cols)."""
+        garr, ng = self._gates(gates)
+        out = np.ascontiguousarray(M, dtype=np.complex128).copy()
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        rc = self.lib.or_apply_circuit(garr, ng, self._dp(p), out.ctypes.data,
+                                       out.shape[0], out.shape[1], out.shape[1])
+        if rc:
+            raise ValueError("bad gate in circuit")
+        return out
+
+    def apply_derivate_to(self, gates, params, M):
+        garr, ng = self._gates(gates)
+        M = np.ascontiguousarray(M, dtype=np.complex128)
+        P = param_count(gates)
+        out = np.zeros((P,) + M.shape, dtype=np.complex128)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        rc = self.lib.or_apply_derivate(garr, ng, self._dp(p), M.ctypes.data,
+                                        M.shape[0], M.shape[1], M.shape[1], out.ctypes.data)
+        if rc:
+            raise ValueError("bad gate in circuit")
+        return out
+
+    def cost(self, M, variant=COST_FROBENIUS, trace_offset=0):
+        M = np.ascontiguousarray(M, dtype=np.complex128)
+        return self.lib.or_cost(M.ctypes.data, M.shape[1], M.shape[1], variant, trace_offset)
+
+    def optimization_problem(self, gates, target, qbit_num, params, variant=COST_FROBENIUS, trace_offset=0):
+        pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        return self.lib.or_optimization_problem(ctypes.byref(pr), self._dp(p))
+
+    def optimization_problem_combined(self, gates, target, qbit_num, params, variant=COST_FROBENIUS,
+                                      trace_offset=0):
+        pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
+        p = np.ascontiguousarray(params, dtype=np.float64)
+        f0 = np.zeros(1)
+        g = np.zeros(pr.P)
+        self.lib.or_optimization_problem_combined(ctypes.byref(pr), self._dp(p), self._dp(f0), self._dp(g))
+        return float(f0[0]), g
+
+    def optimization_problem_batched(self, gates, target, qbit_num, params, variant=COST_FROBENIUS,
+                                     trace_offset=0, nthreads=1):
+        pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, pr.P)
+        out = np.zeros(p.shape[0])
+        self.lib.or_optimization_problem_batched(ctypes.byref(pr), self._dp(p), p.shape[0], nthreads,
+                                                 self._dp(out))
+        return out
+
+    def optimization_problem_combined_batched(self, gates, target, qbit_num, params, variant=COST_FROBENIUS,
+                                              trace_offset=0, nthreads=1):
+        pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, pr.P)
+        f = np.zeros(p.shape[0])
+        g = np.zeros(p.shape)
+        self.lib.or_optimization_problem_combined_batched(ctypes.byref(pr), self._dp(p), p.shape[0],
+                                                          nthreads, self._dp(f), self._dp(g))
+        return f, g
+
+    def bfgs(self, gates, target, qbit_num, x0, max_calls=10000, variant=COST_FROBENIUS, trace_offset=0):
+        pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        calls = ctypes.c_long(0)
+        st = ctypes.c_int(0)
+        f = self.lib.or_bfgs_powell(ctypes.byref(pr), self._dp(x), max_calls, ctypes.byref(calls),
+                                    ctypes.byref(st))
+        return f, x, calls.value, st.value
+
+    def decompose_custom(self, gates, target, qbit_num, seed, initial_guess=1, max_outer_iterations=4,
+                         iteration_loops=1, max_inner_iterations=10000, optimization_tolerance=1e-8,
+                         convergence_threshold=1e-5, variant=COST_FROBENIUS, trace_offset=0):
+        pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
+        x = np.zeros(pr.P)
+        calls = ctypes.c_long(0)
+        f = self.lib.or_decompose_custom(ctypes.byref(pr), seed, initial_guess, max_outer_iterations,
+                                         iteration_loops, max_inner_iterations, optimization_tolerance,
+                                         convergence_threshold, self._dp(x), ctypes.byref(calls))
+        return f, x, calls.value
+
+    def mt_stream(self, seed, n, kind="u32"):
+        state = ctypes.create_string_buffer(624 * 4 + 16)
+        self.lib.or_mt_seed(state, seed)
+        if kind == "u32":
+            return np.array([self.lib.or_mt_next(state) for _ in range(n)], dtype=np.uint64)
+        return np.array([self.lib.or_mt_uniform_2pi(state) for _ in range(n)])
+
+
+# ---------------------------------------------------------------------------
+# Dense brute-force model (textbook definitions, independent of sq_oracle.c)
+# ---------------------------------------------------------------------------
+_X = np.array([[0, 1], [1, 0]], dtype=complex)
+_Y = np.array([[0, -1j], [1j, 0]], dtype=complex)
+_Z = np.array([[1, 0], [0, -1]], dtype=complex)
+
+
+def textbook_1q(gtype, p):
+    """2x2 matrix from the physical angles (2 x raw half-angle)."""
+    if gtype == RX:
+        return scipy.linalg.expm(-1j * (2 * p[0]) / 2 * _X)
+    if gtype in (RY, CRY):
+        return scipy.linalg.expm(-1j * (2 * p[0]) / 2 * _Y)
+    if gtype == RZ:
+        return scipy.linalg.expm(-1j * (2 * p[0]) / 2 * _Z)
+    if gtype == U3:
+        theta, phi, lam = 2 * p[0], p[1], p[2]
+        rz = lambda a: scipy.linalg.expm(-1j * a / 2 * _Z)
+        ry = lambda a: scipy.linalg.expm(-1j * a / 2 * _Y)
+        return np.exp(1j * (phi + lam) / 2) * rz(phi) @ ry(theta) @ rz(lam)
+    if gtype == CNOT:
+        return _X.copy()
+    if gtype == CZ:
+        return _Z.copy()
+    raise ValueError(gtype)
+
+
+def embed(n, u2, target, control=-1):
+    """Full 2^n x 2^n matrix of a (controlled) one-qubit gate; qubit q = bit q."""
+    d = 1 << n
+    full = np.zeros((d, d), dtype=complex)
+    for r in range(d):
+        if control >= 0 and not (r >> control) & 1:
+            full[r, r] = 1.0
+            continue
+        b = (r >> target) & 1
+        r0 = r & ~(1 << target)
+        for bb in (0, 1):
+            full[r0 | (bb << target), r] += u2[bb, b]
+    return full
+
+
+def brute_circuit_matrix(n, gates, params):
+    g = gates_array(gates)
+    C = np.eye(1 << n, dtype=complex)
+    for t, tg, c, ps in g:
+        t = int(t)
+        u = textbook_1q(t, params[ps:ps + PARAM_NUM[t]])
+        ctrl = int(c) if t in (CNOT, CZ, CRY) else -1
+        C = embed(n, u, int(tg), ctrl) @ C
+    return C
+
+
+def brute_apply(n, gates, params, M):
+    return brute_circuit_matrix(n, gates, params) @ M
+
+
+def brute_trace(M, trace_offset=0):
+    m = M.shape[1]
+    return np.sum(M[np.arange(m) + trace_offset, np.arange(m)])
+
+
+def brute_cost(M, variant=COST_FROBENIUS, trace_offset=0):
+    m = M.shape[1]
+    if variant == COST_FROBENIUS:
+        return 1.0 - brute_trace(M, trace_offset).real / m
+    if variant == COST_HS:
+        return 1.0 - abs(brute_trace(M, 0)) ** 2 / m ** 2
+    if variant == COST_INFIDELITY:
+        return 1.0 - (abs(brute_trace(M, 0)) ** 2 / m + 1.0) / (m + 1.0)
+    if variant == COST_ABS_TRACE:
+        return 1.0 - abs(brute_trace(M, trace_offset)) / m
+    raise ValueError(variant)
+
+
+def parameter_shift_derivatives(n, gates, params, M):
+    """dC/dp_k . M by the exact parameter-shift identity for raw half-angles:
+    dU/dp = [U(p + pi/2) - U(p - pi/2)] / 2 holds for every gate here because each
+    raw parameter enters only through sin(p), cos(p) (or e^{ip})."""
+    P = len(params)
+    out = np.zeros((P,) + M.shape, dtype=complex)
+    for k in range(P):
+        pp = np.array(params, dtype=float).copy()
+        pm = pp.copy()
+        pp[k] += np.pi / 2
+        pm[k] -= np.pi / 2
+        out[k] = (brute_apply(n, gates, pp, M) - brute_apply(n, gates, pm, M)) / 2
+    return out

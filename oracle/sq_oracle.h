@@ -1,0 +1,112 @@
+/*
+ * sq_oracle.h -- CPU oracle for the gate-application / cost / gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the SQUANDER
+ * reference algorithms, used by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py as the checker.  The product library never
+ * links, loads or calls it.
+ *
+ * Conventions (all follow the reference):
+ *   - complex128 matrices are row-major, interleaved {re, im}, with an
+ *     explicit row stride in elements (QGDTypes.h:38-43, matrix_base.hpp:130-137);
+ *   - qubit q is bit q of the row index (LSB = qubit 0);
+ *   - gates are applied in list order, gate 0 first (Gates_block.cpp:683-708);
+ *   - parameters are the reference's raw parameters: half-angles for the
+ *     RX/RY/RZ/CRY angle and for the U3 theta (U3.cpp:49-51, RX.cpp:24).
+ */
+#ifndef SQ_ORACLE_H
+#define SQ_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { double re, im; } or_c128;
+
+/* Gate type ids use the reference's gate_type enum values (Gate.h:39-51). */
+enum {
+    OR_CZ = 4, OR_CNOT = 5, OR_U3 = 7, OR_RY = 8, OR_RX = 9, OR_RZ = 10, OR_CRY = 14
+};
+
+/* Cost-function variants use the reference's cost_function_type values
+ * (Optimization_Interface.h:43-45).  OR_COST_ABS_TRACE (1 - |Tr|/m) has no
+ * counterpart in the reference and gets an id outside its range. */
+enum {
+    OR_COST_FROBENIUS = 0, OR_COST_HS = 3, OR_COST_INFIDELITY = 9, OR_COST_ABS_TRACE = 100
+};
+
+typedef struct { int type, target, control, param_start; } or_gate;
+
+int  or_param_num(int type);
+
+/* 2x2 kernels, row-major [u00, u01, u10, u11]. */
+void or_gate_kernel(int type, const double* p, or_c128 k[4]);
+void or_deriv_kernel(int type, const double* p, int pidx, or_c128 k[4]);
+
+/* Pair-row kernel on a rows x cols matrix (apply_kernel_to_input.cpp:35-115). */
+void or_apply_kernel(or_c128* M, int rows, int cols, int stride, const or_c128 k[4],
+                     int target, int control, int deriv);
+/* CNOT row swap and CZ sign flip (apply_dedicated_gate_kernel_to_input.cpp:46-93, 221-254). */
+void or_apply_x(or_c128* M, int rows, int cols, int stride, int target, int control);
+void or_apply_z(or_c128* M, int rows, int cols, int stride, int target, int control);
+
+/* One gate / the whole circuit, in place. Returns 0, or -1 on a bad gate. */
+int or_apply_gate(const or_gate* g, const double* params, or_c128* M, int rows, int cols, int stride);
+int or_apply_circuit(const or_gate* gates, int ng, const double* params,
+                     or_c128* M, int rows, int cols, int stride);
+
+/* apply_derivate_to: out holds P dense rows x cols matrices (row stride = cols),
+ * out[k] = (dC/dp_k) . in.  Returns 0 / -1. */
+int or_apply_derivate(const or_gate* gates, int ng, const double* params,
+                      const or_c128* in, int rows, int cols, int stride, or_c128* out);
+
+/* Trace helpers and the cost variants (N_Qubit_Decomposition_Cost_Function.cpp). */
+or_c128 or_trace_offset(const or_c128* M, int cols, int stride, int trace_offset);
+or_c128 or_trace_plain(const or_c128* M, int cols, int stride);
+double or_cost(const or_c128* M, int cols, int stride, int variant, int trace_offset);
+
+/* A decomposition problem: target (rows x cols), circuit, cost variant. */
+typedef struct {
+    const or_gate* gates; int ng; int P; int qbit_num;
+    const or_c128* target; int rows, cols, stride;
+    int variant, trace_offset;
+} or_problem;
+
+double or_optimization_problem(const or_problem* pr, const double* params);
+/* f0 and grad (P entries) from the materialised derivative matrices
+ * (Optimization_Interface.cpp:1380-1478). */
+int or_optimization_problem_combined(const or_problem* pr, const double* params,
+                                     double* f0, double* grad);
+/* B costs, params is B x P row-major; nthreads <= 1 runs sequentially. */
+void or_optimization_problem_batched(const or_problem* pr, const double* params, int B,
+                                     int nthreads, double* costs);
+/* B cost+grad evaluations, threaded over B. */
+void or_optimization_problem_combined_batched(const or_problem* pr, const double* params, int B,
+                                              int nthreads, double* f0, double* grad);
+
+/* BFGS with Powell's Z update and the Wolfe line search
+ * (BFGS_Powell.cpp:70-386, grad_descend.cpp:127-272, 396-448).
+ * Runs on x in place, returns the final cost; *calls receives the number of
+ * cost+gradient evaluations.  status codes mirror the reference's. */
+double or_bfgs_powell(const or_problem* pr, double* x, long max_calls, long* calls, int* status);
+
+/* mt19937 + uniform_real_distribution(0, 2pi) as consumed by the reference's
+ * decomposition driver (Decomposition_Base.cpp:32-42,426-436), libstdc++ semantics. */
+typedef struct { unsigned int mt[624]; int idx; } or_mt19937;
+void   or_mt_seed(or_mt19937* g, unsigned int seed);
+unsigned int or_mt_next(or_mt19937* g);
+double or_mt_uniform_2pi(or_mt19937* g);
+
+/* N_Qubit_Decomposition_custom + solve_optimization_problem + the BFGS engine
+ * (N_Qubit_Decomposition_custom.cpp:59-190, Decomposition_Base.cpp:375-709,
+ * BFGS.cpp:42-152), whole circuit optimised as one block.  x receives the
+ * optimised parameters; returns the best cost; *calls counts evaluations. */
+double or_decompose_custom(const or_problem* pr, unsigned int seed, int initial_guess,
+                           long max_outer_iterations, long iteration_loops,
+                           long max_inner_iterations, double optimization_tolerance,
+                           double convergence_threshold, double* x, long* calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

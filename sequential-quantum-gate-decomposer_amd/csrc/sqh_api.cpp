@@ -1,0 +1,525 @@
+// sqh_api.cpp -- the C ABI (include/sqhip.h): contexts, device buffers and the
+// host-side orchestration of the kernels for apply_to / apply_derivate_to /
+// optimization_problem{,_batched,_combined,_combined_unitary}.
+#include "sqh_context.h"
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+using namespace sqh;
+
+namespace sqh {
+
+int fail(sqh_ctx* c, int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+int m_pad_for(int N, int m) {
+    const int cw = tile_columns_per_wave(N);
+    if (cw <= 0) return m;
+    if (m >= cw) return (m + cw - 1) / cw * cw;
+    int p = 1;
+    while (p < m) p <<= 1;
+    return p;
+}
+
+int waves_per_batch(int N, int m_pad) {
+    const int cw = tile_columns_per_wave(N);
+    return m_pad >= cw ? m_pad / cw : 1;
+}
+
+bool uses_offset(int variant) { return variant == CV_FROBENIUS || variant == CV_ABS_TRACE; }
+
+static bool valid_variant(int v) {
+    return v == CV_FROBENIUS || v == CV_HS || v == CV_INFIDELITY || v == CV_ABS_TRACE;
+}
+
+int require_circuit(sqh_ctx* c) {
+    if (!c->have_circuit) return fail(c, SQH_ERR_STATE, "no circuit set");
+    return SQH_OK;
+}
+
+int require_target(sqh_ctx* c) {
+    if (!c->have_circuit) return fail(c, SQH_ERR_STATE, "no circuit set");
+    if (!c->have_target) return fail(c, SQH_ERR_STATE, "no target unitary set");
+    if (c->rows != (1 << c->prog.N))
+        return fail(c, SQH_ERR_ARG, "target has %d rows but the circuit acts on %d qubits", c->rows, c->prog.N);
+    const int off = uses_offset(c->variant) ? c->trace_offset : 0;
+    if (c->cols + off > c->rows)
+        return fail(c, SQH_ERR_ARG, "trace offset %d + %d columns exceeds %d rows", off, c->cols, c->rows);
+    return SQH_OK;
+}
+
+int build_kernels(sqh_ctx* c, const double* d_params, int B, bool f32) {
+    const int ns = (int)c->prog.slots.size();
+    const size_t esz = f32 ? 8 : 16;
+    HIPTRY(c, c->d_kern.ensure((size_t)B * std::max(ns, 1) * 4 * esz));
+    HIPTRY(c, launch_build_kernels(f32, d_params, c->prog.P, B, c->d_slots.as<Slot>(), ns, c->d_kern.p, c->stream));
+    return SQH_OK;
+}
+
+TileLaunch base_launch(sqh_ctx* c, bool f32) {
+    TileLaunch t;
+    t.N = c->prog.N;
+    t.f32 = f32;
+    t.ops = c->d_ops.as<Op>();
+    t.nops = (int)c->prog.ops.size();
+    t.lays = c->d_lays.as<Layout>();
+    t.kern = c->d_kern.p;
+    t.ns = (int)c->prog.slots.size();
+    t.slots = c->d_slots.as<Slot>();
+    return t;
+}
+
+// One streaming HBM pass per gate over B matrices, in gate order.  kern holds the
+// [B][ns][4] kernel tables (kern_bstride = 0 when the parameters are shared).
+int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
+                 long long kern_bstride) {
+    int slot = 0;
+    for (const GateRec& g : c->gates) {
+        int kind, control = -1;
+        switch (g.type) {
+        case GT_CNOT: kind = OP_X; control = g.control; break;
+        case GT_CZ: kind = OP_Z; control = g.control; break;
+        case GT_CRY: kind = OP_CU; control = g.control; break;
+        default: kind = OP_U; break;
+        }
+        const int s = (kind == OP_U || kind == OP_CU) ? slot++ : 0;
+        TimedLaunch tl(c);
+        HIPTRY(c, launch_stream_gate(f32, mats, bstride, stride, rows, cols, B, kind, g.target, control, c->d_kern.p,
+                                     kern_bstride, s, c->stream));
+    }
+    return SQH_OK;
+}
+
+// costs (and, with grad, the gradient) for B parameter vectors on the device.
+int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad) {
+    int rc = require_target(c);
+    if (rc) return rc;
+    if (B <= 0) return fail(c, SQH_ERR_ARG, "batch size must be positive");
+    if (precision != SQH_PREC_F64 && precision != SQH_PREC_F32) return fail(c, SQH_ERR_ARG, "bad precision");
+    const bool f32 = precision == SQH_PREC_F32;
+    const int N = c->prog.N, m = c->cols, ns = (int)c->prog.slots.size();
+    const int off = uses_offset(c->variant) ? c->trace_offset : 0;
+    if ((rc = build_kernels(c, d_params, B, f32))) return rc;
+    HIPTRY(c, c->d_diag.ensure((size_t)B * m * 2 * sizeof(double)));
+    const void* U = f32 ? c->d_U32.p : c->d_U64.p;
+    if (N <= kMaxTileQubits) {
+        TileLaunch t = base_launch(c, f32);
+        t.in = U;
+        t.in_bstride = 0;
+        t.in_stride = m;
+        t.m = m;
+        t.m_pad = m_pad_for(N, m);
+        t.B = B;
+        t.kern_bstride = (long long)ns * 4;
+        t.trace_off = off;
+        t.diag = c->d_diag.as<double>();
+        const int wpb = waves_per_batch(N, t.m_pad);
+        if (d_grad) {
+            HIPTRY(c, c->d_mpart.ensure((size_t)B * wpb * std::max(ns, 1) * 8 * sizeof(double)));
+            t.mpart = c->d_mpart.as<double>();
+            {
+                TimedLaunch tl(c);
+                HIPTRY(c, launch_tile_grad(t, c->stream));
+            }
+            HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, t.mpart, wpb, ns, t.slots, d_params, c->prog.P,
+                                    d_grad, c->stream));
+        } else {
+            t.mode = 1;
+            {
+                TimedLaunch tl(c);
+                HIPTRY(c, launch_tile_forward(t, c->stream));
+            }
+            HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, nullptr, 0, ns, t.slots, d_params, c->prog.P,
+                                    nullptr, c->stream));
+        }
+        return SQH_OK;
+    }
+    if (d_grad) return fail(c, SQH_ERR_ARG, "gradient evaluation supports up to %d qubits", kMaxTileQubits);
+    // Streaming path: B copies of the target, one pass per gate, then the diagonal.
+    const size_t esz = f32 ? 8 : 16;
+    const long long msz = (long long)c->rows * m;
+    HIPTRY(c, c->d_mat.ensure((size_t)B * msz * esz));
+    HIPTRY(c, launch_broadcast_copy(f32, U, c->rows, m, m, c->d_mat.p, msz, B, c->stream));
+    if ((rc = stream_apply(c, c->d_mat.p, c->rows, m, m, msz, B, f32, (long long)ns * 4))) return rc;
+    HIPTRY(c, launch_extract_diag(f32, c->d_mat.p, msz, m, m, B, off, c->d_diag.as<double>(), c->stream));
+    HIPTRY(c, launch_finish(c->d_diag.as<double>(), m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(),
+                            d_params, c->prog.P, nullptr, c->stream));
+    return SQH_OK;
+}
+
+int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,
+              long long bstride, int B, bool f32, bool force_stream) {
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    const int N = c->prog.N;
+    if (rows != (1 << N)) return fail(c, SQH_ERR_ARG, "matrix has %d rows, the circuit needs %d", rows, 1 << N);
+    if (cols <= 0 || stride < cols || B <= 0) return fail(c, SQH_ERR_ARG, "bad matrix shape");
+    if (pbstride != 0 && pbstride != c->prog.P) return fail(c, SQH_ERR_ARG, "parameter batch stride must be 0 or P");
+    const int ns = (int)c->prog.slots.size();
+    const int Bk = pbstride ? B : 1;
+    if ((rc = build_kernels(c, d_params, Bk, f32))) return rc;
+    const long long kb = pbstride ? (long long)ns * 4 : 0;
+    if (N <= kMaxTileQubits && !force_stream) {
+        TileLaunch t = base_launch(c, f32);
+        t.mode = 0;
+        t.in = d_inout;
+        t.in_bstride = bstride;
+        t.in_stride = stride;
+        t.out = d_inout;
+        t.out_bstride = bstride;
+        t.out_stride = stride;
+        t.m = cols;
+        t.m_pad = m_pad_for(N, cols);
+        t.B = B;
+        t.kern_bstride = kb;
+        TimedLaunch tl(c);
+        HIPTRY(c, launch_tile_forward(t, c->stream));
+        return SQH_OK;
+    }
+    return stream_apply(c, d_inout, rows, cols, stride, bstride, B, f32, kb);
+}
+
+int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, int cols, int stride, void* d_out) {
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    const int N = c->prog.N;
+    if (rows != (1 << N)) return fail(c, SQH_ERR_ARG, "matrix has %d rows, the circuit needs %d", rows, 1 << N);
+    if (cols <= 0 || stride < cols) return fail(c, SQH_ERR_ARG, "bad matrix shape");
+    if (N > kMaxTileQubits)
+        return fail(c, SQH_ERR_ARG, "apply_derivate_to supports up to %d qubits", kMaxTileQubits);
+    if (c->prog.slots.empty()) return SQH_OK;
+    if ((rc = build_kernels(c, d_params, 1, false))) return rc;
+    TileLaunch t = base_launch(c, false);
+    t.in = d_in;
+    t.in_stride = stride;
+    t.out = d_out;
+    t.out_bstride = (long long)rows * cols;
+    t.m = cols;
+    t.m_pad = m_pad_for(N, cols);
+    t.params = d_params;
+    TimedLaunch tl(c);
+    HIPTRY(c, launch_tile_deriv(t, c->stream));
+    return SQH_OK;
+}
+
+}  // namespace sqh
+
+extern "C" {
+
+const char* sqh_version(void) { return "sqhip 0.1 (gfx950)"; }
+
+int sqh_create(int device, sqh_ctx** out) {
+    if (!out) return SQH_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return SQH_ERR_DEVICE;
+    if (device < 0 || device >= n) return SQH_ERR_ARG;
+    if (hipSetDevice(device) != hipSuccess) return SQH_ERR_DEVICE;
+    sqh_ctx* c = new (std::nothrow) sqh_ctx();
+    if (!c) return SQH_ERR_NOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SQH_ERR_DEVICE;
+    }
+    *out = c;
+    return SQH_OK;
+}
+
+void sqh_destroy(sqh_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    delete c;
+}
+
+const char* sqh_last_error(const sqh_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+void* sqh_get_stream(sqh_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int sqh_synchronize(sqh_ctx* c) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
+int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (n_gates < 0 || (n_gates > 0 && !gates)) return fail(c, SQH_ERR_ARG, "bad gate array");
+    std::vector<GateRec> g(n_gates);
+    for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
+    Program p;
+    std::string err;
+    if (!compile_program(qbit_num, g, p, err)) {
+        c->have_circuit = false;
+        return fail(c, SQH_ERR_GATE, "%s", err.c_str());
+    }
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    HIPTRY(c, c->d_ops.ensure(std::max<size_t>(p.ops.size(), 1) * sizeof(Op)));
+    HIPTRY(c, c->d_lays.ensure(p.layouts.size() * sizeof(Layout)));
+    HIPTRY(c, c->d_slots.ensure(std::max<size_t>(p.slots.size(), 1) * sizeof(Slot)));
+    if (!p.ops.empty())
+        HIPTRY(c, hipMemcpy(c->d_ops.p, p.ops.data(), p.ops.size() * sizeof(Op), hipMemcpyHostToDevice));
+    HIPTRY(c, hipMemcpy(c->d_lays.p, p.layouts.data(), p.layouts.size() * sizeof(Layout), hipMemcpyHostToDevice));
+    if (!p.slots.empty())
+        HIPTRY(c, hipMemcpy(c->d_slots.p, p.slots.data(), p.slots.size() * sizeof(Slot), hipMemcpyHostToDevice));
+    c->prog = std::move(p);
+    c->gates = std::move(g);
+    c->have_circuit = true;
+    c->err.clear();
+    return SQH_OK;
+}
+
+int sqh_get_parameter_num(const sqh_ctx* c) {
+    if (!c || !c->have_circuit) return SQH_ERR_STATE;
+    return c->prog.P;
+}
+
+int sqh_set_target(sqh_ctx* c, const double* U, int rows, int cols, int stride) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!U || rows <= 0 || cols <= 0 || cols > rows || stride < cols || (rows & (rows - 1)))
+        return fail(c, SQH_ERR_ARG, "target must be 2^n x m (m <= 2^n) with stride >= m");
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    const size_t n = (size_t)rows * cols;
+    HIPTRY(c, c->d_U64.ensure(n * 16));
+    HIPTRY(c, c->d_U32.ensure(n * 8));
+    HIPTRY(c, hipMemcpy2D(c->d_U64.p, (size_t)cols * 16, U, (size_t)stride * 16, (size_t)cols * 16, rows,
+                          hipMemcpyHostToDevice));
+    HIPTRY(c, launch_convert(true, c->d_U64.p, c->d_U32.p, (long long)n * 2, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    c->rows = rows;
+    c->cols = cols;
+    c->have_target = true;
+    return SQH_OK;
+}
+
+int sqh_set_trace_offset(sqh_ctx* c, int trace_offset) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (trace_offset < 0) return fail(c, SQH_ERR_ARG, "trace offset must be non-negative");
+    c->trace_offset = trace_offset;
+    return SQH_OK;
+}
+
+int sqh_set_cost_function(sqh_ctx* c, int variant) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!valid_variant(variant)) return fail(c, SQH_ERR_ARG, "cost function variant %d not supported", variant);
+    c->variant = variant;
+    return SQH_OK;
+}
+
+int sqh_apply_to(sqh_ctx* c, const void* params, void* inout, int rows, int cols, int stride, int is_f32) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    if (!inout || (!params && c->prog.P > 0)) return fail(c, SQH_ERR_ARG, "null buffer");
+    if (rows <= 0 || cols <= 0 || stride < cols) return fail(c, SQH_ERR_ARG, "bad matrix shape");
+    HIPTRY(c, hipSetDevice(c->device));
+    const int P = c->prog.P;
+    std::vector<double> p64(std::max(P, 1));
+    for (int i = 0; i < P; ++i) p64[i] = is_f32 ? (double)((const float*)params)[i] : ((const double*)params)[i];
+    const size_t esz = is_f32 ? 8 : 16;
+    HIPTRY(c, c->d_params.ensure(p64.size() * sizeof(double)));
+    HIPTRY(c, c->d_mat.ensure((size_t)rows * cols * esz));
+    HIPTRY(c, hipMemcpyAsync(c->d_params.p, p64.data(), p64.size() * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPTRY(c, hipMemcpy2DAsync(c->d_mat.p, (size_t)cols * esz, inout, (size_t)stride * esz, (size_t)cols * esz, rows,
+                               hipMemcpyHostToDevice, c->stream));
+    if ((rc = apply_dev(c, c->d_params.as<double>(), 0, c->d_mat.p, rows, cols, cols, 0, 1, is_f32 != 0, false)))
+        return rc;
+    HIPTRY(c, hipMemcpy2DAsync(inout, (size_t)stride * esz, c->d_mat.p, (size_t)cols * esz, (size_t)cols * esz, rows,
+                               hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
+int sqh_apply_to_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,
+                     long long bstride, int B, int is_f32) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, false);
+}
+
+int sqh_apply_to_stream_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
+                            int stride, long long bstride, int B, int is_f32) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, true);
+}
+
+int sqh_apply_derivate_to(sqh_ctx* c, const double* params, const double* in, int rows, int cols, int stride,
+                          double* out) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    if (!in || !out || (!params && c->prog.P > 0)) return fail(c, SQH_ERR_ARG, "null buffer");
+    if (rows <= 0 || cols <= 0 || stride < cols) return fail(c, SQH_ERR_ARG, "bad matrix shape");
+    HIPTRY(c, hipSetDevice(c->device));
+    const int P = c->prog.P;
+    if (P == 0) return SQH_OK;
+    const size_t msz = (size_t)rows * cols;
+    HIPTRY(c, c->d_params.ensure((size_t)P * sizeof(double)));
+    HIPTRY(c, c->d_mat.ensure(msz * 16));
+    HIPTRY(c, c->d_out.ensure((size_t)P * msz * 16));
+    HIPTRY(c, hipMemcpyAsync(c->d_params.p, params, (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemcpy2DAsync(c->d_mat.p, (size_t)cols * 16, in, (size_t)stride * 16, (size_t)cols * 16, rows,
+                               hipMemcpyHostToDevice, c->stream));
+    if ((rc = deriv_dev(c, c->d_params.as<double>(), c->d_mat.p, rows, cols, cols, c->d_out.p))) return rc;
+    HIPTRY(c, hipMemcpyAsync(out, c->d_out.p, (size_t)P * msz * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
+int sqh_apply_derivate_to_dev(sqh_ctx* c, const double* d_params, const double* d_in, int rows, int cols, int stride,
+                              double* d_out) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    return deriv_dev(c, d_params, d_in, rows, cols, stride, d_out);
+}
+
+int sqh_optimization_problem_batch_dev(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    return eval_batch(c, d_params, B, precision, d_costs, nullptr);
+}
+
+int sqh_optimization_problem_combined_batch_dev(sqh_ctx* c, const double* d_params, int B, int precision,
+                                                double* d_f0, double* d_grad) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    if (!d_grad) return fail(c, SQH_ERR_ARG, "null gradient buffer");
+    return eval_batch(c, d_params, B, precision, d_f0, d_grad);
+}
+
+static int host_batch(sqh_ctx* c, const double* params, int B, int precision, double* costs, double* grad) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = require_target(c);
+    if (rc) return rc;
+    if (B <= 0 || !costs || (!params && c->prog.P > 0)) return fail(c, SQH_ERR_ARG, "bad batch arguments");
+    HIPTRY(c, hipSetDevice(c->device));
+    const int P = c->prog.P;
+    const size_t np = (size_t)B * std::max(P, 1);
+    HIPTRY(c, c->d_params.ensure(np * sizeof(double)));
+    HIPTRY(c, c->d_res.ensure(((size_t)B + (grad ? np : 0)) * sizeof(double)));
+    if (P > 0)
+        HIPTRY(c, hipMemcpyAsync(c->d_params.p, params, (size_t)B * P * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+    double* d_costs = c->d_res.as<double>();
+    double* d_grad = grad ? d_costs + B : nullptr;
+    if ((rc = eval_batch(c, c->d_params.as<double>(), B, precision, d_costs, d_grad))) return rc;
+    HIPTRY(c, hipMemcpyAsync(costs, d_costs, (size_t)B * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (grad && P > 0)
+        HIPTRY(c, hipMemcpyAsync(grad, d_grad, (size_t)B * P * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
+int sqh_optimization_problem_batch(sqh_ctx* c, const double* params, int B, int precision, double* costs) {
+    return host_batch(c, params, B, precision, costs, nullptr);
+}
+
+int sqh_optimization_problem_combined_batch(sqh_ctx* c, const double* params, int B, int precision, double* f0,
+                                            double* grad) {
+    if (!grad) return c ? fail(c, SQH_ERR_ARG, "null gradient buffer") : SQH_ERR_ARG;
+    return host_batch(c, params, B, precision, f0, grad);
+}
+
+int sqh_optimization_problem_combined_unitary(sqh_ctx* c, const double* params, double* out_unitary,
+                                              double* out_derivs) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = require_target(c);
+    if (rc) return rc;
+    if (!out_unitary || (!params && c->prog.P > 0)) return fail(c, SQH_ERR_ARG, "null buffer");
+    HIPTRY(c, hipSetDevice(c->device));
+    const int P = c->prog.P, rows = c->rows, cols = c->cols;
+    const size_t msz = (size_t)rows * cols;
+    HIPTRY(c, c->d_params.ensure((size_t)std::max(P, 1) * sizeof(double)));
+    HIPTRY(c, c->d_mat.ensure(msz * 16));
+    if (P > 0)
+        HIPTRY(c, hipMemcpyAsync(c->d_params.p, params, (size_t)P * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (P > 0 && out_derivs) {
+        HIPTRY(c, c->d_out.ensure((size_t)P * msz * 16));
+        if ((rc = deriv_dev(c, c->d_params.as<double>(), c->d_U64.p, rows, cols, cols, c->d_out.p))) return rc;
+    }
+    HIPTRY(c, hipMemcpyAsync(c->d_mat.p, c->d_U64.p, msz * 16, hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = apply_dev(c, c->d_params.as<double>(), 0, c->d_mat.p, rows, cols, cols, 0, 1, false, false))) return rc;
+    HIPTRY(c, hipMemcpyAsync(out_unitary, c->d_mat.p, msz * 16, hipMemcpyDeviceToHost, c->stream));
+    if (P > 0 && out_derivs)
+        HIPTRY(c, hipMemcpyAsync(out_derivs, c->d_out.p, (size_t)P * msz * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
+int sqh_set_kernel_timing(sqh_ctx* c, int enable) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->timing = enable != 0;
+    return SQH_OK;
+}
+
+int sqh_get_kernel_timing(sqh_ctx* c, double* total_ms, int* launches) {
+    if (!c || !total_ms || !launches) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (auto& e : c->events) {
+        float ms = 0.0f;
+        HIPTRY(c, hipEventElapsedTime(&ms, e.first, e.second));
+        tot += ms;
+    }
+    *total_ms = tot;
+    *launches = (int)c->events.size();
+    for (auto& e : c->events) {
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    c->events.clear();
+    return SQH_OK;
+}
+
+int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* ops_out, int ops_cap,
+                        void* layouts_out, int layouts_cap, int* n_ops, int* n_layouts) {
+    if (n_gates < 0 || (n_gates > 0 && !gates)) return SQH_ERR_ARG;
+    std::vector<GateRec> g(n_gates);
+    for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
+    Program p;
+    std::string err;
+    if (!compile_program(qbit_num, g, p, err)) return SQH_ERR_GATE;
+    if (n_ops) *n_ops = (int)p.ops.size();
+    if (n_layouts) *n_layouts = (int)p.layouts.size();
+    if (ops_out && ops_cap > 0)
+        std::memcpy(ops_out, p.ops.data(), sizeof(Op) * std::min<size_t>(p.ops.size(), (size_t)ops_cap));
+    if (layouts_out && layouts_cap > 0)
+        std::memcpy(layouts_out, p.layouts.data(),
+                    sizeof(Layout) * std::min<size_t>(p.layouts.size(), (size_t)layouts_cap));
+    return SQH_OK;
+}
+
+}  // extern "C"

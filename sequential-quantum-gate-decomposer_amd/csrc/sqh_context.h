@@ -1,0 +1,124 @@
+// sqh_context.h -- the context behind the opaque sqh_ctx handle, and helpers shared
+// by the C ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/sqhip.h"
+#include "sqh_kernels.h"
+#include "sqh_program.h"
+
+namespace sqh {
+
+// Grow-only device allocation.
+struct DevBuf {
+    void* p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= n) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            n = 0;
+            if (e != hipSuccess) return e;
+        }
+        const size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipMalloc(&p, want);
+        if (e == hipSuccess) n = want;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace sqh
+
+struct sqh_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+
+    bool have_circuit = false;
+    sqh::Program prog;
+    std::vector<sqh::GateRec> gates;
+    sqh::DevBuf d_ops, d_lays, d_slots;
+
+    bool have_target = false;
+    int rows = 0, cols = 0;
+    sqh::DevBuf d_U64, d_U32;
+    int trace_offset = 0;
+    int variant = sqh::CV_FROBENIUS;
+
+    sqh::DevBuf d_params, d_kern, d_diag, d_mpart, d_res, d_mat, d_out;
+
+    // Kernel timers: HIP events around the dominant kernel of each call.
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+
+    ~sqh_ctx() {
+        for (auto& e : events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
+                               &d_res, &d_mat, &d_out})
+            b->release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace sqh {
+
+int fail(sqh_ctx* c, int code, const char* fmt, ...);
+
+#define HIPTRY(ctx, expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return ::sqh::fail((ctx), e_ == hipErrorOutOfMemory ? SQH_ERR_NOMEM : SQH_ERR_DEVICE, "%s: %s", \
+                               #expr, hipGetErrorString(e_));                                              \
+    } while (0)
+
+int m_pad_for(int N, int m);
+int waves_per_batch(int N, int m_pad);
+bool uses_offset(int variant);
+int require_circuit(sqh_ctx* c);
+int require_target(sqh_ctx* c);
+int build_kernels(sqh_ctx* c, const double* d_params, int B, bool f32);
+TileLaunch base_launch(sqh_ctx* c, bool f32);
+int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
+                 long long kern_bstride);
+int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad);
+int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,
+              long long bstride, int B, bool f32, bool force_stream);
+int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, int cols, int stride, void* d_out);
+
+// Brackets one kernel launch with timer events when timing is enabled.
+struct TimedLaunch {
+    sqh_ctx* c;
+    hipEvent_t a = nullptr, b = nullptr;
+    explicit TimedLaunch(sqh_ctx* ctx) : c(ctx) {
+        if (c->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+            (void)hipEventRecord(a, c->stream);
+    }
+    ~TimedLaunch() {
+        if (a && b) {
+            (void)hipEventRecord(b, c->stream);
+            c->events.emplace_back(a, b);
+        }
+    }
+};
+
+}  // namespace sqh

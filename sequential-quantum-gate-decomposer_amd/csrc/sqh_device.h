@@ -1,0 +1,773 @@
+// sqh_device.h -- gfx950 device code (included by the sqh_k_*.hip translation units) for the gate-application / cost / gradient path.
+//
+// Register-tile engine (N <= 10 qubits): one wavefront holds a slab of CW columns of
+// the d x m matrix (d = 2^N); each lane holds NR = 2^K complex entries (K = min(N,4)
+// register qubits) of one column, L = 2^(N-K) lanes share a column.  Gates whose
+// target is a register qubit are pure in-register FMA; the host-compiled program
+// inserts REMAP ops (an LDS round trip inside the wavefront) so every target is a
+// register qubit when its gate runs.  The whole circuit is applied with one HBM read
+// of the input slab and one write (apply_to) or none (cost/gradient: only diagonal
+// entries and per-gate 2x2 correlations leave the wavefront).
+//
+// Streaming kernels (any N): one launch per gate, a coalesced pass over the paired
+// rows of every matrix in HBM -- the unfused path the reference's pair-row kernel
+// (apply_kernel_to_input.cpp:35-115) describes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "sqh_internal.h"
+
+namespace sqh {
+
+template <typename R>
+struct alignas(2 * sizeof(R)) cpx {
+    R re, im;
+};
+
+template <typename R>
+__device__ __forceinline__ cpx<R> mk(R re, R im) {
+    cpx<R> r;
+    r.re = re;
+    r.im = im;
+    return r;
+}
+
+// k0*a + k1*b, complex, fused multiply-adds.
+template <typename R>
+__device__ __forceinline__ cpx<R> mac2(cpx<R> k0, cpx<R> a, cpx<R> k1, cpx<R> b) {
+    cpx<R> r;
+    r.re = fma(k0.re, a.re, fma(-k0.im, a.im, fma(k1.re, b.re, -k1.im * b.im)));
+    r.im = fma(k0.re, a.im, fma(k0.im, a.re, fma(k1.re, b.im, k1.im * b.re)));
+    return r;
+}
+
+// acc += x*y (complex).
+template <typename R>
+__device__ __forceinline__ void cacc(cpx<R>& acc, cpx<R> x, cpx<R> y) {
+    acc.re = fma(x.re, y.re, fma(-x.im, y.im, acc.re));
+    acc.im = fma(x.re, y.im, fma(x.im, y.re, acc.im));
+}
+
+template <int N>
+struct Geo {
+    static constexpr int K = N < 4 ? N : 4;   // register qubits
+    static constexpr int NR = 1 << K;         // complex registers per lane
+    static constexpr int L = 1 << (N - K);    // lanes per column
+    static constexpr int CW = 64 / L;         // columns per wavefront
+    static constexpr int D = 1 << N;
+    static constexpr int LDS = (N > K) ? D * CW : 1;
+};
+
+// ---------------------------------------------------------------------------
+// Gate kernels from raw parameters (device restatement of the 2x2 builders:
+// gate_kernel_templates.h:193-315, 509-601; U3.cpp:89-129; RX/RY/RZ/CRY.cpp).
+// ---------------------------------------------------------------------------
+__device__ inline void dev_gate_kernel(int type, const double* p, cpx<double> k[4]) {
+    double s, c;
+    switch (type) {
+    case GT_U3: {
+        double sp, cp, sl, cl;
+        sincos(p[0], &s, &c);
+        sincos(p[1], &sp, &cp);
+        sincos(p[2], &sl, &cl);
+        const double spl = sp * cl + cp * sl, cpl = cp * cl - sp * sl;
+        k[0] = mk(c, 0.0);
+        k[1] = mk(-s * cl, -s * sl);
+        k[2] = mk(s * cp, s * sp);
+        k[3] = mk(c * cpl, c * spl);
+        return;
+    }
+    case GT_RX:
+        sincos(p[0], &s, &c);
+        k[0] = mk(c, 0.0); k[1] = mk(0.0, -s); k[2] = mk(0.0, -s); k[3] = mk(c, 0.0);
+        return;
+    case GT_RY:
+    case GT_CRY:
+        sincos(p[0], &s, &c);
+        k[0] = mk(c, 0.0); k[1] = mk(-s, 0.0); k[2] = mk(s, 0.0); k[3] = mk(c, 0.0);
+        return;
+    case GT_RZ:
+        sincos(p[0], &s, &c);
+        k[0] = mk(c, -s); k[1] = mk(0.0, 0.0); k[2] = mk(0.0, 0.0); k[3] = mk(c, s);
+        return;
+    default:
+        k[0] = k[1] = k[2] = k[3] = mk(0.0, 0.0);
+    }
+}
+
+// Derivative w.r.t. raw parameter pidx.
+__device__ inline void dev_deriv_kernel(int type, const double* p, int pidx, cpx<double> k[4]) {
+    double s, c;
+    switch (type) {
+    case GT_U3: {
+        double sp, cp, sl, cl;
+        sincos(p[0], &s, &c);
+        sincos(p[1], &sp, &cp);
+        sincos(p[2], &sl, &cl);
+        const double spl = sp * cl + cp * sl, cpl = cp * cl - sp * sl;
+        if (pidx == 0) {
+            k[0] = mk(-s, 0.0); k[1] = mk(-c * cl, -c * sl);
+            k[2] = mk(c * cp, c * sp); k[3] = mk(-s * cpl, -s * spl);
+        } else if (pidx == 1) {
+            k[0] = mk(0.0, 0.0); k[1] = mk(0.0, 0.0);
+            k[2] = mk(-s * sp, s * cp); k[3] = mk(-c * spl, c * cpl);
+        } else {
+            k[0] = mk(0.0, 0.0); k[1] = mk(s * sl, -s * cl);
+            k[2] = mk(0.0, 0.0); k[3] = mk(-c * spl, c * cpl);
+        }
+        return;
+    }
+    case GT_RX:
+        sincos(p[0], &s, &c);
+        k[0] = mk(-s, 0.0); k[1] = mk(0.0, -c); k[2] = mk(0.0, -c); k[3] = mk(-s, 0.0);
+        return;
+    case GT_RY:
+    case GT_CRY:
+        sincos(p[0], &s, &c);
+        k[0] = mk(-s, 0.0); k[1] = mk(-c, 0.0); k[2] = mk(c, 0.0); k[3] = mk(-s, 0.0);
+        return;
+    case GT_RZ:
+        sincos(p[0], &s, &c);
+        k[0] = mk(-s, -c); k[1] = mk(0.0, 0.0); k[2] = mk(0.0, 0.0); k[3] = mk(-s, c);
+        return;
+    default:
+        k[0] = k[1] = k[2] = k[3] = mk(0.0, 0.0);
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ cpx<R> cvt(cpx<double> a) {
+    return mk<R>((R)a.re, (R)a.im);
+}
+
+// ---------------------------------------------------------------------------
+// Register-tile primitives
+// ---------------------------------------------------------------------------
+template <int K, typename F>
+__device__ __forceinline__ void dispatch_t(int t, F&& f) {
+    switch (t) {
+    case 0: if constexpr (K > 0) f(std::integral_constant<int, 0>{}); break;
+    case 1: if constexpr (K > 1) f(std::integral_constant<int, 1>{}); break;
+    case 2: if constexpr (K > 2) f(std::integral_constant<int, 2>{}); break;
+    case 3: if constexpr (K > 3) f(std::integral_constant<int, 3>{}); break;
+    default: break;
+    }
+}
+
+// Mask of registers whose row has the control bit set (bit i <-> register i).
+template <int N>
+__device__ __forceinline__ uint32_t ctrl_mask(int cmode, int c, int s) {
+    constexpr int NR = Geo<N>::NR;
+    constexpr uint32_t all = (NR >= 32) ? 0xffffffffu : ((1u << NR) - 1u);
+    if (cmode == CM_NONE) return all;
+    if (cmode == CM_LANE) return ((s >> c) & 1) ? all : 0u;
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < NR; ++i) m |= ((uint32_t)((i >> c) & 1)) << i;
+    return m;
+}
+
+// v <- K v on the register pairs (i, i | 2^T) whose bit i is set in mask; with
+// DERIV, pairs outside the mask are zeroed (apply_kernel_to_input.cpp:93-97).
+template <int T, bool MASKED, bool DERIV, typename R, int NR>
+__device__ __forceinline__ void pairs_apply(cpx<R> (&v)[NR], const cpx<R> k[4], uint32_t mask) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        if (i & (1 << T)) continue;
+        const int j = i | (1 << T);
+        if (MASKED && !((mask >> i) & 1u)) {
+            if (DERIV) {
+                v[i] = mk<R>(0, 0);
+                v[j] = mk<R>(0, 0);
+            }
+            continue;
+        }
+        const cpx<R> a = v[i], b = v[j];
+        v[i] = mac2(k[0], a, k[1], b);
+        v[j] = mac2(k[2], a, k[3], b);
+    }
+}
+
+template <int T, typename R, int NR>
+__device__ __forceinline__ void pairs_swap(cpx<R> (&v)[NR], uint32_t mask) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        if (i & (1 << T)) continue;
+        const int j = i | (1 << T);
+        if (!((mask >> i) & 1u)) continue;
+        const cpx<R> a = v[i];
+        v[i] = v[j];
+        v[j] = a;
+    }
+}
+
+template <int T, typename R, int NR>
+__device__ __forceinline__ void pairs_negate(cpx<R> (&v)[NR], uint32_t mask) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        if (i & (1 << T)) continue;
+        const int j = i | (1 << T);
+        if (!((mask >> i) & 1u)) continue;
+        v[j].re = -v[j].re;
+        v[j].im = -v[j].im;
+    }
+}
+
+// Correlation of the adjoint tile Y with the state X on the active pairs of target T:
+// acc[a*2+b] += sum Y[r_a] X[r_b].
+template <int T, typename R, int NR>
+__device__ __forceinline__ void pairs_corr(const cpx<R> (&y)[NR], const cpx<R> (&x)[NR], cpx<R> acc[4],
+                                           uint32_t mask, bool masked) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        if (i & (1 << T)) continue;
+        const int j = i | (1 << T);
+        if (masked && !((mask >> i) & 1u)) continue;
+        cacc(acc[0], y[i], x[i]);
+        cacc(acc[1], y[i], x[j]);
+        cacc(acc[2], y[j], x[i]);
+        cacc(acc[3], y[j], x[j]);
+    }
+}
+
+template <int N>
+__device__ __forceinline__ int sub_off(const Layout* l, int s) {
+    int r = 0;
+#pragma unroll
+    for (int j = 0; j < N - Geo<N>::K; ++j) r |= ((s >> j) & 1) << l->sub_pos[j];
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ int lds_idx(int row, int cc) {
+    return (row ^ ((row >> 4) & 15)) * Geo<N>::CW + cc;
+}
+
+// Exchange the slab through LDS from layout lo to layout ln (one wavefront).
+template <int N, typename R>
+__device__ __forceinline__ void remap(cpx<R> (&v)[Geo<N>::NR], const Layout* lo, const Layout* ln, int s,
+                                      int cc, cpx<R>* lds) {
+    if constexpr (N > Geo<N>::K) {
+        const int so = sub_off<N>(lo, s), sn = sub_off<N>(ln, s);
+#pragma unroll
+        for (int i = 0; i < Geo<N>::NR; ++i) lds[lds_idx<N>(so + lo->reg_off[i], cc)] = v[i];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < Geo<N>::NR; ++i) v[i] = lds[lds_idx<N>(sn + ln->reg_off[i], cc)];
+        __syncthreads();
+    }
+}
+
+// Forward execution of one gate op.  kp points at the op's 2x2 kernel.
+template <int N, bool DERIV, typename R>
+__device__ __forceinline__ void exec_gate(cpx<R> (&v)[Geo<N>::NR], const Op op, const cpx<R> k[4], int s) {
+    constexpr int K = Geo<N>::K;
+    switch (op.kind) {
+    case OP_U:
+        dispatch_t<K>(op.t, [&](auto tc) { pairs_apply<decltype(tc)::value, false, false>(v, k, 0u); });
+        break;
+    case OP_CU: {
+        const uint32_t m = ctrl_mask<N>(op.cmode, op.c, s);
+        dispatch_t<K>(op.t, [&](auto tc) { pairs_apply<decltype(tc)::value, true, DERIV>(v, k, m); });
+        break;
+    }
+    case OP_X: {
+        const uint32_t m = ctrl_mask<N>(op.cmode, op.c, s);
+        dispatch_t<K>(op.t, [&](auto tc) { pairs_swap<decltype(tc)::value>(v, m); });
+        break;
+    }
+    case OP_Z: {
+        const uint32_t m = ctrl_mask<N>(op.cmode, op.c, s);
+        dispatch_t<K>(op.t, [&](auto tc) { pairs_negate<decltype(tc)::value>(v, m); });
+        break;
+    }
+    default:
+        break;
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void load_k(const cpx<R>* kp, cpx<R> k[4]) {
+    k[0] = kp[0];
+    k[1] = kp[1];
+    k[2] = kp[2];
+    k[3] = kp[3];
+}
+
+struct LaneInfo {
+    int s, cc, b, col;
+    bool valid;   // the lane holds a real column of a real batch element
+    bool bvalid;  // the lane's batch index is in range (before clamping)
+};
+
+template <int N>
+__device__ __forceinline__ LaneInfo lane_info(int m, int m_pad, int B) {
+    constexpr int CW = Geo<N>::CW;
+    LaneInfo li;
+    const int lane = threadIdx.x;
+    li.s = lane / CW;
+    li.cc = lane % CW;
+    const long long item = (long long)blockIdx.x * CW + li.cc;
+    li.b = (int)(item / m_pad);
+    li.col = (int)(item % m_pad);
+    li.bvalid = li.b < B;
+    li.valid = li.bvalid && (li.col < m);
+    if (!li.bvalid) li.b = B - 1;
+    return li;
+}
+
+// ---------------------------------------------------------------------------
+// Forward tile kernel: apply the whole program, then either store the slab
+// (Gates_block::apply_to) or emit the diagonal entries the trace needs.
+// ---------------------------------------------------------------------------
+template <int N, typename R, int MODE>
+__global__ __launch_bounds__(64) void k_tile_forward(const cpx<R>* in, long long in_bstride,
+                                                     int in_stride, cpx<R>* out,
+                                                     long long out_bstride, int out_stride, int m, int m_pad,
+                                                     int B, const Op* __restrict__ ops, int nops,
+                                                     const Layout* __restrict__ lays,
+                                                     const cpx<R>* __restrict__ kern, long long kern_bstride,
+                                                     int trace_off, double* __restrict__ diag) {
+    using G = Geo<N>;
+    __shared__ cpx<R> lds[G::LDS];
+    const LaneInfo li = lane_info<N>(m, m_pad, B);
+    cpx<R> v[G::NR];
+    {
+        const cpx<R>* src = in + (long long)li.b * in_bstride;
+        const int so = sub_off<N>(&lays[0], li.s);
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[0].reg_off[i];
+            v[i] = li.valid ? src[(long long)row * in_stride + li.col] : mk<R>(0, 0);
+        }
+    }
+    const cpx<R>* kb = kern + (long long)li.b * kern_bstride;
+    int cur = 0;
+    for (int p = 0; p < nops; ++p) {
+        const Op op = ops[p];
+        if (op.kind == OP_REMAP) {
+            const int o = op.idx >> 16, n = op.idx & 0xffff;
+            remap<N, R>(v, &lays[o], &lays[n], li.s, li.cc, lds);
+            cur = n;
+            continue;
+        }
+        cpx<R> k[4];
+        if (op.kind <= OP_CU) load_k(kb + 4 * op.idx, k);
+        exec_gate<N, false, R>(v, op, k, li.s);
+    }
+    const int so = sub_off<N>(&lays[cur], li.s);
+    if constexpr (MODE == 0) {
+        cpx<R>* dst = out + (long long)li.b * out_bstride;
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[cur].reg_off[i];
+            if (li.valid) dst[(long long)row * out_stride + li.col] = v[i];
+        }
+    } else {
+        const int tr = li.col + trace_off;
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[cur].reg_off[i];
+            if (li.valid && row == tr) {
+                diag[2 * ((long long)li.b * m + li.col)] = (double)v[i].re;
+                diag[2 * ((long long)li.b * m + li.col) + 1] = (double)v[i].im;
+            }
+        }
+    }
+}
+
+// Segmented butterfly sum over the lanes that share a batch element.
+template <int CW, typename T>
+__device__ __forceinline__ T seg_sum(T x, int lim) {
+#pragma unroll
+    for (int mk = 1; mk < 64; mk <<= 1) {
+        if (mk < lim || mk >= CW) x += __shfl_xor(x, mk, 64);
+    }
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint cost+gradient tile kernel.  Forward pass to X_N = C.U (diagonal written
+// out), then with Y = E (the trace selector) walk the gates backwards:
+//   X_j = G_j^dagger X_{j+1};  M_j += sum_pairs Y[r_a] X_j[r_b];  Y <- G_j^T Y.
+// dTr/dp = sum_ab D_ab M_ab for each parameter's derivative kernel D, which the
+// finish kernel evaluates.  This replaces the reference's per-parameter derivative
+// matrices (Gates_block.cpp:1012-1150) with O(G d m) work.
+// ---------------------------------------------------------------------------
+template <int N, typename R>
+__global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in, int in_stride, int m, int m_pad,
+                                                  int B, const Op* __restrict__ ops, int nops,
+                                                  const Layout* __restrict__ lays, const cpx<R>* __restrict__ kern,
+                                                  long long kern_bstride, int trace_off, int ns,
+                                                  double* __restrict__ diag, double* __restrict__ mpart) {
+    using G = Geo<N>;
+    constexpr int CW = G::CW;
+    __shared__ cpx<R> lds[G::LDS];
+    const LaneInfo li = lane_info<N>(m, m_pad, B);
+    cpx<R> x[G::NR], y[G::NR];
+    {
+        const int so = sub_off<N>(&lays[0], li.s);
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[0].reg_off[i];
+            x[i] = li.valid ? in[(long long)row * in_stride + li.col] : mk<R>(0, 0);
+        }
+    }
+    const cpx<R>* kb = kern + (long long)li.b * kern_bstride;
+    int cur = 0;
+    for (int p = 0; p < nops; ++p) {
+        const Op op = ops[p];
+        if (op.kind == OP_REMAP) {
+            const int o = op.idx >> 16, n = op.idx & 0xffff;
+            remap<N, R>(x, &lays[o], &lays[n], li.s, li.cc, lds);
+            cur = n;
+            continue;
+        }
+        cpx<R> k[4];
+        if (op.kind <= OP_CU) load_k(kb + 4 * op.idx, k);
+        exec_gate<N, false, R>(x, op, k, li.s);
+    }
+    // Diagonal of C.U and the selector E under the final layout.
+    {
+        const int so = sub_off<N>(&lays[cur], li.s);
+        const int tr = li.col + trace_off;
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[cur].reg_off[i];
+            const bool hit = li.valid && row == tr;
+            if (hit) {
+                diag[2 * ((long long)li.b * m + li.col)] = (double)x[i].re;
+                diag[2 * ((long long)li.b * m + li.col) + 1] = (double)x[i].im;
+            }
+            y[i] = mk<R>(hit ? (R)1 : (R)0, (R)0);
+        }
+    }
+    // Which lanes write the per-wave partials.
+    const int lim = m_pad < CW ? m_pad : CW;
+    const bool leader = li.bvalid && (li.s == 0) && (li.cc % lim == 0);
+    const int wpb = m_pad >= CW ? m_pad / CW : 1;
+    const long long item0 = (long long)blockIdx.x * CW + li.cc;
+    const int wib = m_pad >= CW ? (int)((item0 % m_pad) / CW) : 0;
+    double* mp = mpart + ((long long)li.b * wpb + wib) * (long long)ns * 8;
+
+    for (int p = nops - 1; p >= 0; --p) {
+        const Op op = ops[p];
+        if (op.kind == OP_REMAP) {
+            const int o = op.idx >> 16, n = op.idx & 0xffff;
+            remap<N, R>(x, &lays[n], &lays[o], li.s, li.cc, lds);
+            remap<N, R>(y, &lays[n], &lays[o], li.s, li.cc, lds);
+            continue;
+        }
+        if (op.kind == OP_X || op.kind == OP_Z) {
+            // Self-inverse real symmetric permutation / sign: G^dagger = G^T = G.
+            cpx<R> dummy[4];
+            exec_gate<N, false, R>(x, op, dummy, li.s);
+            exec_gate<N, false, R>(y, op, dummy, li.s);
+            continue;
+        }
+        cpx<R> k[4], kd[4], kt[4];
+        load_k(kb + 4 * op.idx, k);
+        kd[0] = mk(k[0].re, -k[0].im);  // K^dagger
+        kd[1] = mk(k[2].re, -k[2].im);
+        kd[2] = mk(k[1].re, -k[1].im);
+        kd[3] = mk(k[3].re, -k[3].im);
+        kt[0] = k[0];                   // K^T
+        kt[1] = k[2];
+        kt[2] = k[1];
+        kt[3] = k[3];
+        const bool masked = op.kind == OP_CU;
+        const uint32_t mask = ctrl_mask<N>(op.cmode, op.c, li.s);
+        cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+        constexpr int K = G::K;
+        if (masked) {
+            dispatch_t<K>(op.t, [&](auto tc) {
+                constexpr int T = decltype(tc)::value;
+                pairs_apply<T, true, false>(x, kd, mask);
+                pairs_corr<T>(y, x, acc, mask, true);
+                pairs_apply<T, true, false>(y, kt, mask);
+            });
+        } else {
+            dispatch_t<K>(op.t, [&](auto tc) {
+                constexpr int T = decltype(tc)::value;
+                pairs_apply<T, false, false>(x, kd, 0u);
+                pairs_corr<T>(y, x, acc, 0u, false);
+                pairs_apply<T, false, false>(y, kt, 0u);
+            });
+        }
+        double r[8];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            r[2 * e] = seg_sum<CW>((double)acc[e].re, lim);
+            r[2 * e + 1] = seg_sum<CW>((double)acc[e].im, lim);
+        }
+        if (leader) {
+            double* d = mp + (long long)op.idx * 8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) d[e] = r[e];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Materialised derivative matrices (Gates_block::apply_derivate_to,
+// Gates_block.cpp:1012-1150): each wavefront takes one parametric gate and one
+// column slab, runs the prefix, and for each parameter of the gate applies the
+// derivative kernel (control-0 rows zeroed) and the suffix, then stores.
+// ---------------------------------------------------------------------------
+template <int N, typename R>
+__global__ __launch_bounds__(64) void k_tile_deriv(const cpx<R>* __restrict__ in, int in_stride, int m,
+                                                   int m_pad, const Op* __restrict__ ops, int nops,
+                                                   const Layout* __restrict__ lays,
+                                                   const cpx<R>* __restrict__ kern, const Slot* __restrict__ slots,
+                                                   const double* __restrict__ params, cpx<R>* __restrict__ out,
+                                                   long long out_mstride) {
+    using G = Geo<N>;
+    __shared__ cpx<R> lds[G::LDS];
+    const int wpb = m_pad >= G::CW ? m_pad / G::CW : 1;
+    const int slot = blockIdx.x / wpb;
+    const int wave = blockIdx.x % wpb;
+    // lane_info with the slab index of this wave (batch of one).
+    LaneInfo li;
+    li.s = threadIdx.x / G::CW;
+    li.cc = threadIdx.x % G::CW;
+    li.b = 0;
+    li.col = wave * G::CW + li.cc;
+    li.valid = li.col < m;
+    li.bvalid = true;
+    cpx<R> x[G::NR], z[G::NR];
+    {
+        const int so = sub_off<N>(&lays[0], li.s);
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[0].reg_off[i];
+            x[i] = li.valid ? in[(long long)row * in_stride + li.col] : mk<R>(0, 0);
+        }
+    }
+    const Slot sl = slots[slot];
+    int cur = 0;
+    for (int p = 0; p < sl.op; ++p) {
+        const Op op = ops[p];
+        if (op.kind == OP_REMAP) {
+            const int o = op.idx >> 16, n = op.idx & 0xffff;
+            remap<N, R>(x, &lays[o], &lays[n], li.s, li.cc, lds);
+            cur = n;
+            continue;
+        }
+        cpx<R> k[4];
+        if (op.kind <= OP_CU) load_k(kern + 4 * op.idx, k);
+        exec_gate<N, false, R>(x, op, k, li.s);
+    }
+    const int lay_at = cur;
+    const Op dop = ops[sl.op];
+    for (int q = 0; q < sl.nparam; ++q) {
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) z[i] = x[i];
+        cur = lay_at;
+        {
+            cpx<double> kd[4];
+            dev_deriv_kernel(sl.type, params + sl.param_start, q, kd);
+            cpx<R> k[4] = {cvt<R>(kd[0]), cvt<R>(kd[1]), cvt<R>(kd[2]), cvt<R>(kd[3])};
+            Op o2 = dop;
+            o2.kind = OP_CU;  // derivative mode always honours the (possibly absent) control
+            exec_gate<N, true, R>(z, o2, k, li.s);
+        }
+        for (int p = sl.op + 1; p < nops; ++p) {
+            const Op op = ops[p];
+            if (op.kind == OP_REMAP) {
+                const int o = op.idx >> 16, n = op.idx & 0xffff;
+                remap<N, R>(z, &lays[o], &lays[n], li.s, li.cc, lds);
+                cur = n;
+                continue;
+            }
+            cpx<R> k[4];
+            if (op.kind <= OP_CU) load_k(kern + 4 * op.idx, k);
+            exec_gate<N, false, R>(z, op, k, li.s);
+        }
+        cpx<R>* dst = out + (long long)(sl.param_start + q) * out_mstride;
+        const int so = sub_off<N>(&lays[cur], li.s);
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lays[cur].reg_off[i];
+            if (li.valid) dst[(long long)row * m + li.col] = z[i];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Small kernels: kernel table, finish (cost + gradient), streaming gates.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ void k_build_kernels(const double* __restrict__ params, int P, int B, const Slot* __restrict__ slots,
+                                int ns, cpx<R>* __restrict__ kern) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)B * ns) return;
+    const int b = (int)(i / ns), sl = (int)(i % ns);
+    const Slot s = slots[sl];
+    cpx<double> k[4];
+    dev_gate_kernel(s.type, params + (long long)b * P + s.param_start, k);
+    cpx<R>* o = kern + i * 4;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) o[e] = cvt<R>(k[e]);
+}
+
+__device__ inline double block_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    const int nw = blockDim.x / 64;
+    for (int i = 0; i < nw; ++i) t += sh[i];
+    return t;
+}
+
+// Cost variants (N_Qubit_Decomposition_Cost_Function.cpp:137-158, 524-549) and the
+// gradient formulas of optimization_problem_combined (Optimization_Interface.cpp:1401-1450).
+__device__ inline double cost_of(int variant, double tre, double tim, double m) {
+    switch (variant) {
+    case CV_FROBENIUS: return 1.0 - tre / m;
+    case CV_HS: { const double d = 1.0 / m; return 1.0 - d * d * (tre * tre + tim * tim); }
+    case CV_INFIDELITY: return 1.0 - ((tre * tre + tim * tim) / m + 1.0) / (m + 1.0);
+    case CV_ABS_TRACE: return 1.0 - sqrt(tre * tre + tim * tim) / m;
+    default: return __builtin_nan("");
+    }
+}
+
+__device__ inline double grad_of(int variant, double tre, double tim, double kre, double kim, double m) {
+    switch (variant) {
+    case CV_FROBENIUS: return -kre / m;
+    case CV_HS: { const double d = 1.0 / m; return -2.0 * d * d * tre * kre - 2.0 * d * d * tim * kim; }
+    case CV_INFIDELITY: return -2.0 / m / (m + 1.0) * tre * kre - 2.0 / m / (m + 1.0) * tim * kim;
+    case CV_ABS_TRACE: {
+        const double a = sqrt(tre * tre + tim * tim);
+        return a > 0.0 ? -(tre * kre + tim * kim) / (a * m) : 0.0;
+    }
+    default: return __builtin_nan("");
+    }
+}
+
+#ifdef SQH_DEFINE_MISC_KERNELS
+// One workgroup per batch element: trace = sum of the diagonal entries, cost, and
+// (if mpart) the gradient from the per-gate correlations.
+__global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag, int m, int variant,
+                                                double* __restrict__ costs, const double* __restrict__ mpart,
+                                                int wpb, int ns, const Slot* __restrict__ slots,
+                                                const double* __restrict__ params, int P,
+                                                double* __restrict__ grad) {
+    __shared__ double sh[8];
+    const int b = blockIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
+        re += diag[2 * ((long long)b * m + c)];
+        im += diag[2 * ((long long)b * m + c) + 1];
+    }
+    const double tre = block_sum(re, sh);
+    const double tim = block_sum(im, sh);
+    if (threadIdx.x == 0 && costs) costs[b] = cost_of(variant, tre, tim, (double)m);
+    if (!mpart) return;
+    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+        double M[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int w = 0; w < wpb; ++w) {
+            const double* src = mpart + (((long long)b * wpb + w) * ns + s) * 8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) M[e] += src[e];
+        }
+        const Slot sl = slots[s];
+        const double* pb = params + (long long)b * P;
+        for (int q = 0; q < sl.nparam; ++q) {
+            cpx<double> d[4];
+            dev_deriv_kernel(sl.type, pb + sl.param_start, q, d);
+            double kre = 0.0, kim = 0.0;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                kre += d[e].re * M[2 * e] - d[e].im * M[2 * e + 1];
+                kim += d[e].re * M[2 * e + 1] + d[e].im * M[2 * e];
+            }
+            grad[(long long)b * P + sl.param_start + q] = grad_of(variant, tre, tim, kre, kim, (double)m);
+        }
+    }
+}
+
+#endif  // SQH_DEFINE_MISC_KERNELS
+
+// Streaming one-gate pass over B matrices (rows = 2^n, m columns, row stride).
+// Thread -> (matrix b, row pair, column); columns fastest so a wavefront moves
+// contiguous row segments.  Pairs whose control bit is clear exit at once, so a
+// controlled gate touches only the rows it changes.
+template <typename R>
+__global__ __launch_bounds__(256) void k_stream_gate(cpx<R>* __restrict__ mats, long long bstride, int stride,
+                                                     int rows, int m, int B, int kind, int target, int control,
+                                                     const cpx<R>* __restrict__ kern, long long kern_bstride,
+                                                     int slot) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per_b = (long long)(rows >> 1) * m;
+    if (tid >= per_b * B) return;
+    const int b = (int)(tid / per_b);
+    const long long rem = tid % per_b;
+    const int pr = (int)(rem / m), col = (int)(rem % m);
+    const int lo = pr & ((1 << target) - 1);
+    const int r0 = ((pr >> target) << (target + 1)) | lo;
+    const int r1 = r0 | (1 << target);
+    if (control >= 0 && !((r0 >> control) & 1)) return;
+    cpx<R>* M = mats + (long long)b * bstride;
+    cpx<R>* a = M + (long long)r0 * stride + col;
+    cpx<R>* c = M + (long long)r1 * stride + col;
+    if (kind == OP_Z) {
+        cpx<R> y = *c;
+        y.re = -y.re;
+        y.im = -y.im;
+        *c = y;
+        return;
+    }
+    const cpx<R> x0 = *a, x1 = *c;
+    if (kind == OP_X) {
+        *a = x1;
+        *c = x0;
+        return;
+    }
+    const cpx<R>* kp = kern + (long long)b * kern_bstride + 4 * slot;
+    const cpx<R> k0 = kp[0], k1 = kp[1], k2 = kp[2], k3 = kp[3];
+    *a = mac2(k0, x0, k1, x1);
+    *c = mac2(k2, x0, k3, x1);
+}
+
+template <typename R>
+__global__ void k_extract_diag(const cpx<R>* __restrict__ mats, long long bstride, int stride, int m, int B,
+                               int trace_off, double* __restrict__ diag) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)B * m) return;
+    const int b = (int)(i / m), c = (int)(i % m);
+    const cpx<R> v = mats[(long long)b * bstride + (long long)(c + trace_off) * stride + c];
+    diag[2 * i] = (double)v.re;
+    diag[2 * i + 1] = (double)v.im;
+}
+
+template <typename R>
+__global__ void k_broadcast_copy(const cpx<R>* __restrict__ src, int rows, int m, int sstride,
+                                 cpx<R>* __restrict__ dst, long long dbstride, int B) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)rows * m;
+    if (i >= per * B) return;
+    const int b = (int)(i / per);
+    const long long r = i % per;
+    const int row = (int)(r / m), col = (int)(r % m);
+    dst[(long long)b * dbstride + (long long)row * m + col] = src[(long long)row * sstride + col];
+}
+
+#ifdef SQH_DEFINE_MISC_KERNELS
+__global__ void k_c128_to_c64(const double* __restrict__ src, float* __restrict__ dst, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+__global__ void k_c64_to_c128(const float* __restrict__ src, double* __restrict__ dst, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (double)src[i];
+}
+
+#endif  // SQH_DEFINE_MISC_KERNELS
+
+}  // namespace sqh

@@ -1,0 +1,69 @@
+// sqh_internal.h -- types shared by the host-side circuit compiler and the HIP kernels.
+//
+// A circuit (an ordered gate list, Gates_block.h:44-1025) is compiled on the host
+// into a "tile program": a list of ops executed by every wavefront on the column
+// slab of the matrix it holds in registers.  Each lane holds NR = 2^K complex
+// elements of one column; the K "register qubits" are the row-index bits that
+// vary across those registers, the remaining N-K row bits select the lane
+// (the "sub" index) inside the column's lane group.  A gate whose target is not a
+// register qubit is preceded by a REMAP op that exchanges the slab through LDS into
+// a layout where it is.
+#pragma once
+#include <cstdint>
+
+namespace sqh {
+
+// Reference gate_type values (Gate.h:39-51).
+enum GateType : int32_t {
+    GT_CZ = 4, GT_CNOT = 5, GT_U3 = 7, GT_RY = 8, GT_RX = 9, GT_RZ = 10, GT_CRY = 14
+};
+
+// Reference cost_function_type values (Optimization_Interface.h:43-45); ABS_TRACE
+// (1 - |Tr|/m) has no reference counterpart.
+enum CostVariant : int32_t {
+    CV_FROBENIUS = 0, CV_HS = 3, CV_INFIDELITY = 9, CV_ABS_TRACE = 100
+};
+
+enum OpKind : uint8_t {
+    OP_U = 0,      // uncontrolled 2x2 from the kernel table (slot = idx)
+    OP_CU = 1,     // controlled 2x2 (CRY), slot = idx
+    OP_X = 2,      // CNOT: controlled row swap
+    OP_Z = 3,      // CZ: controlled sign flip of the target-bit-set row
+    OP_REMAP = 4,  // idx = (old_layout << 16) | new_layout
+};
+
+// Control modes: none, control qubit is a register bit (c = register position),
+// control qubit is a lane bit (c = sub-index bit).
+enum CtrlMode : uint8_t { CM_NONE = 0, CM_REG = 1, CM_LANE = 2 };
+
+struct alignas(8) Op {
+    uint8_t kind, t, cmode, c;
+    int32_t idx;
+};
+
+// A layout: reg_off[i] is the row-index contribution of register i, sub_pos[j] the
+// qubit carried by bit j of the lane's sub index.
+struct alignas(16) Layout {
+    int16_t reg_off[16];
+    int8_t sub_pos[8];
+    int32_t pad[2];
+};
+
+// One parametric gate ("slot"): its type, first raw parameter, parameter count,
+// and the index of its op in the program.
+struct alignas(16) Slot {
+    int32_t type, param_start, nparam, op;
+};
+
+constexpr int kMaxTileQubits = 10;   // one column fits one wavefront up to 2^10 rows
+
+inline int param_num_of(int type) {
+    switch (type) {
+    case GT_U3: return 3;
+    case GT_RX: case GT_RY: case GT_RZ: case GT_CRY: return 1;
+    case GT_CNOT: case GT_CZ: return 0;
+    default: return -1;
+    }
+}
+
+}  // namespace sqh

@@ -1,0 +1,102 @@
+// sqh_k_misc.hip -- precision dispatch of the tile kernels and launchers of the
+// small kernels (kernel tables, finish, streaming gates, copies).
+#define SQH_DEFINE_MISC_KERNELS 1
+#include "sqh_launch_common.h"
+
+namespace sqh {
+
+hipError_t launch_tile_forward_double(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_forward_float(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_grad_double(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_grad_float(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_deriv_double(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_deriv_float(const TileLaunch& t, hipStream_t st);
+
+hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st) {
+    return t.f32 ? launch_tile_forward_float(t, st) : launch_tile_forward_double(t, st);
+}
+hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st) {
+    return t.f32 ? launch_tile_grad_float(t, st) : launch_tile_grad_double(t, st);
+}
+hipError_t launch_tile_deriv(const TileLaunch& t, hipStream_t st) {
+    return t.f32 ? launch_tile_deriv_float(t, st) : launch_tile_deriv_double(t, st);
+}
+
+int tile_columns_per_wave(int N) {
+    if (N < 1 || N > kMaxTileQubits) return 0;
+    const int K = N < 4 ? N : 4;
+    return 64 >> (N - K);
+}
+
+hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, const Slot* slots, int ns, void* kern,
+                                hipStream_t st) {
+    const long long n = (long long)B * ns;
+    if (n == 0) return hipSuccess;
+    if (f32)
+        hipLaunchKernelGGL(k_build_kernels<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st, params, P, B, slots,
+                           ns, (cpx<float>*)kern);
+    else
+        hipLaunchKernelGGL(k_build_kernels<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st, params, P, B, slots,
+                           ns, (cpx<double>*)kern);
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,
+                         int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st) {
+    hipLaunchKernelGGL(k_finish, dim3(B), dim3(256), 0, st, diag, m, variant, costs, mpart, wpb, ns, slots, params, P,
+                       grad);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_gate(bool f32, void* mats, long long bstride, int stride, int rows, int m, int B, int kind,
+                              int target, int control, const void* kern, long long kern_bstride, int slot,
+                              hipStream_t st) {
+    const long long n = (long long)(rows >> 1) * m * B;
+    if (n == 0) return hipSuccess;
+    if (f32)
+        hipLaunchKernelGGL(k_stream_gate<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st, (cpx<float>*)mats,
+                           bstride, stride, rows, m, B, kind, target, control, (const cpx<float>*)kern, kern_bstride,
+                           slot);
+    else
+        hipLaunchKernelGGL(k_stream_gate<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st, (cpx<double>*)mats,
+                           bstride, stride, rows, m, B, kind, target, control, (const cpx<double>*)kern,
+                           kern_bstride, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_extract_diag(bool f32, const void* mats, long long bstride, int stride, int m, int B,
+                               int trace_off, double* diag, hipStream_t st) {
+    const long long n = (long long)B * m;
+    if (f32)
+        hipLaunchKernelGGL(k_extract_diag<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const cpx<float>*)mats, bstride, stride, m, B, trace_off, diag);
+    else
+        hipLaunchKernelGGL(k_extract_diag<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const cpx<double>*)mats, bstride, stride, m, B, trace_off, diag);
+    return hipGetLastError();
+}
+
+hipError_t launch_broadcast_copy(bool f32, const void* src, int rows, int m, int sstride, void* dst,
+                                 long long dbstride, int B, hipStream_t st) {
+    const long long n = (long long)rows * m * B;
+    if (f32)
+        hipLaunchKernelGGL(k_broadcast_copy<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const cpx<float>*)src, rows, m, sstride, (cpx<float>*)dst, dbstride, B);
+    else
+        hipLaunchKernelGGL(k_broadcast_copy<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
+                           (const cpx<double>*)src, rows, m, sstride, (cpx<double>*)dst, dbstride, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(bool to_f32, const void* src, void* dst, long long nreal, hipStream_t st) {
+    if (nreal == 0) return hipSuccess;
+    if (to_f32)
+        hipLaunchKernelGGL(k_c128_to_c64, dim3(blocks_for(nreal, 256)), dim3(256), 0, st, (const double*)src,
+                           (float*)dst, nreal);
+    else
+        hipLaunchKernelGGL(k_c64_to_c128, dim3(blocks_for(nreal, 256)), dim3(256), 0, st, (const float*)src,
+                           (double*)dst, nreal);
+    return hipGetLastError();
+}
+
+}  // namespace sqh

@@ -1,0 +1,55 @@
+// sqh_kernels.h -- host-side launchers for the kernels in sqh_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sqh_internal.h"
+
+namespace sqh {
+
+// Arguments of one register-tile launch.  Matrices are interleaved complex
+// (complex128, or complex64 when f32), row-major with a row stride in elements.
+struct TileLaunch {
+    int N = 0;                 // qubits (rows = 2^N)
+    bool f32 = false;          // complex64 tiles with FP32 arithmetic
+    int mode = 0;              // forward: 0 = store matrix, 1 = diagonal only
+    const void* in = nullptr;  // input slab base
+    long long in_bstride = 0;  // elements between batch inputs (0: shared input)
+    int in_stride = 0;
+    void* out = nullptr;       // forward store / derivative output
+    long long out_bstride = 0; // elements between batch outputs (derivative: between matrices)
+    int out_stride = 0;
+    int m = 0;                 // columns
+    int m_pad = 0;             // padded columns per batch element
+    int B = 1;                 // batch elements
+    const Op* ops = nullptr;
+    int nops = 0;
+    const Layout* lays = nullptr;
+    const void* kern = nullptr;  // [B][ns][4] kernels
+    long long kern_bstride = 0;  // complex entries between batch kernel tables (0: shared)
+    int trace_off = 0;
+    int ns = 0;                  // parametric gate count
+    double* diag = nullptr;      // [B][m] complex diagonal entries
+    double* mpart = nullptr;     // [B][wpb][ns][8] gradient correlations
+    const Slot* slots = nullptr;
+    const double* params = nullptr;
+};
+
+hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st);
+hipError_t launch_tile_deriv(const TileLaunch& t, hipStream_t st);
+int tile_columns_per_wave(int N);
+
+hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, const Slot* slots, int ns, void* kern,
+                                hipStream_t st);
+hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,
+                         int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st);
+hipError_t launch_stream_gate(bool f32, void* mats, long long bstride, int stride, int rows, int m, int B, int kind,
+                              int target, int control, const void* kern, long long kern_bstride, int slot,
+                              hipStream_t st);
+hipError_t launch_extract_diag(bool f32, const void* mats, long long bstride, int stride, int m, int B,
+                               int trace_off, double* diag, hipStream_t st);
+hipError_t launch_broadcast_copy(bool f32, const void* src, int rows, int m, int sstride, void* dst,
+                                 long long dbstride, int B, hipStream_t st);
+hipError_t launch_convert(bool to_f32, const void* src, void* dst, long long nreal, hipStream_t st);
+
+}  // namespace sqh

@@ -1,0 +1,33 @@
+// sqh_program.h -- host-side compilation of a gate list into a register-tile program.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "sqh_internal.h"
+
+namespace sqh {
+
+struct GateRec {
+    int type, target, control, param_start;
+};
+
+struct Program {
+    int N = 0;   // qubits
+    int K = 0;   // register qubits
+    int P = 0;   // parameters
+    std::vector<Op> ops;
+    std::vector<Layout> layouts;
+    std::vector<Slot> slots;
+    int remaps = 0;
+};
+
+// Validates the gate list (types, qubit indices, parameter ranges) the way the
+// reference's gate constructors and Gates_block::add_gate do (U3.cpp:41-46,
+// CRY.cpp:60-66, Gates_block.cpp:2500-2525) and compiles it.  Returns false with
+// err set on a bad gate.
+bool compile_program(int N, const std::vector<GateRec>& gates, Program& prog, std::string& err);
+
+// Layout of a qubit placement: pos[j] is the qubit at position j (registers first).
+Layout make_layout(int N, int K, const std::vector<int>& pos);
+
+}  // namespace sqh

@@ -1,0 +1,192 @@
+"""ctypes binding of libsqhip.so (the C ABI declared in include/sqhip.h).
+
+The library is the product: there is no CPU fallback.  Loading fails loudly when
+libsqhip.so has not been built, and every compute call fails loudly when no GPU
+is present.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsqhip.so")
+
+SQH_OK = 0
+SQH_ERR_ARG, SQH_ERR_GATE, SQH_ERR_STATE, SQH_ERR_DEVICE, SQH_ERR_NOMEM = -1, -2, -3, -4, -5
+
+# Gate-type ids: the reference's gate_type enum (Gate.h:39-51).
+GATE_CZ, GATE_CNOT, GATE_U3, GATE_RY, GATE_RX, GATE_RZ, GATE_CRY = 4, 5, 7, 8, 9, 10, 14
+PARAM_NUM = {GATE_U3: 3, GATE_RX: 1, GATE_RY: 1, GATE_RZ: 1, GATE_CRY: 1, GATE_CNOT: 0, GATE_CZ: 0}
+GATE_NAMES = {GATE_U3: "U3", GATE_RX: "RX", GATE_RY: "RY", GATE_RZ: "RZ", GATE_CRY: "CRY",
+              GATE_CNOT: "CNOT", GATE_CZ: "CZ"}
+
+# Cost variants: the reference's cost_function_type values + ABS_TRACE.
+COST_FROBENIUS, COST_HILBERT_SCHMIDT, COST_INFIDELITY, COST_ABS_TRACE = 0, 3, 9, 100
+PREC_F64, PREC_F32 = 0, 1
+
+# Exported symbols (checked by the CPU test suite against include/sqhip.h).
+EXPORTS = [
+    "sqh_create", "sqh_destroy", "sqh_last_error", "sqh_get_stream", "sqh_synchronize", "sqh_version",
+    "sqh_set_circuit", "sqh_get_parameter_num", "sqh_set_target", "sqh_set_trace_offset",
+    "sqh_set_cost_function", "sqh_apply_to", "sqh_apply_to_dev", "sqh_apply_to_stream_dev",
+    "sqh_apply_derivate_to", "sqh_apply_derivate_to_dev", "sqh_optimization_problem_batch",
+    "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
+    "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
+    "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing",
+]
+
+
+class SqhError(Exception):
+    """A failed library call (the reference raises a plain Exception with the message)."""
+
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class _Gate(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("target", ctypes.c_int32),
+                ("control", ctypes.c_int32), ("param_start", ctypes.c_int32)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def lib():
+    """Load libsqhip.so once; raise if it is missing."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "(the HIP library has no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        ll = ctypes.c_longlong
+        sig = {
+            "sqh_create": ([i, ctypes.POINTER(vp)], i),
+            "sqh_destroy": ([vp], None),
+            "sqh_last_error": ([vp], ctypes.c_char_p),
+            "sqh_get_stream": ([vp], vp),
+            "sqh_synchronize": ([vp], i),
+            "sqh_version": ([], ctypes.c_char_p),
+            "sqh_set_circuit": ([vp, i, ctypes.POINTER(_Gate), i], i),
+            "sqh_get_parameter_num": ([vp], i),
+            "sqh_set_target": ([vp, vp, i, i, i], i),
+            "sqh_set_trace_offset": ([vp, i], i),
+            "sqh_set_cost_function": ([vp, i], i),
+            "sqh_apply_to": ([vp, vp, vp, i, i, i, i], i),
+            "sqh_apply_to_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
+            "sqh_apply_to_stream_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
+            "sqh_apply_derivate_to": ([vp, vp, vp, i, i, i, vp], i),
+            "sqh_apply_derivate_to_dev": ([vp, vp, vp, i, i, i, vp], i),
+            "sqh_optimization_problem_batch": ([vp, vp, i, i, vp], i),
+            "sqh_optimization_problem_batch_dev": ([vp, vp, i, i, vp], i),
+            "sqh_optimization_problem_combined_batch": ([vp, vp, i, i, vp, vp], i),
+            "sqh_optimization_problem_combined_batch_dev": ([vp, vp, i, i, vp, vp], i),
+            "sqh_optimization_problem_combined_unitary": ([vp, vp, vp, vp], i),
+            "sqh_set_kernel_timing": ([vp, i], i),
+            "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
+            "sqh_inspect_program": ([i, ctypes.POINTER(_Gate), i, vp, i, vp, i, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int)], i),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+        return L
+
+
+def gate_array(gates):
+    g = np.ascontiguousarray(np.asarray(gates, dtype=np.int32).reshape(-1, 4))
+    arr = (_Gate * max(len(g), 1))()
+    for k, (t, tg, c, ps) in enumerate(g):
+        arr[k].type, arr[k].target, arr[k].control, arr[k].param_start = int(t), int(tg), int(c), int(ps)
+    return arr, len(g)
+
+
+def inspect_program(qbit_num, gates):
+    """The register-tile program the library compiles for a gate list:
+    (ops[n,8] uint8 records, layouts[n,48] uint8 records)."""
+    L = lib()
+    garr, ng = gate_array(gates)
+    nops, nl = ctypes.c_int(0), ctypes.c_int(0)
+    rc = L.sqh_inspect_program(qbit_num, garr, ng, None, 0, None, 0, ctypes.byref(nops), ctypes.byref(nl))
+    if rc:
+        raise SqhError(rc, "program compilation failed")
+    ops = np.zeros((max(nops.value, 1), 8), dtype=np.uint8)
+    lays = np.zeros((max(nl.value, 1), 48), dtype=np.uint8)
+    rc = L.sqh_inspect_program(qbit_num, garr, ng, ops.ctypes.data, nops.value, lays.ctypes.data, nl.value,
+                               ctypes.byref(nops), ctypes.byref(nl))
+    if rc:
+        raise SqhError(rc, "program compilation failed")
+    return ops[:nops.value], lays[:nl.value]
+
+
+class Context:
+    """One sqh_ctx: a device, its stream, and the circuit/target state."""
+
+    def __init__(self, device: int = 0):
+        self._L = lib()
+        h = ctypes.c_void_p()
+        rc = self._L.sqh_create(device, ctypes.byref(h))
+        if rc != SQH_OK:
+            raise SqhError(rc, f"sqh_create(device={device}) failed with status {rc}: no usable GPU")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.sqh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, rc):
+        if rc != SQH_OK:
+            msg = self._L.sqh_last_error(self.h)
+            raise SqhError(rc, msg.decode() if msg else f"status {rc}")
+
+    @property
+    def stream(self) -> int:
+        return self._L.sqh_get_stream(self.h) or 0
+
+    def synchronize(self):
+        self.check(self._L.sqh_synchronize(self.h))
+
+    def set_circuit(self, qbit_num, gates):
+        garr, ng = gate_array(gates)
+        self.check(self._L.sqh_set_circuit(self.h, qbit_num, garr, ng))
+
+    def parameter_num(self) -> int:
+        return int(self._L.sqh_get_parameter_num(self.h))
+
+    def set_target(self, U):
+        U = np.ascontiguousarray(U, dtype=np.complex128)
+        self.check(self._L.sqh_set_target(self.h, U.ctypes.data, U.shape[0], U.shape[1], U.shape[1]))
+
+    def set_trace_offset(self, off):
+        self.check(self._L.sqh_set_trace_offset(self.h, int(off)))
+
+    def set_cost_function(self, variant):
+        self.check(self._L.sqh_set_cost_function(self.h, int(variant)))
+
+    def set_kernel_timing(self, enable=True):
+        self.check(self._L.sqh_set_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_timing(self):
+        """(total kernel milliseconds, launches) since the last query; resets the timers."""
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int(0)
+        self.check(self._L.sqh_get_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,173 @@
+"""``N_Qubit_Decomposition_custom``: the reference decomposition object's
+cost / gradient surface on the GPU.
+
+Method names and semantics follow the reference's Python wrapper
+(squander/decomposition/qgd_N_Qubit_Decompositions_Wrapper.cpp:1446-1714,
+method table :3125-3211) and Optimization_Interface
+(Optimization_Interface.cpp:617-1561):
+
+* ``Optimization_Problem(p)``            -> cost (float)
+* ``Optimization_Problem_Grad(p)``       -> gradient (P,)
+* ``Optimization_Problem_Combined(p)``   -> (cost, gradient)
+* ``Optimization_Problem_Batch(P2d)``    -> costs (B,)
+* ``Optimization_Problem_Combined_Unitary(p)`` -> (C.Umtx, [dC/dp_k . Umtx])
+
+The target passed to the constructor is ``Umtx``; the circuit is applied to it and
+the cost is 1 - Re Tr(C.Umtx)/m by default (FROBENIUS_NORM), so callers pass the
+adjoint of the unitary to decompose, as the reference's tests do
+(tests/decomposition/test_decomposition.py:62).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _native as nat
+from .circuit import Circuit
+
+_COST_NAMES = {"FROBENIUS_NORM": nat.COST_FROBENIUS, "HILBERT_SCHMIDT_TEST": nat.COST_HILBERT_SCHMIDT,
+               "INFIDELITY": nat.COST_INFIDELITY, "ABS_TRACE": nat.COST_ABS_TRACE}
+
+
+class N_Qubit_Decomposition_custom:
+    def __init__(self, Umtx, qbit_num=None, initial_guess="RANDOM", config=None, accelerator_num=0):
+        U = np.ascontiguousarray(Umtx, dtype=np.complex128)
+        if U.ndim != 2:
+            raise Exception("Umtx must be a matrix")
+        rows = U.shape[0]
+        n = int(round(np.log2(rows)))
+        if (1 << n) != rows:
+            raise Exception("Umtx must have 2^n rows")
+        if qbit_num is not None and int(qbit_num) != n:
+            raise Exception("qbit_num does not match the size of Umtx")
+        self.qbit_num = n
+        self.Umtx = U
+        self.config = dict(config or {})
+        self.initial_guess = str(initial_guess).upper() if initial_guess is not None else "RANDOM"
+        self.trace_offset = 0
+        self.cost_variant = nat.COST_FROBENIUS
+        self.circuit = None
+        self.optimized_parameters = None
+        self.num_iters = 0
+        self.decomposition_error = None
+        self.optimization_tolerance = float(self.config.get("optimization_tolerance", 1e-7))
+        self.max_iterations = None
+        self._ctx = nat.Context(0)
+        self._ctx.set_target(U)
+
+    # -- configuration -------------------------------------------------------
+    def set_Gate_Structure(self, circuit: Circuit):
+        if not isinstance(circuit, Circuit):
+            raise Exception("set_Gate_Structure expects a Circuit")
+        if circuit.qbit_num != self.qbit_num:
+            raise Exception("the gate structure acts on a different number of qubits")
+        self.circuit = circuit
+        self._ctx.set_circuit(self.qbit_num, circuit.gate_array())
+
+    def get_Circuit(self):
+        return self.circuit
+
+    def get_Parameter_Num(self) -> int:
+        return self.circuit.get_Parameter_Num() if self.circuit is not None else 0
+
+    def get_Qbit_Num(self) -> int:
+        return self.qbit_num
+
+    def set_Trace_Offset(self, trace_offset):
+        self._ctx.set_trace_offset(int(trace_offset))
+        self.trace_offset = int(trace_offset)
+
+    def get_Trace_Offset(self) -> int:
+        return self.trace_offset
+
+    def set_Cost_Function_Variant(self, costfnc=0):
+        if isinstance(costfnc, str):
+            costfnc = _COST_NAMES[costfnc.upper()]
+        self._ctx.set_cost_function(int(costfnc))
+        self.cost_variant = int(costfnc)
+
+    def set_Optimization_Tolerance(self, tol):
+        self.optimization_tolerance = float(tol)
+
+    def set_Max_Iterations(self, max_iters):
+        self.max_iterations = int(max_iters)
+
+    def get_Num_of_Iters(self) -> int:
+        return self.num_iters
+
+    def get_Unitary(self):
+        return self.Umtx.copy()
+
+    # -- the hot path -----------------------------------------------------------
+    def _check_params(self, p, batch=False):
+        if self.circuit is None:
+            raise Exception("no gate structure set")
+        p = np.asarray(p)
+        if p.dtype == np.float32:
+            p = p.astype(np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        P = self.get_Parameter_Num()
+        if batch:
+            if p.ndim != 2 or p.shape[1] != P:
+                raise Exception(f"Optimization_Problem_Batch: parameters must be B x {P}")
+        elif p.size != P:
+            raise Exception(f"Optimization_Interface::optimization_problem: Number of free paramaters should be {P}, "
+                            f"but got {p.size}")
+        return p
+
+    def Optimization_Problem(self, parameters, precision=nat.PREC_F64) -> float:
+        p = self._check_params(parameters)
+        out = np.zeros(1)
+        self._ctx.check(self._ctx._L.sqh_optimization_problem_batch(self._ctx.h, p.ctypes.data, 1, precision,
+                                                                     out.ctypes.data))
+        self.num_iters += 1
+        return float(out[0])
+
+    def Optimization_Problem_Batch(self, parameters, precision=nat.PREC_F64):
+        p = self._check_params(parameters, batch=True)
+        out = np.zeros(p.shape[0])
+        self._ctx.check(self._ctx._L.sqh_optimization_problem_batch(self._ctx.h, p.ctypes.data, p.shape[0],
+                                                                     precision, out.ctypes.data))
+        self.num_iters += p.shape[0]
+        return out
+
+    def Optimization_Problem_Combined_Batch(self, parameters, precision=nat.PREC_F64):
+        p = self._check_params(parameters, batch=True)
+        f = np.zeros(p.shape[0])
+        g = np.zeros(p.shape)
+        self._ctx.check(self._ctx._L.sqh_optimization_problem_combined_batch(
+            self._ctx.h, p.ctypes.data, p.shape[0], precision, f.ctypes.data, g.ctypes.data))
+        self.num_iters += p.shape[0] * (p.shape[1] + 1)
+        return f, g
+
+    def Optimization_Problem_Combined(self, parameters, precision=nat.PREC_F64):
+        p = self._check_params(parameters)
+        f, g = self.Optimization_Problem_Combined_Batch(p.reshape(1, -1), precision)
+        return float(f[0]), g[0]
+
+    def Optimization_Problem_Grad(self, parameters, precision=nat.PREC_F64):
+        return self.Optimization_Problem_Combined(parameters, precision)[1]
+
+    def Optimization_Problem_Combined_Unitary(self, parameters):
+        p = self._check_params(parameters)
+        P = p.size
+        U = np.empty(self.Umtx.shape, dtype=np.complex128)
+        D = np.empty((P,) + self.Umtx.shape, dtype=np.complex128)
+        self._ctx.check(self._ctx._L.sqh_optimization_problem_combined_unitary(
+            self._ctx.h, p.ctypes.data, U.ctypes.data, D.ctypes.data if P else None))
+        return U, [D[k] for k in range(P)]
+
+    def get_Matrix(self, parameters):
+        return self.circuit.get_Matrix(np.ascontiguousarray(parameters, dtype=np.float64))
+
+    # -- results -------------------------------------------------------------------
+    def get_Optimized_Parameters(self):
+        return None if self.optimized_parameters is None else self.optimized_parameters.copy()
+
+    def set_Optimized_Parameters(self, parameters):
+        self.optimized_parameters = self._check_params(parameters).copy()
+
+    def get_Decomposition_Error(self):
+        """Cost of the optimised parameters (calc_decomposition_error)."""
+        if self.optimized_parameters is None:
+            raise Exception("no optimized parameters")
+        return self.Optimization_Problem(self.optimized_parameters)

@@ -1,0 +1,80 @@
+"""CPU checks of the host-side tile-program compiler and the kernels' algorithm.
+
+The op/layout records come from libsqhip itself (sqh_inspect_program); the
+emulator in tile_emulator.py executes them as the HIP kernels do and is compared
+with the oracle: forward application (Gates_block::apply_to), the trace entries,
+and the adjoint gradient.  No GPU is needed.
+"""
+import numpy as np
+import pytest
+
+import oracle as orc
+import tile_emulator as te
+from conftest import rel_fro
+from squander_hip import _native as nat
+from squander_hip import workloads as W
+
+
+def kernels_for(oracle, gates, params):
+    ks = []
+    for t, tg, c, ps in gates:
+        if nat.PARAM_NUM[int(t)]:
+            ks.append(oracle.gate_kernel(int(t), params[ps:ps + nat.PARAM_NUM[int(t)]]).reshape(-1))
+    return ks
+
+
+@pytest.mark.parametrize("n", list(range(1, 11)))
+def test_forward_program_matches_oracle(oracle, n):
+    gates = W.mixed_gate_circuit(n, 6 * n + 10, seed=300 + n)
+    P = W.param_count(gates)
+    p = np.random.default_rng(n).uniform(0, 2 * np.pi, P)
+    ops, lays = te.decode(*nat.inspect_program(n, gates))
+    d = 1 << n
+    m = min(d, 48)
+    X = np.random.default_rng(7).standard_normal((d, m)) + 1j * np.random.default_rng(8).standard_normal((d, m))
+    out, _ = te.run_forward(n, ops, lays, kernels_for(oracle, gates, p), X)
+    assert rel_fro(out, oracle.apply_to(gates, p, X)) < 1e-13
+
+
+def test_program_remaps_only_when_needed():
+    # 8-qubit ansatz: every op target must be a register qubit; remaps are far fewer than gates.
+    gates = W.ansatz_8q_40()
+    ops, lays = te.decode(*nat.inspect_program(8, gates))
+    nremap = sum(1 for o in ops if o[0] == te.OP_REMAP)
+    assert len(ops) - nremap == len(gates)
+    assert nremap < len(gates) // 3
+    assert all(o[1] < 4 for o in ops if o[0] != te.OP_REMAP)
+
+
+@pytest.mark.parametrize("n,off,variant", [(3, 0, orc.COST_FROBENIUS), (4, 0, orc.COST_HS), (5, 0, orc.COST_ABS_TRACE),
+                                           (6, 8, orc.COST_FROBENIUS), (8, 0, orc.COST_INFIDELITY)])
+def test_adjoint_gradient_matches_oracle(oracle, n, off, variant):
+    gates = W.mixed_gate_circuit(n, 5 * n + 8, seed=400 + n)
+    P = W.param_count(gates)
+    p = np.random.default_rng(3 * n).uniform(0, 2 * np.pi, P)
+    d = 1 << n
+    m = d - off
+    U = W.haar_unitary(d, 50 + n)[:, :m].copy()
+    ops, lays = te.decode(*nat.inspect_program(n, gates))
+    toff = off if variant in (orc.COST_FROBENIUS, orc.COST_ABS_TRACE) else 0
+    trace, Ms = te.run_adjoint(n, ops, lays, kernels_for(oracle, gates, p), U, toff)
+    # gradient from the correlations, as the finish kernel forms it
+    grad = np.zeros(P)
+    slot = 0
+    for t, tg, c, ps in gates:
+        for q in range(nat.PARAM_NUM[int(t)]):
+            D = oracle.deriv_kernel(int(t), p[ps:ps + nat.PARAM_NUM[int(t)]], q).reshape(-1)
+            tk = np.sum(D * Ms[slot])
+            if variant == orc.COST_FROBENIUS:
+                grad[ps + q] = -tk.real / m
+            elif variant == orc.COST_HS:
+                grad[ps + q] = -2.0 / m ** 2 * (trace.real * tk.real + trace.imag * tk.imag)
+            elif variant == orc.COST_INFIDELITY:
+                grad[ps + q] = -2.0 / m / (m + 1) * (trace.real * tk.real + trace.imag * tk.imag)
+            else:
+                grad[ps + q] = -(trace.real * tk.real + trace.imag * tk.imag) / (abs(trace) * m)
+        if nat.PARAM_NUM[int(t)]:
+            slot += 1
+    f_ref, g_ref = oracle.optimization_problem_combined(gates, U, n, p, variant, off)
+    assert abs(orc.brute_cost(oracle.apply_to(gates, p, U), variant, off) - f_ref) < 1e-13
+    assert np.abs(grad - g_ref).max() < 1e-12
