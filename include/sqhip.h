@@ -140,7 +140,7 @@ int sqh_get_kernel_timing(sqh_ctx* ctx, double* total_ms, int* launches);
 
 /* ---- introspection (host only, no device needed) ---------------------------- */
 /* Compiles a gate list into the register-tile program the kernels execute and
- * copies out its op records (8 bytes each) and layout records (48 bytes each);
+ * copies out its op records (8 bytes each) and layout records (96 bytes each);
  * n_ops / n_layouts receive the counts.  Buffers may be NULL to query the counts.
  * No counterpart in the reference: it exists for tooling and the CPU test suite. */
 int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* ops_out, int ops_cap,

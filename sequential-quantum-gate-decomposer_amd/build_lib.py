@@ -21,6 +21,9 @@ ARCH = os.environ.get("SQH_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          # Keep SimplifyCFG from merging the per-target-qubit switch arms into
+          # pointer phis: those force the register tile into scratch memory.
+          "-mllvm", "-simplifycfg-sink-common=false",
           "-Wall", "-Wno-unused-function", "-Wno-unknown-pragmas"]
 
 

@@ -70,7 +70,7 @@ TileLaunch base_launch(sqh_ctx* c, bool f32) {
     TileLaunch t;
     t.N = c->prog.N;
     t.f32 = f32;
-    t.ops = c->d_ops.as<Op>();
+    t.ops = c->d_ops.as<Op>() + kOpPad;
     t.nops = (int)c->prog.ops.size();
     t.lays = c->d_lays.as<Layout>();
     t.kern = c->d_kern.p;
@@ -269,11 +269,18 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
     }
     HIPTRY(c, hipSetDevice(c->device));
     HIPTRY(c, hipStreamSynchronize(c->stream));
-    HIPTRY(c, c->d_ops.ensure(std::max<size_t>(p.ops.size(), 1) * sizeof(Op)));
+    // Device copy of the ops with kOpPad no-op records on each side.
+    std::vector<Op> padded(p.ops.size() + 2 * kOpPad);
+    Op nop;
+    nop.kind = OP_NOP;
+    nop.t = nop.cmode = nop.c = 0;
+    nop.idx = 0;
+    for (auto& o : padded) o = nop;
+    std::copy(p.ops.begin(), p.ops.end(), padded.begin() + kOpPad);
+    HIPTRY(c, c->d_ops.ensure(padded.size() * sizeof(Op)));
     HIPTRY(c, c->d_lays.ensure(p.layouts.size() * sizeof(Layout)));
     HIPTRY(c, c->d_slots.ensure(std::max<size_t>(p.slots.size(), 1) * sizeof(Slot)));
-    if (!p.ops.empty())
-        HIPTRY(c, hipMemcpy(c->d_ops.p, p.ops.data(), p.ops.size() * sizeof(Op), hipMemcpyHostToDevice));
+    HIPTRY(c, hipMemcpy(c->d_ops.p, padded.data(), padded.size() * sizeof(Op), hipMemcpyHostToDevice));
     HIPTRY(c, hipMemcpy(c->d_lays.p, p.layouts.data(), p.layouts.size() * sizeof(Layout), hipMemcpyHostToDevice));
     if (!p.slots.empty())
         HIPTRY(c, hipMemcpy(c->d_slots.p, p.slots.data(), p.slots.size() * sizeof(Slot), hipMemcpyHostToDevice));

@@ -145,16 +145,13 @@ __device__ __forceinline__ cpx<R> cvt(cpx<double> a) {
 // ---------------------------------------------------------------------------
 // Register-tile primitives
 // ---------------------------------------------------------------------------
-template <int K, typename F>
-__device__ __forceinline__ void dispatch_t(int t, F&& f) {
-    switch (t) {
-    case 0: if constexpr (K > 0) f(std::integral_constant<int, 0>{}); break;
-    case 1: if constexpr (K > 1) f(std::integral_constant<int, 1>{}); break;
-    case 2: if constexpr (K > 2) f(std::integral_constant<int, 2>{}); break;
-    case 3: if constexpr (K > 3) f(std::integral_constant<int, 3>{}); break;
-    default: break;
-    }
-}
+// One switch arm per (op kind, local target): case KIND*4 + T runs STMT with a
+// compile-time T.  A single jump table replaces nested dispatch.
+#define SQH_ARMS(KIND, K, STMT)                                              \
+    case (KIND) * 4 + 0: { if constexpr ((K) > 0) { constexpr int T = 0; STMT; } } break; \
+    case (KIND) * 4 + 1: { if constexpr ((K) > 1) { constexpr int T = 1; STMT; } } break; \
+    case (KIND) * 4 + 2: { if constexpr ((K) > 2) { constexpr int T = 2; STMT; } } break; \
+    case (KIND) * 4 + 3: { if constexpr ((K) > 3) { constexpr int T = 3; STMT; } } break;
 
 // Mask of registers whose row has the control bit set (bit i <-> register i).
 template <int N>
@@ -260,33 +257,54 @@ __device__ __forceinline__ void remap(cpx<R> (&v)[Geo<N>::NR], const Layout* lo,
     }
 }
 
-// Forward execution of one gate op.  kp points at the op's 2x2 kernel.
+// Forward execution of one gate op with its 2x2 kernel k.
 template <int N, bool DERIV, typename R>
 __device__ __forceinline__ void exec_gate(cpx<R> (&v)[Geo<N>::NR], const Op op, const cpx<R> k[4], int s) {
     constexpr int K = Geo<N>::K;
-    switch (op.kind) {
-    case OP_U:
-        dispatch_t<K>(op.t, [&](auto tc) { pairs_apply<decltype(tc)::value, false, false>(v, k, 0u); });
-        break;
-    case OP_CU: {
-        const uint32_t m = ctrl_mask<N>(op.cmode, op.c, s);
-        dispatch_t<K>(op.t, [&](auto tc) { pairs_apply<decltype(tc)::value, true, DERIV>(v, k, m); });
-        break;
-    }
-    case OP_X: {
-        const uint32_t m = ctrl_mask<N>(op.cmode, op.c, s);
-        dispatch_t<K>(op.t, [&](auto tc) { pairs_swap<decltype(tc)::value>(v, m); });
-        break;
-    }
-    case OP_Z: {
-        const uint32_t m = ctrl_mask<N>(op.cmode, op.c, s);
-        dispatch_t<K>(op.t, [&](auto tc) { pairs_negate<decltype(tc)::value>(v, m); });
-        break;
-    }
+    switch (op.kind * 4 + op.t) {
+        SQH_ARMS(OP_U, K, (pairs_apply<T, false, false>(v, k, 0u)))
+        SQH_ARMS(OP_CU, K, (pairs_apply<T, true, DERIV>(v, k, ctrl_mask<N>(op.cmode, op.c, s))))
+        SQH_ARMS(OP_X, K, (pairs_swap<T>(v, ctrl_mask<N>(op.cmode, op.c, s))))
+        SQH_ARMS(OP_Z, K, (pairs_negate<T>(v, ctrl_mask<N>(op.cmode, op.c, s))))
     default:
         break;
     }
 }
+
+// Kernel-table slot of an op (0 for ops without a kernel, so a prefetch of it is
+// always in bounds).  Callers clamp look-ahead positions to the padding.
+__device__ __forceinline__ int kslot(const Op op) { return op.kind <= OP_CU ? op.idx : 0; }
+
+// Software-pipelined walk over a program: the op record two ahead and the kernel
+// of the next op are loaded while the current op computes.
+template <typename R>
+struct OpStream {
+    const Op* ops;
+    const cpx<R>* kb;
+    int nops;
+    Op cur, nxt;
+    cpx<R> k[4];
+    __device__ __forceinline__ OpStream(const Op* ops_, const cpx<R>* kb_, int nops_, int first, int step)
+        : ops(ops_), kb(kb_), nops(nops_) {
+        cur = at(first);
+        nxt = at(first + step);
+        load(kslot(cur), k);
+    }
+    // ops[-kOpPad .. nops + kOpPad) are readable (no-op padding on both sides).
+    __device__ __forceinline__ Op at(int p) const { return ops[p]; }
+    __device__ __forceinline__ void load(int slot, cpx<R> out[4]) const {
+        const cpx<R>* kp = kb + 4 * slot;
+        out[0] = kp[0];
+        out[1] = kp[1];
+        out[2] = kp[2];
+        out[3] = kp[3];
+    }
+    // Issue the loads for the following ops; returns the kernel of the next op.
+    __device__ __forceinline__ void prefetch(int p_after_next, cpx<R> knext[4], Op& after) const {
+        after = at(p_after_next);
+        load(kslot(nxt), knext);
+    }
+};
 
 template <typename R>
 __device__ __forceinline__ void load_k(const cpx<R>* kp, cpx<R> k[4]) {
@@ -302,19 +320,22 @@ struct LaneInfo {
     bool bvalid;  // the lane's batch index is in range (before clamping)
 };
 
-template <int N>
-__device__ __forceinline__ LaneInfo lane_info(int m, int m_pad, int B) {
+// UNI: every lane of the wavefront belongs to one batch element (m_pad is a multiple
+// of CW), so the batch index is made wave-uniform and per-element tables are read
+// through the scalar cache.
+template <int N, bool UNI = false>
+__device__ __forceinline__ LaneInfo lane_info(int m, int m_pad, int B, int lane, long long wave) {
     constexpr int CW = Geo<N>::CW;
     LaneInfo li;
-    const int lane = threadIdx.x;
     li.s = lane / CW;
     li.cc = lane % CW;
-    const long long item = (long long)blockIdx.x * CW + li.cc;
+    const long long item = wave * CW + li.cc;
     li.b = (int)(item / m_pad);
     li.col = (int)(item % m_pad);
     li.bvalid = li.b < B;
     li.valid = li.bvalid && (li.col < m);
     if (!li.bvalid) li.b = B - 1;
+    if constexpr (UNI) li.b = __builtin_amdgcn_readfirstlane(li.b);
     return li;
 }
 
@@ -322,17 +343,23 @@ __device__ __forceinline__ LaneInfo lane_info(int m, int m_pad, int B) {
 // Forward tile kernel: apply the whole program, then either store the slab
 // (Gates_block::apply_to) or emit the diagonal entries the trace needs.
 // ---------------------------------------------------------------------------
-template <int N, typename R, int MODE>
-__global__ __launch_bounds__(64) void k_tile_forward(const cpx<R>* in, long long in_bstride,
-                                                     int in_stride, cpx<R>* out,
-                                                     long long out_bstride, int out_stride, int m, int m_pad,
-                                                     int B, const Op* __restrict__ ops, int nops,
-                                                     const Layout* __restrict__ lays,
-                                                     const cpx<R>* __restrict__ kern, long long kern_bstride,
-                                                     int trace_off, double* __restrict__ diag) {
+// WPG > 1: a workgroup of WPG wavefronts that share one batch element (requires
+// UNI and waves-per-element divisible by WPG); the element's kernel table (ns
+// entries) is staged in LDS after the WPG remap buffers (dynamic shared memory).
+template <int N, typename R, int MODE, bool UNI, int WPG>
+__global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, long long in_bstride,
+                                                           int in_stride, cpx<R>* out,
+                                                           long long out_bstride, int out_stride, int m, int m_pad,
+                                                           int B, const Op* __restrict__ ops, int nops,
+                                                           const Layout* __restrict__ lays,
+                                                           const cpx<R>* __restrict__ kern, long long kern_bstride,
+                                                           int trace_off, double* __restrict__ diag, int ns) {
     using G = Geo<N>;
-    __shared__ cpx<R> lds[G::LDS];
-    const LaneInfo li = lane_info<N>(m, m_pad, B);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
+    const int wave = (WPG > 1) ? (int)(threadIdx.x >> 6) : 0;
+    cpx<R>* lds = smem + wave * G::LDS;
+    const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x & 63, (long long)blockIdx.x * WPG + wave);
     cpx<R> v[G::NR];
     {
         const cpx<R>* src = in + (long long)li.b * in_bstride;
@@ -344,18 +371,30 @@ __global__ __launch_bounds__(64) void k_tile_forward(const cpx<R>* in, long long
         }
     }
     const cpx<R>* kb = kern + (long long)li.b * kern_bstride;
+    if constexpr (WPG > 1) {
+        cpx<R>* ktab = smem + WPG * G::LDS;
+        for (int i = threadIdx.x; i < 4 * ns; i += 64 * WPG) ktab[i] = kb[i];
+        __syncthreads();
+        kb = ktab;
+    }
     int cur = 0;
+    OpStream<R> os(ops, kb, nops, 0, 1);
     for (int p = 0; p < nops; ++p) {
-        const Op op = ops[p];
+        cpx<R> kn[4];
+        Op after;
+        os.prefetch(p + 2, kn, after);
+        const Op op = os.cur;
         if (op.kind == OP_REMAP) {
             const int o = op.idx >> 16, n = op.idx & 0xffff;
             remap<N, R>(v, &lays[o], &lays[n], li.s, li.cc, lds);
             cur = n;
-            continue;
+        } else {
+            exec_gate<N, false, R>(v, op, os.k, li.s);
         }
-        cpx<R> k[4];
-        if (op.kind <= OP_CU) load_k(kb + 4 * op.idx, k);
-        exec_gate<N, false, R>(v, op, k, li.s);
+        os.cur = os.nxt;
+        os.nxt = after;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) os.k[e] = kn[e];
     }
     const int so = sub_off<N>(&lays[cur], li.s);
     if constexpr (MODE == 0) {
@@ -396,7 +435,7 @@ __device__ __forceinline__ T seg_sum(T x, int lim) {
 // finish kernel evaluates.  This replaces the reference's per-parameter derivative
 // matrices (Gates_block.cpp:1012-1150) with O(G d m) work.
 // ---------------------------------------------------------------------------
-template <int N, typename R>
+template <int N, typename R, bool UNI>
 __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in, int in_stride, int m, int m_pad,
                                                   int B, const Op* __restrict__ ops, int nops,
                                                   const Layout* __restrict__ lays, const cpx<R>* __restrict__ kern,
@@ -405,7 +444,7 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     using G = Geo<N>;
     constexpr int CW = G::CW;
     __shared__ cpx<R> lds[G::LDS];
-    const LaneInfo li = lane_info<N>(m, m_pad, B);
+    const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x, blockIdx.x);
     cpx<R> x[G::NR], y[G::NR];
     {
         const int so = sub_off<N>(&lays[0], li.s);
@@ -417,17 +456,25 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     }
     const cpx<R>* kb = kern + (long long)li.b * kern_bstride;
     int cur = 0;
-    for (int p = 0; p < nops; ++p) {
-        const Op op = ops[p];
-        if (op.kind == OP_REMAP) {
-            const int o = op.idx >> 16, n = op.idx & 0xffff;
-            remap<N, R>(x, &lays[o], &lays[n], li.s, li.cc, lds);
-            cur = n;
-            continue;
+    {
+        OpStream<R> os(ops, kb, nops, 0, 1);
+        for (int p = 0; p < nops; ++p) {
+            cpx<R> kn[4];
+            Op after;
+            os.prefetch(p + 2, kn, after);
+            const Op op = os.cur;
+            if (op.kind == OP_REMAP) {
+                const int o = op.idx >> 16, n = op.idx & 0xffff;
+                remap<N, R>(x, &lays[o], &lays[n], li.s, li.cc, lds);
+                cur = n;
+            } else {
+                exec_gate<N, false, R>(x, op, os.k, li.s);
+            }
+            os.cur = os.nxt;
+            os.nxt = after;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) os.k[e] = kn[e];
         }
-        cpx<R> k[4];
-        if (op.kind <= OP_CU) load_k(kb + 4 * op.idx, k);
-        exec_gate<N, false, R>(x, op, k, li.s);
     }
     // Diagonal of C.U and the selector E under the final layout.
     {
@@ -452,65 +499,62 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     const int wib = m_pad >= CW ? (int)((item0 % m_pad) / CW) : 0;
     double* mp = mpart + ((long long)li.b * wpb + wib) * (long long)ns * 8;
 
+    OpStream<R> os(ops, kb, nops, nops - 1, -1);
     for (int p = nops - 1; p >= 0; --p) {
-        const Op op = ops[p];
+        cpx<R> kn[4];
+        Op after;
+        os.prefetch(p - 2, kn, after);
+        const Op op = os.cur;
         if (op.kind == OP_REMAP) {
             const int o = op.idx >> 16, n = op.idx & 0xffff;
             remap<N, R>(x, &lays[n], &lays[o], li.s, li.cc, lds);
             remap<N, R>(y, &lays[n], &lays[o], li.s, li.cc, lds);
-            continue;
-        }
-        if (op.kind == OP_X || op.kind == OP_Z) {
+        } else if (op.kind == OP_X || op.kind == OP_Z) {
             // Self-inverse real symmetric permutation / sign: G^dagger = G^T = G.
-            cpx<R> dummy[4];
-            exec_gate<N, false, R>(x, op, dummy, li.s);
-            exec_gate<N, false, R>(y, op, dummy, li.s);
-            continue;
-        }
-        cpx<R> k[4], kd[4], kt[4];
-        load_k(kb + 4 * op.idx, k);
-        kd[0] = mk(k[0].re, -k[0].im);  // K^dagger
-        kd[1] = mk(k[2].re, -k[2].im);
-        kd[2] = mk(k[1].re, -k[1].im);
-        kd[3] = mk(k[3].re, -k[3].im);
-        kt[0] = k[0];                   // K^T
-        kt[1] = k[2];
-        kt[2] = k[1];
-        kt[3] = k[3];
-        const bool masked = op.kind == OP_CU;
-        const uint32_t mask = ctrl_mask<N>(op.cmode, op.c, li.s);
-        cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
-        constexpr int K = G::K;
-        if (masked) {
-            dispatch_t<K>(op.t, [&](auto tc) {
-                constexpr int T = decltype(tc)::value;
-                pairs_apply<T, true, false>(x, kd, mask);
-                pairs_corr<T>(y, x, acc, mask, true);
-                pairs_apply<T, true, false>(y, kt, mask);
-            });
+            exec_gate<N, false, R>(x, op, os.k, li.s);
+            exec_gate<N, false, R>(y, op, os.k, li.s);
         } else {
-            dispatch_t<K>(op.t, [&](auto tc) {
-                constexpr int T = decltype(tc)::value;
-                pairs_apply<T, false, false>(x, kd, 0u);
-                pairs_corr<T>(y, x, acc, 0u, false);
-                pairs_apply<T, false, false>(y, kt, 0u);
-            });
-        }
-        double r[8];
+            const cpx<R>* k = os.k;
+            cpx<R> kd[4], kt[4];
+            kd[0] = mk(k[0].re, -k[0].im);  // K^dagger
+            kd[1] = mk(k[2].re, -k[2].im);
+            kd[2] = mk(k[1].re, -k[1].im);
+            kd[3] = mk(k[3].re, -k[3].im);
+            kt[0] = k[0];                   // K^T
+            kt[1] = k[2];
+            kt[2] = k[1];
+            kt[3] = k[3];
+            cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+            constexpr int K = G::K;
+            switch (op.kind * 4 + op.t) {
+                SQH_ARMS(OP_U, K, (pairs_apply<T, false, false>(x, kd, 0u), pairs_corr<T>(y, x, acc, 0u, false),
+                                   pairs_apply<T, false, false>(y, kt, 0u)))
+                SQH_ARMS(OP_CU, K,
+                         (pairs_apply<T, true, false>(x, kd, ctrl_mask<N>(op.cmode, op.c, li.s)),
+                          pairs_corr<T>(y, x, acc, ctrl_mask<N>(op.cmode, op.c, li.s), true),
+                          pairs_apply<T, true, false>(y, kt, ctrl_mask<N>(op.cmode, op.c, li.s))))
+            default:
+                break;
+            }
+            double r[8];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            r[2 * e] = seg_sum<CW>((double)acc[e].re, lim);
-            r[2 * e + 1] = seg_sum<CW>((double)acc[e].im, lim);
-        }
-        if (leader) {
-            double* d = mp + (long long)op.idx * 8;
+            for (int e = 0; e < 4; ++e) {
+                r[2 * e] = seg_sum<CW>((double)acc[e].re, lim);
+                r[2 * e + 1] = seg_sum<CW>((double)acc[e].im, lim);
+            }
+            if (leader) {
+                double* d = mp + (long long)op.idx * 8;
 #pragma unroll
-            for (int e = 0; e < 8; ++e) d[e] = r[e];
+                for (int e = 0; e < 8; ++e) d[e] = r[e];
+            }
         }
+        os.cur = os.nxt;
+        os.nxt = after;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) os.k[e] = kn[e];
     }
 }
 
-// ---------------------------------------------------------------------------
 // Materialised derivative matrices (Gates_block::apply_derivate_to,
 // Gates_block.cpp:1012-1150): each wavefront takes one parametric gate and one
 // column slab, runs the prefix, and for each parameter of the gate applies the

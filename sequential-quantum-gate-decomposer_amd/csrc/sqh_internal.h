@@ -30,7 +30,12 @@ enum OpKind : uint8_t {
     OP_X = 2,      // CNOT: controlled row swap
     OP_Z = 3,      // CZ: controlled sign flip of the target-bit-set row
     OP_REMAP = 4,  // idx = (old_layout << 16) | new_layout
+    OP_NOP = 5,    // padding around the program (never executed, only prefetched)
 };
+
+// Padding records on each side of the device copy of a program, so the kernels'
+// look-ahead loads need no bounds checks.
+constexpr int kOpPad = 2;
 
 // Control modes: none, control qubit is a register bit (c = register position),
 // control qubit is a lane bit (c = sub-index bit).
@@ -44,9 +49,8 @@ struct alignas(8) Op {
 // A layout: reg_off[i] is the row-index contribution of register i, sub_pos[j] the
 // qubit carried by bit j of the lane's sub index.
 struct alignas(16) Layout {
-    int16_t reg_off[16];
-    int8_t sub_pos[8];
-    int32_t pad[2];
+    int32_t reg_off[16];
+    int32_t sub_pos[8];
 };
 
 // One parametric gate ("slot"): its type, first raw parameter, parameter count,

@@ -22,21 +22,76 @@ static inline unsigned blocks_for(long long n, int bs) { return (unsigned)((n + 
     default: return hipErrorInvalidValue;     \
     }
 
+// A wavefront holds one batch element when the padded column count is a multiple
+// of its column count (only possible above 4 qubits).
+template <int N>
+inline bool uniform_waves(const TileLaunch& t) {
+    return N > 4 && t.m_pad % Geo<N>::CW == 0;
+}
+
+// The uniform path groups 4 (else 2) wavefronts of one batch element per workgroup
+// when the element spans a multiple of that many wavefronts and its kernel table
+// fits this LDS budget.
+constexpr size_t kKernelTableLdsBytes = 32 * 1024;
+
+template <int N, typename R, int MODE, bool UNI, int WPG>
+hipError_t launch_forward_nmu(const TileLaunch& t, hipStream_t st, unsigned waves) {
+    const unsigned grid = (waves + WPG - 1) / WPG;
+    const size_t shm = sizeof(cpx<R>) * ((size_t)WPG * Geo<N>::LDS + (WPG > 1 ? 4 * (size_t)t.ns : 0));
+    hipLaunchKernelGGL((k_tile_forward<N, R, MODE, UNI, WPG>), dim3(grid), dim3(64 * WPG), shm, st,
+                       (const cpx<R>*)t.in, t.in_bstride, t.in_stride, (cpx<R>*)t.out, t.out_bstride, t.out_stride,
+                       t.m, t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride,
+                       t.trace_off, t.diag, t.ns);
+    return hipGetLastError();
+}
+
+// Largest dynamic LDS a workgroup may use on the current device (cached per device).
+inline size_t max_lds_per_group() {
+    static size_t cache[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 64 * 1024;
+    if (!cache[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            v = 64 * 1024;
+        cache[dev] = (size_t)v;
+    }
+    return cache[dev];
+}
+
+template <int N, typename R, int WPG>
+inline bool group_fits(const TileLaunch& t) {
+    const size_t shm = sizeof(cpx<R>) * ((size_t)WPG * Geo<N>::LDS + 4 * (size_t)t.ns);
+    const int wpb = t.m_pad / Geo<N>::CW;
+    return t.ns > 0 && wpb % WPG == 0 && sizeof(cpx<R>) * 4 * (size_t)t.ns <= kKernelTableLdsBytes &&
+           shm <= max_lds_per_group();
+}
+
+template <int N, typename R, int MODE>
+hipError_t launch_forward_nm(const TileLaunch& t, hipStream_t st, unsigned waves) {
+    if constexpr (N > 4) {
+        if (uniform_waves<N>(t)) {
+            if (group_fits<N, R, 4>(t)) return launch_forward_nmu<N, R, MODE, true, 4>(t, st, waves);
+            if (group_fits<N, R, 2>(t)) return launch_forward_nmu<N, R, MODE, true, 2>(t, st, waves);
+            return launch_forward_nmu<N, R, MODE, true, 1>(t, st, waves);
+        }
+    }
+    return launch_forward_nmu<N, R, MODE, false, 1>(t, st, waves);
+}
+
 template <int N, typename R>
 hipError_t launch_forward_n(const TileLaunch& t, hipStream_t st) {
     using G = Geo<N>;
     const long long items = (long long)t.B * t.m_pad;
-    const unsigned grid = blocks_for(items, G::CW);
-    if (t.mode == 0)
-        hipLaunchKernelGGL((k_tile_forward<N, R, 0>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in,
-                           t.in_bstride, t.in_stride, (cpx<R>*)t.out, t.out_bstride, t.out_stride, t.m, t.m_pad,
-                           t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride, t.trace_off,
-                           t.diag);
-    else
-        hipLaunchKernelGGL((k_tile_forward<N, R, 1>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in,
-                           t.in_bstride, t.in_stride, (cpx<R>*)t.out, t.out_bstride, t.out_stride, t.m, t.m_pad,
-                           t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride, t.trace_off,
-                           t.diag);
+    const unsigned waves = blocks_for(items, G::CW);
+    return t.mode == 0 ? launch_forward_nm<N, R, 0>(t, st, waves) : launch_forward_nm<N, R, 1>(t, st, waves);
+}
+
+template <int N, typename R, bool UNI>
+hipError_t launch_grad_nu(const TileLaunch& t, hipStream_t st, unsigned grid) {
+    hipLaunchKernelGGL((k_tile_grad<N, R, UNI>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in, t.in_stride, t.m,
+                       t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride, t.trace_off,
+                       t.ns, t.diag, t.mpart);
     return hipGetLastError();
 }
 
@@ -45,10 +100,10 @@ hipError_t launch_grad_n(const TileLaunch& t, hipStream_t st) {
     using G = Geo<N>;
     const long long items = (long long)t.B * t.m_pad;
     const unsigned grid = blocks_for(items, G::CW);
-    hipLaunchKernelGGL((k_tile_grad<N, R>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in, t.in_stride, t.m,
-                       t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride, t.trace_off,
-                       t.ns, t.diag, t.mpart);
-    return hipGetLastError();
+    if constexpr (N > 4) {
+        if (uniform_waves<N>(t)) return launch_grad_nu<N, R, true>(t, st, grid);
+    }
+    return launch_grad_nu<N, R, false>(t, st, grid);
 }
 
 template <int N, typename R>

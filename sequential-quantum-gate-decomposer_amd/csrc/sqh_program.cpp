@@ -19,9 +19,9 @@ Layout make_layout(int N, int K, const std::vector<int>& pos) {
         int off = 0;
         for (int j = 0; j < K; ++j)
             if ((i >> j) & 1) off |= 1 << pos[j];
-        l.reg_off[i] = (int16_t)off;
+        l.reg_off[i] = off;
     }
-    for (int j = 0; j < N - K; ++j) l.sub_pos[j] = (int8_t)pos[K + j];
+    for (int j = 0; j < N - K; ++j) l.sub_pos[j] = pos[K + j];
     return l;
 }
 

@@ -114,7 +114,7 @@ def gate_array(gates):
 
 def inspect_program(qbit_num, gates):
     """The register-tile program the library compiles for a gate list:
-    (ops[n,8] uint8 records, layouts[n,48] uint8 records)."""
+    (ops[n,8] uint8 records, layouts[n,96] uint8 records)."""
     L = lib()
     garr, ng = gate_array(gates)
     nops, nl = ctypes.c_int(0), ctypes.c_int(0)
@@ -122,7 +122,7 @@ def inspect_program(qbit_num, gates):
     if rc:
         raise SqhError(rc, "program compilation failed")
     ops = np.zeros((max(nops.value, 1), 8), dtype=np.uint8)
-    lays = np.zeros((max(nl.value, 1), 48), dtype=np.uint8)
+    lays = np.zeros((max(nl.value, 1), 96), dtype=np.uint8)
     rc = L.sqh_inspect_program(qbit_num, garr, ng, ops.ctypes.data, nops.value, lays.ctypes.data, nl.value,
                                ctypes.byref(nops), ctypes.byref(nl))
     if rc:

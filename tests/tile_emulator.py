@@ -23,8 +23,8 @@ def decode(ops_raw, lays_raw):
         ops.append((int(r[0]), int(r[1]), int(r[2]), int(r[3]), idx))
     lays = []
     for r in lays_raw:
-        reg_off = np.frombuffer(bytes(r[0:32]), dtype="<i2").astype(np.int64)
-        sub_pos = np.frombuffer(bytes(r[32:40]), dtype="i1").astype(np.int64)
+        reg_off = np.frombuffer(bytes(r[0:64]), dtype="<i4").astype(np.int64)
+        sub_pos = np.frombuffer(bytes(r[64:96]), dtype="<i4").astype(np.int64)
         lays.append((reg_off, sub_pos))
     return ops, lays
 
