@@ -129,6 +129,36 @@ int sqh_optimization_problem_combined_batch_dev(sqh_ctx* ctx, const double* d_pa
 int sqh_optimization_problem_combined_unitary(sqh_ctx* ctx, const double* params, double* out_unitary,
                                               double* out_derivs);
 
+/* ---- batched decomposition (the optimiser that calls the path) -------------- */
+/* Options of sqh_decompose_batch; the defaults of the reference's custom
+ * decomposition at <= 5 qubits are max_inner_iterations = 10000,
+ * max_outer_iterations = 4, convergence_threshold = 1e-5
+ * (Optimization_Interface.cpp:1667-1670, N_Qubit_Decomposition_custom.cpp:59-76,
+ * Decomposition_Base.cpp:131-224). */
+typedef struct sqh_decompose_options {
+    long long max_inner_iterations;   /* cost+gradient evaluations per BFGS run */
+    int max_outer_iterations;         /* BFGS restarts from the best point */
+    double optimization_tolerance;    /* stop once the FP64 cost is below this */
+    double convergence_threshold;     /* relative spread of the last 20 minima */
+    int mixed_precision;              /* 1: a first BFGS stage in complex64 / FP32 */
+    double switch_cost;               /* ... handing over to FP64 below this FP32 cost */
+    int rounds_per_graph;             /* lockstep rounds per captured hipGraph (0: 16) */
+    long long max_rounds;             /* safety cap on rounds (0: derived from the limits) */
+} sqh_decompose_options;
+
+/* n_runs independent decompositions of the context's circuit, run r on target
+ * run_target[r] (targets: n_targets x rows x cols complex128, each the Umtx of
+ * its problem) from start point x0[r] (n_runs x P).  Each run is the reference's
+ * driver for a custom circuit: BFGS_Powell with the Wolfe line search
+ * (BFGS_Powell.cpp:70-386, grad_descend.cpp:127-272) restarted from the best point
+ * (Decomposition_Base.cpp:375-709, BFGS.cpp:42-152, one iteration loop).  All runs
+ * advance in lockstep on the GPU, one batched cost+gradient launch per round.
+ * Outputs: best parameters, best FP64 cost, status of the last BFGS run (-1 if the
+ * round cap stopped it), evaluations per run, and the number of rounds. */
+int sqh_decompose_batch(sqh_ctx* ctx, const double* targets, int n_targets, int rows, int cols,
+                        const int* run_target, int n_runs, const double* x0, const sqh_decompose_options* opts,
+                        double* x_out, double* f_out, int* status_out, long long* evals_out, long long* rounds_out);
+
 /* ---- kernel timers --------------------------------------------------------- */
 /* When enabled, every call brackets its dominant kernel (the register-tile kernel,
  * or the derivative kernel) with HIP events on the context's stream.

@@ -359,6 +359,7 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
     cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
     const int wave = (WPG > 1) ? (int)(threadIdx.x >> 6) : 0;
     cpx<R>* lds = smem + wave * G::LDS;
+    if (((long long)blockIdx.x * WPG + wave) * G::CW / m_pad >= B) return;  // wavefront past the batch
     const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x & 63, (long long)blockIdx.x * WPG + wave);
     cpx<R> v[G::NR];
     {
@@ -440,11 +441,18 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
                                                   int B, const Op* __restrict__ ops, int nops,
                                                   const Layout* __restrict__ lays, const cpx<R>* __restrict__ kern,
                                                   long long kern_bstride, int trace_off, int ns,
-                                                  double* __restrict__ diag, double* __restrict__ mpart) {
+                                                  double* __restrict__ diag, double* __restrict__ mpart,
+                                                  BatchIndex bi) {
     using G = Geo<N>;
     constexpr int CW = G::CW;
     __shared__ cpx<R> lds[G::LDS];
+    if (bi.count) {
+        B = *bi.count;
+        if (B <= 0) return;
+    }
+    if ((long long)blockIdx.x * CW / m_pad >= B) return;  // whole wavefront past the live elements
     const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x, blockIdx.x);
+    if (bi.count) in += (long long)bi.run_target[bi.elem_run[li.b]] * bi.tgt_stride;
     cpx<R> x[G::NR], y[G::NR];
     {
         const int so = sub_off<N>(&lays[0], li.s);
@@ -644,13 +652,15 @@ __global__ __launch_bounds__(64) void k_tile_deriv(const cpx<R>* __restrict__ in
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ void k_build_kernels(const double* __restrict__ params, int P, int B, const Slot* __restrict__ slots,
-                                int ns, cpx<R>* __restrict__ kern) {
+                                int ns, cpx<R>* __restrict__ kern, BatchIndex bi) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (bi.count) B = *bi.count;
     if (i >= (long long)B * ns) return;
     const int b = (int)(i / ns), sl = (int)(i % ns);
+    const int r = bi.count ? bi.elem_run[b] : b;
     const Slot s = slots[sl];
     cpx<double> k[4];
-    dev_gate_kernel(s.type, params + (long long)b * P + s.param_start, k);
+    dev_gate_kernel(s.type, params + (long long)r * P + s.param_start, k);
     cpx<R>* o = kern + i * 4;
 #pragma unroll
     for (int e = 0; e < 4; ++e) o[e] = cvt<R>(k[e]);
@@ -700,9 +710,12 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag,
                                                 double* __restrict__ costs, const double* __restrict__ mpart,
                                                 int wpb, int ns, const Slot* __restrict__ slots,
                                                 const double* __restrict__ params, int P,
-                                                double* __restrict__ grad) {
+                                                double* __restrict__ grad, BatchIndex bi) {
     __shared__ double sh[8];
     const int b = blockIdx.x;
+    if (bi.count && b >= *bi.count) return;
+    // Results go to the run's slots; params/costs/grad are indexed by run.
+    const int r = bi.count ? bi.elem_run[b] : b;
     double re = 0.0, im = 0.0;
     for (int c = threadIdx.x; c < m; c += blockDim.x) {
         re += diag[2 * ((long long)b * m + c)];
@@ -710,7 +723,7 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag,
     }
     const double tre = block_sum(re, sh);
     const double tim = block_sum(im, sh);
-    if (threadIdx.x == 0 && costs) costs[b] = cost_of(variant, tre, tim, (double)m);
+    if (threadIdx.x == 0 && costs) costs[r] = cost_of(variant, tre, tim, (double)m);
     if (!mpart) return;
     for (int s = threadIdx.x; s < ns; s += blockDim.x) {
         double M[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -720,7 +733,7 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag,
             for (int e = 0; e < 8; ++e) M[e] += src[e];
         }
         const Slot sl = slots[s];
-        const double* pb = params + (long long)b * P;
+        const double* pb = params + (long long)r * P;
         for (int q = 0; q < sl.nparam; ++q) {
             cpx<double> d[4];
             dev_deriv_kernel(sl.type, pb + sl.param_start, q, d);
@@ -730,7 +743,7 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag,
                 kre += d[e].re * M[2 * e] - d[e].im * M[2 * e + 1];
                 kim += d[e].re * M[2 * e + 1] + d[e].im * M[2 * e];
             }
-            grad[(long long)b * P + sl.param_start + q] = grad_of(variant, tre, tim, kre, kim, (double)m);
+            grad[(long long)r * P + sl.param_start + q] = grad_of(variant, tre, tim, kre, kim, (double)m);
         }
     }
 }

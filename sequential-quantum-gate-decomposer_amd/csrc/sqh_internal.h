@@ -59,6 +59,17 @@ struct alignas(16) Slot {
     int32_t type, param_start, nparam, op;
 };
 
+// Optional indirection for batched optimisation: launch element j is run
+// elem_run[j], whose target matrix is run_target[run] (targets tgt_stride complex
+// entries apart); only the first *count elements are live.  All null: element j
+// is batch element j on the shared input.
+struct BatchIndex {
+    const int* count = nullptr;
+    const int* elem_run = nullptr;
+    const int* run_target = nullptr;
+    long long tgt_stride = 0;
+};
+
 constexpr int kMaxTileQubits = 10;   // one column fits one wavefront up to 2^10 rows
 
 inline int param_num_of(int type) {

@@ -1,0 +1,400 @@
+// sqh_k_bfgs.hip -- kernels of the batched BFGS optimiser (state machine per run).
+#include <cfloat>
+
+#include "sqh_bfgs_device.h"
+#include "sqh_kernels.h"
+
+namespace sqh {
+
+constexpr int kBfgsThreads = 256;
+constexpr int kBfgsMaxP = 1024;
+constexpr double kNumPrecision = 1.42e-14;  // Grad_Descend::num_precision (grad_descend.cpp:40)
+
+__device__ inline double wg_sum(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int i = 0; i < kBfgsThreads / 64; ++i) t += sh[i];
+    return t;
+}
+
+__device__ inline double wg_min(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double t = sh[0];
+    for (int i = 1; i < kBfgsThreads / 64; ++i) t = fmin(t, sh[i]);
+    return t;
+}
+
+__global__ void k_bfgs_init(RunState* __restrict__ st, double* __restrict__ vecs, const double* __restrict__ x0,
+                            int B, int P, int mixed, int* __restrict__ counts) {
+    const int r = blockIdx.x;
+    if (r >= B) return;
+    double* V = vecs + (size_t)r * RV_COUNT * P;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) V[RV_BEST * P + k] = x0[(size_t)r * P + k];
+    if (threadIdx.x == 0) {
+        RunState S;
+        S.phase = PH_OUTER_START;
+        S.status = BS_INITIAL;
+        S.prec = mixed ? 1 : 0;
+        S.outer = 0;
+        S.ls_iter = 0;
+        S.finished = 0;
+        S.pad0 = S.pad1 = 0;
+        S.calls = S.total_calls = S.calls32 = 0;
+        S.f = S.fprev = 0.0;
+        S.norm_Z = -1.0;
+        S.step = S.f_lowest = S.f_highest = S.step_highest = S.step_lowest = S.step_best = S.f_best = 0.0;
+        S.dg_abs_best = S.dg_highest = S.dg0 = S.maxstep = 0.0;
+        S.cur_min = DBL_MAX;
+        for (int i = 0; i < kMinWindow; ++i) S.minvec[i] = 0.0;
+        st[r] = S;
+    }
+    if (r == 0 && threadIdx.x == 0) counts[0] = counts[1] = counts[2] = 0;
+}
+
+__global__ void k_bfgs_reset_counts(int* __restrict__ counts) {
+    if (threadIdx.x == 0) {
+        counts[0] = 0;
+        counts[1] = 0;
+    }
+}
+
+// BFGS_Powell::BFGS_iteration (BFGS_Powell.cpp:202-386) across the workgroup.
+__device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int P, double& norm_Z, double* sh,
+                            double* zs) {
+    const int tid = threadIdx.x, nt = blockDim.x;
+    double* x = V + RV_X * P;
+    double* g = V + RV_G * P;
+    double* x0 = V + RV_X0 * P;
+    double* g0 = V + RV_G0 * P;
+    double* delta = V + RV_DELTA * P;
+    double* gamma = V + RV_GAMMA * P;
+    const double* ztg = V + RV_ZTG * P;
+    double nd = 0.0, t0 = 0.0, dtg = 0.0;
+    for (int k = tid; k < P; k += nt) {
+        const double dk = x[k] - x0[k], gk = g[k] - g0[k];
+        delta[k] = dk;
+        gamma[k] = gk;
+        nd += dk * dk;
+        t0 += g0[k] * dk;
+        dtg += gk * dk;
+    }
+    nd = wg_sum(nd, sh);
+    t0 = wg_sum(t0, sh);
+    dtg = wg_sum(dtg, sh);
+    if (dtg < fabs(t0) * 0.1) return;
+    dtg = fmax(kNumPrecision, dtg);
+    for (int k = tid; k < P; k += nt) zs[k] = ztg[k];
+    __syncthreads();
+    // Givens rotations folding Z^T g into its first entry (Powell 1987, Eq. 1.7).
+    for (int c = P - 1; c > 0; --c) {
+        const double zc = zs[c], zm = zs[c - 1];
+        if (zc == 0.0) continue;
+        const double a = sqrt(zm * zm + zc * zc);
+        const double cv = zm / a, sv = zc / a;
+        for (int row = tid; row < P; row += nt) {
+            double* z = Z + (size_t)row * P + (c - 1);
+            const double z0 = z[0], z1 = z[1];
+            z[0] = cv * z0 + sv * z1;
+            z[1] = cv * z1 - sv * z0;
+        }
+        __syncthreads();
+        if (tid == 0) zs[c - 1] = a;
+        __syncthreads();
+    }
+    if (norm_Z < 0.0) {
+        norm_Z = nd / dtg;
+    } else {
+        const double t = sqrt(norm_Z * nd / dtg);
+        norm_Z = fmin(norm_Z, t);
+        norm_Z = fmax(norm_Z, t * 0.1);
+    }
+    const double sq = sqrt(dtg);
+    for (int row = tid; row < P; row += nt)
+        Z[(size_t)row * P] = fabs(delta[row]) > kNumPrecision ? delta[row] / sq : 0.0;
+    __syncthreads();
+    for (int c = 1 + tid; c < P; c += nt) {
+        double s = 0.0;
+        for (int row = 0; row < P; ++row) s += gamma[row] * Z[(size_t)row * P + c];
+        zs[c] = s / dtg;
+    }
+    __syncthreads();
+    for (int c = 1 + tid; c < P; c += nt) {
+        const double gz = zs[c];
+        double nz = 0.0;
+        for (int row = 0; row < P; ++row) {
+            double* z = Z + (size_t)row * P + c;
+            const double v = *z - gz * delta[row];
+            *z = v;
+            nz += v * v;
+        }
+        if (nz < norm_Z && nz > kNumPrecision) {
+            const double sc = sqrt(norm_Z / nz);
+            for (int row = 0; row < P; ++row) Z[(size_t)row * P + c] *= sc;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restrict__ states,
+                                                               double* __restrict__ vecs, double* __restrict__ Zs,
+                                                               const double* __restrict__ fe,
+                                                               const double* __restrict__ ge, double* __restrict__ xt,
+                                                               int* __restrict__ lists, int* __restrict__ counts,
+                                                               BfgsOpts o, int B) {
+    __shared__ double sh[8];
+    __shared__ double zs[kBfgsMaxP];
+    const int r = blockIdx.x;
+    if (r >= B) return;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int P = o.P;
+    RunState S = states[r];
+    if (S.phase == PH_FINISHED) return;
+    double* V = vecs + (size_t)r * RV_COUNT * P;
+    double* x = V + RV_X * P;
+    double* g = V + RV_G * P;
+    double* d = V + RV_D * P;
+    double* x0 = V + RV_X0 * P;
+    double* g0 = V + RV_G0 * P;
+    double* gbest = V + RV_GBEST * P;
+    double* ztg = V + RV_ZTG * P;
+    double* best = V + RV_BEST * P;
+    double* Z = Zs + (size_t)r * P * P;
+    double* xr = xt + (size_t)r * P;
+    const double* gr = ge + (size_t)r * P;
+    bool request = false;
+    for (int guard = 0; guard < 16 && !request && S.phase != PH_FINISHED; ++guard) {
+        switch (S.phase) {
+        case PH_OUTER_START: {
+            for (int k = tid; k < P; k += nt) {
+                x[k] = best[k];
+                xr[k] = best[k];
+            }
+            for (size_t i = tid; i < (size_t)P * P; i += nt) Z[i] = (i / P == i % P) ? 1.0 : 0.0;
+            S.calls = 0;
+            S.norm_Z = -1.0;
+            S.status = BS_VARS_INIT;
+            S.phase = PH_INIT_WAIT;
+            request = true;
+            break;
+        }
+        case PH_INIT_WAIT: {
+            S.f = fe[r];
+            for (int k = tid; k < P; k += nt) g[k] = gr[k];
+            S.calls++;
+            S.fprev = fabs(S.f + S.f + 1.0);
+            S.phase = PH_DIRECTION;
+            __syncthreads();
+            break;
+        }
+        case PH_DIRECTION: {
+            if (S.prec == 1 && S.f < o.switch_cost) {
+                S.status = BS_MINIMUM;
+                S.phase = PH_BFGS_END;
+                break;
+            }
+            // d = -Z Z^T g (BFGS_Powell.cpp:396-439)
+            for (int c = tid; c < P; c += nt) {
+                double s = 0.0;
+                for (int row = 0; row < P; ++row) s += Z[(size_t)row * P + c] * g[row];
+                ztg[c] = s;
+            }
+            __syncthreads();
+            for (int row = tid; row < P; row += nt) {
+                double s = 0.0;
+                for (int c = 0; c < P; ++c) s += Z[(size_t)row * P + c] * ztg[c];
+                d[row] = -s;
+            }
+            __syncthreads();
+            double dg = 0.0, gn = 0.0, ms = INFINITY;
+            for (int k = tid; k < P; k += nt) {
+                dg += d[k] * g[k];
+                gn += g[k] * g[k];
+                if (fabs(d[k]) >= 1e-5) ms = fmin(ms, fabs(2.0 * M_PI / d[k]));
+            }
+            dg = wg_sum(dg, sh);
+            gn = wg_sum(gn, sh);
+            ms = wg_min(ms, sh);
+            double maxstep = 0.0;
+            if (dg < 0.0) maxstep = (ms == INFINITY) ? 0.0 : ms;
+            if (gn <= 1e-19 * 1e-19) { S.status = BS_MINIMUM; S.phase = PH_BFGS_END; break; }
+            if (dg >= 0.0) { S.status = BS_NO_DESCENT; S.phase = PH_BFGS_END; break; }
+            if (S.f >= S.fprev) { S.status = BS_MINIMUM; S.phase = PH_BFGS_END; break; }
+            S.fprev = S.f;
+            if (S.calls >= o.max_calls) { S.status = BS_MAX_ITER; S.phase = PH_BFGS_END; break; }
+            // Line search set-up (grad_descend.cpp:127-160).
+            S.maxstep = maxstep;
+            S.step = fmin(1.0, maxstep);
+            S.f_lowest = S.f_highest = S.f;
+            S.step_highest = maxstep;
+            S.step_lowest = 0.0;
+            S.step_best = 0.0;
+            S.f_best = S.f;
+            S.dg_abs_best = fabs(dg);
+            S.dg_highest = 0.0;
+            S.dg0 = dg;
+            S.ls_iter = 0;
+            for (int k = tid; k < P; k += nt) {
+                x0[k] = x[k];
+                g0[k] = g[k];
+                gbest[k] = g[k];
+                xr[k] = x[k] + S.step * d[k];
+            }
+            S.phase = PH_LS_WAIT;
+            request = true;
+            break;
+        }
+        case PH_LS_WAIT: {
+            // One probe of Grad_Descend::line_search (grad_descend.cpp:163-250).
+            S.f = fe[r];
+            double dgc = 0.0;
+            for (int k = tid; k < P; k += nt) {
+                x[k] = xr[k];
+                g[k] = gr[k];
+                dgc += d[k] * gr[k];
+            }
+            S.calls++;
+            dgc = wg_sum(dgc, sh);
+            if (S.f < S.f_best || fabs(dgc) < S.dg_abs_best) {
+                S.step_best = S.step;
+                S.f_best = S.f;
+                S.dg_abs_best = fabs(dgc);
+                for (int k = tid; k < P; k += nt) gbest[k] = g[k];
+            }
+            bool brk = false;
+            if (S.calls == o.max_calls) {
+                brk = true;
+            } else {
+                if (S.f < S.f_lowest + S.step * 0.1 * S.dg0) {
+                    if (dgc >= S.dg0 * 0.7) {
+                        brk = true;
+                    } else {
+                        S.step_lowest = S.step;
+                        S.f_lowest = S.f;
+                    }
+                } else {
+                    S.step_highest = S.step;
+                    S.f_highest = S.f;
+                    S.dg_highest = dgc;
+                }
+                if (!brk) {
+                    if (S.ls_iter == 0 || S.step_lowest > 0.0) {
+                        const double fit = (S.f_highest - S.f_lowest) / (S.step_highest - S.step_lowest) * 2.0 -
+                                           S.dg_highest;
+                        double scale = -fit * 0.5 / (S.dg_highest - fit);
+                        scale = fmax(0.1, scale);
+                        S.step = S.step_lowest + scale * (S.step_highest - S.step_lowest);
+                    } else {
+                        S.step = S.step * 0.1;
+                        if (S.step < kNumPrecision) brk = true;
+                    }
+                }
+                if (!brk) {
+                    S.ls_iter++;
+                    if (S.ls_iter >= 50) brk = true;
+                }
+            }
+            __syncthreads();
+            if (!brk) {
+                for (int k = tid; k < P; k += nt) xr[k] = x0[k] + S.step * d[k];
+                request = true;
+                break;
+            }
+            // End of the line search (grad_descend.cpp:256-270).
+            if (S.step != S.step_best) {
+                S.step = S.step_best;
+                S.f = S.f_best;
+                for (int k = tid; k < P; k += nt) {
+                    g[k] = gbest[k];
+                    x[k] = x0[k] + S.step * d[k];
+                }
+            }
+            __syncthreads();
+            if (S.step == 0.0) {
+                S.status = BS_ZERO_STEP;
+                S.phase = PH_BFGS_END;
+                break;
+            }
+            bfgs_update(Z, V, P, S.norm_Z, sh, zs);
+            S.phase = PH_DIRECTION;
+            break;
+        }
+        case PH_BFGS_END: {
+            S.total_calls += S.calls;
+            if (S.prec == 1) {
+                // The complex64 stage hands its point to the FP64 stage.
+                for (int k = tid; k < P; k += nt) best[k] = x[k];
+                S.calls32 += S.calls;
+                S.prec = 0;
+                S.phase = PH_OUTER_START;
+                __syncthreads();
+                break;
+            }
+            if (S.cur_min > S.f) {
+                S.cur_min = S.f;
+                for (int k = tid; k < P; k += nt) best[k] = x[k];
+            }
+            double mean = 0.0;
+            for (int i = kMinWindow - 1; i > 0; --i) {
+                S.minvec[i] = S.minvec[i - 1];
+                mean += S.minvec[i - 1];
+            }
+            S.minvec[0] = S.cur_min;
+            mean = (mean + S.cur_min) / kMinWindow;
+            double sd = 0.0;
+            for (int i = 0; i < kMinWindow; ++i) {
+                const double t = S.minvec[i] - mean;
+                sd += t * t;
+            }
+            sd = sqrt(sd / (kMinWindow - 1));
+            S.outer++;
+            __syncthreads();
+            if (fabs(sd / S.minvec[kMinWindow - 1]) < o.conv_thr || fabs(S.cur_min) < o.tol || S.outer >= o.max_outer) {
+                S.phase = PH_FINISHED;
+                S.finished = 1;
+                if (tid == 0) atomicAdd(&counts[2], 1);
+            } else {
+                S.phase = PH_OUTER_START;
+            }
+            break;
+        }
+        default:
+            break;
+        }
+    }
+    if (request && tid == 0) {
+        const int j = atomicAdd(&counts[S.prec], 1);
+        lists[(size_t)S.prec * B + j] = r;
+    }
+    if (tid == 0) states[r] = S;
+}
+
+hipError_t launch_bfgs_init(void* st, double* vecs, const double* x0, int B, int P, int mixed, int* counts,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_bfgs_init, dim3(B), dim3(256), 0, s, (RunState*)st, vecs, x0, B, P, mixed, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_bfgs_reset_counts(int* counts, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfgs_reset_counts, dim3(1), dim3(64), 0, s, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_bfgs_advance(void* st, double* vecs, double* Z, const double* fe, const double* ge, double* xt,
+                               int* lists, int* counts, const BfgsOpts& o, int B, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfgs_advance, dim3(B), dim3(kBfgsThreads), 0, s, (RunState*)st, vecs, Z, fe, ge, xt, lists,
+                       counts, o, B);
+    return hipGetLastError();
+}
+
+size_t bfgs_state_bytes() { return sizeof(RunState); }
+int bfgs_max_params() { return kBfgsMaxP; }
+
+}  // namespace sqh

@@ -29,22 +29,23 @@ int tile_columns_per_wave(int N) {
 }
 
 hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, const Slot* slots, int ns, void* kern,
-                                hipStream_t st) {
+                                hipStream_t st, BatchIndex bi) {
     const long long n = (long long)B * ns;
     if (n == 0) return hipSuccess;
     if (f32)
         hipLaunchKernelGGL(k_build_kernels<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st, params, P, B, slots,
-                           ns, (cpx<float>*)kern);
+                           ns, (cpx<float>*)kern, bi);
     else
         hipLaunchKernelGGL(k_build_kernels<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st, params, P, B, slots,
-                           ns, (cpx<double>*)kern);
+                           ns, (cpx<double>*)kern, bi);
     return hipGetLastError();
 }
 
 hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,
-                         int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st) {
+                         int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st,
+                         BatchIndex bi) {
     hipLaunchKernelGGL(k_finish, dim3(B), dim3(256), 0, st, diag, m, variant, costs, mpart, wpb, ns, slots, params, P,
-                       grad);
+                       grad, bi);
     return hipGetLastError();
 }
 

@@ -32,6 +32,7 @@ struct TileLaunch {
     double* mpart = nullptr;     // [B][wpb][ns][8] gradient correlations
     const Slot* slots = nullptr;
     const double* params = nullptr;
+    BatchIndex bi;             // optional element -> run -> target indirection
 };
 
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st);
@@ -40,9 +41,10 @@ hipError_t launch_tile_deriv(const TileLaunch& t, hipStream_t st);
 int tile_columns_per_wave(int N);
 
 hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, const Slot* slots, int ns, void* kern,
-                                hipStream_t st);
+                                hipStream_t st, BatchIndex bi = BatchIndex());
 hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,
-                         int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st);
+                         int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st,
+                         BatchIndex bi = BatchIndex());
 hipError_t launch_stream_gate(bool f32, void* mats, long long bstride, int stride, int rows, int m, int B, int kind,
                               int target, int control, const void* kern, long long kern_bstride, int slot,
                               hipStream_t st);

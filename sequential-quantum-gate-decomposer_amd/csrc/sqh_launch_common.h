@@ -91,7 +91,7 @@ template <int N, typename R, bool UNI>
 hipError_t launch_grad_nu(const TileLaunch& t, hipStream_t st, unsigned grid) {
     hipLaunchKernelGGL((k_tile_grad<N, R, UNI>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in, t.in_stride, t.m,
                        t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride, t.trace_off,
-                       t.ns, t.diag, t.mpart);
+                       t.ns, t.diag, t.mpart, t.bi);
     return hipGetLastError();
 }
 

@@ -36,8 +36,16 @@ EXPORTS = [
     "sqh_apply_derivate_to", "sqh_apply_derivate_to_dev", "sqh_optimization_problem_batch",
     "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
-    "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing",
+    "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
 ]
+
+
+class DecomposeOptions(ctypes.Structure):
+    """sqh_decompose_options (include/sqhip.h)."""
+    _fields_ = [("max_inner_iterations", ctypes.c_longlong), ("max_outer_iterations", ctypes.c_int),
+                ("optimization_tolerance", ctypes.c_double), ("convergence_threshold", ctypes.c_double),
+                ("mixed_precision", ctypes.c_int), ("switch_cost", ctypes.c_double),
+                ("rounds_per_graph", ctypes.c_int), ("max_rounds", ctypes.c_longlong)]
 
 
 class SqhError(Exception):
@@ -92,6 +100,8 @@ def lib():
             "sqh_optimization_problem_combined_batch_dev": ([vp, vp, i, i, vp, vp], i),
             "sqh_optimization_problem_combined_unitary": ([vp, vp, vp, vp], i),
             "sqh_set_kernel_timing": ([vp, i], i),
+            "sqh_decompose_batch": ([vp, vp, i, i, i, vp, i, vp, ctypes.POINTER(DecomposeOptions), vp, vp, vp, vp,
+                                     ctypes.POINTER(ll)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_program": ([i, ctypes.POINTER(_Gate), i, vp, i, vp, i, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)], i),

@@ -159,6 +159,41 @@ class N_Qubit_Decomposition_custom:
     def get_Matrix(self, parameters):
         return self.circuit.get_Matrix(np.ascontiguousarray(parameters, dtype=np.float64))
 
+    # -- the optimiser ----------------------------------------------------------------
+    def _options(self, **kw):
+        from .optimizer import default_options
+        cfg = dict(self.config)
+        cfg.update(kw)
+        if self.max_iterations is not None:
+            cfg.setdefault("max_inner_iterations", self.max_iterations)
+        cfg.setdefault("optimization_tolerance", self.optimization_tolerance)
+        keys = ("max_inner_iterations", "max_outer_iterations", "optimization_tolerance", "convergence_threshold",
+                "mixed_precision", "switch_cost", "rounds_per_graph", "max_rounds")
+        if "max_inner_iterations_bfgs" in cfg:
+            cfg["max_inner_iterations"] = cfg["max_inner_iterations_bfgs"]
+        return default_options(self.qbit_num, **{k: cfg[k] for k in keys if k in cfg})
+
+    def Start_Decomposition(self, **kw):
+        """Decompose Umtx with the gate structure (N_Qubit_Decomposition_custom::
+        start_decomposition, N_Qubit_Decomposition_custom.cpp:116-190): initial guess from
+        config ``random_seed`` (Decomposition_Base.cpp:415-446), BFGS restarted from the best
+        point, all on the GPU.  Extra keyword options: mixed_precision, switch_cost."""
+        from .optimizer import decompose_batch, initial_guess
+        if self.circuit is None:
+            raise Exception("no gate structure set")
+        P = self.get_Parameter_Num()
+        if self.optimized_parameters is not None and self.optimized_parameters.size == P:
+            x0 = self.optimized_parameters.copy()
+        else:
+            seed = int(self.config.get("random_seed", np.random.SeedSequence().entropy & 0xFFFFFFFF))
+            x0 = initial_guess(P, seed, self.initial_guess)
+        res = decompose_batch(self._ctx, self.Umtx, np.zeros(1, dtype=np.int32), x0[None, :], self._options(**kw))
+        self.optimized_parameters = res["x"][0].copy()
+        self.num_iters += int(res["evals"][0]) * (P + 1)
+        self.decomposition_error = float(res["f"][0])
+        self.last_run = res
+        return 0
+
     # -- results -------------------------------------------------------------------
     def get_Optimized_Parameters(self):
         return None if self.optimized_parameters is None else self.optimized_parameters.copy()

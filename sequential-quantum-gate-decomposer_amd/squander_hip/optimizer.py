@@ -1,0 +1,75 @@
+"""Batched decomposition on the GPU (sqh_decompose_batch) and the reference
+driver's initial guesses.
+
+Initial guesses follow Decomposition_Base::solve_optimization_problem
+(Decomposition_Base.cpp:415-446): a std::mt19937 seeded with config
+``random_seed`` feeds std::uniform_real_distribution(0, 2*pi); RANDOM uses the
+draws, CLOSE_TO_ZERO divides them by 100, ZEROS uses zeros.  libstdc++ builds each
+double from two 32-bit draws (low word first), which is reproduced here on top of
+numpy's MT19937 with the standard's integer seeding.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _native as nat
+
+GUESS = {"ZEROS": 0, "RANDOM": 1, "CLOSE_TO_ZERO": 2}
+
+
+def mt19937_uniform_2pi(seed: int, n: int) -> np.ndarray:
+    """n draws of uniform_real_distribution<double>(0, 2pi) from std::mt19937(seed)."""
+    bg = np.random.RandomState(int(seed) & 0xFFFFFFFF)._bit_generator
+    raw = np.asarray(bg.random_raw(2 * n), dtype=np.uint64).reshape(n, 2).astype(np.float64)
+    u = (raw[:, 0] + raw[:, 1] * 4294967296.0) / (4294967296.0 * 4294967296.0)
+    u = np.where(u >= 1.0, np.nextafter(1.0, 0.0), u)
+    return u * (2.0 * np.pi)
+
+
+def initial_guess(P: int, seed: int, kind: str = "RANDOM") -> np.ndarray:
+    kind = kind.upper()
+    if kind == "ZEROS":
+        return np.zeros(P)
+    draws = mt19937_uniform_2pi(seed, P)
+    return draws / 100.0 if kind == "CLOSE_TO_ZERO" else draws
+
+
+def default_options(qbit_num: int, **kw) -> nat.DecomposeOptions:
+    """Reference defaults of a custom decomposition (BFGS, 4 outer iterations at
+    <= 5 qubits, 1 above; 10000 evaluations per BFGS run; tolerance 1e-7;
+    convergence threshold 1e-5)."""
+    o = nat.DecomposeOptions()
+    o.max_inner_iterations = int(kw.get("max_inner_iterations", 10000))
+    o.max_outer_iterations = int(kw.get("max_outer_iterations", 4 if qbit_num <= 5 else 1))
+    o.optimization_tolerance = float(kw.get("optimization_tolerance", 1e-7))
+    o.convergence_threshold = float(kw.get("convergence_threshold", 1e-5))
+    o.mixed_precision = int(bool(kw.get("mixed_precision", False)))
+    o.switch_cost = float(kw.get("switch_cost", 1e-4))
+    o.rounds_per_graph = int(kw.get("rounds_per_graph", 16))
+    o.max_rounds = int(kw.get("max_rounds", 0))
+    return o
+
+
+def decompose_batch(ctx: nat.Context, targets, run_target, x0, options: nat.DecomposeOptions):
+    """Run len(run_target) decompositions of the context's circuit in lockstep.
+
+    targets: (T, rows, cols) complex128 (each problem's Umtx); run_target: (R,)
+    target index per run; x0: (R, P) start points.  Returns a dict with x (R, P),
+    f (R,), status (R,), evals (R,) and rounds."""
+    T = np.ascontiguousarray(targets, dtype=np.complex128)
+    if T.ndim == 2:
+        T = T[None]
+    rt = np.ascontiguousarray(run_target, dtype=np.int32)
+    X0 = np.ascontiguousarray(x0, dtype=np.float64)
+    R, P = X0.shape
+    x = np.empty((R, P))
+    f = np.empty(R)
+    st = np.empty(R, dtype=np.int32)
+    ev = np.empty(R, dtype=np.int64)
+    rounds = ctypes.c_longlong(0)
+    ctx.check(ctx._L.sqh_decompose_batch(ctx.h, T.ctypes.data, T.shape[0], T.shape[1], T.shape[2], rt.ctypes.data, R,
+                                         X0.ctypes.data, ctypes.byref(options), x.ctypes.data, f.ctypes.data,
+                                         st.ctypes.data, ev.ctypes.data, ctypes.byref(rounds)))
+    return {"x": x, "f": f, "status": st, "evals": ev, "rounds": int(rounds.value)}

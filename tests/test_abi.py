@@ -96,3 +96,9 @@ def test_no_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(nat.SqhError):
         nat.Context(0)
+
+
+def test_initial_guess_matches_oracle_mt19937(oracle):
+    from squander_hip import mt19937_uniform_2pi
+    for seed in (1, 42, 2026):
+        assert np.array_equal(mt19937_uniform_2pi(seed, 50), oracle.mt_stream(seed, 50, kind="uniform"))
