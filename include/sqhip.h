@@ -159,6 +159,28 @@ int sqh_decompose_batch(sqh_ctx* ctx, const double* targets, int n_targets, int 
                         const int* run_target, int n_runs, const double* x0, const sqh_decompose_options* opts,
                         double* x_out, double* f_out, int* status_out, long long* evals_out, long long* rounds_out);
 
+/* ---- warm-start network ----------------------------------------------------- */
+/* A multilayer perceptron in -> hidden (GELU) x n_hidden -> out with BF16 weights
+ * and activations, FP32 accumulation on the matrix cores and FP32 master weights,
+ * trained with Adam on mean-squared error.  It maps a target unitary to start angles
+ * for the batched optimiser.  No counterpart in the reference (its nn module only
+ * generates data: squander/src-cpp/nn/include/NN.h:58-146).  Parameters travel as a
+ * flat float32 array: per layer W (out x in, row-major) then b (out). */
+typedef struct sqh_mlp sqh_mlp;
+int sqh_mlp_create(sqh_ctx* ctx, int in_dim, int hidden, int n_hidden, int out_dim, unsigned long long seed,
+                   sqh_mlp** out);
+void sqh_mlp_destroy(sqh_mlp* mlp);
+long long sqh_mlp_param_count(const sqh_mlp* mlp);
+int sqh_mlp_get_params(sqh_mlp* mlp, float* params);
+int sqh_mlp_set_params(sqh_mlp* mlp, const float* params);
+/* y (B x out, fp32) = network(x (B x in, fp32)); device pointers, asynchronous. */
+int sqh_mlp_forward_dev(sqh_mlp* mlp, const float* d_x, int B, float* d_y);
+/* One Adam step on the batch (x, targets t); *loss (if given) receives the batch loss
+ * before the step (synchronises). */
+int sqh_mlp_train_step_dev(sqh_mlp* mlp, const float* d_x, const float* d_t, int B, float lr, double* loss);
+/* Loss and gradients (flat, like the parameters) of a host batch, without updating. */
+int sqh_mlp_gradients(sqh_mlp* mlp, const float* x, const float* t, int B, float* grads, double* loss);
+
 /* ---- kernel timers --------------------------------------------------------- */
 /* When enabled, every call brackets its dominant kernel (the register-tile kernel,
  * or the derivative kernel) with HIP events on the context's stream.

@@ -37,6 +37,8 @@ EXPORTS = [
     "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
     "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
+    "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
+    "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients",
 ]
 
 
@@ -100,6 +102,14 @@ def lib():
             "sqh_optimization_problem_combined_batch_dev": ([vp, vp, i, i, vp, vp], i),
             "sqh_optimization_problem_combined_unitary": ([vp, vp, vp, vp], i),
             "sqh_set_kernel_timing": ([vp, i], i),
+            "sqh_mlp_create": ([vp, i, i, i, i, ctypes.c_ulonglong, ctypes.POINTER(vp)], i),
+            "sqh_mlp_destroy": ([vp], None),
+            "sqh_mlp_param_count": ([vp], ll),
+            "sqh_mlp_get_params": ([vp, vp], i),
+            "sqh_mlp_set_params": ([vp, vp], i),
+            "sqh_mlp_forward_dev": ([vp, vp, i, vp], i),
+            "sqh_mlp_train_step_dev": ([vp, vp, vp, i, ctypes.c_float, d], i),
+            "sqh_mlp_gradients": ([vp, vp, vp, i, vp, d], i),
             "sqh_decompose_batch": ([vp, vp, i, i, i, vp, i, vp, ctypes.POINTER(DecomposeOptions), vp, vp, vp, vp,
                                      ctypes.POINTER(ll)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),

