@@ -205,6 +205,27 @@ def main():
                              "launches": snl, "bytes_per_pass": bytes_pass * B,
                              "note": "algorithmic bytes over summed per-gate kernel time"}
     log(f"unfused stream: {gbs:.0f} GB/s")
+
+    # fused streaming path: runs of gates as 16x16 blocks on 4 qubits (FP64 matrix cores)
+    def fused_step():
+        ctx.check(L.sqh_apply_to_fused_dev(ctx.h, params.data_ptr(), P, mats.data_ptr(), d, d, d, d * d, B, 0))
+
+    fused_step()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fused_step()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    fkms, fnl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    blk_bytes = 2.0 * 16 * d * d * B     # read + write of every matrix per block pass
+    out["fused_stream"] = {"evals_per_s": B * reps / (t1 - t0), "blocks_per_pass": fnl // reps,
+                           "achieved_GBs": blk_bytes * fnl / (fkms / 1e3) / 1e9,
+                           "algorithmic_TFLOPs": flops_eval * B * reps / (fkms / 1e3) / 1e12,
+                           "kernel": "k_block_mfma4 (v_mfma_f64_16x16x4f64)"}
+    log(f"fused stream: {out['fused_stream']['evals_per_s']:.0f} evals/s")
     del mats
 
     # single-vector cost + gradient latency (8 qubits, 264 angles)

@@ -107,6 +107,17 @@ int sqh_apply_to_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d
 int sqh_apply_to_stream_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
                             int stride, long long bstride, int B, int is_f32);
 
+/* The same result with runs of gates fused into dense 16x16 blocks on <= 4 qubits
+ * (one HBM pass per block, FP64 blocks on the matrix cores); needs >= 4 qubits. */
+int sqh_apply_to_fused_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
+                           int stride, long long bstride, int B, int is_f32);
+/* apply_large_kernel_to_input (squander/src-cpp/gates/kernels/apply_large_kernel_to_input.cpp:82-102):
+ * a dense 2^k x 2^k block (row-major complex128, or complex64 with is_f32) on the k
+ * involved qubits (sorted ascending; local index bit i <-> qubits[i]) of B matrices,
+ * 1 <= k <= 5, device pointers; qubits is a host array. */
+int sqh_apply_large_kernel_dev(sqh_ctx* ctx, const void* d_block, const int* qubits, int k, void* d_inout, int rows,
+                               int cols, int stride, long long bstride, int B, int is_f32);
+
 /* ---- Gates_block::apply_derivate_to / apply_to_combined ---------------------- */
 /* out receives P dense rows x cols complex128 matrices: out[k] = (dC/dp_k) . in. */
 int sqh_apply_derivate_to(sqh_ctx* ctx, const double* params, const double* in, int rows, int cols, int stride,

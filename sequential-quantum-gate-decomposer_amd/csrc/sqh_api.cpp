@@ -100,6 +100,32 @@ int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lo
     return SQH_OK;
 }
 
+// The circuit as fused dense blocks (<= 4 qubits each): one HBM pass per block,
+// matrix cores for the 4-qubit FP64 blocks.  kern holds the kernel tables built for
+// this call (per matrix when kern_bstride != 0).
+int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
+                long long kern_bstride) {
+    const FusionPlan& fp = c->fplan;
+    const int nb = (int)fp.qubits.size();
+    if (nb == 0) return SQH_OK;
+    const int Bk = kern_bstride ? B : 1;
+    const size_t esz = f32 ? 8 : 16;
+    HIPTRY(c, c->d_fused.ensure((size_t)Bk * nb * 256 * esz));
+    HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, c->d_kern.p, kern_bstride, Bk,
+                                 c->d_fused.p, c->stream));
+    for (int blk = 0; blk < nb; ++blk) {
+        const char* F = static_cast<const char*>(c->d_fused.p) + (size_t)blk * 256 * esz;
+        const int D = 1 << fp.k;
+        // blocks are stored 16 x 16; a smaller k uses the leading D x D corner only when k == 4
+        if (fp.k != 4) return fail(c, SQH_ERR_ARG, "fused streaming needs at least 4 qubits");
+        (void)D;
+        TimedLaunch tl(c);
+        HIPTRY(c, launch_block_apply(f32, true, mats, bstride, stride, rows, cols, B, F,
+                                     Bk > 1 ? (long long)nb * 256 : 0, fp.qubits[blk].data(), fp.k, c->stream));
+    }
+    return SQH_OK;
+}
+
 // costs (and, with grad, the gradient) for B parameter vectors on the device.
 int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad) {
     int rc = require_target(c);
@@ -150,7 +176,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
     const long long msz = (long long)c->rows * m;
     HIPTRY(c, c->d_mat.ensure((size_t)B * msz * esz));
     HIPTRY(c, launch_broadcast_copy(f32, U, c->rows, m, m, c->d_mat.p, msz, B, c->stream));
-    if ((rc = stream_apply(c, c->d_mat.p, c->rows, m, m, msz, B, f32, (long long)ns * 4))) return rc;
+    if ((rc = fused_apply(c, c->d_mat.p, c->rows, m, m, msz, B, f32, (long long)ns * 4))) return rc;
     HIPTRY(c, launch_extract_diag(f32, c->d_mat.p, msz, m, m, B, off, c->d_diag.as<double>(), c->stream));
     HIPTRY(c, launch_finish(c->d_diag.as<double>(), m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(),
                             d_params, c->prog.P, nullptr, c->stream));
@@ -158,7 +184,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
 }
 
 int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,
-              long long bstride, int B, bool f32, bool force_stream) {
+              long long bstride, int B, bool f32, int path) {
     int rc = require_circuit(c);
     if (rc) return rc;
     const int N = c->prog.N;
@@ -169,7 +195,8 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
     const int Bk = pbstride ? B : 1;
     if ((rc = build_kernels(c, d_params, Bk, f32))) return rc;
     const long long kb = pbstride ? (long long)ns * 4 : 0;
-    if (N <= kMaxTileQubits && !force_stream) {
+    if (path == 0 && N > kMaxTileQubits) path = N >= 4 ? 2 : 1;
+    if (path == 0) {
         TileLaunch t = base_launch(c, f32);
         t.mode = 0;
         t.in = d_inout;
@@ -186,6 +213,7 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
         HIPTRY(c, launch_tile_forward(t, c->stream));
         return SQH_OK;
     }
+    if (path == 2) return fused_apply(c, d_inout, rows, cols, stride, bstride, B, f32, kb);
     return stream_apply(c, d_inout, rows, cols, stride, bstride, B, f32, kb);
 }
 
@@ -284,6 +312,14 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
     HIPTRY(c, hipMemcpy(c->d_lays.p, p.layouts.data(), p.layouts.size() * sizeof(Layout), hipMemcpyHostToDevice));
     if (!p.slots.empty())
         HIPTRY(c, hipMemcpy(c->d_slots.p, p.slots.data(), p.slots.size() * sizeof(Slot), hipMemcpyHostToDevice));
+    FusionPlan fp = plan_fusion(qbit_num, g);
+    static_assert(sizeof(FusedRec) == 8, "fused record layout");
+    HIPTRY(c, c->d_fgates.ensure(std::max<size_t>(fp.gates.size(), 1) * sizeof(FusedRec)));
+    HIPTRY(c, c->d_ffirst.ensure(fp.first.size() * sizeof(int)));
+    if (!fp.gates.empty())
+        HIPTRY(c, hipMemcpy(c->d_fgates.p, fp.gates.data(), fp.gates.size() * sizeof(FusedRec), hipMemcpyHostToDevice));
+    HIPTRY(c, hipMemcpy(c->d_ffirst.p, fp.first.data(), fp.first.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->fplan = std::move(fp);
     c->prog = std::move(p);
     c->gates = std::move(g);
     c->have_circuit = true;
@@ -350,7 +386,7 @@ int sqh_apply_to(sqh_ctx* c, const void* params, void* inout, int rows, int cols
                              c->stream));
     HIPTRY(c, hipMemcpy2DAsync(c->d_mat.p, (size_t)cols * esz, inout, (size_t)stride * esz, (size_t)cols * esz, rows,
                                hipMemcpyHostToDevice, c->stream));
-    if ((rc = apply_dev(c, c->d_params.as<double>(), 0, c->d_mat.p, rows, cols, cols, 0, 1, is_f32 != 0, false)))
+    if ((rc = apply_dev(c, c->d_params.as<double>(), 0, c->d_mat.p, rows, cols, cols, 0, 1, is_f32 != 0, 0)))
         return rc;
     HIPTRY(c, hipMemcpy2DAsync(inout, (size_t)stride * esz, c->d_mat.p, (size_t)cols * esz, (size_t)cols * esz, rows,
                                hipMemcpyDeviceToHost, c->stream));
@@ -363,7 +399,7 @@ int sqh_apply_to_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_i
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     HIPTRY(c, hipSetDevice(c->device));
-    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, false);
+    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, 0);
 }
 
 int sqh_apply_to_stream_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
@@ -371,7 +407,33 @@ int sqh_apply_to_stream_dev(sqh_ctx* c, const double* d_params, int pbstride, vo
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     HIPTRY(c, hipSetDevice(c->device));
-    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, true);
+    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, 1);
+}
+
+int sqh_apply_to_fused_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
+                           int stride, long long bstride, int B, int is_f32) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    if (c->have_circuit && c->prog.N < 4) return fail(c, SQH_ERR_ARG, "fused streaming needs at least 4 qubits");
+    return apply_dev(c, d_params, pbstride, d_inout, rows, cols, stride, bstride, B, is_f32 != 0, 2);
+}
+
+int sqh_apply_large_kernel_dev(sqh_ctx* c, const void* d_block, const int* qubits, int k, void* d_inout, int rows,
+                               int cols, int stride, long long bstride, int B, int is_f32) {
+    if (!c || !d_block || !qubits || !d_inout) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (k < 1 || k > 5 || rows < (1 << k) || (rows & (rows - 1)) || cols <= 0 || stride < cols || B <= 0)
+        return fail(c, SQH_ERR_ARG, "bad block or matrix shape");
+    int n = 0;
+    while ((1 << n) < rows) ++n;
+    for (int j = 0; j < k; ++j)
+        if (qubits[j] < 0 || qubits[j] >= n || (j > 0 && qubits[j] <= qubits[j - 1]))
+            return fail(c, SQH_ERR_ARG, "involved qubits must be distinct, sorted and below %d", n);
+    HIPTRY(c, hipSetDevice(c->device));
+    HIPTRY(c, launch_block_apply(is_f32 != 0, true, d_inout, bstride, stride, rows, cols, B, d_block, 0, qubits, k,
+                                 c->stream));
+    return SQH_OK;
 }
 
 int sqh_apply_derivate_to(sqh_ctx* c, const double* params, const double* in, int rows, int cols, int stride,
@@ -475,7 +537,7 @@ int sqh_optimization_problem_combined_unitary(sqh_ctx* c, const double* params, 
         if ((rc = deriv_dev(c, c->d_params.as<double>(), c->d_U64.p, rows, cols, cols, c->d_out.p))) return rc;
     }
     HIPTRY(c, hipMemcpyAsync(c->d_mat.p, c->d_U64.p, msz * 16, hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = apply_dev(c, c->d_params.as<double>(), 0, c->d_mat.p, rows, cols, cols, 0, 1, false, false))) return rc;
+    if ((rc = apply_dev(c, c->d_params.as<double>(), 0, c->d_mat.p, rows, cols, cols, 0, 1, false, 0))) return rc;
     HIPTRY(c, hipMemcpyAsync(out_unitary, c->d_mat.p, msz * 16, hipMemcpyDeviceToHost, c->stream));
     if (P > 0 && out_derivs)
         HIPTRY(c, hipMemcpyAsync(out_derivs, c->d_out.p, (size_t)P * msz * 16, hipMemcpyDeviceToHost, c->stream));

@@ -63,6 +63,10 @@ struct sqh_ctx {
 
     sqh::DevBuf d_params, d_kern, d_diag, d_mpart, d_res, d_mat, d_out;
 
+    // Dense-block fusion of the circuit for the streaming path.
+    sqh::FusionPlan fplan;
+    sqh::DevBuf d_fgates, d_ffirst, d_fused;
+
     // Kernel timers: HIP events around the dominant kernel of each call.
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -73,7 +77,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused})
             b->release();
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -101,9 +105,19 @@ TileLaunch base_launch(sqh_ctx* c, bool f32);
 int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
                  long long kern_bstride);
 int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad);
+// path: 0 register tile (streaming above the tile limit), 1 one pass per gate,
+// 2 fused dense blocks.
 int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,
-              long long bstride, int B, bool f32, bool force_stream);
+              long long bstride, int B, bool f32, int path);
 int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, int cols, int stride, void* d_out);
+int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
+                long long kern_bstride);
+
+hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,
+                              int B, const void* F, long long fbstride, const int* qubits, int k, hipStream_t s);
+hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
+                              long long kern_bstride, int Bk, void* out, hipStream_t s);
+size_t fused_gate_bytes();
 
 // Brackets one kernel launch with timer events when timing is enabled.
 struct TimedLaunch {

@@ -1,0 +1,238 @@
+// sqh_k_block.hip -- dense 2^k x 2^k blocks on k qubits, applied in one HBM pass
+// (apply_large_kernel_to_input, apply_large_kernel_to_input.cpp:82-102, 124+: local
+// index bit i <-> the i-th smallest involved qubit), and the fusion of gate runs into
+// such blocks.
+//
+// k = 4 in FP64 runs on the matrix cores: a wavefront computes a 16 x 16 output tile
+// Y = F X for 16 columns with 16 v_mfma_f64_16x16x4 (four real products, K = 16 in
+// steps of 4); F stays in registers for the whole wavefront.  Other k (and complex64)
+// use one thread per (group, column) with F in LDS.
+#include <hip/hip_runtime.h>
+
+#include "sqh_device.h"
+#include "sqh_kernels.h"
+
+namespace sqh {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+struct BlockQubits {
+    int q[5];   // sorted ascending
+    int k;
+};
+
+// Row of local index i in group g: insert the k involved bits (ascending) into g.
+__device__ __forceinline__ long long block_base(long long g, const BlockQubits& bq) {
+    for (int j = 0; j < bq.k; ++j) {
+        const int q = bq.q[j];
+        g = ((g >> q) << (q + 1)) | (g & ((1LL << q) - 1));
+    }
+    return g;
+}
+
+__device__ __forceinline__ int block_off(int i, const BlockQubits& bq) {
+    int o = 0;
+    for (int j = 0; j < bq.k; ++j)
+        if ((i >> j) & 1) o |= 1 << bq.q[j];
+    return o;
+}
+
+// Generic: thread -> (matrix b, group g, column c).  F: [B_F][D][D] complex (D = 2^k),
+// per-matrix when fbstride != 0.
+template <typename R, int K>
+__global__ __launch_bounds__(256) void k_block_fma(cpx<R>* __restrict__ mats, long long bstride, int stride, int rows,
+                                                   int m, int B, const cpx<R>* __restrict__ F, long long fbstride,
+                                                   BlockQubits bq) {
+    constexpr int D = 1 << K;
+    __shared__ cpx<R> Fs[D * D];
+    const long long groups = rows >> K;
+    const long long per_b = groups * m;
+    const long long tid0 = (long long)blockIdx.x * blockDim.x;
+    const int b0 = (int)(tid0 / per_b);
+    // All threads of a block share one batch element when per_b is a multiple of 256
+    // (host guarantees: else fbstride == 0).
+    const cpx<R>* Fb = F + (fbstride ? (long long)(b0 < B ? b0 : B - 1) * fbstride : 0);
+    for (int i = threadIdx.x; i < D * D; i += blockDim.x) Fs[i] = Fb[i];
+    __syncthreads();
+    const long long tid = tid0 + threadIdx.x;
+    if (tid >= per_b * B) return;
+    const int b = (int)(tid / per_b);
+    const long long rem = tid % per_b;
+    const long long g = rem / m;
+    const int col = (int)(rem % m);
+    cpx<R>* M = mats + (long long)b * bstride;
+    const long long base = block_base(g, bq);
+    cpx<R> x[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) x[i] = M[(base + block_off(i, bq)) * stride + col];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+        cpx<R> y = mk<R>(0, 0);
+#pragma unroll
+        for (int j = 0; j < D; ++j) cacc(y, Fs[i * D + j], x[j]);
+        M[(base + block_off(i, bq)) * stride + col] = y;
+    }
+}
+
+// k = 4, FP64, matrix cores.  Wavefront -> (matrix b, group g, 16-column tile).
+__global__ __launch_bounds__(256) void k_block_mfma4(cpx<double>* __restrict__ mats, long long bstride, int stride,
+                                                     int rows, int m, int B, const cpx<double>* __restrict__ F,
+                                                     long long fbstride, BlockQubits bq) {
+    const int lane = threadIdx.x & 63;
+    const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long groups = rows >> 4;
+    const long long tiles = (m + 15) / 16;
+    const long long per_b = groups * tiles;
+    if (wave >= per_b * B) return;
+    const int b = (int)(wave / per_b);
+    const long long rem = wave % per_b;
+    const long long g = rem / tiles;
+    const int c0 = (int)(rem % tiles) * 16;
+    const cpx<double>* Fb = F + (long long)b * fbstride;
+    // A operand of step s: A[i][k] = F[lane%16][4s + lane/16]
+    double ar[4], ai[4], ain[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const cpx<double> f = Fb[(lane & 15) * 16 + 4 * s + (lane >> 4)];
+        ar[s] = f.re;
+        ai[s] = f.im;
+        ain[s] = -f.im;
+    }
+    cpx<double>* M = mats + (long long)b * bstride;
+    const long long base = block_base(g, bq);
+    const int col = c0 + (lane & 15);
+    const bool cok = col < m;
+    // B operand of step s: B[k][n] = X[4s + lane/16][c0 + lane%16]
+    double xr[4], xi[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        const int i = 4 * s + (lane >> 4);
+        const cpx<double> v = cok ? M[(base + block_off(i, bq)) * stride + col] : mk(0.0, 0.0);
+        xr[s] = v.re;
+        xi[s] = v.im;
+    }
+    f64x4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s], xr[s], yr, 0, 0, 0);
+        yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ain[s], xi[s], yr, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s], xi[s], yi, 0, 0, 0);
+        yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[s], xr[s], yi, 0, 0, 0);
+    }
+    // D[i][n]: i = 4*r + lane/16, n = lane % 16 (v_mfma_f64_16x16x4f64 output layout)
+    if (cok) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 4 * r + (lane >> 4);
+            M[(base + block_off(i, bq)) * stride + col] = mk(yr[r], yi[r]);
+        }
+    }
+}
+
+// Fused block matrices: thread -> (batch element b, block, column j of the block);
+// the block's gates (local qubits 0..3, the block's slots) are applied to e_j.
+struct FusedGate {
+    int8_t kind, t, c, pad;  // OP_U / OP_CU / OP_X / OP_Z; local target; local control (-1 none)
+    int32_t slot;            // parametric slot (kernel table index) or -1
+};
+
+template <typename R>
+__global__ void k_fuse_blocks(const FusedGate* __restrict__ fg, const int* __restrict__ block_first, int nblocks,
+                              const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
+                              cpx<R>* __restrict__ out) {
+    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= (long long)Bk * nblocks * 16) return;
+    const int j = (int)(tid % 16);
+    const int blk = (int)((tid / 16) % nblocks);
+    const int b = (int)(tid / (16 * nblocks));
+    cpx<R> v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) v[i] = mk<R>(i == j ? (R)1 : (R)0, (R)0);
+    const cpx<R>* kb = kern + (long long)b * kern_bstride;
+    for (int p = block_first[blk]; p < block_first[blk + 1]; ++p) {
+        const FusedGate gt = fg[p];
+        Op op;
+        op.kind = (uint8_t)gt.kind;
+        op.t = (uint8_t)gt.t;
+        op.cmode = gt.c >= 0 ? CM_REG : CM_NONE;
+        op.c = (uint8_t)(gt.c >= 0 ? gt.c : 0);
+        op.idx = gt.slot;
+        cpx<R> k[4];
+        if (gt.slot >= 0) {
+            k[0] = kb[4 * gt.slot];
+            k[1] = kb[4 * gt.slot + 1];
+            k[2] = kb[4 * gt.slot + 2];
+            k[3] = kb[4 * gt.slot + 3];
+        }
+        exec_gate<4, false, R>(v, op, k, 0);
+    }
+    // out[b][blk][i][j]
+    cpx<R>* o = out + ((long long)b * nblocks + blk) * 256;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) o[i * 16 + j] = v[i];
+}
+
+// Launchers -----------------------------------------------------------------------
+
+template <typename R, int K>
+static hipError_t launch_fma_k(void* mats, long long bstride, int stride, int rows, int m, int B, const void* F,
+                               long long fbstride, const BlockQubits& bq, hipStream_t s) {
+    const long long n = (long long)(rows >> K) * m * B;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL((k_block_fma<R, K>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (cpx<R>*)mats,
+                       bstride, stride, rows, m, B, (const cpx<R>*)F, fbstride, bq);
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_fma(void* mats, long long bstride, int stride, int rows, int m, int B, const void* F,
+                             long long fbstride, const BlockQubits& bq, hipStream_t s) {
+    switch (bq.k) {
+    case 1: return launch_fma_k<R, 1>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s);
+    case 2: return launch_fma_k<R, 2>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s);
+    case 3: return launch_fma_k<R, 3>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s);
+    case 4: return launch_fma_k<R, 4>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s);
+    case 5: return launch_fma_k<R, 5>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// Dense block on the k sorted qubits of every matrix.  F holds one block, or one per
+// matrix (fbstride = 4^k).  use_mfma selects the matrix-core kernel for k = 4 FP64.
+hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,
+                              int B, const void* F, long long fbstride, const int* qubits, int k, hipStream_t s) {
+    if (k < 1 || k > 5 || (1 << k) > rows) return hipErrorInvalidValue;
+    BlockQubits bq;
+    bq.k = k;
+    for (int j = 0; j < 5; ++j) bq.q[j] = j < k ? qubits[j] : 0;
+    if (!f32 && use_mfma && k == 4) {
+        const long long waves = (long long)(rows >> 4) * ((m + 15) / 16) * B;
+        hipLaunchKernelGGL(k_block_mfma4, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (cpx<double>*)mats,
+                           bstride, stride, rows, m, B, (const cpx<double>*)F, fbstride, bq);
+        return hipGetLastError();
+    }
+    // The FMA kernel stages one block per workgroup: per-matrix blocks need each
+    // workgroup inside one matrix.
+    if (fbstride != 0 && (((long long)(rows >> k) * m) % 256) != 0) return hipErrorInvalidValue;
+    return f32 ? launch_fma<float>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s)
+               : launch_fma<double>(mats, bstride, stride, rows, m, B, F, fbstride, bq, s);
+}
+
+hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
+                              long long kern_bstride, int Bk, void* out, hipStream_t s) {
+    const long long n = (long long)Bk * nblocks * 16;
+    if (n == 0) return hipSuccess;
+    if (f32)
+        hipLaunchKernelGGL(k_fuse_blocks<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           (const FusedGate*)fg, block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk,
+                           (cpx<float>*)out);
+    else
+        hipLaunchKernelGGL(k_fuse_blocks<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                           (const FusedGate*)fg, block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk,
+                           (cpx<double>*)out);
+    return hipGetLastError();
+}
+
+size_t fused_gate_bytes() { return sizeof(FusedGate); }
+
+}  // namespace sqh

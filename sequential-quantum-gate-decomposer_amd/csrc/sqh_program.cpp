@@ -150,4 +150,62 @@ bool compile_program(int N, const std::vector<GateRec>& gates, Program& prog, st
     return true;
 }
 
+FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates) {
+    FusionPlan fp;
+    const int kmax = N < 4 ? N : 4;
+    fp.k = kmax;
+    std::vector<int> cur;   // qubits of the open block
+    std::vector<size_t> members;
+    int slot = 0;
+    std::vector<int> slot_of(gates.size(), -1);
+    for (size_t i = 0; i < gates.size(); ++i)
+        if (gates[i].type != GT_CNOT && gates[i].type != GT_CZ) slot_of[i] = slot++;
+    auto close = [&]() {
+        if (members.empty()) return;
+        // pad to kmax qubits with the lowest unused ones
+        for (int q = 0; q < N && (int)cur.size() < kmax; ++q)
+            if (std::find(cur.begin(), cur.end(), q) == cur.end()) cur.push_back(q);
+        std::sort(cur.begin(), cur.end());
+        std::array<int, 4> qs = {0, 0, 0, 0};
+        for (int j = 0; j < kmax; ++j) qs[j] = cur[j];
+        fp.first.push_back((int)fp.gates.size());
+        fp.qubits.push_back(qs);
+        auto local = [&](int q) { return (int)(std::find(cur.begin(), cur.end(), q) - cur.begin()); };
+        for (size_t gi : members) {
+            const GateRec& g = gates[gi];
+            FusedRec r;
+            r.pad = 0;
+            r.t = (int8_t)local(g.target);
+            r.c = -1;
+            r.slot = slot_of[gi];
+            switch (g.type) {
+            case GT_CNOT: r.kind = OP_X; r.c = (int8_t)local(g.control); break;
+            case GT_CZ: r.kind = OP_Z; r.c = (int8_t)local(g.control); break;
+            case GT_CRY: r.kind = OP_CU; r.c = (int8_t)local(g.control); break;
+            default: r.kind = OP_U; break;
+            }
+            fp.gates.push_back(r);
+        }
+        cur.clear();
+        members.clear();
+    };
+    for (size_t i = 0; i < gates.size(); ++i) {
+        const GateRec& g = gates[i];
+        std::vector<int> q = {g.target};
+        if (g.type == GT_CNOT || g.type == GT_CZ || g.type == GT_CRY) q.push_back(g.control);
+        std::vector<int> uni = cur;
+        for (int x : q)
+            if (std::find(uni.begin(), uni.end(), x) == uni.end()) uni.push_back(x);
+        if ((int)uni.size() > kmax) {
+            close();
+            uni = q;
+        }
+        cur = uni;
+        members.push_back(i);
+    }
+    close();
+    fp.first.push_back((int)fp.gates.size());
+    return fp;
+}
+
 }  // namespace sqh

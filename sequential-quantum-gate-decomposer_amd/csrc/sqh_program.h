@@ -1,5 +1,6 @@
 // sqh_program.h -- host-side compilation of a gate list into a register-tile program.
 #pragma once
+#include <array>
 #include <string>
 #include <vector>
 
@@ -20,6 +21,22 @@ struct Program {
     std::vector<Slot> slots;
     int remaps = 0;
 };
+
+// Fusion of consecutive gates into dense blocks on at most 4 qubits (padded to 4
+// qubits when the circuit has them, so every block runs on the matrix cores).
+struct FusedRec {
+    int8_t kind, t, c, pad;  // OpKind, local target, local control (-1 none)
+    int32_t slot;            // parametric slot or -1
+};
+
+struct FusionPlan {
+    std::vector<FusedRec> gates;
+    std::vector<int> first;             // block b covers gates[first[b] .. first[b+1])
+    std::vector<std::array<int, 4>> qubits;  // sorted qubits of each block
+    int k = 0;                          // qubits per block
+};
+
+FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates);
 
 // Validates the gate list (types, qubit indices, parameter ranges) the way the
 // reference's gate constructors and Gates_block::add_gate do (U3.cpp:41-46,

@@ -38,7 +38,8 @@ EXPORTS = [
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
     "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
-    "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients",
+    "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
+    "sqh_apply_large_kernel_dev",
 ]
 
 
@@ -76,8 +77,17 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
                               "(the HIP library has no CPU fallback)")
+        # PyTorch-ROCm bundles its own libamdhip64 (soname libamdhip64.so.7) and its
+        # HIP library names it "libamdhip64.so", which does not match a runtime already
+        # loaded from /opt/rocm.  Loading torch first lets this library's
+        # "libamdhip64.so.7" dependency bind to torch's copy, so the process holds one
+        # HIP/HSA runtime and device pointers from either side are valid in both.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
-        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        vp, i, d =ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
         ll = ctypes.c_longlong
         sig = {
             "sqh_create": ([i, ctypes.POINTER(vp)], i),
@@ -94,6 +104,8 @@ def lib():
             "sqh_apply_to": ([vp, vp, vp, i, i, i, i], i),
             "sqh_apply_to_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
             "sqh_apply_to_stream_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
+            "sqh_apply_to_fused_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
+            "sqh_apply_large_kernel_dev": ([vp, vp, vp, i, vp, i, i, i, ll, i, i], i),
             "sqh_apply_derivate_to": ([vp, vp, vp, i, i, i, vp], i),
             "sqh_apply_derivate_to_dev": ([vp, vp, vp, i, i, i, vp], i),
             "sqh_optimization_problem_batch": ([vp, vp, i, i, vp], i),

@@ -191,3 +191,69 @@ def test_wrong_parameter_count_raises():
     dec.set_Gate_Structure(circuit_of(3, gates))
     with pytest.raises(Exception):
         dec.Optimization_Problem(np.zeros(W.param_count(gates) + 1))
+
+
+def _dense_block_full(n, F, qubits):
+    """Full 2^n operator of a dense block on sorted qubits (local bit j <-> qubits[j])."""
+    d = 1 << n
+    k = len(qubits)
+    full = np.zeros((d, d), dtype=complex)
+    for r in range(d):
+        li = sum(((r >> q) & 1) << j for j, q in enumerate(qubits))
+        base = r & ~sum(1 << q for q in qubits)
+        for lo in range(1 << k):
+            ro = base | sum(((lo >> j) & 1) << q for j, q in enumerate(qubits))
+            full[ro, r] += F[lo, li]
+    return full
+
+
+@pytest.mark.parametrize("n,k", [(3, 1), (4, 2), (5, 3), (6, 4), (8, 4), (7, 5)])
+def test_large_kernel_matches_dense_reference(n, k):
+    import torch
+    from squander_hip import Context
+    say(f"dense block n={n} k={k}")
+    rng = np.random.default_rng(10 * n + k)
+    qubits = np.sort(rng.choice(n, size=k, replace=False)).astype(np.int32)
+    F = (rng.standard_normal((1 << k, 1 << k)) + 1j * rng.standard_normal((1 << k, 1 << k)))
+    d = 1 << n
+    B, m = 3, max(1, d // 2 + 3)
+    X = rng.standard_normal((B, d, m)) + 1j * rng.standard_normal((B, d, m))
+    ref = np.einsum("ij,bjk->bik", _dense_block_full(n, F, list(qubits)), X)
+    ctx = Context(0)
+    for f32 in (False, True):
+        dt = np.complex64 if f32 else np.complex128
+        Fd = torch.from_numpy(F.astype(dt)).cuda()
+        Xd = torch.from_numpy(X.astype(dt)).cuda()
+        torch.cuda.synchronize()
+        ctx.check(ctx._L.sqh_apply_large_kernel_dev(ctx.h, Fd.data_ptr(), qubits.ctypes.data, k, Xd.data_ptr(), d, m,
+                                                    m, d * m, B, 1 if f32 else 0))
+        ctx.synchronize()
+        err = rel_fro(Xd.cpu().numpy(), ref)
+        assert err < (1e-5 if f32 else 1e-12), (f32, err)
+
+
+@pytest.mark.parametrize("n", [4, 6, 8, 10, 12])
+def test_fused_streaming_matches_oracle(oracle, n):
+    import torch
+    from squander_hip import Context
+    say(f"fused streaming n={n}")
+    gates = W.mixed_gate_circuit(n, 5 * n + 10, seed=1100 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    m = 48 if n >= 10 else d
+    B = 3
+    ps = W.random_params(P, n, batch=B)
+    X = np.random.default_rng(n).standard_normal((B, d, m)) + 0j
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    for per_b in (True, False):
+        pd = torch.from_numpy(ps if per_b else ps[:1].copy()).cuda()
+        Xd = torch.from_numpy(X.copy()).cuda()
+        torch.cuda.synchronize()
+        ctx.check(ctx._L.sqh_apply_to_fused_dev(ctx.h, pd.data_ptr(), P if per_b else 0, Xd.data_ptr(), d, m, m,
+                                                d * m, B, 0))
+        ctx.synchronize()
+        got = Xd.cpu().numpy()
+        for b in range(B):
+            ref = oracle.apply_to(gates, ps[b] if per_b else ps[0], X[b])
+            assert rel_fro(got[b], ref) < 1e-12

@@ -56,16 +56,22 @@ def test_batch_matches_oracle_driver(oracle, n, layers):
         assert (fo < 1e-8) == (fg < 1e-8) or abs(fo - fg) < 1e-10
 
 
-def test_start_decomposition_reachable_3q():
-    say("Start_Decomposition 3q")
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_start_decomposition_matches_oracle_driver(oracle, seed):
+    """N_Qubit_Decomposition_custom.Start_Decomposition with config random_seed gives
+    the oracle driver's result from the same seed (a random start may end in a local
+    minimum; both must end in the same one)."""
+    say(f"Start_Decomposition 3q seed {seed}")
     gates = W.ansatz_layers(3, W.alternating_pairs(3, 6))
     T = reachable_targets(3, gates, 1, 7)[0]
-    dec = N_Qubit_Decomposition_custom(T, config={"random_seed": 3, "optimization_tolerance": 1e-10})
+    dec = N_Qubit_Decomposition_custom(T, config={"random_seed": seed, "optimization_tolerance": 1e-10})
     dec.set_Gate_Structure(Circuit.from_gate_array(3, gates))
     dec.Start_Decomposition()
     err = dec.get_Decomposition_Error()
-    say(f"  error {err:.3e}, evals {dec.last_run['evals'][0]}")
-    assert err < 1e-8
+    fo, xo, co = oracle.decompose_custom(gates, T, 3, seed, max_outer_iterations=4, optimization_tolerance=1e-10)
+    say(f"  error {err:.3e}, evals {dec.last_run['evals'][0]}; oracle {fo:.3e}, evals {co}")
+    assert abs(err - oracle.optimization_problem(gates, T, 3, dec.get_Optimized_Parameters())) < 1e-12
+    assert (fo < 1e-10) == (err < 1e-10) or abs(err - fo) < 1e-10
 
 
 def test_mixed_precision_batch_4q(oracle):
