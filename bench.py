@@ -21,6 +21,11 @@ sys.path.insert(0, os.path.join(ROOT, "sequential-quantum-gate-decomposer_amd"))
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
+# HBM bytes per launch of the headline kernel from two rocprofv3 --pmc runs of this
+# script (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 256 MiB device copy
+# in the same runs): profiles/k_tile_forward_k_stream_gate_k_block_mfma4_8q_b512.md
+TILE_FORWARD_8Q_B512_TRAFFIC = (11910.7 * 2.0 + 2048.0) * 1024
+
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -76,6 +81,8 @@ def main():
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-main", action="store_true", help="time the headline step only (for profiling)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -155,10 +162,20 @@ def main():
                    "qubits": n, "batch": B, "gates": int(len(gates)), "params": P},
         "err_vs_fp64_oracle": err64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": TILE_FORWARD_8Q_B512_TRAFFIC if B == 512 else None,
                      "kernel": "k_tile_forward<8,double,1>", "kernel_avg_ms": k_avg_s * 1e3,
                      "algorithmic_flops_per_launch": flops_eval * B},
     }
+    if args.copy_calibration:
+        nbytes = 256 << 20
+        src = torch.ones(nbytes, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_calibration"] = {"bytes_read": nbytes, "bytes_written": nbytes, "launches": 3}
+        del src, dst
     if args.only_main:
         print(json.dumps(out), flush=True)
         return
