@@ -170,6 +170,49 @@ int sqh_decompose_batch(sqh_ctx* ctx, const double* targets, int n_targets, int 
                         const int* run_target, int n_runs, const double* x0, const sqh_decompose_options* opts,
                         double* x_out, double* f_out, int* status_out, long long* evals_out, long long* rounds_out);
 
+/* One decomposition problem of sqh_decompose_problems: its own circuit (gate list as
+ * in sqh_set_circuit), its square 2^qbit_num target (the problem's Umtx, complex128,
+ * row-major, stride 2^qbit_num) and its start point (the circuit's parameter count). */
+typedef struct sqh_problem {
+    int qbit_num;
+    int n_gates;
+    const sqh_gate* gates;
+    const double* target;
+    const double* x0;
+} sqh_problem;
+
+/* n_problems independent decompositions with different circuits and qubit counts
+ * (1..10 qubits each) as one lockstep batch: the driver of sqh_decompose_batch per
+ * problem (Decomposition_Base.cpp:375-709 for each), with the cost function the
+ * context is set to (trace offset 0), and every problem's cost+gradient evaluated in
+ * one launch per qubit count and precision per round.  x_out receives the problems'
+ * parameter vectors one after another; the other outputs follow sqh_decompose_batch. */
+int sqh_decompose_problems(sqh_ctx* ctx, const sqh_problem* problems, int n_problems,
+                           const sqh_decompose_options* opts, double* x_out, double* f_out, int* status_out,
+                           long long* evals_out, long long* rounds_out);
+
+/* Options of sqh_agents; the reference defaults in brackets (AGENTS.cpp:98-232,
+ * Optimization_Interface.cpp:62-64). */
+typedef struct sqh_agents_options {
+    int agent_num;                   /* agent_num [64] */
+    long long max_iterations;        /* max_inner_iterations(_agent) */
+    double optimization_tolerance;   /* optimization_tolerance(_agent) */
+    int agent_lifetime;              /* agent_lifetime [1000] */
+    double exploration_rate;         /* agent_exploration_rate [0.2] */
+    double agent_randomization_rate; /* agent_randomization_rate [0.2] */
+    double radius;                   /* Randomized_Radius [1.0] */
+    double randomization_rate;       /* randomization_rate [0.3] */
+    int convergence_length;          /* convergence_length [20] */
+} sqh_agents_options;
+
+/* Optimization_Interface::solve_layer_optimization_problem_AGENTS (AGENTS.cpp:42-870)
+ * on the context's circuit and target, FROBENIUS cost (its three-point sinusoid
+ * line search), the optimiser's std::mt19937 seeded with seed.  x: in the solution
+ * guess, out optimized_parameters (P doubles, host).  *f_out: current_minimum;
+ * *iters_out: iterations run; *evals_out: cost evaluations (all batched on the GPU). */
+int sqh_agents(sqh_ctx* ctx, double* x, const sqh_agents_options* opts, unsigned int seed, double* f_out,
+               long long* iters_out, long long* evals_out);
+
 /* ---- warm-start network ----------------------------------------------------- */
 /* A multilayer perceptron in -> hidden (GELU) x n_hidden -> out with BF16 weights
  * and activations, FP32 accumulation on the matrix cores and FP32 master weights,

@@ -70,6 +70,28 @@ def param_count(gates) -> int:
     return int(sum(PARAM_NUM[int(t)] for t in g[:, 0]))
 
 
+class AgentsOpts(ctypes.Structure):
+    """or_agents_opts (reference defaults)."""
+    _fields_ = [("agent_num", ctypes.c_int), ("max_iterations", ctypes.c_long), ("tolerance", ctypes.c_double),
+                ("lifetime", ctypes.c_int), ("exploration_rate", ctypes.c_double),
+                ("agent_randomization_rate", ctypes.c_double), ("radius", ctypes.c_double),
+                ("randomization_rate", ctypes.c_double), ("convergence_length", ctypes.c_int)]
+
+
+def agents_opts(**kw):
+    o = AgentsOpts()
+    o.agent_num = int(kw.get("agent_num", 64))
+    o.max_iterations = int(kw.get("max_iterations", 10000))
+    o.tolerance = float(kw.get("tolerance", 1e-8))
+    o.lifetime = int(kw.get("lifetime", 1000))
+    o.exploration_rate = float(kw.get("exploration_rate", 0.2))
+    o.agent_randomization_rate = float(kw.get("agent_randomization_rate", 0.2))
+    o.radius = float(kw.get("radius", 1.0))
+    o.randomization_rate = float(kw.get("randomization_rate", 0.3))
+    o.convergence_length = int(kw.get("convergence_length", 20))
+    return o
+
+
 class Oracle:
     """ctypes binding of sq_oracle.c."""
 
@@ -98,6 +120,15 @@ class Oracle:
                                           ctypes.c_long, ctypes.c_double, ctypes.c_double, dp,
                                           ctypes.POINTER(ctypes.c_long)]
         L.or_decompose_custom.restype = ctypes.c_double
+        L.or_mt_uniform_int.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.or_mt_uniform_int.restype = ctypes.c_int
+        L.or_mt_canonical.argtypes = [ctypes.c_void_p]
+        L.or_mt_canonical.restype = ctypes.c_double
+        L.or_mt_uniform.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]
+        L.or_mt_uniform.restype = ctypes.c_double
+        L.or_agents.argtypes = [pp, ctypes.c_uint, ctypes.POINTER(AgentsOpts), dp, ctypes.POINTER(ctypes.c_long),
+                                ctypes.POINTER(ctypes.c_long)]
+        L.or_agents.restype = ctypes.c_double
         L.or_mt_seed.argtypes = [ctypes.c_void_p, ctypes.c_uint]
         L.or_mt_next.argtypes = [ctypes.c_void_p]
         L.or_mt_next.restype = ctypes.c_uint
@@ -217,6 +248,32 @@ class Oracle:
                                          iteration_loops, max_inner_iterations, optimization_tolerance,
                                          convergence_threshold, self._dp(x), ctypes.byref(calls))
         return f, x, calls.value
+
+    def agents(self, gates, target, qbit_num, seed, x0, variant=COST_FROBENIUS, **kw):
+        """AGENTS from the solution guess x0 with mt19937(seed); returns
+        (current_minimum, optimized parameters, iterations, evaluations)."""
+        pr, keep = self._problem(gates, target, qbit_num, variant, 0)
+        x = np.array(x0, dtype=np.float64, copy=True)
+        it, ev = ctypes.c_long(0), ctypes.c_long(0)
+        o = agents_opts(**kw)
+        f = self.lib.or_agents(ctypes.byref(pr), seed, ctypes.byref(o), self._dp(x), ctypes.byref(it),
+                               ctypes.byref(ev))
+        return f, x, it.value, ev.value
+
+    def mt_draws(self, seed, plan):
+        """Interleaved libstdc++ draws from one mt19937(seed): plan is a list of
+        ("int", n) / ("canonical",) / ("uniform", a, b)."""
+        state = ctypes.create_string_buffer(624 * 4 + 16)
+        self.lib.or_mt_seed(state, seed)
+        out = []
+        for step in plan:
+            if step[0] == "int":
+                out.append(self.lib.or_mt_uniform_int(state, step[1]))
+            elif step[0] == "canonical":
+                out.append(self.lib.or_mt_canonical(state))
+            else:
+                out.append(self.lib.or_mt_uniform(state, step[1], step[2]))
+        return out
 
     def mt_stream(self, seed, n, kind="u32"):
         state = ctypes.create_string_buffer(624 * 4 + 16)

@@ -671,3 +671,154 @@ double or_decompose_custom(const or_problem* pr, unsigned int seed, int initial_
     free(opt); free(guess); free(best);
     return current_minimum;
 }
+
+/* ---- libstdc++ distributions ------------------------------------------------ */
+
+int or_mt_uniform_int(or_mt19937* g, int n) {
+    /* uniform_int_distribution::operator() with a 32-bit engine range: _S_nd<uint64>
+     * (Lemire, "Fast random integer generation in an interval", 2019). */
+    const unsigned int range = (unsigned int)n;
+    unsigned long long product = (unsigned long long)or_mt_next(g) * range;
+    unsigned int low = (unsigned int)product;
+    if (low < range) {
+        const unsigned int threshold = (unsigned int)(-range) % range;
+        while (low < threshold) {
+            product = (unsigned long long)or_mt_next(g) * range;
+            low = (unsigned int)product;
+        }
+    }
+    return (int)(product >> 32);
+}
+
+double or_mt_canonical(or_mt19937* g) {
+    const double r = 4294967296.0;
+    double sum = (double)or_mt_next(g);
+    sum += (double)or_mt_next(g) * r;
+    double u = sum / (r * r);
+    if (u >= 1.0) u = nextafter(1.0, 0.0);
+    return u;
+}
+
+double or_mt_uniform(or_mt19937* g, double a, double b) { return or_mt_canonical(g) * (b - a) + a; }
+
+/* ---- AGENTS -------------------------------------------------------------------- */
+
+/* Optimization_Interface::randomize_parameters (Optimization_Interface.cpp:576-602),
+ * non-VQE branch. */
+static void or_randomize_parameters(or_mt19937* g, const double* in, double* out, int P, double f0, double radius,
+                                    double rate) {
+    for (int j = 0; j < P; j++) {
+        if (or_mt_canonical(g) <= rate)
+            out[j] = in[j] + or_mt_uniform(g, -2.0 * M_PI, 2.0 * M_PI) * sqrt(f0) * radius;
+        else
+            out[j] = in[j];
+    }
+}
+
+double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* o, double* x, long* iters,
+                 long* evals) {
+    const int P = pr->P, A = o->agent_num;
+    or_mt19937 gen;
+    or_mt_seed(&gen, seed);
+    long n_eval = 0;
+    double* best = (double*)malloc(sizeof(double) * (size_t)(P > 0 ? P : 1));
+    double* ag = (double*)malloc(sizeof(double) * (size_t)A * (P > 0 ? P : 1));
+    double* fmin_ag = (double*)malloc(sizeof(double) * (size_t)A);
+    double* f_pi2 = (double*)malloc(sizeof(double) * (size_t)A);
+    double* f_pi = (double*)malloc(sizeof(double) * (size_t)A);
+    double* saved = (double*)malloc(sizeof(double) * (size_t)A);
+    int* pidx = (int*)malloc(sizeof(int) * (size_t)A);
+    double* rnd = (double*)malloc(sizeof(double) * 2 * (size_t)A);
+    double* win = (double*)calloc((size_t)o->convergence_length, sizeof(double));
+    memcpy(best, x, sizeof(double) * P);
+    /* AGENTS.cpp:98 */
+    double current_minimum = or_optimization_problem(pr, best);
+    n_eval++;
+    /* AGENTS.cpp:280-296: initial parameter index of every agent */
+    for (int a = 0; a < A; a++) pidx[a] = or_mt_uniform_int(&gen, P);
+    /* AGENTS.cpp:300-340: agent 0 starts at the guess, the others randomised around it */
+    for (int a = 0; a < A; a++) {
+        if (a == 0) memcpy(ag, x, sizeof(double) * P);
+        else or_randomize_parameters(&gen, best, ag + (size_t)a * P, P, current_minimum, o->radius,
+                                     o->randomization_rate);
+    }
+    or_optimization_problem_batched(pr, ag, A, 1, fmin_ag);
+    n_eval += A;
+    double mean = 0.0, var = DBL_MAX;
+    int widx = 0, most = 0, terminate = 0;
+    long it;
+    for (it = 0; it < o->max_iterations; it++) {
+        /* AGENTS.cpp:372-390 */
+        for (int a = 0; a < A; a++) {
+            pidx[a] = or_mt_uniform_int(&gen, P);
+            saved[a] = ag[(size_t)a * P + pidx[a]];
+        }
+        /* three-point line search on the sinusoid f(p) = offset + A cos(p - phi)
+         * (AGENTS.cpp:395-443) */
+        for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += M_PI / 2;
+        or_optimization_problem_batched(pr, ag, A, 1, f_pi2);
+        for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += M_PI / 2;
+        or_optimization_problem_batched(pr, ag, A, 1, f_pi);
+        n_eval += 2 * A;
+        for (int a = 0; a < A; a++) {
+            const double f0 = fmin_ag[a];
+            const double a_cos = (f0 - f_pi[a]) / 2;
+            const double offset = (f0 + f_pi[a]) / 2;
+            const double a_sin = offset - f_pi2[a];
+            const double phi0 = atan2(a_sin, a_cos);
+            const double shift = phi0 > 0 ? M_PI - phi0 : -phi0 - M_PI;
+            const double amp = sqrt(a_sin * a_sin + a_cos * a_cos);
+            ag[(size_t)a * P + pidx[a]] = saved[a] + shift;
+            fmin_ag[a] = offset - amp;
+        }
+        /* AGENTS.cpp:696-703 */
+        for (int i = 0; i < 2 * A; i++) rnd[i] = or_mt_canonical(&gen);
+        const int period = (it % o->lifetime) == 0;
+        if (period) {  /* AGENTS.cpp:737-742 */
+            or_optimization_problem_batched(pr, ag, A, 1, fmin_ag);
+            n_eval += A;
+        }
+        for (int a = 0; a < A; a++) {  /* AGENTS.cpp:747-850 */
+            double* xa = ag + (size_t)a * P;
+            if (fmin_ag[a] < o->tolerance) {
+                terminate = 1;
+                current_minimum = fmin_ag[a];
+                most = a;
+                memcpy(best, xa, sizeof(double) * P);
+            }
+            if (!period) continue;
+            if (fmin_ag[a] <= current_minimum) {
+                most = a;
+                memcpy(best, xa, sizeof(double) * P);
+                current_minimum = fmin_ag[a];
+            } else if (rnd[2 * a] < o->exploration_rate && a != most) {
+                fmin_ag[a] = fmin_ag[most];
+                memcpy(xa, ag + (size_t)most * P, sizeof(double) * P);
+                if (rnd[2 * a + 1] < o->agent_randomization_rate) {
+                    or_randomize_parameters(&gen, best, xa, P, current_minimum, o->radius, o->randomization_rate);
+                    fmin_ag[a] = or_optimization_problem(pr, xa);
+                    n_eval++;
+                }
+            }
+            if (a == 0) {  /* convergence window (AGENTS.cpp:818-836) */
+                const int L = o->convergence_length;
+                mean = mean + (current_minimum - win[widx]) / L;
+                win[widx] = current_minimum;
+                widx = (widx + 1) % L;
+                var = 0.0;
+                for (int i = 0; i < L; i++) var += (win[i] - mean) * (win[i] - mean);
+                var = sqrt(var) / L;
+                if (fabs(mean - current_minimum) < 1e-7 && var < 1e-7) terminate = 1;
+            }
+        }
+        if (terminate) {
+            it++;
+            break;
+        }
+    }
+    memcpy(x, best, sizeof(double) * P);
+    if (iters) *iters = it;
+    if (evals) *evals = n_eval;
+    free(best); free(ag); free(fmin_ag); free(f_pi2); free(f_pi); free(saved); free(pidx); free(rnd); free(win);
+    return current_minimum;
+}

@@ -106,6 +106,37 @@ double or_decompose_custom(const or_problem* pr, unsigned int seed, int initial_
                            long max_inner_iterations, double optimization_tolerance,
                            double convergence_threshold, double* x, long* calls);
 
+/* libstdc++ distributions on mt19937 (GCC 11 bits/uniform_int_dist.h and
+ * bits/random.tcc): uniform_int_distribution<int>(0, n-1) by the nearly
+ * divisionless method on the 64-bit product of one 32-bit draw and n;
+ * generate_canonical<double, 53> from two draws; uniform_real_distribution(a, b)
+ * = a + (b - a) * canonical. */
+int    or_mt_uniform_int(or_mt19937* g, int n);
+double or_mt_canonical(or_mt19937* g);
+double or_mt_uniform(or_mt19937* g, double a, double b);
+
+/* Options of the AGENTS engine with the reference defaults in brackets
+ * (AGENTS.cpp:98-232; Optimization_Interface.cpp:62-64). */
+typedef struct {
+    int agent_num;                   /* agent_num [64] */
+    long max_iterations;             /* max_inner_iterations(_agent) */
+    double tolerance;                /* optimization_tolerance(_agent) */
+    int lifetime;                    /* agent_lifetime [1000] */
+    double exploration_rate;         /* agent_exploration_rate [0.2] */
+    double agent_randomization_rate; /* agent_randomization_rate [0.2] */
+    double radius;                   /* Randomized_Radius [1.0] */
+    double randomization_rate;       /* randomization_rate [0.3] */
+    int convergence_length;          /* convergence_length [20] */
+} or_agents_opts;
+
+/* Optimization_Interface::solve_layer_optimization_problem_AGENTS
+ * (AGENTS.cpp:42-870) for the FROBENIUS cost (the three-point sinusoid line search
+ * of AGENTS.cpp:402-443) with the optimiser's mt19937 freshly seeded with seed and
+ * the solution guess in x.  x receives optimized_parameters; returns
+ * current_minimum; *iters the iterations run, *evals the cost evaluations. */
+double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* o, double* x, long* iters,
+                 long* evals);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,11 +1,18 @@
 // sqh_bfgs.cpp -- host driver of the batched decomposition optimiser.
 //
-// n_runs independent decompositions (each a target index and a start point) run as
-// lockstep rounds on the context's stream: advance every run's BFGS state machine
-// (consuming the evaluation it asked for), then evaluate all requested points in one
-// batched cost+gradient launch per precision.  rounds_per_graph rounds are captured
-// once as a hipGraph and replayed until every run has finished.
+// n_runs independent decompositions (each a circuit, a target and a start point) run
+// as lockstep rounds on the context's stream: advance every run's BFGS state machine
+// (consuming the evaluation it asked for), then evaluate all requested points with
+// one batched cost+gradient launch per qubit-count group and precision.
+// rounds_per_graph rounds are captured once as a hipGraph and replayed until every
+// run has finished.
+//
+// sqh_decompose_batch: every run uses the context's circuit.
+// sqh_decompose_problems: every run brings its own circuit (any qubit count up to
+// the tile limit); runs with the same qubit count share a launch, each wavefront
+// reading its run's program through a per-run table.
 #include <cstring>
+#include <map>
 #include <vector>
 
 #include "sqh_bfgs_device.h"
@@ -13,120 +20,132 @@
 
 namespace sqh {
 
-hipError_t launch_bfgs_init(void* st, double* vecs, const double* x0, int B, int P, int mixed, int* counts,
-                            hipStream_t s);
-hipError_t launch_bfgs_reset_counts(int* counts, hipStream_t s);
+hipError_t launch_bfgs_init(void* st, double* vecs, const double* x0, int B, int Ps, const int* run_P, int mixed,
+                            int* counts, int n_groups, hipStream_t s);
+hipError_t launch_bfgs_reset_counts(int* counts, int n_groups, hipStream_t s);
 hipError_t launch_bfgs_advance(void* st, double* vecs, double* Z, const double* fe, const double* ge, double* xt,
-                               int* lists, int* counts, const BfgsOpts& o, int B, hipStream_t s);
+                               int* lists, int* counts, const int* run_group, const BfgsOpts& o, int B,
+                               hipStream_t s);
 size_t bfgs_state_bytes();
 int bfgs_max_params();
 
 namespace {
 
+// One launch group: runs on the same number of qubits.
+struct Group {
+    int N = 0, rows = 0, cols = 0;
+    int B = 0;          // runs in the group (grid bound of its launches)
+    int n_targets = 0;  // target matrices of the group
+    int ns = 0;         // slot stride of the kernel and correlation tables
+    int m_pad = 0, wpb = 0;
+    const Op* ops = nullptr;  // single circuit: its ops; per-run circuits: concatenation
+    int nops = 0;
+    const Layout* lays = nullptr;
+    const Slot* slots = nullptr;
+    const ProgRef* run_prog = nullptr;
+    DevBuf tgt64, tgt32, d_ops, d_lays, d_slots;
+    ~Group() {
+        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots}) b->release();
+    }
+};
+
 struct BatchBufs {
-    DevBuf tgt64, tgt32, run_target, x0, states, vecs, Z, fe, ge, xt, lists, counts, kern64, kern32, diag, mpart;
+    DevBuf run_target, run_group, run_P, run_prog, x0, states, vecs, Z, fe, ge, xt, lists, counts, kern64, kern32,
+        diag, mpart;
     ~BatchBufs() {
-        for (DevBuf* b : {&tgt64, &tgt32, &run_target, &x0, &states, &vecs, &Z, &fe, &ge, &xt, &lists, &counts,
-                          &kern64, &kern32, &diag, &mpart})
+        for (DevBuf* b : {&run_target, &run_group, &run_P, &run_prog, &x0, &states, &vecs, &Z, &fe, &ge, &xt, &lists,
+                          &counts, &kern64, &kern32, &diag, &mpart})
             b->release();
     }
 };
 
 // One lockstep round: advance all runs, then evaluate the requested points.
-int enqueue_round(sqh_ctx* c, BatchBufs& bb, const BfgsOpts& o, int B, int rows, int cols, bool mixed) {
-    const Program& pr = c->prog;
-    const int N = pr.N, P = pr.P, ns = (int)pr.slots.size();
-    HIPTRY(c, launch_bfgs_reset_counts(bb.counts.as<int>(), c->stream));
+int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const BfgsOpts& o, int B, bool mixed,
+                  bool grouped) {
+    const int G = (int)groups.size();
+    HIPTRY(c, launch_bfgs_reset_counts(bb.counts.as<int>(), G, c->stream));
     HIPTRY(c, launch_bfgs_advance(bb.states.p, bb.vecs.as<double>(), bb.Z.as<double>(), bb.fe.as<double>(),
-                                  bb.ge.as<double>(), bb.xt.as<double>(), bb.lists.as<int>(), bb.counts.as<int>(), o,
-                                  B, c->stream));
-    const int off = uses_offset(c->variant) ? c->trace_offset : 0;
-    for (int p = 0; p < (mixed ? 2 : 1); ++p) {
-        const bool f32 = p == 1;
-        TileLaunch t = base_launch(c, f32);
-        void* kern = f32 ? bb.kern32.p : bb.kern64.p;
-        t.kern = kern;
-        t.in = f32 ? bb.tgt32.p : bb.tgt64.p;
-        t.in_stride = cols;
-        t.m = cols;
-        t.m_pad = m_pad_for(N, cols);
-        t.B = B;
-        t.kern_bstride = (long long)ns * 4;
-        t.trace_off = off;
-        t.diag = bb.diag.as<double>();
-        t.mpart = bb.mpart.as<double>();
-        t.bi.count = bb.counts.as<int>() + p;
-        t.bi.elem_run = bb.lists.as<int>() + (size_t)p * B;
-        t.bi.run_target = bb.run_target.as<int>();
-        t.bi.tgt_stride = (long long)rows * cols;
-        const int wpb = waves_per_batch(N, t.m_pad);
-        HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), P, B, pr.slots.empty() ? nullptr : c->d_slots.as<Slot>(),
-                                       ns, kern, c->stream, t.bi));
-        HIPTRY(c, launch_tile_grad(t, c->stream));
-        HIPTRY(c, launch_finish(t.diag, cols, B, c->variant, bb.fe.as<double>(), t.mpart, wpb, ns,
-                                c->d_slots.as<Slot>(), bb.xt.as<double>(), P, bb.ge.as<double>(), c->stream, t.bi));
+                                  bb.ge.as<double>(), bb.xt.as<double>(), bb.lists.as<int>(), bb.counts.as<int>(),
+                                  grouped ? bb.run_group.as<int>() : nullptr, o, B, c->stream));
+    for (int g = 0; g < G; ++g) {
+        Group& gr = groups[g];
+        for (int p = 0; p < (mixed ? 2 : 1); ++p) {
+            const bool f32 = p == 1;
+            TileLaunch t;
+            t.N = gr.N;
+            t.f32 = f32;
+            t.ops = gr.ops;
+            t.nops = gr.nops;
+            t.lays = gr.lays;
+            t.slots = gr.slots;
+            t.ns = gr.ns;
+            void* kern = f32 ? bb.kern32.p : bb.kern64.p;
+            t.kern = kern;
+            t.in = f32 ? gr.tgt32.p : gr.tgt64.p;
+            t.in_stride = gr.cols;
+            t.m = gr.cols;
+            t.m_pad = gr.m_pad;
+            t.B = gr.B;
+            t.kern_bstride = (long long)gr.ns * 4;
+            t.trace_off = uses_offset(c->variant) && !grouped ? c->trace_offset : 0;
+            t.diag = bb.diag.as<double>();
+            t.mpart = bb.mpart.as<double>();
+            t.bi.count = bb.counts.as<int>() + 4 + 2 * g + p;
+            t.bi.elem_run = bb.lists.as<int>() + (size_t)(2 * g + p) * B;
+            t.bi.run_target = bb.run_target.as<int>();
+            t.bi.tgt_stride = (long long)gr.rows * gr.cols;
+            t.bi.run_prog = gr.run_prog;
+            HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns, kern, c->stream, t.bi));
+            HIPTRY(c, launch_tile_grad(t, c->stream));
+            HIPTRY(c, launch_finish(t.diag, gr.cols, gr.B, c->variant, bb.fe.as<double>(), t.mpart, gr.wpb, gr.ns,
+                                    gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), c->stream, t.bi));
+        }
     }
     return SQH_OK;
 }
 
-}  // namespace
-}  // namespace sqh
-
-using namespace sqh;
-
-extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targets, int rows, int cols,
-                                   const int* run_target, int n_runs, const double* x0,
-                                   const sqh_decompose_options* opts, double* x_out, double* f_out, int* status_out,
-                                   long long* evals_out, long long* rounds_out) {
-    if (!c) return SQH_ERR_ARG;
-    std::lock_guard<std::mutex> lk(c->mu);
-    int rc = require_circuit(c);
-    if (rc) return rc;
-    const Program& pr = c->prog;
-    const int N = pr.N, P = pr.P, ns = (int)pr.slots.size();
-    if (!targets || !run_target || !x0 || !opts || !x_out || !f_out) return fail(c, SQH_ERR_ARG, "null buffer");
-    if (n_targets <= 0 || n_runs <= 0) return fail(c, SQH_ERR_ARG, "need at least one target and one run");
-    if (rows != (1 << N) || cols <= 0 || cols > rows)
-        return fail(c, SQH_ERR_ARG, "targets must be %d x m with m <= %d", 1 << N, 1 << N);
-    if (N > kMaxTileQubits) return fail(c, SQH_ERR_ARG, "batched optimisation supports up to %d qubits", kMaxTileQubits);
-    if (P <= 0 || P > bfgs_max_params())
-        return fail(c, SQH_ERR_ARG, "batched optimisation needs 1..%d parameters", bfgs_max_params());
-    const int off = uses_offset(c->variant) ? c->trace_offset : 0;
-    if (cols + off > rows) return fail(c, SQH_ERR_ARG, "trace offset exceeds the target");
-    for (int r = 0; r < n_runs; ++r)
-        if (run_target[r] < 0 || run_target[r] >= n_targets) return fail(c, SQH_ERR_ARG, "run %d: bad target index", r);
-    if (opts->max_inner_iterations <= 0 || opts->max_outer_iterations <= 0)
-        return fail(c, SQH_ERR_ARG, "iteration limits must be positive");
-    HIPTRY(c, hipSetDevice(c->device));
-
-    const int B = n_runs;
+// Runs the optimiser over B runs in the given groups.  x0: B x Ps (row stride Ps);
+// run_target: index into the run's group targets; run_P / run_group: per run (null
+// for the single-circuit case).  Outputs follow sqh_decompose_batch, with x_out
+// rows of stride Ps.
+int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* run_target, const int* run_P,
+              const int* run_group, const double* x0, const sqh_decompose_options* opts, double* x_out,
+              int x_stride, double* f_out, int* status_out, long long* evals_out, long long* rounds_out) {
+    const bool grouped = run_group != nullptr;
     const bool mixed = opts->mixed_precision != 0;
-    const size_t msz = (size_t)rows * cols;
-    const int m_pad = m_pad_for(N, cols);
-    const int wpb = waves_per_batch(N, m_pad);
+    const int G = (int)groups.size();
+    size_t kern_elems = 1, diag_elems = 1, mpart_elems = 1;
+    for (Group& g : groups) {
+        kern_elems = std::max(kern_elems, (size_t)g.B * std::max(g.ns, 1) * 4);
+        diag_elems = std::max(diag_elems, (size_t)g.B * g.cols * 2);
+        mpart_elems = std::max(mpart_elems, (size_t)g.B * g.wpb * std::max(g.ns, 1) * 8);
+    }
     BatchBufs bb;
-    HIPTRY(c, bb.tgt64.ensure((size_t)n_targets * msz * 16));
-    HIPTRY(c, bb.tgt32.ensure((size_t)n_targets * msz * 8));
     HIPTRY(c, bb.run_target.ensure((size_t)B * sizeof(int)));
-    HIPTRY(c, bb.x0.ensure((size_t)B * P * sizeof(double)));
+    HIPTRY(c, bb.x0.ensure((size_t)B * Ps * sizeof(double)));
     HIPTRY(c, bb.states.ensure((size_t)B * bfgs_state_bytes()));
-    HIPTRY(c, bb.vecs.ensure((size_t)B * RV_COUNT * P * sizeof(double)));
-    HIPTRY(c, bb.Z.ensure((size_t)B * P * P * sizeof(double)));
+    HIPTRY(c, bb.vecs.ensure((size_t)B * RV_COUNT * Ps * sizeof(double)));
+    HIPTRY(c, bb.Z.ensure((size_t)B * Ps * Ps * sizeof(double)));
     HIPTRY(c, bb.fe.ensure((size_t)B * sizeof(double)));
-    HIPTRY(c, bb.ge.ensure((size_t)B * P * sizeof(double)));
-    HIPTRY(c, bb.xt.ensure((size_t)B * P * sizeof(double)));
-    HIPTRY(c, bb.lists.ensure((size_t)2 * B * sizeof(int)));
-    HIPTRY(c, bb.counts.ensure(4 * sizeof(int)));
-    HIPTRY(c, bb.kern64.ensure((size_t)B * std::max(ns, 1) * 4 * 16));
-    HIPTRY(c, bb.kern32.ensure((size_t)B * std::max(ns, 1) * 4 * 8));
-    HIPTRY(c, bb.diag.ensure((size_t)B * cols * 2 * sizeof(double)));
-    HIPTRY(c, bb.mpart.ensure((size_t)B * wpb * std::max(ns, 1) * 8 * sizeof(double)));
-    HIPTRY(c, hipMemcpyAsync(bb.tgt64.p, targets, (size_t)n_targets * msz * 16, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, launch_convert(true, bb.tgt64.p, bb.tgt32.p, (long long)n_targets * msz * 2, c->stream));
+    HIPTRY(c, bb.ge.ensure((size_t)B * Ps * sizeof(double)));
+    HIPTRY(c, bb.xt.ensure((size_t)B * Ps * sizeof(double)));
+    HIPTRY(c, bb.lists.ensure((size_t)2 * G * B * sizeof(int)));
+    HIPTRY(c, bb.counts.ensure((size_t)(4 + 2 * G) * sizeof(int)));
+    HIPTRY(c, bb.kern64.ensure(kern_elems * 16));
+    HIPTRY(c, bb.kern32.ensure(kern_elems * 8));
+    HIPTRY(c, bb.diag.ensure(diag_elems * sizeof(double)));
+    HIPTRY(c, bb.mpart.ensure(mpart_elems * sizeof(double)));
     HIPTRY(c, hipMemcpyAsync(bb.run_target.p, run_target, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipMemcpyAsync(bb.x0.p, x0, (size_t)B * P * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, launch_bfgs_init(bb.states.p, bb.vecs.as<double>(), bb.x0.as<double>(), B, P, mixed ? 1 : 0,
-                               bb.counts.as<int>(), c->stream));
+    if (grouped) {
+        HIPTRY(c, bb.run_group.ensure((size_t)B * sizeof(int)));
+        HIPTRY(c, bb.run_P.ensure((size_t)B * sizeof(int)));
+        HIPTRY(c, hipMemcpyAsync(bb.run_group.p, run_group, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipMemcpyAsync(bb.run_P.p, run_P, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPTRY(c, hipMemcpyAsync(bb.x0.p, x0, (size_t)B * Ps * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, launch_bfgs_init(bb.states.p, bb.vecs.as<double>(), bb.x0.as<double>(), B, Ps,
+                               grouped ? bb.run_P.as<int>() : nullptr, mixed ? 1 : 0, bb.counts.as<int>(), G,
+                               c->stream));
 
     BfgsOpts o;
     o.max_calls = opts->max_inner_iterations;
@@ -135,7 +154,7 @@ extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targ
     o.tol = opts->optimization_tolerance;
     o.conv_thr = opts->convergence_threshold;
     o.switch_cost = opts->switch_cost;
-    o.P = P;
+    o.P = Ps;
 
     const int R = opts->rounds_per_graph > 0 ? opts->rounds_per_graph : 16;
     const long long max_rounds =
@@ -146,7 +165,7 @@ extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targ
     HIPTRY(c, hipStreamSynchronize(c->stream));
     HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < R; ++i) {
-        rc = enqueue_round(c, bb, o, B, rows, cols, mixed);
+        const int rc = enqueue_round(c, bb, groups, o, B, mixed, grouped);
         if (rc) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(c->stream, &g);
@@ -180,16 +199,201 @@ extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targ
 
     std::vector<char> st((size_t)B * bfgs_state_bytes());
     HIPTRY(c, hipMemcpy(st.data(), bb.states.p, st.size(), hipMemcpyDeviceToHost));
-    std::vector<double> vecs((size_t)B * RV_COUNT * P);
+    std::vector<double> vecs((size_t)B * RV_COUNT * Ps);
     HIPTRY(c, hipMemcpy(vecs.data(), bb.vecs.p, vecs.size() * sizeof(double), hipMemcpyDeviceToHost));
     for (int r = 0; r < B; ++r) {
         RunState S;
         std::memcpy(&S, st.data() + (size_t)r * bfgs_state_bytes(), sizeof(RunState));
-        std::memcpy(x_out + (size_t)r * P, vecs.data() + ((size_t)r * RV_COUNT + RV_BEST) * P, P * sizeof(double));
+        std::memcpy(x_out + (size_t)r * x_stride, vecs.data() + ((size_t)r * RV_COUNT + RV_BEST) * Ps,
+                    (size_t)S.P * sizeof(double));
         f_out[r] = S.cur_min;
         if (status_out) status_out[r] = S.finished ? S.status : -1;
         if (evals_out) evals_out[r] = S.total_calls + (S.finished ? 0 : S.calls);
     }
     if (rounds_out) *rounds_out = rounds;
     return SQH_OK;
+}
+
+int check_opts(sqh_ctx* c, const sqh_decompose_options* opts) {
+    if (opts->max_inner_iterations <= 0 || opts->max_outer_iterations <= 0)
+        return fail(c, SQH_ERR_ARG, "iteration limits must be positive");
+    return SQH_OK;
+}
+
+// Uploads n_targets complex128 targets of a group and their complex64 copies.
+int upload_targets(sqh_ctx* c, Group& g, const double* const* targets) {
+    const size_t msz = (size_t)g.rows * g.cols;
+    HIPTRY(c, g.tgt64.ensure((size_t)g.n_targets * msz * 16));
+    HIPTRY(c, g.tgt32.ensure((size_t)g.n_targets * msz * 8));
+    for (int i = 0; i < g.n_targets; ++i)
+        HIPTRY(c, hipMemcpyAsync(g.tgt64.as<char>() + (size_t)i * msz * 16, targets[i], msz * 16,
+                                 hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, launch_convert(true, g.tgt64.p, g.tgt32.p, (long long)g.n_targets * msz * 2, c->stream));
+    return SQH_OK;
+}
+
+}  // namespace
+}  // namespace sqh
+
+using namespace sqh;
+
+extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targets, int rows, int cols,
+                                   const int* run_target, int n_runs, const double* x0,
+                                   const sqh_decompose_options* opts, double* x_out, double* f_out, int* status_out,
+                                   long long* evals_out, long long* rounds_out) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    const Program& pr = c->prog;
+    const int N = pr.N, P = pr.P, ns = (int)pr.slots.size();
+    if (!targets || !run_target || !x0 || !opts || !x_out || !f_out) return fail(c, SQH_ERR_ARG, "null buffer");
+    if (n_targets <= 0 || n_runs <= 0) return fail(c, SQH_ERR_ARG, "need at least one target and one run");
+    if (rows != (1 << N) || cols <= 0 || cols > rows)
+        return fail(c, SQH_ERR_ARG, "targets must be %d x m with m <= %d", 1 << N, 1 << N);
+    if (N > kMaxTileQubits) return fail(c, SQH_ERR_ARG, "batched optimisation supports up to %d qubits", kMaxTileQubits);
+    if (P <= 0 || P > bfgs_max_params())
+        return fail(c, SQH_ERR_ARG, "batched optimisation needs 1..%d parameters", bfgs_max_params());
+    const int off = uses_offset(c->variant) ? c->trace_offset : 0;
+    if (cols + off > rows) return fail(c, SQH_ERR_ARG, "trace offset exceeds the target");
+    for (int r = 0; r < n_runs; ++r)
+        if (run_target[r] < 0 || run_target[r] >= n_targets) return fail(c, SQH_ERR_ARG, "run %d: bad target index", r);
+    if ((rc = check_opts(c, opts))) return rc;
+    HIPTRY(c, hipSetDevice(c->device));
+
+    std::vector<Group> groups(1);
+    Group& g = groups[0];
+    g.N = N;
+    g.rows = rows;
+    g.cols = cols;
+    g.B = n_runs;
+    g.n_targets = n_targets;
+    g.ns = ns;
+    g.m_pad = m_pad_for(N, cols);
+    g.wpb = waves_per_batch(N, g.m_pad);
+    g.ops = c->d_ops.as<Op>() + kOpPad;
+    g.nops = (int)pr.ops.size();
+    g.lays = c->d_lays.as<Layout>();
+    g.slots = pr.slots.empty() ? nullptr : c->d_slots.as<Slot>();
+    std::vector<const double*> tp(n_targets);
+    for (int i = 0; i < n_targets; ++i) tp[i] = targets + (size_t)i * rows * cols * 2;
+    if ((rc = upload_targets(c, g, tp.data()))) return rc;
+    return run_batch(c, groups, n_runs, P, run_target, nullptr, nullptr, x0, opts, x_out, P, f_out, status_out,
+                     evals_out, rounds_out);
+}
+
+extern "C" int sqh_decompose_problems(sqh_ctx* c, const sqh_problem* problems, int n_problems,
+                                      const sqh_decompose_options* opts, double* x_out, double* f_out,
+                                      int* status_out, long long* evals_out, long long* rounds_out) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!problems || n_problems <= 0 || !opts || !x_out || !f_out)
+        return fail(c, SQH_ERR_ARG, "need problems, options and output buffers");
+    int rc = check_opts(c, opts);
+    if (rc) return rc;
+    // Compile every problem's circuit and sort the runs into qubit-count groups.
+    std::vector<Program> progs(n_problems);
+    std::map<int, std::vector<int>> by_n;
+    int Ps = 1;
+    for (int i = 0; i < n_problems; ++i) {
+        const sqh_problem& q = problems[i];
+        if (q.qbit_num < 1 || q.qbit_num > kMaxTileQubits)
+            return fail(c, SQH_ERR_ARG, "problem %d: qubit count must be 1..%d", i, kMaxTileQubits);
+        if (!q.target || !q.x0 || (q.n_gates > 0 && !q.gates))
+            return fail(c, SQH_ERR_ARG, "problem %d: null gates, target or start point", i);
+        std::vector<GateRec> gl(q.n_gates);
+        for (int k = 0; k < q.n_gates; ++k)
+            gl[k] = GateRec{q.gates[k].type, q.gates[k].target, q.gates[k].control, q.gates[k].param_start};
+        std::string err;
+        if (!compile_program(q.qbit_num, gl, progs[i], err)) return fail(c, SQH_ERR_GATE, "problem %d: %s", i, err.c_str());
+        if (progs[i].P <= 0 || progs[i].P > bfgs_max_params())
+            return fail(c, SQH_ERR_ARG, "problem %d: batched optimisation needs 1..%d parameters", i,
+                        bfgs_max_params());
+        Ps = std::max(Ps, progs[i].P);
+        by_n[q.qbit_num].push_back(i);
+    }
+    HIPTRY(c, hipSetDevice(c->device));
+    std::vector<Group> groups(by_n.size());
+    std::vector<int> run_target(n_problems), run_group(n_problems), run_P(n_problems);
+    std::vector<ProgRef> refs(n_problems);
+    std::vector<double> x0((size_t)n_problems * Ps, 0.0);
+    int gi = 0;
+    for (auto& kv : by_n) {
+        Group& g = groups[gi];
+        const int N = kv.first;
+        const std::vector<int>& members = kv.second;
+        g.N = N;
+        g.rows = g.cols = 1 << N;
+        g.B = (int)members.size();
+        g.n_targets = g.B;
+        g.m_pad = m_pad_for(N, g.cols);
+        const int cw = tile_columns_per_wave(N);
+        g.m_pad = (g.m_pad + cw - 1) / cw * cw;  // one element per wavefront
+        g.wpb = waves_per_batch(N, g.m_pad);
+        Op nop;
+        nop.kind = OP_NOP;
+        nop.t = nop.cmode = nop.c = 0;
+        nop.idx = 0;
+        std::vector<Op> ops(kOpPad, nop);
+        std::vector<Layout> lays;
+        std::vector<Slot> slots;
+        std::vector<const double*> tp;
+        int ns = 1;
+        for (int j = 0; j < g.B; ++j) {
+            const int i = members[j];
+            const Program& p = progs[i];
+            ProgRef& pg = refs[i];
+            pg.op_off = (int)ops.size();
+            pg.nops = (int)p.ops.size();
+            pg.lay_off = (int)lays.size();
+            pg.slot_off = (int)slots.size();
+            pg.ns = (int)p.slots.size();
+            pg.pad0 = pg.pad1 = pg.pad2 = 0;
+            ops.insert(ops.end(), p.ops.begin(), p.ops.end());
+            ops.insert(ops.end(), kOpPad, nop);
+            lays.insert(lays.end(), p.layouts.begin(), p.layouts.end());
+            slots.insert(slots.end(), p.slots.begin(), p.slots.end());
+            ns = std::max(ns, pg.ns);
+            run_target[i] = j;
+            run_group[i] = gi;
+            run_P[i] = p.P;
+            std::memcpy(&x0[(size_t)i * Ps], problems[i].x0, (size_t)p.P * sizeof(double));
+            tp.push_back(problems[i].target);
+        }
+        g.ns = ns;
+        HIPTRY(c, g.d_ops.ensure(ops.size() * sizeof(Op)));
+        HIPTRY(c, g.d_lays.ensure(std::max<size_t>(lays.size(), 1) * sizeof(Layout)));
+        HIPTRY(c, g.d_slots.ensure(std::max<size_t>(slots.size(), 1) * sizeof(Slot)));
+        HIPTRY(c, hipMemcpyAsync(g.d_ops.p, ops.data(), ops.size() * sizeof(Op), hipMemcpyHostToDevice, c->stream));
+        if (!lays.empty())
+            HIPTRY(c, hipMemcpyAsync(g.d_lays.p, lays.data(), lays.size() * sizeof(Layout), hipMemcpyHostToDevice,
+                                     c->stream));
+        if (!slots.empty())
+            HIPTRY(c, hipMemcpyAsync(g.d_slots.p, slots.data(), slots.size() * sizeof(Slot), hipMemcpyHostToDevice,
+                                     c->stream));
+        g.ops = g.d_ops.as<Op>();  // offsets in the run table point past each program's leading pad
+        g.nops = 0;
+        g.lays = g.d_lays.as<Layout>();
+        g.slots = g.d_slots.as<Slot>();
+        if ((rc = upload_targets(c, g, tp.data()))) return rc;
+        ++gi;
+    }
+    DevBuf d_refs;
+    HIPTRY(c, d_refs.ensure((size_t)n_problems * sizeof(ProgRef)));
+    HIPTRY(c, hipMemcpyAsync(d_refs.p, refs.data(), (size_t)n_problems * sizeof(ProgRef), hipMemcpyHostToDevice,
+                             c->stream));
+    for (Group& g : groups) g.run_prog = d_refs.as<ProgRef>();
+    std::vector<double> xo((size_t)n_problems * Ps);
+    rc = run_batch(c, groups, n_problems, Ps, run_target.data(), run_P.data(), run_group.data(), x0.data(), opts,
+                   xo.data(), Ps, f_out, status_out, evals_out, rounds_out);
+    if (rc == SQH_OK) {
+        size_t o = 0;
+        for (int i = 0; i < n_problems; ++i) {
+            std::memcpy(x_out + o, &xo[(size_t)i * Ps], (size_t)run_P[i] * sizeof(double));
+            o += run_P[i];
+        }
+    }
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    d_refs.release();
+    return rc;
 }

@@ -39,12 +39,12 @@ struct BfgsOpts {
     double tol;              // optimization_tolerance on the FP64 cost
     double conv_thr;         // convergence_threshold of the minimum window
     double switch_cost;      // FP32 -> FP64 switch once the FP32 cost is below this
-    int P;
+    int P;                   // stride of the per-run vectors (largest parameter count)
 };
 
 struct alignas(16) RunState {
     int phase, status, prec, outer;
-    int ls_iter, finished, pad0, pad1;
+    int ls_iter, finished, P, pad1;  // P: this run's parameter count
     long long calls, total_calls, calls32;
     double f, fprev, norm_Z;
     double step, f_lowest, f_highest, step_highest, step_lowest, step_best, f_best, dg_abs_best, dg_highest, dg0,

@@ -453,6 +453,14 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     if ((long long)blockIdx.x * CW / m_pad >= B) return;  // whole wavefront past the live elements
     const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x, blockIdx.x);
     if (bi.count) in += (long long)bi.run_target[bi.elem_run[li.b]] * bi.tgt_stride;
+    if constexpr (UNI) {
+        if (bi.run_prog) {  // this wavefront's run has its own circuit
+            const ProgRef pg = bi.run_prog[bi.elem_run[li.b]];
+            ops += pg.op_off;
+            nops = pg.nops;
+            lays += pg.lay_off;
+        }
+    }
     cpx<R> x[G::NR], y[G::NR];
     {
         const int so = sub_off<N>(&lays[0], li.s);
@@ -658,6 +666,11 @@ __global__ void k_build_kernels(const double* __restrict__ params, int P, int B,
     if (i >= (long long)B * ns) return;
     const int b = (int)(i / ns), sl = (int)(i % ns);
     const int r = bi.count ? bi.elem_run[b] : b;
+    if (bi.run_prog) {  // ns is the table stride; the run has pg.ns slots of its own
+        const ProgRef pg = bi.run_prog[r];
+        if (sl >= pg.ns) return;
+        slots += pg.slot_off;
+    }
     const Slot s = slots[sl];
     cpx<double> k[4];
     dev_gate_kernel(s.type, params + (long long)r * P + s.param_start, k);
@@ -725,7 +738,13 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag,
     const double tim = block_sum(im, sh);
     if (threadIdx.x == 0 && costs) costs[r] = cost_of(variant, tre, tim, (double)m);
     if (!mpart) return;
-    for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    int nsr = ns;  // ns: stride of the correlation table
+    if (bi.run_prog) {
+        const ProgRef pg = bi.run_prog[r];
+        slots += pg.slot_off;
+        nsr = pg.ns;
+    }
+    for (int s = threadIdx.x; s < nsr; s += blockDim.x) {
         double M[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (int w = 0; w < wpb; ++w) {
             const double* src = mpart + (((long long)b * wpb + w) * ns + s) * 8;

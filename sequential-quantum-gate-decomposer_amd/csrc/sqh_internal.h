@@ -63,11 +63,19 @@ struct alignas(16) Slot {
 // elem_run[j], whose target matrix is run_target[run] (targets tgt_stride complex
 // entries apart); only the first *count elements are live.  All null: element j
 // is batch element j on the shared input.
+// Program of one run when the runs of one launch carry different circuits on the
+// same number of qubits: offsets into the launch's concatenated op, layout and slot
+// arrays (each program's ops padded by kOpPad no-ops on both sides).
+struct ProgRef {
+    int32_t op_off, nops, lay_off, slot_off, ns, pad0, pad1, pad2;
+};
+
 struct BatchIndex {
     const int* count = nullptr;
     const int* elem_run = nullptr;
     const int* run_target = nullptr;
     long long tgt_stride = 0;
+    const ProgRef* run_prog = nullptr;  // per run; requires one element per wavefront
 };
 
 constexpr int kMaxTileQubits = 10;   // one column fits one wavefront up to 2^10 rows

@@ -32,12 +32,17 @@ __device__ inline double wg_min(double v, double* sh) {
     return t;
 }
 
+// Ps: stride of the per-run vectors (the largest parameter count); run_P: each
+// run's parameter count (null: all Ps).  counts: [2] finished runs, [4 + 2g + p]
+// evaluation requests of group g in precision p.
 __global__ void k_bfgs_init(RunState* __restrict__ st, double* __restrict__ vecs, const double* __restrict__ x0,
-                            int B, int P, int mixed, int* __restrict__ counts) {
+                            int B, int Ps, const int* __restrict__ run_P, int mixed, int* __restrict__ counts,
+                            int n_counts) {
     const int r = blockIdx.x;
     if (r >= B) return;
-    double* V = vecs + (size_t)r * RV_COUNT * P;
-    for (int k = threadIdx.x; k < P; k += blockDim.x) V[RV_BEST * P + k] = x0[(size_t)r * P + k];
+    const int P = run_P ? run_P[r] : Ps;
+    double* V = vecs + (size_t)r * RV_COUNT * Ps;
+    for (int k = threadIdx.x; k < P; k += blockDim.x) V[RV_BEST * Ps + k] = x0[(size_t)r * Ps + k];
     if (threadIdx.x == 0) {
         RunState S;
         S.phase = PH_OUTER_START;
@@ -46,7 +51,8 @@ __global__ void k_bfgs_init(RunState* __restrict__ st, double* __restrict__ vecs
         S.outer = 0;
         S.ls_iter = 0;
         S.finished = 0;
-        S.pad0 = S.pad1 = 0;
+        S.P = P;
+        S.pad1 = 0;
         S.calls = S.total_calls = S.calls32 = 0;
         S.f = S.fprev = 0.0;
         S.norm_Z = -1.0;
@@ -56,27 +62,27 @@ __global__ void k_bfgs_init(RunState* __restrict__ st, double* __restrict__ vecs
         for (int i = 0; i < kMinWindow; ++i) S.minvec[i] = 0.0;
         st[r] = S;
     }
-    if (r == 0 && threadIdx.x == 0) counts[0] = counts[1] = counts[2] = 0;
+    if (r == 0)
+        for (int i = threadIdx.x; i < n_counts; i += blockDim.x) counts[i] = 0;
 }
 
-__global__ void k_bfgs_reset_counts(int* __restrict__ counts) {
-    if (threadIdx.x == 0) {
-        counts[0] = 0;
-        counts[1] = 0;
-    }
+// Clears the evaluation-request counters (counts[4 ..]) before each round.
+__global__ void k_bfgs_reset_counts(int* __restrict__ counts, int n_groups) {
+    for (int i = threadIdx.x; i < 2 * n_groups; i += blockDim.x) counts[4 + i] = 0;
 }
 
 // BFGS_Powell::BFGS_iteration (BFGS_Powell.cpp:202-386) across the workgroup.
-__device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int P, double& norm_Z, double* sh,
-                            double* zs) {
+// V: the run's vectors with stride Ps; Z: P x P row-major.
+__device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int P, int Ps, double& norm_Z,
+                            double* sh, double* zs) {
     const int tid = threadIdx.x, nt = blockDim.x;
-    double* x = V + RV_X * P;
-    double* g = V + RV_G * P;
-    double* x0 = V + RV_X0 * P;
-    double* g0 = V + RV_G0 * P;
-    double* delta = V + RV_DELTA * P;
-    double* gamma = V + RV_GAMMA * P;
-    const double* ztg = V + RV_ZTG * P;
+    double* x = V + RV_X * Ps;
+    double* g = V + RV_G * Ps;
+    double* x0 = V + RV_X0 * Ps;
+    double* g0 = V + RV_G0 * Ps;
+    double* delta = V + RV_DELTA * Ps;
+    double* gamma = V + RV_GAMMA * Ps;
+    const double* ztg = V + RV_ZTG * Ps;
     double nd = 0.0, t0 = 0.0, dtg = 0.0;
     for (int k = tid; k < P; k += nt) {
         const double dk = x[k] - x0[k], gk = g[k] - g0[k];
@@ -148,27 +154,29 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                                                                const double* __restrict__ fe,
                                                                const double* __restrict__ ge, double* __restrict__ xt,
                                                                int* __restrict__ lists, int* __restrict__ counts,
-                                                               BfgsOpts o, int B) {
+                                                               const int* __restrict__ run_group, BfgsOpts o,
+                                                               int B) {
     __shared__ double sh[8];
     __shared__ double zs[kBfgsMaxP];
     const int r = blockIdx.x;
     if (r >= B) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int P = o.P;
+    const int Ps = o.P;  // stride of the per-run vectors
     RunState S = states[r];
     if (S.phase == PH_FINISHED) return;
-    double* V = vecs + (size_t)r * RV_COUNT * P;
-    double* x = V + RV_X * P;
-    double* g = V + RV_G * P;
-    double* d = V + RV_D * P;
-    double* x0 = V + RV_X0 * P;
-    double* g0 = V + RV_G0 * P;
-    double* gbest = V + RV_GBEST * P;
-    double* ztg = V + RV_ZTG * P;
-    double* best = V + RV_BEST * P;
-    double* Z = Zs + (size_t)r * P * P;
-    double* xr = xt + (size_t)r * P;
-    const double* gr = ge + (size_t)r * P;
+    const int P = S.P;
+    double* V = vecs + (size_t)r * RV_COUNT * Ps;
+    double* x = V + RV_X * Ps;
+    double* g = V + RV_G * Ps;
+    double* d = V + RV_D * Ps;
+    double* x0 = V + RV_X0 * Ps;
+    double* g0 = V + RV_G0 * Ps;
+    double* gbest = V + RV_GBEST * Ps;
+    double* ztg = V + RV_ZTG * Ps;
+    double* best = V + RV_BEST * Ps;
+    double* Z = Zs + (size_t)r * Ps * Ps;
+    double* xr = xt + (size_t)r * Ps;
+    const double* gr = ge + (size_t)r * Ps;
     bool request = false;
     for (int guard = 0; guard < 16 && !request && S.phase != PH_FINISHED; ++guard) {
         switch (S.phase) {
@@ -322,7 +330,7 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                 S.phase = PH_BFGS_END;
                 break;
             }
-            bfgs_update(Z, V, P, S.norm_Z, sh, zs);
+            bfgs_update(Z, V, P, Ps, S.norm_Z, sh, zs);
             S.phase = PH_DIRECTION;
             break;
         }
@@ -370,27 +378,30 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
         }
     }
     if (request && tid == 0) {
-        const int j = atomicAdd(&counts[S.prec], 1);
-        lists[(size_t)S.prec * B + j] = r;
+        const int q = 2 * (run_group ? run_group[r] : 0) + S.prec;
+        const int j = atomicAdd(&counts[4 + q], 1);
+        lists[(size_t)q * B + j] = r;
     }
     if (tid == 0) states[r] = S;
 }
 
-hipError_t launch_bfgs_init(void* st, double* vecs, const double* x0, int B, int P, int mixed, int* counts,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_bfgs_init, dim3(B), dim3(256), 0, s, (RunState*)st, vecs, x0, B, P, mixed, counts);
+hipError_t launch_bfgs_init(void* st, double* vecs, const double* x0, int B, int Ps, const int* run_P, int mixed,
+                            int* counts, int n_groups, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfgs_init, dim3(B), dim3(256), 0, s, (RunState*)st, vecs, x0, B, Ps, run_P, mixed, counts,
+                       4 + 2 * n_groups);
     return hipGetLastError();
 }
 
-hipError_t launch_bfgs_reset_counts(int* counts, hipStream_t s) {
-    hipLaunchKernelGGL(k_bfgs_reset_counts, dim3(1), dim3(64), 0, s, counts);
+hipError_t launch_bfgs_reset_counts(int* counts, int n_groups, hipStream_t s) {
+    hipLaunchKernelGGL(k_bfgs_reset_counts, dim3(1), dim3(64), 0, s, counts, n_groups);
     return hipGetLastError();
 }
 
 hipError_t launch_bfgs_advance(void* st, double* vecs, double* Z, const double* fe, const double* ge, double* xt,
-                               int* lists, int* counts, const BfgsOpts& o, int B, hipStream_t s) {
+                               int* lists, int* counts, const int* run_group, const BfgsOpts& o, int B,
+                               hipStream_t s) {
     hipLaunchKernelGGL(k_bfgs_advance, dim3(B), dim3(kBfgsThreads), 0, s, (RunState*)st, vecs, Z, fe, ge, xt, lists,
-                       counts, o, B);
+                       counts, run_group, o, B);
     return hipGetLastError();
 }
 

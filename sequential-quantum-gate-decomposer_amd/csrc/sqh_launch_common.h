@@ -100,9 +100,10 @@ hipError_t launch_grad_n(const TileLaunch& t, hipStream_t st) {
     using G = Geo<N>;
     const long long items = (long long)t.B * t.m_pad;
     const unsigned grid = blocks_for(items, G::CW);
-    if constexpr (N > 4) {
-        if (uniform_waves<N>(t)) return launch_grad_nu<N, R, true>(t, st, grid);
-    }
+    // Per-run circuits need one element per wavefront (the host pads columns to a
+    // whole wavefront for them, also at 4 qubits and below).
+    if (uniform_waves<N>(t) || (t.bi.run_prog && t.m_pad % G::CW == 0)) return launch_grad_nu<N, R, true>(t, st, grid);
+    if (t.bi.run_prog) return hipErrorInvalidValue;
     return launch_grad_nu<N, R, false>(t, st, grid);
 }
 

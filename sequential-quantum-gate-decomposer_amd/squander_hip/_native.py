@@ -39,7 +39,7 @@ EXPORTS = [
     "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
-    "sqh_apply_large_kernel_dev",
+    "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents",
 ]
 
 
@@ -62,6 +62,20 @@ class SqhError(Exception):
 class _Gate(ctypes.Structure):
     _fields_ = [("type", ctypes.c_int32), ("target", ctypes.c_int32),
                 ("control", ctypes.c_int32), ("param_start", ctypes.c_int32)]
+
+
+class AgentsOptions(ctypes.Structure):
+    """sqh_agents_options (include/sqhip.h)."""
+    _fields_ = [("agent_num", ctypes.c_int), ("max_iterations", ctypes.c_longlong),
+                ("optimization_tolerance", ctypes.c_double), ("agent_lifetime", ctypes.c_int),
+                ("exploration_rate", ctypes.c_double), ("agent_randomization_rate", ctypes.c_double),
+                ("radius", ctypes.c_double), ("randomization_rate", ctypes.c_double),
+                ("convergence_length", ctypes.c_int)]
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("qbit_num", ctypes.c_int), ("n_gates", ctypes.c_int), ("gates", ctypes.POINTER(_Gate)),
+                ("target", ctypes.c_void_p), ("x0", ctypes.c_void_p)]
 
 
 _lib = None
@@ -87,7 +101,7 @@ def lib():
         except ImportError:
             pass
         L = ctypes.CDLL(LIB_PATH)
-        vp, i, d =ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
+        vp, i, d = ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double)
         ll = ctypes.c_longlong
         sig = {
             "sqh_create": ([i, ctypes.POINTER(vp)], i),
@@ -124,6 +138,10 @@ def lib():
             "sqh_mlp_gradients": ([vp, vp, vp, i, vp, d], i),
             "sqh_decompose_batch": ([vp, vp, i, i, i, vp, i, vp, ctypes.POINTER(DecomposeOptions), vp, vp, vp, vp,
                                      ctypes.POINTER(ll)], i),
+            "sqh_decompose_problems": ([vp, ctypes.POINTER(Problem), i, ctypes.POINTER(DecomposeOptions), vp, vp, vp,
+                                        vp, ctypes.POINTER(ll)], i),
+            "sqh_agents": ([vp, vp, ctypes.POINTER(AgentsOptions), ctypes.c_uint, d, ctypes.POINTER(ll),
+                            ctypes.POINTER(ll)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_program": ([i, ctypes.POINTER(_Gate), i, vp, i, vp, i, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)], i),

@@ -73,3 +73,66 @@ def decompose_batch(ctx: nat.Context, targets, run_target, x0, options: nat.Deco
                                          X0.ctypes.data, ctypes.byref(options), x.ctypes.data, f.ctypes.data,
                                          st.ctypes.data, ev.ctypes.data, ctypes.byref(rounds)))
     return {"x": x, "f": f, "status": st, "evals": ev, "rounds": int(rounds.value)}
+
+
+def decompose_problems(ctx: nat.Context, problems, options: nat.DecomposeOptions):
+    """Independent decompositions with their own circuits and qubit counts in one
+    lockstep batch.  problems: sequence of (qbit_num, gates (G, 4) int array,
+    Umtx (2^n, 2^n) complex128, x0 (P,)).  Returns a dict with x (list of arrays), f,
+    status, evals and rounds."""
+    keep = []
+    arr = (nat.Problem * len(problems))()
+    Ps = []
+    for i, (n, gates, U, x0) in enumerate(problems):
+        g, ng = nat.gate_array(gates)
+        Um = np.ascontiguousarray(U, dtype=np.complex128)
+        if Um.shape != (1 << n, 1 << n):
+            raise ValueError(f"problem {i}: target must be {1 << n} x {1 << n}")
+        xs = np.ascontiguousarray(x0, dtype=np.float64)
+        keep += [g, Um, xs]
+        arr[i].qbit_num = n
+        arr[i].n_gates = ng
+        arr[i].gates = ctypes.cast(g, ctypes.POINTER(nat._Gate))
+        arr[i].target = Um.ctypes.data
+        arr[i].x0 = xs.ctypes.data
+        Ps.append(xs.size)
+    R = len(problems)
+    xo = np.empty(int(sum(Ps)))
+    f = np.empty(R)
+    st = np.empty(R, dtype=np.int32)
+    ev = np.empty(R, dtype=np.int64)
+    rounds = ctypes.c_longlong(0)
+    ctx.check(ctx._L.sqh_decompose_problems(ctx.h, arr, R, ctypes.byref(options), xo.ctypes.data, f.ctypes.data,
+                                            st.ctypes.data, ev.ctypes.data, ctypes.byref(rounds)))
+    xs_out, o = [], 0
+    for p in Ps:
+        xs_out.append(xo[o:o + p].copy())
+        o += p
+    return {"x": xs_out, "f": f, "status": st, "evals": ev, "rounds": int(rounds.value)}
+
+
+def agents_options(**kw) -> nat.AgentsOptions:
+    """AGENTS options with the reference defaults (AGENTS.cpp:98-232)."""
+    o = nat.AgentsOptions()
+    o.agent_num = int(kw.get("agent_num", 64))
+    o.max_iterations = int(kw.get("max_iterations", kw.get("max_inner_iterations", 10000)))
+    o.optimization_tolerance = float(kw.get("optimization_tolerance", 1e-8))
+    o.agent_lifetime = int(kw.get("agent_lifetime", 1000))
+    o.exploration_rate = float(kw.get("agent_exploration_rate", 0.2))
+    o.agent_randomization_rate = float(kw.get("agent_randomization_rate", 0.2))
+    o.radius = float(kw.get("Randomized_Radius", 1.0))
+    o.randomization_rate = float(kw.get("randomization_rate", 0.3))
+    o.convergence_length = int(kw.get("convergence_length", 20))
+    return o
+
+
+def agents(ctx: nat.Context, x0, seed: int, options: nat.AgentsOptions | None = None):
+    """AGENTS on the context's circuit and target from the guess x0; returns
+    (current_minimum, optimized parameters, iterations, evaluations)."""
+    x = np.array(x0, dtype=np.float64, copy=True)
+    f = ctypes.c_double(0.0)
+    it, ev = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    o = options if options is not None else agents_options()
+    ctx.check(ctx._L.sqh_agents(ctx.h, x.ctypes.data, ctypes.byref(o), seed, ctypes.byref(f), ctypes.byref(it),
+                                ctypes.byref(ev)))
+    return f.value, x, int(it.value), int(ev.value)

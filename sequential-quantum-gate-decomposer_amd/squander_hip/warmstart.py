@@ -31,6 +31,63 @@ def target_features(V: np.ndarray) -> np.ndarray:
     return np.concatenate([Vn.real.reshape(n, -1), Vn.imag.reshape(n, -1)], axis=1).astype(np.float32)
 
 
+def canonical_angles(gates, theta: np.ndarray) -> np.ndarray:
+    """Representative of theta's class of exactly equal circuits (same matrix, same
+    global phase), so that training labels are one function of the target.
+
+    U3(t, p, l) = [[cos t, -sin t e^{il}], [sin t e^{ip}, cos t e^{i(p+l)}]]
+    (gate_kernel_templates.h:509-521) satisfies
+      U3(t, p, l) = diag(1, e^{ip}) U3(t, 0, l)  and  U3(t, p, l) diag(1, e^{im}) = U3(t, p, l + m),
+      U3(t, p, l) = U3(-t, p + pi, l + pi),   U3(t + pi, p, l) = -U3(t, p, l).
+    A diagonal on a CNOT's control commutes with the CNOT, so the phi of a U3 whose
+    qubit is next the control of a CNOT that is followed (on that qubit) by a U3 moves
+    to that U3's lambda.  Every theta is folded into [0, pi/2]; the product of the
+    signs the folds take out goes back into the last U3 as theta + pi.
+    Only U3 and CNOT/CZ gates are rewritten; other parametric gates are left alone."""
+    g = np.asarray(gates, dtype=np.int64).reshape(-1, 4)
+    x = np.array(theta, dtype=np.float64, copy=True)
+    single = x.ndim == 1
+    if single:
+        x = x[None, :]
+    u3 = [i for i in range(len(g)) if g[i, 0] == nat.GATE_U3]
+    if not u3:
+        return theta
+    sign = np.ones(x.shape[0])
+    for i in u3:
+        ps = g[i, 3]
+        t = x[:, ps]
+        k = np.floor(t / np.pi)
+        t0 = t - k * np.pi
+        sign *= np.where(np.mod(k, 2) == 1, -1.0, 1.0)
+        hi = t0 > np.pi / 2
+        x[:, ps] = np.where(hi, np.pi - t0, t0)
+        x[:, ps + 1] += np.where(hi, np.pi, 0.0)
+        x[:, ps + 2] += np.where(hi, np.pi, 0.0)
+        sign *= np.where(hi, -1.0, 1.0)
+
+    def next_on(q, start):
+        for j in range(start, len(g)):
+            t, tg, c = g[j, 0], g[j, 1], g[j, 2]
+            if tg == q or (c == q and t in (nat.GATE_CNOT, nat.GATE_CZ, nat.GATE_CRY)):
+                return j
+        return None
+
+    for i in u3:
+        q = g[i, 1]
+        j = next_on(q, i + 1)
+        if j is None or g[j, 0] not in (nat.GATE_CNOT, nat.GATE_CZ) or g[j, 2] != q:
+            continue
+        k = next_on(q, j + 1)
+        if k is None or g[k, 0] != nat.GATE_U3:
+            continue
+        x[:, g[k, 3] + 2] += x[:, g[i, 3] + 1]
+        x[:, g[i, 3] + 1] = 0.0
+    last = g[u3[-1], 3]
+    x[:, last] += np.where(sign < 0, np.pi, 0.0)
+    x = np.mod(x, 2.0 * np.pi)
+    return x[0] if single else x
+
+
 def angles_to_sincos(theta: np.ndarray) -> np.ndarray:
     return np.concatenate([np.sin(theta), np.cos(theta)], axis=1).astype(np.float32)
 
@@ -81,12 +138,14 @@ class CircuitBatch:
 
 
 def generate_pairs(cb: CircuitBatch, count: int, seed: int):
-    """(features, labels, theta*, V, verification costs) of count seeded reachable targets."""
+    """(features, sin/cos labels, canonical angles, V, verification costs) of count
+    seeded reachable targets V = C(theta*), theta* uniform in [0, 2pi)."""
     rng = np.random.default_rng(seed)
     theta = rng.uniform(0.0, 2.0 * np.pi, (count, cb.P))
     V = cb.unitaries(theta)
-    costs = cb.verify(theta, V)
-    return target_features(V), angles_to_sincos(theta), theta, V, costs
+    label = canonical_angles(cb.gates, theta)
+    costs = cb.verify(label, V)
+    return target_features(V), angles_to_sincos(label), label, V, costs
 
 
 class WarmStart:

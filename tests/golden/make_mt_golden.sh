@@ -1,7 +1,11 @@
 #!/bin/sh
-# Writes mt19937_uniform_2pi.txt: draws of std::uniform_real_distribution<>(0, 2*M_PI)
-# from std::mt19937(seed) as libstdc++ produces them (the RNG the reference's
-# decomposition driver consumes: Decomposition_Base.cpp:32-42, 426-436).
+# Writes the libstdc++ golden draws of std::mt19937 that the reference's optimisers
+# consume (compiled here with g++; no reference source involved):
+#  mt19937_uniform_2pi.txt: std::uniform_real_distribution<>(0, 2*M_PI), seed 1
+#    (Decomposition_Base.cpp:32-42, 426-436: the RANDOM initial guess);
+#  mt19937_agents_draws.txt: interleaved std::uniform_int_distribution<>(0, 83),
+#    std::uniform_real_distribution<>(0.0, 1.0) and (-2*M_PI, 2*M_PI), seed 7
+#    (AGENTS.cpp:280-296, 696-703; Optimization_Interface.cpp:576-602).
 set -e
 HERE=$(cd "$(dirname "$0")" && pwd)
 TMP=$(mktemp -d)
@@ -19,6 +23,29 @@ int main() {
     return 0;
 }
 CPP
+cat > "$TMP/gen2.cpp" <<'CPP'
+#include <cmath>
+#include <cstdio>
+#include <random>
+int main() {
+    const unsigned seed = 7;
+    const int n = 200;
+    std::mt19937 gen(seed);
+    std::uniform_int_distribution<> di(0, 83);
+    std::uniform_real_distribution<> dp(0.0, 1.0);
+    std::uniform_real_distribution<> dr(-2 * M_PI, 2 * M_PI);
+    std::printf("%u %d\n", seed, n);
+    for (int i = 0; i < n; ++i) {
+        const int a = di(gen);
+        const double b = dp(gen);
+        const double c = dr(gen);
+        std::printf("%d %.17g %.17g\n", a, b, c);
+    }
+    return 0;
+}
+CPP
 g++ -O2 -o "$TMP/gen" "$TMP/gen.cpp"
+g++ -O2 -o "$TMP/gen2" "$TMP/gen2.cpp"
 "$TMP/gen" > "$HERE/mt19937_uniform_2pi.txt"
+"$TMP/gen2" > "$HERE/mt19937_agents_draws.txt"
 rm -rf "$TMP"

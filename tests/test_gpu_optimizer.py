@@ -95,3 +95,97 @@ def test_mixed_precision_batch_4q(oracle):
     smx = np.mean(resmx["f"] < 1e-8)
     say(f"  success FP64 {s64:.2f} ({res64['rounds']} rounds), mixed {smx:.2f} ({resmx['rounds']} rounds)")
     assert s64 > 0 and smx > 0
+
+
+def test_decompose_problems_matches_oracle_driver(oracle):
+    """Problems with their own circuits and qubit counts in one lockstep batch, each
+    compared with the oracle's driver from the same mt19937 start point."""
+    from squander_hip import decompose_problems
+    specs = [(2, 2, 1), (3, 4, 2), (4, 3, 3), (5, 2, 4), (3, 6, 5), (4, 5, 6), (2, 3, 7), (5, 4, 8), (6, 2, 9)]
+    probs, ref = [], []
+    for n, layers, seed in specs:
+        say(f"  problem n={n} layers={layers} seed={seed}")
+        gates = W.ansatz_layers(n, W.random_adjacent_pairs(n, layers, seed))
+        P = W.param_count(gates)
+        T = reachable_targets(n, gates, 1, 300 + seed)[0]
+        probs.append((n, gates, T, initial_guess(P, seed)))
+        ref.append(oracle.decompose_custom(gates, T, n, seed, max_outer_iterations=4, optimization_tolerance=1e-8))
+    ctx = nat.Context(0)
+    res = decompose_problems(ctx, probs, default_options(4, optimization_tolerance=1e-8, max_outer_iterations=4))
+    for i, ((n, gates, T, _), (fo, xo, co)) in enumerate(zip(probs, ref)):
+        fg = res["f"][i]
+        f_check = oracle.optimization_problem(gates, T, n, res["x"][i])
+        say(f"  problem {i} (n={n}): gpu f={fg:.3e} evals={res['evals'][i]}  oracle f={fo:.3e} evals={co}")
+        assert res["x"][i].shape == (W.param_count(gates),)
+        assert abs(f_check - fg) < 1e-12
+        assert (fo < 1e-8) == (fg < 1e-8) or abs(fo - fg) < 1e-10
+
+
+def test_decompose_problems_mixed_precision_uneven(oracle):
+    """The uneven workload generator with complex64 descent and FP64 acceptance:
+    every reported cost is the FP64 oracle cost of the returned parameters."""
+    from squander_hip import decompose_problems
+    from squander_hip.uneven_bench import make_problems
+
+    def matrix(n, gates, theta):
+        return oracle.apply_to(gates, theta, np.eye(1 << n))
+
+    probs = make_problems(24, seed=11, matrix=matrix)
+    say(f"  uneven mixed batch of {len(probs)}: qubits {[p[0] for p in probs]}")
+    ctx = nat.Context(0)
+    res = decompose_problems(ctx, [p[:4] for p in probs],
+                             default_options(4, optimization_tolerance=1e-8, mixed_precision=True))
+    for i, (n, gates, U, _, kind) in enumerate(probs):
+        f_check = oracle.optimization_problem(gates, U, n, res["x"][i])
+        assert abs(f_check - res["f"][i]) < 1e-12, (i, f_check, res["f"][i])
+    reach = np.array([p[4] == "reachable" for p in probs])
+    say(f"  reachable success {np.mean(res['f'][reach] < 1e-8):.2f}, haar {np.mean(res['f'][~reach] < 1e-8):.2f}")
+    assert np.mean(res["f"][reach] < 1e-8) > 0
+
+
+def test_decompose_problems_rejects_bad_problem():
+    from squander_hip import decompose_problems
+    ctx = nat.Context(0)
+    gates = W.ansatz_layers(3, [0, 1])
+    P = W.param_count(gates)
+    bad = np.array([[W.GATE_U3, 5, -1, 0]], dtype=np.int32)  # qubit 5 of a 3-qubit problem
+    with pytest.raises(nat.SqhError):
+        decompose_problems(ctx, [(3, gates, np.eye(8), np.zeros(P)), (3, bad, np.eye(8), np.zeros(3))],
+                           default_options(3))
+    with pytest.raises(ValueError):
+        decompose_problems(ctx, [(3, gates, np.eye(4), np.zeros(P))], default_options(3))
+
+
+@pytest.mark.parametrize("n,layers,seed", [(2, 2, 1), (3, 4, 2)])
+def test_agents_matches_oracle(oracle, n, layers, seed):
+    """AGENTS with the batched GPU cost follows the oracle's AGENTS from the same
+    generator seed and guess: same iteration count, same minimum, same parameters."""
+    from squander_hip import agents, agents_options
+    say(f"AGENTS n={n} layers={layers} seed={seed}")
+    gates = W.ansatz_layers(n, W.alternating_pairs(n, layers))
+    P = W.param_count(gates)
+    T = reachable_targets(n, gates, 1, 500 + seed)[0]
+    x0 = np.random.default_rng(seed + 100).uniform(0, 2 * np.pi, P)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(T)
+    fg, xg, itg, evg = agents(ctx, x0, seed, agents_options(max_iterations=3000, optimization_tolerance=1e-8))
+    fo, xo, ito, evo = oracle.agents(gates, T, n, seed, x0, agent_num=64, max_iterations=3000, lifetime=1000,
+                                     tolerance=1e-8)
+    say(f"  gpu f={fg:.3e} it={itg} evals={evg}; oracle f={fo:.3e} it={ito} evals={evo}")
+    assert abs(oracle.optimization_problem(gates, T, n, xg) - fg) < 1e-10
+    assert itg == ito and evg == evo
+    assert abs(fg - fo) < 1e-10
+    d = np.mod(xg - xo + np.pi, 2 * np.pi) - np.pi
+    assert np.max(np.abs(d)) < 1e-6
+
+
+def test_agents_rejects_other_costs():
+    from squander_hip import agents
+    gates = W.ansatz_layers(2, [0])
+    ctx = nat.Context(0)
+    ctx.set_circuit(2, gates)
+    ctx.set_target(np.eye(4))
+    ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT)
+    with pytest.raises(nat.SqhError):
+        agents(ctx, np.zeros(W.param_count(gates)), 1)
