@@ -73,6 +73,152 @@ def cpu_baseline(gates, U, n, P, seconds=10.0):
                       f"{dt:.1f} s)"}
 
 
+def ten_qubit_leg(steps, o):
+    """10 qubits, U3 layer + 60 random adjacent CNOT layers (390 angles), 1024 x 1024
+    complex128 Haar target, batch of 64: complex128 and complex64 cost evaluations."""
+    import numpy as np
+    import torch
+
+    from squander_hip import PREC_F32, PREC_F64, Context
+    from squander_hip import workloads as W
+    n, d, B = 10, 1024, 64
+    gates = W.ansatz_10q_60(2027)
+    P = W.param_count(gates)
+    U = W.haar_unitary(d, 2027).conj().T.copy()
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    params = torch.from_numpy(W.random_params(P, 8, batch=B)).cuda()
+    costs = torch.empty(B, dtype=torch.float64, device="cuda")
+    flops, _ = gate_work(gates, d, d, 16)
+    res = {"workload": "10 qubits, 60 CNOT layers, 390 angles, batch 64", "algorithmic_flops_per_eval": flops}
+    ref = o.optimization_problem_batched(gates, U, n, params[:2].cpu().numpy(), nthreads=2)
+    for name, prec in (("c128", PREC_F64), ("c64", PREC_F32)):
+        step = lambda: ctx.check(ctx._L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, prec,
+                                                                          costs.data_ptr()))
+        step()
+        ctx.synchronize()
+        ctx.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        k = max(2, steps // 4)
+        for _ in range(k):
+            step()
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        kms, nl = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+        peak = FP64_PEAK_TFLOPS if prec == PREC_F64 else 157.3
+        tf = flops * B / (kms / max(nl, 1) / 1e3) / 1e12
+        res[name] = {"evals_per_s": B * k / dt, "kernel_avg_ms": kms / max(nl, 1), "achieved_TFLOPs": tf,
+                     "peak_TFLOPs": peak, "frac": tf / peak,
+                     "err_vs_fp64_oracle": float(np.abs(costs[:2].cpu().numpy() - ref).max())}
+        log(f"10q {name}: {res[name]['evals_per_s']:.0f} evals/s, {tf:.1f} TFLOP/s")
+    return res
+
+
+def derivate_leg(gates, P, U):
+    """apply_derivate_to of the 8-qubit ansatz on a 256 x 256 matrix: 264 derivative
+    matrices of 1 MiB written to HBM per call."""
+    import torch
+
+    from squander_hip import Context
+    from squander_hip import workloads as W
+    d = 256
+    ctx = Context(0)
+    ctx.set_circuit(8, gates)
+    p = torch.from_numpy(W.random_params(P, 3)).cuda()
+    M = torch.from_numpy(U.copy()).cuda()
+    out = torch.empty((P, d, d), dtype=torch.complex128, device="cuda")
+    call = lambda: ctx.check(ctx._L.sqh_apply_derivate_to_dev(ctx.h, p.data_ptr(), M.data_ptr(), d, d, d,
+                                                              out.data_ptr()))
+    call()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        call()
+    ctx.synchronize()
+    dt = (time.perf_counter() - t0) / 5
+    kms, nl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    wbytes = P * d * d * 16
+    k_s = kms / max(nl, 1) / 1e3
+    flops_eval, _ = gate_work(gates, d, d, 16)
+    # every derivative matrix is one replay of the circuit with one gate replaced by
+    # its derivative kernel: P x (forward flops) per call, against 264 MiB written
+    flops = P * flops_eval
+    return {"ms": dt * 1e3, "kernel_avg_ms": k_s * 1e3, "bound": "mfma (FP64)",
+            "algorithmic_flops": flops, "achieved_TFLOPs": flops / k_s / 1e12,
+            "frac_of_fp64_peak": flops / k_s / 1e12 / FP64_PEAK_TFLOPS, "bytes_written": wbytes,
+            "write_GBs": wbytes / k_s / 1e9, "kernel": "k_tile_deriv<8,double>"}
+
+
+def uneven_leg():
+    from squander_hip import uneven_bench
+    res, _, _ = uneven_bench.run(count=256, seed=2028, tol=1e-8, mixed=True, log=log)
+    res["workload"] = ("256 decompositions, qubits uniform 3..6, U3 layer + 4..20 random adjacent CNOT layers, "
+                       "half Haar / half ansatz-reachable targets, complex64 descent + FP64 acceptance")
+    return res
+
+
+def solves_leg():
+    from squander_hip import solve_bench
+    res = solve_bench.run(n_train=200_000, n_reach=1024, n_haar=128, epochs=20, tol=1e-8, n_plain=64, log=log)
+    res["workload"] = ("4 qubits, U3 layer + 12 alternating CNOT layers (84 angles); warm-start network trained on "
+                       "200k generated pairs for 20 epochs (full-size evaluation: 4096 reachable + 1024 Haar)")
+    return res
+
+
+def c1_leg(o, seconds=10.0):
+    """The plain decomposition of 3-qubit targets on the host cores (the oracle's
+    restated BFGS driver, one target per thread) beside the GPU's batched driver."""
+    import numpy as np
+    from concurrent.futures import ThreadPoolExecutor
+
+    from squander_hip import Context, decompose_batch, default_options, initial_guess
+    from squander_hip import workloads as W
+    n = 3
+    gates = W.ansatz_3q(12)
+    P = W.param_count(gates)
+    # half Haar targets (a generic 3-qubit unitary needs >= 14 CNOTs, so these rarely
+    # solve at 12), half reachable targets built from seeded angles
+    Ts = [W.haar_unitary(8, 5000 + i).conj().T.copy() for i in range(128)]
+    Ts += [o.apply_to(gates, W.random_params(P, 6000 + i), np.eye(8)).conj().T.copy() for i in range(128)]
+    kind = np.array([0] * 128 + [1] * 128)
+    threads = min(os.cpu_count() or 1, 64)
+    order = np.random.default_rng(0).permutation(len(Ts))  # mix Haar and reachable in the time-bounded sample
+    done, ok = 0, {0: 0, 1: 0}
+    seen = {0: 0, 1: 0}
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        while time.perf_counter() - t0 < seconds and done < len(Ts):
+            batch = [int(i) for i in order[done:min(done + threads, len(Ts))]]
+            fs = list(ex.map(lambda i: o.decompose_custom(gates, Ts[i], n, 11 + i, optimization_tolerance=1e-8)[0],
+                             batch))
+            for i, f in zip(batch, fs):
+                seen[kind[i]] += 1
+                ok[kind[i]] += int(f <= 1e-8)
+            done += len(batch)
+    cpu_dt = time.perf_counter() - t0
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    x0 = np.array([initial_guess(P, 11 + i) for i in range(len(Ts))])
+    t0 = time.perf_counter()
+    res = decompose_batch(ctx, np.array(Ts), np.arange(len(Ts), dtype=np.int32), x0,
+                          default_options(n, optimization_tolerance=1e-8))
+    gpu_dt = time.perf_counter() - t0
+    return {"workload": "3-qubit Haar targets, U3 layer + 12 alternating CNOT layers, random starts, cost <= 1e-8",
+            "cpu": {"targets": done, "seconds": cpu_dt, "decompositions_per_s": done / cpu_dt,
+                    "solves_per_s": (ok[0] + ok[1]) / cpu_dt,
+                    "success_reachable": ok[1] / max(seen[1], 1), "success_haar": ok[0] / max(seen[0], 1),
+                    "cores": threads, "kind": "port"},
+            "gpu_batched": {"targets": len(Ts), "seconds": gpu_dt, "decompositions_per_s": len(Ts) / gpu_dt,
+                            "solves_per_s": float(np.sum(res["f"] <= 1e-8)) / gpu_dt,
+                            "success_reachable": float(np.mean(res["f"][kind == 1] <= 1e-8)),
+                            "success_haar": float(np.mean(res["f"][kind == 0] <= 1e-8)),
+                            "same_start_points_as_cpu": True}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +227,7 @@ def main():
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-main", action="store_true", help="time the headline step only (for profiling)")
+    ap.add_argument("--skip-solves", action="store_true", help="skip the decomposition legs (4q solves, uneven batch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -266,9 +413,15 @@ def main():
     out["grad_8q_single"] = {"ms": (t1 - t0) / 20 * 1e3, "err_grad_vs_oracle": float(np.abs(g0.cpu().numpy() - gr).max()),
                              "err_cost_vs_oracle": float(abs(f0.item() - fr))}
 
+    out["ten_qubit_b64"] = ten_qubit_leg(args.steps, o)
+    out["apply_derivate_8q"] = derivate_leg(gates, P, U)
+    if not args.skip_solves:
+        out["uneven_256"] = uneven_leg()
+        out["solves_4q"] = solves_leg()
     if not args.skip_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(gates, U, n, P)
+        out["c1_3q_plain_solves"] = c1_leg(o)
     print(json.dumps(out), flush=True)
 
 

@@ -108,6 +108,8 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
         # start quality: FP64 cost at the predicted and at the random start points
         c_pred = cb.verify(x_pred, V)
         c_rand = cb.verify(x_rand, V)
+        c_pred_pf = cb.verify(x_pred, V, phase_free=True)
+        c_rand_pf = cb.verify(x_rand, V, phase_free=True)
         r = {"targets": n,
              "plain_single": _leg(res_s, t_s, tol, {"targets_measured": k}),
              "batched_fp64": _leg(res_b, t_b, tol),
@@ -117,7 +119,9 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
                                "evals_p50": float(np.median(res_l["evals"])),
                                "evals_p99": float(np.percentile(res_l["evals"], 99)),
                                "predict_s": t_pred, "fallback_runs": int(miss.size), "fallback_s": t_fb},
-             "start_cost_median": {"predicted": float(np.median(c_pred)), "random": float(np.median(c_rand))}}
+             "start_cost_median": {"predicted": float(np.median(c_pred)), "random": float(np.median(c_rand)),
+                                   "predicted_phase_free": float(np.median(c_pred_pf)),
+                                   "random_phase_free": float(np.median(c_rand_pf))}}
         r["speedup_vs_plain_single"] = r["learned_mixed"]["solves_per_s"] / max(r["plain_single"]["solves_per_s"], 1e-30)
         r["speedup_vs_batched_fp64"] = r["learned_mixed"]["solves_per_s"] / max(r["batched_fp64"]["solves_per_s"], 1e-30)
         r["max_fp64_cost_of_learned_solves"] = (float(np.max(cb.verify(res_l["x"][ok_l], V[ok_l])))
@@ -126,7 +130,8 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
             f"batched fp64 {r['batched_fp64']['success']:.3f} @ {r['batched_fp64']['solves_per_s']:.0f}/s; "
             f"learned+mixed {r['learned_mixed']['success']:.3f} (pred {r['learned_mixed']['success_from_prediction']:.3f}) "
             f"@ {r['learned_mixed']['solves_per_s']:.0f}/s; start cost pred {r['start_cost_median']['predicted']:.3f} "
-            f"rand {r['start_cost_median']['random']:.3f}")
+            f"rand {r['start_cost_median']['random']:.3f} (phase-free pred "
+            f"{r['start_cost_median']['predicted_phase_free']:.3f} rand {r['start_cost_median']['random_phase_free']:.3f})")
         results[name] = r
     out["results"] = results
     return out

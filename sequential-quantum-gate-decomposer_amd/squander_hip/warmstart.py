@@ -88,6 +88,97 @@ def canonical_angles(gates, theta: np.ndarray) -> np.ndarray:
     return x[0] if single else x
 
 
+def _u3_matrices(t, p, l):
+    """(B, 2, 2) U3 kernels from raw parameters (gate_kernel_templates.h:509-521)."""
+    c, s = np.cos(t), np.sin(t)
+    M = np.empty(t.shape + (2, 2), dtype=np.complex128)
+    M[..., 0, 0] = c
+    M[..., 0, 1] = -np.exp(1j * l) * s
+    M[..., 1, 0] = np.exp(1j * p) * s
+    M[..., 1, 1] = np.exp(1j * (p + l)) * c
+    return M
+
+
+def _u3_params(M):
+    """Raw U3 parameters (t in [0, pi/2]) of e^{-i g} M for the phase g that makes
+    M[0, 0] real and non-negative: every U(2) matrix is a phase times such a U3."""
+    a = M[..., 0, 0]
+    ph = np.where(np.abs(a) > 1e-300, np.conj(a) / np.maximum(np.abs(a), 1e-300), 1.0)
+    g = M * ph[..., None, None]
+    c = np.clip(np.real(g[..., 0, 0]), 0.0, 1.0)
+    s = np.abs(g[..., 1, 0])
+    t = np.arctan2(s, c)
+    p = np.where(s > 1e-12, np.angle(g[..., 1, 0]), 0.0)
+    # lambda from M01 when sin t > 0, else from M11 = e^{i(p+l)} cos t
+    l = np.where(s > 1e-12, np.angle(-g[..., 0, 1]), np.angle(g[..., 1, 1]) - p)
+    return t, np.mod(p, 2 * np.pi), np.mod(l, 2 * np.pi)
+
+
+def _rx(a):
+    c, s = np.cos(a / 2), np.sin(a / 2)
+    R = np.empty(a.shape + (2, 2), dtype=np.complex128)
+    R[..., 0, 0] = c
+    R[..., 0, 1] = -1j * s
+    R[..., 1, 0] = -1j * s
+    R[..., 1, 1] = c
+    return R
+
+
+def projective_canonical_angles(gates, theta: np.ndarray) -> np.ndarray:
+    """Representative of theta's class of circuits equal up to a global phase, for
+    U3 + CNOT circuits: a diagonal on a CNOT's control and an Rx on its target
+    commute with the CNOT, so before every CNOT the control-side U3 is reduced to
+    phi = 0 (its diag(1, e^{i phi}) moves into the next U3 on that qubit) and the
+    target-side U3 M is replaced by Rx(-a) M with a chosen so that the Bloch vector
+    M|0> lies in the x-z half plane z >= 0 (Rx(a) moves into the next U3 on that
+    qubit); every U3 is then re-read with theta in [0, pi/2].  The circuit matrix
+    changes by a global phase only.  Gates other than U3 and CNOT must not occur."""
+    g = np.asarray(gates, dtype=np.int64).reshape(-1, 4)
+    x = np.array(theta, dtype=np.float64, copy=True)
+    single = x.ndim == 1
+    if single:
+        x = x[None, :]
+    if not np.all(np.isin(g[:, 0], (nat.GATE_U3, nat.GATE_CNOT))):
+        raise ValueError("projective canonical form needs a U3 + CNOT circuit")
+    mats = {i: _u3_matrices(x[:, g[i, 3]], x[:, g[i, 3] + 1], x[:, g[i, 3] + 2])
+            for i in range(len(g)) if g[i, 0] == nat.GATE_U3}
+
+    def next_u3(q, start):
+        for j in range(start, len(g)):
+            if g[j, 1] == q or (g[j, 0] == nat.GATE_CNOT and g[j, 2] == q):
+                return j if g[j, 0] == nat.GATE_U3 else None
+        return None
+
+    pending = {}
+    for i in range(len(g)):
+        if g[i, 0] == nat.GATE_U3:
+            pending[int(g[i, 1])] = i
+            continue
+        t, c = int(g[i, 1]), int(g[i, 2])
+        ic, jc = pending.pop(c, None), next_u3(c, i + 1)
+        if ic is not None and jc is not None:
+            tt, pp, ll = _u3_params(mats[ic])
+            mats[ic] = _u3_matrices(tt, np.zeros_like(pp), ll)
+            D = np.zeros(pp.shape + (2, 2), dtype=np.complex128)
+            D[..., 0, 0] = 1.0
+            D[..., 1, 1] = np.exp(1j * pp)
+            mats[jc] = mats[jc] @ D
+        it, jt = pending.pop(t, None), next_u3(t, i + 1)
+        if it is not None and jt is not None:
+            M = mats[it]
+            v0, v1 = M[..., 0, 0], M[..., 1, 0]
+            vy = 2 * np.imag(np.conj(v0) * v1)
+            vz = np.abs(v0) ** 2 - np.abs(v1) ** 2
+            a = np.arctan2(-vy, vz)  # Rx(-a) v has y = 0, z >= 0
+            mats[it] = _rx(-a) @ M
+            mats[jt] = mats[jt] @ _rx(a)
+    for i, M in mats.items():
+        tt, pp, ll = _u3_params(M)
+        ps = g[i, 3]
+        x[:, ps], x[:, ps + 1], x[:, ps + 2] = tt, pp, ll
+    return x[0] if single else x
+
+
 def angles_to_sincos(theta: np.ndarray) -> np.ndarray:
     return np.concatenate([np.sin(theta), np.cos(theta)], axis=1).astype(np.float32)
 
@@ -127,24 +218,34 @@ class CircuitBatch:
             out.append(self.apply(p, np.broadcast_to(np.eye(d, dtype=np.complex128), (p.shape[0], d, d))))
         return np.concatenate(out)
 
-    def verify(self, params: np.ndarray, V: np.ndarray, chunk: int = 16384) -> np.ndarray:
-        """The decomposer's FROBENIUS cost 1 - Re Tr(C(theta) V^dagger)/d of each pair."""
+    def verify(self, params: np.ndarray, V: np.ndarray, chunk: int = 16384, phase_free: bool = False) -> np.ndarray:
+        """The decomposer's FROBENIUS cost 1 - Re Tr(C(theta) V^dagger)/d of each pair
+        (phase_free: 1 - |Tr(C(theta) V^dagger)|/d, blind to a global phase)."""
         d = 1 << self.n
         costs = []
         for s in range(0, params.shape[0], chunk):
             R = self.apply(params[s:s + chunk], np.conj(np.transpose(V[s:s + chunk], (0, 2, 1))))
-            costs.append(1.0 - np.real(np.trace(R, axis1=1, axis2=2)) / d)
+            tr = np.trace(R, axis1=1, axis2=2)
+            costs.append(1.0 - (np.abs(tr) if phase_free else np.real(tr)) / d)
         return np.concatenate(costs)
 
 
 def generate_pairs(cb: CircuitBatch, count: int, seed: int):
     """(features, sin/cos labels, canonical angles, V, verification costs) of count
-    seeded reachable targets V = C(theta*), theta* uniform in [0, 2pi)."""
+    seeded reachable targets V = C(theta*), theta* uniform in [0, 2pi).  For U3 +
+    CNOT circuits the label is the projective canonical form (verified with the
+    phase-blind cost, since the features are phase-normalised too); otherwise the
+    exact canonical form (verified with the FROBENIUS cost)."""
     rng = np.random.default_rng(seed)
     theta = rng.uniform(0.0, 2.0 * np.pi, (count, cb.P))
     V = cb.unitaries(theta)
-    label = canonical_angles(cb.gates, theta)
-    costs = cb.verify(label, V)
+    g = np.asarray(cb.gates).reshape(-1, 4)
+    if np.all(np.isin(g[:, 0], (nat.GATE_U3, nat.GATE_CNOT))):
+        label = projective_canonical_angles(cb.gates, theta)
+        costs = cb.verify(label, V, phase_free=True)
+    else:
+        label = canonical_angles(cb.gates, theta)
+        costs = cb.verify(label, V)
     return target_features(V), angles_to_sincos(label), label, V, costs
 
 
