@@ -73,6 +73,29 @@ def cpu_baseline(gates, U, n, P, seconds=10.0):
                       f"{dt:.1f} s)"}
 
 
+def generic_leg(gates, U, params, costs, steps, flops):
+    """The same batch with the generic tile kernel (one program walked op by op)."""
+    from squander_hip import PREC_F64, Context
+    ctx = Context(0)
+    ctx.set_circuit(8, gates)
+    ctx.set_target(U)
+    B = params.shape[0]
+    step = lambda: ctx.check(ctx._L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64,
+                                                                      costs.data_ptr()))
+    step()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    kms, nl = ctx.kernel_timing()
+    k_s = kms / max(nl, 1) / 1e3
+    return {"evals_per_s": B * steps / dt, "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": flops / k_s / 1e12,
+            "frac": flops / k_s / 1e12 / FP64_PEAK_TFLOPS, "kernel": "k_tile_forward<8,double,1>"}
+
+
 def ten_qubit_leg(steps, o):
     """10 qubits, U3 layer + 60 random adjacent CNOT layers (390 angles), 1024 x 1024
     complex128 Haar target, batch of 64: complex128 and complex64 cost evaluations."""
@@ -88,6 +111,7 @@ def ten_qubit_leg(steps, o):
     ctx = Context(0)
     ctx.set_circuit(n, gates)
     ctx.set_target(U)
+    ctx.set_jit(True)
     params = torch.from_numpy(W.random_params(P, 8, batch=B)).cuda()
     costs = torch.empty(B, dtype=torch.float64, device="cuda")
     flops, _ = gate_work(gates, d, d, 16)
@@ -228,6 +252,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-main", action="store_true", help="time the headline step only (for profiling)")
     ap.add_argument("--skip-solves", action="store_true", help="skip the decomposition legs (4q solves, uneven batch)")
+    ap.add_argument("--generic", action="store_true", help="headline with the generic (not circuit-specialised) kernel")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -255,6 +280,7 @@ def main():
     ctx = Context(0)
     ctx.set_circuit(n, gates)
     ctx.set_target(U)
+    ctx.set_jit(not args.generic)  # circuit-specialised kernels (compiled in the first warm-up step)
     L = ctx._L
     params = torch.from_numpy(W.random_params(P, 7, batch=B)).to(dev)
     costs = torch.empty(B, dtype=torch.float64, device=dev)
@@ -264,9 +290,12 @@ def main():
         ctx.check(L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, prec, costs.data_ptr()))
 
     log(f"warmup {args.warmup} steps")
-    for _ in range(args.warmup):
+    t_w = time.perf_counter()
+    for _ in range(max(args.warmup, 1)):
         step()
     ctx.synchronize()
+    jit_ms, _ = ctx.jit_stats()
+    log(f"warm-up done in {time.perf_counter() - t_w:.1f} s (run-time kernel compilation {jit_ms:.0f} ms)")
     ctx.kernel_timing()
     ctx.set_kernel_timing(True)
     log(f"timing {args.steps} steps of B={B} cost evaluations (FP64)")
@@ -310,10 +339,14 @@ def main():
         "err_vs_fp64_oracle": err64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": TILE_FORWARD_8Q_B512_TRAFFIC if B == 512 else None,
-                     "kernel": "k_tile_forward<8,double,1>", "kernel_avg_ms": k_avg_s * 1e3,
-                     "algorithmic_flops_per_launch": flops_eval * B},
+                     "traffic": TILE_FORWARD_8Q_B512_TRAFFIC if (B == 512 and args.generic) else None,
+                     "kernel": ("k_tile_forward<8,double,1>" if args.generic else
+                                "sqh_jit_kernel (k_tile_forward specialised to the circuit, diagonal mode)"),
+                     "kernel_avg_ms": k_avg_s * 1e3, "algorithmic_flops_per_launch": flops_eval * B},
+        "kernel_compile_ms": jit_ms,
     }
+    if not args.generic and not args.only_main:
+        out["generic_kernel"] = generic_leg(gates, U, params, costs, args.steps, flops_eval * B)
     if args.copy_calibration:
         nbytes = 256 << 20
         src = torch.ones(nbytes, dtype=torch.uint8, device=dev)

@@ -213,6 +213,21 @@ typedef struct sqh_agents_options {
 int sqh_agents(sqh_ctx* ctx, double* x, const sqh_agents_options* opts, unsigned int seed, double* f_out,
                long long* iters_out, long long* evals_out);
 
+/* Run-time specialised tile kernels.  With enable = 1 the context compiles its
+ * circuit, on first use, into straight-line forward and gradient kernels (hiprtc,
+ * cached per circuit) and uses them for apply_to, the batched costs and gradients and
+ * sqh_decompose_batch; results equal the generic kernels' to rounding.  No
+ * counterpart in the reference (which has one generic loop).  sqh_get_jit_stats
+ * reports the compile time spent so far. */
+int sqh_set_jit(sqh_ctx* ctx, int enable);
+int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
+/* Introspection (no GPU needed): generate and compile, for gfx950, the run-time
+ * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient;
+ * uniform = one batch element per wavefront).  Returns SQH_OK when it compiles;
+ * log receives the compiler log. */
+int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32, int uniform,
+                          char* log, int log_cap, long long* code_bytes, long long* source_bytes);
+
 /* ---- warm-start network ----------------------------------------------------- */
 /* A multilayer perceptron in -> hidden (GELU) x n_hidden -> out with BF16 weights
  * and activations, FP32 accumulation on the matrix cores and FP32 master weights,

@@ -155,7 +155,11 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
             t.mpart = c->d_mpart.as<double>();
             {
                 TimedLaunch tl(c);
-                HIPTRY(c, launch_tile_grad(t, c->stream));
+                if (jit_wants(c, 2)) {
+                    if ((rc = jit_launch(c, t, 2, c->stream))) return rc;
+                } else {
+                    HIPTRY(c, launch_tile_grad(t, c->stream));
+                }
             }
             HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, t.mpart, wpb, ns, t.slots, d_params, c->prog.P,
                                     d_grad, c->stream));
@@ -163,7 +167,11 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
             t.mode = 1;
             {
                 TimedLaunch tl(c);
-                HIPTRY(c, launch_tile_forward(t, c->stream));
+                if (jit_wants(c, 1)) {
+                    if ((rc = jit_launch(c, t, 1, c->stream))) return rc;
+                } else {
+                    HIPTRY(c, launch_tile_forward(t, c->stream));
+                }
             }
             HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, nullptr, 0, ns, t.slots, d_params, c->prog.P,
                                     nullptr, c->stream));
@@ -210,6 +218,7 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
         t.B = B;
         t.kern_bstride = kb;
         TimedLaunch tl(c);
+        if (jit_wants(c, 0)) return jit_launch(c, t, 0, c->stream);
         HIPTRY(c, launch_tile_forward(t, c->stream));
         return SQH_OK;
     }
@@ -323,6 +332,7 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
     c->prog = std::move(p);
     c->gates = std::move(g);
     c->have_circuit = true;
+    c->prog_version++;
     c->err.clear();
     return SQH_OK;
 }
@@ -542,6 +552,20 @@ int sqh_optimization_problem_combined_unitary(sqh_ctx* c, const double* params, 
     if (P > 0 && out_derivs)
         HIPTRY(c, hipMemcpyAsync(out_derivs, c->d_out.p, (size_t)P * msz * 16, hipMemcpyDeviceToHost, c->stream));
     HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
+int sqh_set_jit(sqh_ctx* c, int enable) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->jit = enable != 0;
+    return SQH_OK;
+}
+
+int sqh_get_jit_stats(sqh_ctx* c, double* compile_ms, int* compiles) {
+    if (!c || !compile_ms || !compiles) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    *compile_ms = jit_compile_ms(c, compiles);
     return SQH_OK;
 }
 

@@ -96,7 +96,12 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             t.bi.tgt_stride = (long long)gr.rows * gr.cols;
             t.bi.run_prog = gr.run_prog;
             HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns, kern, c->stream, t.bi));
-            HIPTRY(c, launch_tile_grad(t, c->stream));
+            if (!grouped && jit_wants(c, 2)) {
+                const int rc = jit_launch(c, t, 2, c->stream);
+                if (rc) return rc;
+            } else {
+                HIPTRY(c, launch_tile_grad(t, c->stream));
+            }
             HIPTRY(c, launch_finish(t.diag, gr.cols, gr.B, c->variant, bb.fe.as<double>(), t.mpart, gr.wpb, gr.ns,
                                     gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), c->stream, t.bi));
         }
@@ -162,6 +167,14 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
                              : (long long)opts->max_outer_iterations * (opts->max_inner_iterations + 2) * (mixed ? 2 : 1);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    if (!grouped && jit_wants(c, 2)) {
+        // compile the run-time kernels outside the capture (a module load is not a
+        // stream operation)
+        for (int p = 0; p < (mixed ? 2 : 1); ++p) {
+            const int rc = jit_prepare(c, groups[0].N, groups[0].m_pad, 2, p == 1);
+            if (rc) return rc;
+        }
+    }
     HIPTRY(c, hipStreamSynchronize(c->stream));
     HIPTRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     for (int i = 0; i < R; ++i) {

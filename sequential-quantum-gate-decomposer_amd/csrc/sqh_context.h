@@ -42,6 +42,9 @@ struct DevBuf {
     }
 };
 
+struct JitCache;
+void jit_release(sqh_ctx* c);
+
 }  // namespace sqh
 
 struct sqh_ctx {
@@ -51,6 +54,7 @@ struct sqh_ctx {
     std::string err;
 
     bool have_circuit = false;
+    long long prog_version = 0;  // bumped by every sqh_set_circuit (keys the run-time kernels)
     sqh::Program prog;
     std::vector<sqh::GateRec> gates;
     sqh::DevBuf d_ops, d_lays, d_slots;
@@ -67,11 +71,16 @@ struct sqh_ctx {
     sqh::FusionPlan fplan;
     sqh::DevBuf d_fgates, d_ffirst, d_fused;
 
+    // Run-time specialised tile kernels (sqh_set_jit).
+    bool jit = false;
+    sqh::JitCache* jit_cache = nullptr;
+
     // Kernel timers: HIP events around the dominant kernel of each call.
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
 
     ~sqh_ctx() {
+        sqh::jit_release(this);
         for (auto& e : events) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
@@ -112,6 +121,14 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
 int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, int cols, int stride, void* d_out);
 int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
                 long long kern_bstride);
+// Run-time specialised kernel for the current program: kind 0 forward store, 1
+// forward diagonal, 2 gradient (sqh_jit.cpp).
+int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s);
+// Whether the context uses a run-time kernel of this kind for its program.
+bool jit_wants(sqh_ctx* c, int kind);
+// Compiles the kernel jit_launch would use for these shapes (before a graph capture).
+int jit_prepare(sqh_ctx* c, int N, int m_pad, int kind, bool f32);
+double jit_compile_ms(sqh_ctx* c, int* compiles);
 
 hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,
                               int B, const void* F, long long fbstride, const int* qubits, int k, hipStream_t s);

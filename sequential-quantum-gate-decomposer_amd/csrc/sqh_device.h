@@ -13,9 +13,9 @@
 // rows of every matrix in HBM -- the unfused path the reference's pair-row kernel
 // (apply_kernel_to_input.cpp:35-115) describes.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
-
-#include <type_traits>
+#endif
 
 #include "sqh_internal.h"
 

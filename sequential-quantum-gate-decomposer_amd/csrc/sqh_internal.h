@@ -9,7 +9,14 @@
 // register qubit is preceded by a REMAP op that exchanges the slab through LDS into
 // a layout where it is.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <cstdint>
+#else
+using __hip_internal::int32_t;
+using __hip_internal::uint32_t;
+using __hip_internal::uint8_t;
+using __hip_internal::int8_t;
+#endif
 
 namespace sqh {
 
@@ -59,10 +66,6 @@ struct alignas(16) Slot {
     int32_t type, param_start, nparam, op;
 };
 
-// Optional indirection for batched optimisation: launch element j is run
-// elem_run[j], whose target matrix is run_target[run] (targets tgt_stride complex
-// entries apart); only the first *count elements are live.  All null: element j
-// is batch element j on the shared input.
 // Program of one run when the runs of one launch carry different circuits on the
 // same number of qubits: offsets into the launch's concatenated op, layout and slot
 // arrays (each program's ops padded by kOpPad no-ops on both sides).
@@ -70,12 +73,34 @@ struct ProgRef {
     int32_t op_off, nops, lay_off, slot_off, ns, pad0, pad1, pad2;
 };
 
+// Optional indirection for batched optimisation: launch element j is run
+// elem_run[j], whose target matrix is run_target[run] (targets tgt_stride complex
+// entries apart); only the first *count elements are live.  All null: element j
+// is batch element j on the shared input.
 struct BatchIndex {
     const int* count = nullptr;
     const int* elem_run = nullptr;
     const int* run_target = nullptr;
     long long tgt_stride = 0;
     const ProgRef* run_prog = nullptr;  // per run; requires one element per wavefront
+};
+
+// Arguments of a run-time compiled tile kernel (one struct, passed by value).
+struct JitArgs {
+    const void* in;
+    long long in_bstride;
+    int in_stride, m, m_pad, B;
+    void* out;
+    long long out_bstride;
+    int out_stride, trace_off, ns, pad0;
+    const void* kern;
+    long long kern_bstride;
+    double* diag;
+    double* mpart;
+    const int* count;
+    const int* elem_run;
+    const int* run_target;
+    long long tgt_stride;
 };
 
 constexpr int kMaxTileQubits = 10;   // one column fits one wavefront up to 2^10 rows

@@ -1,0 +1,440 @@
+// sqh_jit.cpp -- run-time specialised tile kernels.
+//
+// The generic tile kernels walk the compiled program op by op: every gate goes
+// through a dispatch on (kind, target), its control mask is a run-time value, and the
+// switch arms merge back into one register assignment, which costs register copies
+// and per-pair selects on top of the 2x2 arithmetic.  For a circuit that is evaluated
+// many times (optimiser loops, batched costs) the program is fixed, so this file
+// emits it as straight-line HIP: each op becomes a call of the same device helpers
+// with its target, control mask and layout offsets as literals, hiprtc compiles it
+// for the device (with the library's own code-generation flags), and the module is
+// cached per context and program.  Register-controlled CNOTs become register
+// renaming, kernel-table loads get constant offsets, and nothing is dispatched.
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "sqh_context.h"
+
+namespace sqh {
+
+extern const char* const kJitHeaderInternal;  // sqh_internal.h (generated at build time)
+extern const char* const kJitHeaderDevice;    // sqh_device.h
+
+namespace {
+
+int geo_k(int N) { return N < 4 ? N : 4; }
+
+// Register mask of a control (same rule as ctrl_mask on the device), as source text.
+std::string mask_expr(int N, const Op& op) {
+    const int NR = 1 << geo_k(N);
+    const unsigned all = NR >= 32 ? 0xffffffffu : ((1u << NR) - 1u);
+    std::ostringstream o;
+    if (op.cmode == CM_NONE) {
+        o << all << "u";
+    } else if (op.cmode == CM_LANE) {
+        o << "(((s >> " << int(op.c) << ") & 1) ? " << all << "u : 0u)";
+    } else {
+        unsigned m = 0;
+        for (int i = 0; i < NR; ++i) m |= (unsigned)((i >> op.c) & 1) << i;
+        o << m << "u";
+    }
+    return o.str();
+}
+
+std::string sub_expr(int N, const Layout& l) {
+    const int nsub = N - geo_k(N);
+    if (nsub == 0) return "0";
+    std::ostringstream o;
+    for (int j = 0; j < nsub; ++j) {
+        if (j) o << " | ";
+        o << "(((s >> " << j << ") & 1) << " << l.sub_pos[j] << ")";
+    }
+    return o.str();
+}
+
+struct Gen {
+    const Program& pr;
+    int N, NR;
+    std::ostringstream o;
+    explicit Gen(const Program& p) : pr(p), N(p.N), NR(1 << geo_k(p.N)) {}
+
+    // Kernel-table loads run one parametric op ahead of their use: op p's kernel is
+    // loaded into k<p> while the previous parametric op computes, and a scheduling
+    // barrier after every op keeps the compiler from hoisting all loads to the top
+    // (which runs out of scalar registers and spills them).
+    std::vector<int> order_fwd() const {
+        std::vector<int> v;
+        for (int p = 0; p < (int)pr.ops.size(); ++p)
+            if (pr.ops[p].kind == OP_U || pr.ops[p].kind == OP_CU) v.push_back(p);
+        return v;
+    }
+    void declare_k(int p) { o << "  cpx<R> k" << p << "[4]; load_k(kb + " << 4 * pr.ops[p].idx << ", k" << p << ");\n"; }
+    void declare_b(int p) {
+        o << "  cpx<R> kb" << p << "[4]; load_k(kb + " << 4 * pr.ops[p].idx << ", kb" << p << ");\n";
+    }
+    void barrier() { o << "  __builtin_amdgcn_sched_barrier(0);\n"; }
+
+    void remap(const char* v, int from, int to) {
+        if (N <= geo_k(N)) return;
+        const Layout& a = pr.layouts[from];
+        const Layout& b = pr.layouts[to];
+        o << "  { const int so = " << sub_expr(N, a) << ", sn = " << sub_expr(N, b) << ";\n";
+        for (int i = 0; i < NR; ++i)
+            o << "    lds[lds_idx<N>(so + " << a.reg_off[i] << ", cc)] = " << v << "[" << i << "];\n";
+        o << "    __syncthreads();\n";
+        for (int i = 0; i < NR; ++i)
+            o << "    " << v << "[" << i << "] = lds[lds_idx<N>(sn + " << b.reg_off[i] << ", cc)];\n";
+        o << "    __syncthreads(); }\n";
+    }
+
+    // Forward execution of op p (not a remap) on tile v; a parametric op uses k<p>.
+    void gate(const char* v, int p) {
+        const Op& op = pr.ops[p];
+        const int t = op.t;
+        switch (op.kind) {
+        case OP_U:
+            o << "  pairs_apply<" << t << ", false, false>(" << v << ", k" << p << ", 0u);\n";
+            break;
+        case OP_CU:
+            o << "  pairs_apply<" << t << ", true, false>(" << v << ", k" << p << ", " << mask_expr(N, op) << ");\n";
+            break;
+        case OP_X:
+            o << "  pairs_swap<" << t << ">(" << v << ", " << mask_expr(N, op) << ");\n";
+            break;
+        case OP_Z:
+            o << "  pairs_negate<" << t << ">(" << v << ", " << mask_expr(N, op) << ");\n";
+            break;
+        default:
+            break;
+        }
+    }
+
+    int forward(const char* v) {
+        int cur = 0;
+        const std::vector<int> par = order_fwd();
+        size_t next = 0;  // next parametric op whose kernel is not loaded yet
+        if (next < par.size()) declare_k(par[next++]);
+        for (int p = 0; p < (int)pr.ops.size(); ++p) {
+            const Op& op = pr.ops[p];
+            if (op.kind == OP_REMAP) {
+                const int a = op.idx >> 16, b = op.idx & 0xffff;
+                remap(v, a, b);
+                cur = b;
+            } else {
+                if ((op.kind == OP_U || op.kind == OP_CU) && next < par.size()) declare_k(par[next++]);
+                gate(v, p);
+            }
+            barrier();
+        }
+        return cur;
+    }
+
+    void load(const char* v, const char* src, const char* stride) {
+        const Layout& l0 = pr.layouts[0];
+        o << "  { const int so = " << sub_expr(N, l0) << ";\n";
+        for (int i = 0; i < NR; ++i)
+            o << "    " << v << "[" << i << "] = li.valid ? " << src << "[(long long)(so + " << l0.reg_off[i] << ") * "
+              << stride << " + li.col] : mk<R>(0, 0);\n";
+        o << "  }\n";
+    }
+
+    void prologue(bool uni, bool f32, const char* name) {
+        o << "#include \"sqh_device.h\"\nusing namespace sqh;\ntypedef " << (f32 ? "float" : "double")
+          << " R;\nconstexpr int N = " << N << ";\ntypedef Geo<N> G;\n"
+          << "extern \"C\" __global__ __launch_bounds__(64) void " << name << "(JitArgs a) {\n"
+          << "  __shared__ cpx<R> lds[G::LDS];\n"
+          << "  int B = a.B;\n"
+          << "  if (a.count) { B = *a.count; if (B <= 0) return; }\n"
+          << "  if ((long long)blockIdx.x * G::CW / a.m_pad >= B) return;\n"
+          << "  const LaneInfo li = lane_info<N, " << (uni ? "true" : "false")
+          << ">(a.m, a.m_pad, B, threadIdx.x, blockIdx.x);\n"
+          << "  const int s = li.s, cc = li.cc;\n"
+          << "  (void)s; (void)cc; (void)lds;\n"
+          << "  const cpx<R>* in = (const cpx<R>*)a.in + (long long)li.b * a.in_bstride;\n"
+          << "  if (a.count) in = (const cpx<R>*)a.in + (long long)a.run_target[a.elem_run[li.b]] * a.tgt_stride;\n"
+          << "  const cpx<R>* kb = (const cpx<R>*)a.kern + (long long)li.b * a.kern_bstride;\n";
+    }
+
+    // Straight-line k_tile_forward (mode 0: store; 1: diagonal entries).
+    std::string forward_kernel(bool uni, bool f32, int mode) {
+        prologue(uni, f32, "sqh_jit_kernel");
+        o << "  cpx<R> v[G::NR];\n";
+        load("v", "in", "a.in_stride");
+        const int cur = forward("v");
+        const Layout& lf = pr.layouts[cur];
+        o << "  const int sf = " << sub_expr(N, lf) << ";\n";
+        if (mode == 0) {
+            o << "  cpx<R>* dst = (cpx<R>*)a.out + (long long)li.b * a.out_bstride;\n";
+            for (int i = 0; i < NR; ++i)
+                o << "  if (li.valid) dst[(long long)(sf + " << lf.reg_off[i] << ") * a.out_stride + li.col] = v[" << i
+                  << "];\n";
+        } else {
+            o << "  const int tr = li.col + a.trace_off;\n";
+            for (int i = 0; i < NR; ++i)
+                o << "  if (li.valid && sf + " << lf.reg_off[i] << " == tr) { a.diag[2 * ((long long)li.b * a.m + li.col)] = "
+                  << "(double)v[" << i << "].re; a.diag[2 * ((long long)li.b * a.m + li.col) + 1] = (double)v[" << i
+                  << "].im; }\n";
+        }
+        o << "}\n";
+        return o.str();
+    }
+
+    // Straight-line k_tile_grad: forward, diagonal and selector, adjoint sweep.
+    std::string grad_kernel(bool uni, bool f32) {
+        prologue(uni, f32, "sqh_jit_kernel");
+        o << "  cpx<R> x[G::NR], y[G::NR];\n";
+        load("x", "in", "a.in_stride");
+        const int cur = forward("x");
+        const Layout& lf = pr.layouts[cur];
+        o << "  { const int sf = " << sub_expr(N, lf) << ";\n    const int tr = li.col + a.trace_off;\n";
+        for (int i = 0; i < NR; ++i) {
+            o << "    { const bool hit = li.valid && sf + " << lf.reg_off[i] << " == tr;\n"
+              << "      if (hit) { a.diag[2 * ((long long)li.b * a.m + li.col)] = (double)x[" << i
+              << "].re; a.diag[2 * ((long long)li.b * a.m + li.col) + 1] = (double)x[" << i << "].im; }\n"
+              << "      y[" << i << "] = mk<R>(hit ? (R)1 : (R)0, (R)0); }\n";
+        }
+        o << "  }\n"
+          << "  const int lim = a.m_pad < G::CW ? a.m_pad : G::CW;\n"
+          << "  const bool leader = li.bvalid && (li.s == 0) && (li.cc % lim == 0);\n"
+          << "  const int wpb = a.m_pad >= G::CW ? a.m_pad / G::CW : 1;\n"
+          << "  const long long item0 = (long long)blockIdx.x * G::CW + li.cc;\n"
+          << "  const int wib = a.m_pad >= G::CW ? (int)((item0 % a.m_pad) / G::CW) : 0;\n"
+          << "  double* mp = a.mpart + ((long long)li.b * wpb + wib) * (long long)a.ns * 8;\n";
+        std::vector<int> par = order_fwd();
+        std::reverse(par.begin(), par.end());
+        size_t next = 0;
+        if (next < par.size()) {
+            o << "  {\n";
+            declare_b(par[next++]);
+        }
+        for (int p = (int)pr.ops.size() - 1; p >= 0; --p) {
+            const Op& op = pr.ops[p];
+            if (op.kind == OP_REMAP) {
+                const int oa = op.idx >> 16, nb = op.idx & 0xffff;
+                remap("x", nb, oa);
+                remap("y", nb, oa);
+            } else if (op.kind == OP_X || op.kind == OP_Z) {
+                gate("x", p);
+                gate("y", p);
+            } else if (op.kind == OP_U || op.kind == OP_CU) {
+                if (next < par.size()) declare_b(par[next++]);
+                const bool masked = op.kind == OP_CU;
+                const std::string mk = masked ? mask_expr(N, op) : std::string("0u");
+                const char* mb = masked ? "true" : "false";
+                o << "  { const cpx<R>* k = kb" << p << ";\n"
+                  << "    cpx<R> kd[4] = {mk(k[0].re, -k[0].im), mk(k[2].re, -k[2].im), mk(k[1].re, -k[1].im), "
+                     "mk(k[3].re, -k[3].im)};\n"
+                  << "    cpx<R> kt[4] = {k[0], k[2], k[1], k[3]};\n"
+                  << "    cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};\n"
+                  << "    const unsigned msk = " << mk << ";\n"
+                  << "    pairs_apply<" << int(op.t) << ", " << mb << ", false>(x, kd, msk);\n"
+                  << "    pairs_corr<" << int(op.t) << ">(y, x, acc, msk, " << mb << ");\n"
+                  << "    pairs_apply<" << int(op.t) << ", " << mb << ", false>(y, kt, msk);\n"
+                  << "    double r[8];\n"
+                  << "    for (int e = 0; e < 4; ++e) { r[2 * e] = seg_sum<G::CW>((double)acc[e].re, lim); "
+                     "r[2 * e + 1] = seg_sum<G::CW>((double)acc[e].im, lim); }\n"
+                  << "    if (leader) { double* d = mp + " << 8 * op.idx << "; for (int e = 0; e < 8; ++e) d[e] = r[e]; }\n"
+                  << "  }\n";
+            }
+            barrier();
+        }
+        if (!par.empty()) o << "  }\n";
+        o << "}\n";
+        return o.str();
+    }
+};
+
+}  // namespace
+
+// One compiled kernel of the cache.
+struct JitEntry {
+    long long prog_version;
+    int kind;  // 0: forward store, 1: forward diagonal, 2: gradient
+    bool f32, uni;
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+};
+
+struct JitCache {
+    std::vector<JitEntry> entries;
+    double compile_ms = 0.0;
+    int compiles = 0;
+    ~JitCache() {
+        for (auto& e : entries)
+            if (e.mod) (void)hipModuleUnload(e.mod);
+    }
+};
+
+void jit_release(sqh_ctx* c) {
+    delete c->jit_cache;
+    c->jit_cache = nullptr;
+}
+
+namespace {
+
+std::string jit_source(const Program& pr, int kind, bool f32, bool uni) {
+    Gen g(pr);
+    return kind == 2 ? g.grad_kernel(uni, f32) : g.forward_kernel(uni, f32, kind);
+}
+
+// hiprtc compile of a generated source for arch; code object in code, log on failure.
+bool jit_compile(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& log) {
+    hiprtcProgram p = nullptr;
+    const char* hs[] = {kJitHeaderInternal, kJitHeaderDevice};
+    const char* hn[] = {"sqh_internal.h", "sqh_device.h"};
+    if (hiprtcCreateProgram(&p, src.c_str(), "sqh_jit.hip", 2, hs, hn) != HIPRTC_SUCCESS) {
+        log = "hiprtcCreateProgram failed";
+        return false;
+    }
+    const std::string a = "--offload-arch=" + arch;
+    // the library's own code-generation flags (build_lib.py)
+    const char* opts[] = {a.c_str(), "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops", "-mllvm", "-simplifycfg-sink-common=false"};
+    const hiprtcResult rr = hiprtcCompileProgram(p, 9, opts);
+    if (rr != HIPRTC_SUCCESS) {
+        size_t ls = 0;
+        (void)hiprtcGetProgramLogSize(p, &ls);
+        log.assign(ls + 1, '\0');
+        (void)hiprtcGetProgramLog(p, &log[0]);
+        (void)hiprtcDestroyProgram(&p);
+        return false;
+    }
+    size_t cs = 0;
+    (void)hiprtcGetCodeSize(p, &cs);
+    code.resize(cs);
+    (void)hiprtcGetCode(p, code.data());
+    (void)hiprtcDestroyProgram(&p);
+    return true;
+}
+
+}  // namespace
+
+// Compiles (once) and returns the kernel for the current program; nullptr with the
+// context's error set on failure.
+static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni) {
+    if (!c->jit_cache) c->jit_cache = new JitCache();
+    JitCache& jc = *c->jit_cache;
+    for (auto& e : jc.entries)
+        if (e.prog_version == c->prog_version && e.kind == kind && e.f32 == f32 && e.uni == uni) return e.fn;
+    const auto t0 = std::chrono::steady_clock::now();
+    hipDeviceProp_t prop;
+    std::string arch = "gfx950";
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess) {
+        std::string name = prop.gcnArchName;
+        arch = name.substr(0, name.find(':'));
+    }
+    std::vector<char> code;
+    std::string log;
+    if (!jit_compile(jit_source(c->prog, kind, f32, uni), arch, code, log)) {
+        fail(c, SQH_ERR_DEVICE, "run-time kernel compilation failed: %.400s", log.c_str());
+        return nullptr;
+    }
+    JitEntry e;
+    e.prog_version = c->prog_version;
+    e.kind = kind;
+    e.f32 = f32;
+    e.uni = uni;
+    hipError_t he = hipModuleLoadData(&e.mod, code.data());
+    if (he == hipSuccess) he = hipModuleGetFunction(&e.fn, e.mod, "sqh_jit_kernel");
+    if (he != hipSuccess) {
+        if (e.mod) (void)hipModuleUnload(e.mod);
+        fail(c, SQH_ERR_DEVICE, "loading the run-time kernel: %s", hipGetErrorString(he));
+        return nullptr;
+    }
+    jc.entries.push_back(e);
+    jc.compiles++;
+    jc.compile_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return e.fn;
+}
+
+int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
+    const int cw = tile_columns_per_wave(t.N);
+    const bool uni = t.N > 4 && t.m_pad % cw == 0;
+    if (t.bi.run_prog) return fail(c, SQH_ERR_ARG, "run-time kernels serve one circuit");
+    hipFunction_t fn = jit_get(c, kind, t.f32, uni);
+    if (!fn) return SQH_ERR_DEVICE;
+    JitArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.in = t.in;
+    a.in_bstride = t.in_bstride;
+    a.in_stride = t.in_stride;
+    a.m = t.m;
+    a.m_pad = t.m_pad;
+    a.B = t.B;
+    a.out = t.out;
+    a.out_bstride = t.out_bstride;
+    a.out_stride = t.out_stride;
+    a.trace_off = t.trace_off;
+    a.ns = t.ns;
+    a.kern = t.kern;
+    a.kern_bstride = t.kern_bstride;
+    a.diag = t.diag;
+    a.mpart = t.mpart;
+    a.count = t.bi.count;
+    a.elem_run = t.bi.elem_run;
+    a.run_target = t.bi.run_target;
+    a.tgt_stride = t.bi.tgt_stride;
+    const long long items = (long long)t.B * t.m_pad;
+    const unsigned grid = (unsigned)((items + cw - 1) / cw);
+    if (grid == 0) return SQH_OK;
+    void* params[] = {&a};
+    HIPTRY(c, hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, 0, s, params, nullptr));
+    return SQH_OK;
+}
+
+// Gradient kernels of long programs compile slowly (the straight-line adjoint sweep
+// of the 128-gate 8-qubit circuit takes about a minute); those keep the generic kernel.
+constexpr size_t kJitMaxGradOps = 96;
+
+bool jit_wants(sqh_ctx* c, int kind) {
+    if (!c->jit || c->prog.N > kMaxTileQubits) return false;
+    return kind != 2 || c->prog.ops.size() <= kJitMaxGradOps;
+}
+
+int jit_prepare(sqh_ctx* c, int N, int m_pad, int kind, bool f32) {
+    const int cw = tile_columns_per_wave(N);
+    const bool uni = N > 4 && m_pad % cw == 0;
+    return jit_get(c, kind, f32, uni) ? SQH_OK : SQH_ERR_DEVICE;
+}
+
+double jit_compile_ms(sqh_ctx* c, int* compiles) {
+    if (!c->jit_cache) {
+        if (compiles) *compiles = 0;
+        return 0.0;
+    }
+    if (compiles) *compiles = c->jit_cache->compiles;
+    return c->jit_cache->compile_ms;
+}
+
+}  // namespace sqh
+
+using namespace sqh;
+
+extern "C" int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32,
+                                     int uniform, char* log, int log_cap, long long* code_bytes,
+                                     long long* source_bytes) {
+    if (n_gates < 0 || (n_gates > 0 && !gates) || kind < 0 || kind > 2) return SQH_ERR_ARG;
+    std::vector<GateRec> g(n_gates);
+    for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
+    Program pr;
+    std::string err;
+    if (!compile_program(qbit_num, g, pr, err) || pr.N > kMaxTileQubits) {
+        if (log && log_cap > 0) std::snprintf(log, (size_t)log_cap, "%s", err.c_str());
+        return SQH_ERR_GATE;
+    }
+    const std::string src = jit_source(pr, kind, is_f32 != 0, uniform != 0);
+    if (source_bytes) *source_bytes = (long long)src.size();
+    std::vector<char> code;
+    std::string lg;
+    const bool ok = jit_compile(src, "gfx950", code, lg);
+    if (code_bytes) *code_bytes = (long long)code.size();
+    if (log && log_cap > 0) std::snprintf(log, (size_t)log_cap, "%s", lg.c_str());
+    return ok ? SQH_OK : SQH_ERR_DEVICE;
+}

@@ -40,6 +40,7 @@ EXPORTS = [
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents",
+    "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check",
 ]
 
 
@@ -142,6 +143,10 @@ def lib():
                                         vp, ctypes.POINTER(ll)], i),
             "sqh_agents": ([vp, vp, ctypes.POINTER(AgentsOptions), ctypes.c_uint, d, ctypes.POINTER(ll),
                             ctypes.POINTER(ll)], i),
+            "sqh_set_jit": ([vp, i], i),
+            "sqh_jit_compile_check": ([i, ctypes.POINTER(_Gate), i, i, i, i, ctypes.c_char_p, i,
+                                       ctypes.POINTER(ll), ctypes.POINTER(ll)], i),
+            "sqh_get_jit_stats": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_program": ([i, ctypes.POINTER(_Gate), i, vp, i, vp, i, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)], i),
@@ -231,6 +236,15 @@ class Context:
     def set_cost_function(self, variant):
         self.check(self._L.sqh_set_cost_function(self.h, int(variant)))
 
+    def set_jit(self, enable=True):
+        """Use run-time specialised kernels for this context's circuit."""
+        self.check(self._L.sqh_set_jit(self.h, int(bool(enable))))
+
+    def jit_stats(self):
+        ms, n = ctypes.c_double(0.0), ctypes.c_int(0)
+        self.check(self._L.sqh_get_jit_stats(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def set_kernel_timing(self, enable=True):
         self.check(self._L.sqh_set_kernel_timing(self.h, 1 if enable else 0))
 
@@ -240,3 +254,15 @@ class Context:
         n = ctypes.c_int(0)
         self.check(self._L.sqh_get_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+def jit_compile_check(qbit_num, gates, kind=1, is_f32=False, uniform=True):
+    """Generate and compile (for gfx950, no GPU needed) the run-time specialised kernel
+    of a gate list; returns (ok, log, code bytes, source bytes)."""
+    L = lib()
+    garr, ng = gate_array(gates)
+    buf = ctypes.create_string_buffer(4096)
+    cb, sb = ctypes.c_longlong(0), ctypes.c_longlong(0)
+    rc = L.sqh_jit_compile_check(qbit_num, garr, ng, kind, int(is_f32), int(uniform), buf, 4096, ctypes.byref(cb),
+                                 ctypes.byref(sb))
+    return rc == 0, buf.value.decode(errors="replace"), cb.value, sb.value

@@ -135,3 +135,19 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
         results[name] = r
     out["results"] = results
     return out
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="4-qubit solves/s: plain vs batched vs learned + batched")
+    ap.add_argument("--train", type=int, default=200_000)
+    ap.add_argument("--reach", type=int, default=4096)
+    ap.add_argument("--haar", type=int, default=1024)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--plain", type=int, default=128, help="targets timed on the one-at-a-time plain path")
+    a = ap.parse_args()
+    res = run(n_train=a.train, n_reach=a.reach, n_haar=a.haar, epochs=a.epochs, n_plain=a.plain,
+              log=lambda *m: print(*m, file=sys.stderr, flush=True))
+    print(json.dumps(res))

@@ -67,3 +67,16 @@ def run(count=256, seed=2028, tol=1e-8, mixed=True, log=print):
     log(f"uneven batch: {count} problems in {dt:.2f}s, success reachable {out['reachable']['success']:.3f} "
         f"haar {out['haar']['success']:.3f}, {out['solves_per_s']:.1f} solves/s")
     return out, probs, res
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+    import sys
+    ap = argparse.ArgumentParser(description="uneven batch of independent decompositions on the GPU")
+    ap.add_argument("--count", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=2028)
+    ap.add_argument("--fp64", action="store_true", help="FP64 throughout (default: complex64 descent)")
+    a = ap.parse_args()
+    res, _, _ = run(count=a.count, seed=a.seed, mixed=not a.fp64, log=lambda *m: print(*m, file=sys.stderr))
+    print(json.dumps(res))
