@@ -135,6 +135,13 @@ int sqh_optimization_problem_combined_batch(sqh_ctx* ctx, const double* params, 
                                             double* f0, double* grad);
 int sqh_optimization_problem_combined_batch_dev(sqh_ctx* ctx, const double* d_params, int B, int precision,
                                                 double* d_f0, double* d_grad);
+/* Cost and gradient of B parameter vectors, element b against its own square target
+ * targets[target_index[b]] (device arrays: n_targets x 2^n x 2^n complex128, B ints,
+ * B x P parameters; trace offset 0).  The batched form of optimization_problem_combined
+ * over many problems of one circuit (the caller of a learned start's training loop). */
+int sqh_optimization_problem_combined_targets_dev(sqh_ctx* ctx, const double* d_targets, int n_targets,
+                                                  const int* d_target_index, const double* d_params, int B,
+                                                  int precision, double* d_f0, double* d_grad);
 /* optimization_problem_combined_unitary: C.Umtx (rows x cols dense) and the P
  * derivative matrices (P x rows x cols). */
 int sqh_optimization_problem_combined_unitary(sqh_ctx* ctx, const double* params, double* out_unitary,
@@ -248,6 +255,10 @@ int sqh_mlp_forward_dev(sqh_mlp* mlp, const float* d_x, int B, float* d_y);
  * before the step (synchronises). */
 int sqh_mlp_train_step_dev(sqh_mlp* mlp, const float* d_x, const float* d_t, int B, float lr, double* loss);
 /* Loss and gradients (flat, like the parameters) of a host batch, without updating. */
+/* One Adam step from a caller-supplied gradient of the loss with respect to the
+ * network's output (d_dy: B x out_dim FP32, device), e.g. the gradient of the
+ * decomposition cost at the predicted angles chained through the output coding. */
+int sqh_mlp_backward_step_dev(sqh_mlp* mlp, const float* d_x, const float* d_dy, int B, float lr);
 int sqh_mlp_gradients(sqh_mlp* mlp, const float* x, const float* t, int B, float* grads, double* loss);
 
 /* ---- kernel timers --------------------------------------------------------- */

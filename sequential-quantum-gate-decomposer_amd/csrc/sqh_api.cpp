@@ -494,6 +494,69 @@ int sqh_optimization_problem_combined_batch_dev(sqh_ctx* c, const double* d_para
     return eval_batch(c, d_params, B, precision, d_f0, d_grad);
 }
 
+int sqh_optimization_problem_combined_targets_dev(sqh_ctx* c, const double* d_targets, int n_targets,
+                                                  const int* d_target_index, const double* d_params, int B,
+                                                  int precision, double* d_f0, double* d_grad) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    const int N = c->prog.N, ns = (int)c->prog.slots.size();
+    if (!d_targets || n_targets <= 0 || !d_target_index || B <= 0 || !d_f0 || !d_grad)
+        return fail(c, SQH_ERR_ARG, "need targets, indices, a batch and output buffers");
+    if (N > kMaxTileQubits) return fail(c, SQH_ERR_ARG, "gradient evaluation supports up to %d qubits", kMaxTileQubits);
+    if (precision != SQH_PREC_F64 && precision != SQH_PREC_F32) return fail(c, SQH_ERR_ARG, "bad precision");
+    const bool f32 = precision == SQH_PREC_F32;
+    const int d = 1 << N;
+    const long long msz = (long long)d * d;
+    HIPTRY(c, hipSetDevice(c->device));
+    const void* tg = d_targets;
+    if (f32) {
+        HIPTRY(c, c->d_mat.ensure((size_t)n_targets * msz * 8));
+        HIPTRY(c, launch_convert(true, d_targets, c->d_mat.p, (long long)n_targets * msz * 2, c->stream));
+        tg = c->d_mat.p;
+    }
+    // element b -> run b -> target d_target_index[b]
+    if (c->idx_B != B) {
+        std::vector<int> h(B + 1);
+        h[0] = B;
+        for (int b = 0; b < B; ++b) h[b + 1] = b;
+        HIPTRY(c, c->d_idx.ensure(h.size() * sizeof(int)));
+        HIPTRY(c, hipMemcpyAsync(c->d_idx.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipStreamSynchronize(c->stream));
+        c->idx_B = B;
+    }
+    if ((rc = build_kernels(c, d_params, B, f32))) return rc;
+    HIPTRY(c, c->d_diag.ensure((size_t)B * d * 2 * sizeof(double)));
+    TileLaunch t = base_launch(c, f32);
+    t.in = tg;
+    t.in_stride = d;
+    t.m = d;
+    t.m_pad = m_pad_for(N, d);
+    t.B = B;
+    t.kern_bstride = (long long)ns * 4;
+    t.trace_off = 0;
+    t.diag = c->d_diag.as<double>();
+    const int wpb = waves_per_batch(N, t.m_pad);
+    HIPTRY(c, c->d_mpart.ensure((size_t)B * wpb * std::max(ns, 1) * 8 * sizeof(double)));
+    t.mpart = c->d_mpart.as<double>();
+    t.bi.count = c->d_idx.as<int>();
+    t.bi.elem_run = c->d_idx.as<int>() + 1;
+    t.bi.run_target = d_target_index;
+    t.bi.tgt_stride = msz;
+    {
+        TimedLaunch tl(c);
+        if (jit_wants(c, 2)) {
+            if ((rc = jit_launch(c, t, 2, c->stream))) return rc;
+        } else {
+            HIPTRY(c, launch_tile_grad(t, c->stream));
+        }
+    }
+    HIPTRY(c, launch_finish(t.diag, d, B, c->variant, d_f0, t.mpart, wpb, ns, t.slots, d_params, c->prog.P, d_grad,
+                            c->stream, t.bi));
+    return SQH_OK;
+}
+
 static int host_batch(sqh_ctx* c, const double* params, int B, int precision, double* costs, double* grad) {
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);

@@ -66,6 +66,8 @@ struct sqh_ctx {
     int variant = sqh::CV_FROBENIUS;
 
     sqh::DevBuf d_params, d_kern, d_diag, d_mpart, d_res, d_mat, d_out;
+    sqh::DevBuf d_idx;  // [count][0..B) identity element list of the per-target evaluation
+    int idx_B = -1;
 
     // Dense-block fusion of the circuit for the streaming path.
     sqh::FusionPlan fplan;
@@ -86,7 +88,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx})
             b->release();
         if (stream) (void)hipStreamDestroy(stream);
     }

@@ -72,9 +72,13 @@ __global__ void k_bfgs_reset_counts(int* __restrict__ counts, int n_groups) {
 }
 
 // BFGS_Powell::BFGS_iteration (BFGS_Powell.cpp:202-386) across the workgroup.
-// V: the run's vectors with stride Ps; Z: P x P row-major.
+// Z is P x P stored column-major (Z[row][c] at c * P + row), so a column is
+// contiguous: the Givens sweep and the column updates read whole columns with
+// consecutive threads on consecutive rows.  The rotation angles of the sweep depend
+// only on the vector being folded (Z^T g), so one thread computes all of them first
+// and every thread then applies the whole sequence to its own rows without barriers.
 __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int P, int Ps, double& norm_Z,
-                            double* sh, double* zs) {
+                            double* sh, double* zs, double* rc, double* rs) {
     const int tid = threadIdx.x, nt = blockDim.x;
     double* x = V + RV_X * Ps;
     double* g = V + RV_G * Ps;
@@ -97,24 +101,41 @@ __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
     dtg = wg_sum(dtg, sh);
     if (dtg < fabs(t0) * 0.1) return;
     dtg = fmax(kNumPrecision, dtg);
-    for (int k = tid; k < P; k += nt) zs[k] = ztg[k];
-    __syncthreads();
-    // Givens rotations folding Z^T g into its first entry (Powell 1987, Eq. 1.7).
-    for (int c = P - 1; c > 0; --c) {
-        const double zc = zs[c], zm = zs[c - 1];
-        if (zc == 0.0) continue;
-        const double a = sqrt(zm * zm + zc * zc);
-        const double cv = zm / a, sv = zc / a;
-        for (int row = tid; row < P; row += nt) {
-            double* z = Z + (size_t)row * P + (c - 1);
-            const double z0 = z[0], z1 = z[1];
-            z[0] = cv * z0 + sv * z1;
-            z[1] = cv * z1 - sv * z0;
+    // Givens rotations folding Z^T g into its first entry (Powell 1987, Eq. 1.7):
+    // angles first (sequential in c), then the column pairs (c-1, c) of every row.
+    if (tid == 0) {
+        double carry = ztg[P - 1];
+        for (int c = P - 1; c > 0; --c) {
+            const double zc = carry, zm = ztg[c - 1];
+            if (zc == 0.0) {
+                rc[c] = 1.0;
+                rs[c] = 0.0;
+                carry = zm;
+                continue;
+            }
+            const double a = sqrt(zm * zm + zc * zc);
+            rc[c] = zm / a;
+            rs[c] = zc / a;
+            carry = a;
         }
-        __syncthreads();
-        if (tid == 0) zs[c - 1] = a;
-        __syncthreads();
     }
+    __syncthreads();
+    for (int row = tid; row < P; row += nt) {
+        double z1 = Z[(size_t)(P - 1) * P + row];
+        for (int c = P - 1; c > 0; --c) {
+            const double cv = rc[c], sv = rs[c];
+            const double z0 = Z[(size_t)(c - 1) * P + row];
+            if (sv == 0.0 && cv == 1.0) {
+                Z[(size_t)c * P + row] = z1;
+                z1 = z0;
+                continue;
+            }
+            Z[(size_t)c * P + row] = cv * z1 - sv * z0;
+            z1 = cv * z0 + sv * z1;
+        }
+        Z[row] = z1;
+    }
+    __syncthreads();
     if (norm_Z < 0.0) {
         norm_Z = nd / dtg;
     } else {
@@ -123,27 +144,27 @@ __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
         norm_Z = fmax(norm_Z, t * 0.1);
     }
     const double sq = sqrt(dtg);
-    for (int row = tid; row < P; row += nt)
-        Z[(size_t)row * P] = fabs(delta[row]) > kNumPrecision ? delta[row] / sq : 0.0;
+    for (int row = tid; row < P; row += nt) Z[row] = fabs(delta[row]) > kNumPrecision ? delta[row] / sq : 0.0;
     __syncthreads();
     for (int c = 1 + tid; c < P; c += nt) {
+        const double* zc = Z + (size_t)c * P;
         double s = 0.0;
-        for (int row = 0; row < P; ++row) s += gamma[row] * Z[(size_t)row * P + c];
+        for (int row = 0; row < P; ++row) s += gamma[row] * zc[row];
         zs[c] = s / dtg;
     }
     __syncthreads();
     for (int c = 1 + tid; c < P; c += nt) {
         const double gz = zs[c];
+        double* zc = Z + (size_t)c * P;
         double nz = 0.0;
         for (int row = 0; row < P; ++row) {
-            double* z = Z + (size_t)row * P + c;
-            const double v = *z - gz * delta[row];
-            *z = v;
+            const double v = zc[row] - gz * delta[row];
+            zc[row] = v;
             nz += v * v;
         }
         if (nz < norm_Z && nz > kNumPrecision) {
             const double sc = sqrt(norm_Z / nz);
-            for (int row = 0; row < P; ++row) Z[(size_t)row * P + c] *= sc;
+            for (int row = 0; row < P; ++row) zc[row] *= sc;
         }
     }
     __syncthreads();
@@ -158,6 +179,8 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                                                                int B) {
     __shared__ double sh[8];
     __shared__ double zs[kBfgsMaxP];
+    __shared__ double rot_c[kBfgsMaxP];
+    __shared__ double rot_s[kBfgsMaxP];
     const int r = blockIdx.x;
     if (r >= B) return;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -208,16 +231,17 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                 S.phase = PH_BFGS_END;
                 break;
             }
-            // d = -Z Z^T g (BFGS_Powell.cpp:396-439)
+            // d = -Z Z^T g (BFGS_Powell.cpp:396-439); Z column-major
             for (int c = tid; c < P; c += nt) {
+                const double* zc = Z + (size_t)c * P;
                 double s = 0.0;
-                for (int row = 0; row < P; ++row) s += Z[(size_t)row * P + c] * g[row];
+                for (int row = 0; row < P; ++row) s += zc[row] * g[row];
                 ztg[c] = s;
             }
             __syncthreads();
             for (int row = tid; row < P; row += nt) {
                 double s = 0.0;
-                for (int c = 0; c < P; ++c) s += Z[(size_t)row * P + c] * ztg[c];
+                for (int c = 0; c < P; ++c) s += Z[(size_t)c * P + row] * ztg[c];
                 d[row] = -s;
             }
             __syncthreads();
@@ -330,7 +354,7 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                 S.phase = PH_BFGS_END;
                 break;
             }
-            bfgs_update(Z, V, P, Ps, S.norm_Z, sh, zs);
+            bfgs_update(Z, V, P, Ps, S.norm_Z, sh, zs, rot_c, rot_s);
             S.phase = PH_DIRECTION;
             break;
         }

@@ -148,6 +148,18 @@ __global__ __launch_bounds__(256) void k_mse(const float* __restrict__ y, int ld
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// Output gradient supplied by the caller (dy: B x out, FP32) into the padded FP32 and
+// BF16 buffers of the last layer; padding rows and columns are zero.
+__global__ __launch_bounds__(256) void k_out_grad(const float* __restrict__ dy, int B, int out, int ldy, int Bp,
+                                                  float* __restrict__ dyf, bf16* __restrict__ dyb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)Bp * ldy) return;
+    const int r = (int)(i / ldy), c = (int)(i % ldy);
+    const float g = (r < B && c < out) ? dy[(size_t)r * out + c] : 0.0f;
+    dyf[i] = g;
+    dyb[i] = (bf16)g;
+}
+
 // db[n] = sum over the first B rows of dz[:, n].
 __global__ void k_bias_grad(const float* __restrict__ dz, int ld, int B, int N, float* __restrict__ db) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
@@ -223,6 +235,13 @@ hipError_t mlp_mse(const float* y, int ldy, const float* t, int B, int out, int 
     const unsigned g = (unsigned)((n + 255) / 256);
     *nblocks = (int)g;
     hipLaunchKernelGGL(k_mse, dim3(g), dim3(256), 0, s, y, ldy, t, B, out, Bp, dyf, (bf16*)dyb, part);
+    return hipGetLastError();
+}
+
+hipError_t mlp_out_grad(const float* dy, int B, int out, int ldy, int Bp, float* dyf, void* dyb, hipStream_t s) {
+    const size_t n = (size_t)Bp * ldy;
+    hipLaunchKernelGGL(k_out_grad, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, dy, B, out, ldy, Bp, dyf,
+                       (bf16*)dyb);
     return hipGetLastError();
 }
 

@@ -22,6 +22,7 @@ hipError_t mlp_to_bf16_padded(const float* x, int B, int in, void* out, int Bp, 
 hipError_t mlp_mse(const float* y, int ldy, const float* t, int B, int out, int Bp, float* dyf, void* dyb,
                    double* part, int* nblocks, hipStream_t s);
 hipError_t mlp_bias_grad(const float* dz, int ld, int B, int N, float* db, hipStream_t s);
+hipError_t mlp_out_grad(const float* dy, int B, int out, int ldy, int Bp, float* dyf, void* dyb, hipStream_t s);
 hipError_t mlp_adam(float* w, void* wb, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2,
                     float eps, int step, hipStream_t s);
 hipError_t mlp_f32_to_bf16(const float* a, void* b, size_t n, hipStream_t s);
@@ -78,13 +79,10 @@ int forward(sqh_mlp* m, const float* d_x, int B, int Bp) {
     return SQH_OK;
 }
 
-// Loss, gradients into gW/gb; the loss sum goes to *loss (synchronises).
-int backward(sqh_mlp* m, const float* d_t, int B, int Bp, double* loss) {
+// Gradients into gW/gb from the output gradient already in dZf/dZb[L-1].
+int backward_layers(sqh_mlp* m, int B, int Bp) {
     sqh_ctx* c = m->ctx;
     const int L = m->L;
-    int nb = 0;
-    HIPTRY(c, mlp_mse(m->Z[L - 1].as<float>(), m->dpad[L], d_t, B, m->dim[L], Bp, m->dZf[L - 1].as<float>(),
-                      m->dZb[L - 1].p, m->part.as<double>(), &nb, m->st()));
     for (int l = L - 1; l >= 0; --l) {
         HIPTRY(c, mlp_gemm_backward_weights(m->dZb[l].p, m->dpad[l + 1], m->H[l].p, m->dpad[l], Bp,
                                             m->gW[l].as<float>(), m->st()));
@@ -95,6 +93,31 @@ int backward(sqh_mlp* m, const float* d_t, int B, int Bp, double* loss) {
                                              m->Z[l - 1].as<float>(), m->dZf[l - 1].as<float>(), m->dZb[l - 1].p,
                                              m->st()));
     }
+    return SQH_OK;
+}
+
+int adam_step(sqh_mlp* m, float lr) {
+    sqh_ctx* c = m->ctx;
+    m->step++;
+    for (int l = 0; l < m->L; ++l) {
+        const size_t nw = (size_t)m->dpad[l + 1] * m->dpad[l];
+        HIPTRY(c, mlp_adam(m->W[l].as<float>(), m->Wb[l].p, m->gW[l].as<float>(), m->mW[l].as<float>(),
+                           m->vW[l].as<float>(), nw, lr, 0.9f, 0.999f, 1e-8f, m->step, m->st()));
+        HIPTRY(c, mlp_adam(m->b[l].as<float>(), nullptr, m->gb[l].as<float>(), m->mb[l].as<float>(),
+                           m->vb[l].as<float>(), (size_t)m->dpad[l + 1], lr, 0.9f, 0.999f, 1e-8f, m->step, m->st()));
+    }
+    return SQH_OK;
+}
+
+// Loss, gradients into gW/gb; the loss sum goes to *loss (synchronises).
+int backward(sqh_mlp* m, const float* d_t, int B, int Bp, double* loss) {
+    sqh_ctx* c = m->ctx;
+    const int L = m->L;
+    int nb = 0;
+    HIPTRY(c, mlp_mse(m->Z[L - 1].as<float>(), m->dpad[L], d_t, B, m->dim[L], Bp, m->dZf[L - 1].as<float>(),
+                      m->dZb[L - 1].p, m->part.as<double>(), &nb, m->st()));
+    int rc = backward_layers(m, B, Bp);
+    if (rc) return rc;
     if (loss) {
         std::vector<double> part(nb);
         HIPTRY(c, hipMemcpyAsync(part.data(), m->part.p, nb * sizeof(double), hipMemcpyDeviceToHost, m->st()));
@@ -247,15 +270,23 @@ int sqh_mlp_train_step_dev(sqh_mlp* m, const float* d_x, const float* d_t, int B
     if (rc) return rc;
     if ((rc = forward(m, d_x, B, Bp))) return rc;
     if ((rc = backward(m, d_t, B, Bp, loss))) return rc;
-    m->step++;
-    for (int l = 0; l < m->L; ++l) {
-        const size_t nw = (size_t)m->dpad[l + 1] * m->dpad[l];
-        HIPTRY(c, mlp_adam(m->W[l].as<float>(), m->Wb[l].p, m->gW[l].as<float>(), m->mW[l].as<float>(),
-                           m->vW[l].as<float>(), nw, lr, 0.9f, 0.999f, 1e-8f, m->step, m->st()));
-        HIPTRY(c, mlp_adam(m->b[l].as<float>(), nullptr, m->gb[l].as<float>(), m->mb[l].as<float>(),
-                           m->vb[l].as<float>(), (size_t)m->dpad[l + 1], lr, 0.9f, 0.999f, 1e-8f, m->step, m->st()));
-    }
-    return SQH_OK;
+    return adam_step(m, lr);
+}
+
+int sqh_mlp_backward_step_dev(sqh_mlp* m, const float* d_x, const float* d_dy, int B, float lr) {
+    if (!m || !d_x || !d_dy || B <= 0) return SQH_ERR_ARG;
+    sqh_ctx* c = m->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    HIPTRY(c, hipSetDevice(c->device));
+    const int Bp = pad64(B);
+    int rc = ensure_batch(m, Bp);
+    if (rc) return rc;
+    if ((rc = forward(m, d_x, B, Bp))) return rc;
+    const int L = m->L;
+    HIPTRY(c, mlp_out_grad(d_dy, B, m->dim[L], m->dpad[L], Bp, m->dZf[L - 1].as<float>(), m->dZb[L - 1].p,
+                           m->st()));
+    if ((rc = backward_layers(m, B, Bp))) return rc;
+    return adam_step(m, lr);
 }
 
 int sqh_mlp_gradients(sqh_mlp* m, const float* x, const float* t, int B, float* grads, double* loss) {

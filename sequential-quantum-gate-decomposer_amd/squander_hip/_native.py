@@ -41,6 +41,7 @@ EXPORTS = [
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check",
+    "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
 
 
@@ -144,6 +145,8 @@ def lib():
             "sqh_agents": ([vp, vp, ctypes.POINTER(AgentsOptions), ctypes.c_uint, d, ctypes.POINTER(ll),
                             ctypes.POINTER(ll)], i),
             "sqh_set_jit": ([vp, i], i),
+            "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
+            "sqh_mlp_backward_step_dev": ([vp, vp, vp, i, ctypes.c_float], i),
             "sqh_jit_compile_check": ([i, ctypes.POINTER(_Gate), i, i, i, i, ctypes.c_char_p, i,
                                        ctypes.POINTER(ll), ctypes.POINTER(ll)], i),
             "sqh_get_jit_stats": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
@@ -195,6 +198,7 @@ class Context:
         if rc != SQH_OK:
             raise SqhError(rc, f"sqh_create(device={device}) failed with status {rc}: no usable GPU")
         self.h = h
+        self.device = device
 
     def close(self):
         if getattr(self, "h", None):

@@ -56,6 +56,10 @@ class WarmStartMLP:
                                                            ctypes.byref(loss) if want_loss else None))
         return loss.value
 
+    def backward_step_dev(self, x_ptr: int, dy_ptr: int, B: int, lr: float):
+        """One Adam step from dLoss/dOutput (B x out FP32 on the device)."""
+        self.ctx.check(self.ctx._L.sqh_mlp_backward_step_dev(self.h, x_ptr, dy_ptr, B, lr))
+
     def gradients(self, x, t):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)

@@ -52,12 +52,13 @@ def plain_single(ctx, Umtx, x0, tol):
 
 
 def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=2026, log=print,
-        hidden=1024, n_hidden=3, n_plain=128, lr=1e-3):
+        hidden=1024, n_hidden=3, n_plain=128, lr=1e-3, ft_epochs=0, ft_lr=1e-4, jit=True):
     """Returns a dict with the three legs per target set:
     plain_single (the reference flow on the GPU, measured on the first n_plain targets),
     batched_fp64 (all targets in one batched FP64 run from the same random starts) and
     learned_mixed (network start, batched mixed precision, random-start fallback)."""
     ctx = nat.Context(0)
+    ctx.set_jit(jit)
     gates = ansatz_4q_12()
     cb = CircuitBatch(ctx, 4, gates)
     P = cb.P
@@ -66,7 +67,7 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
     # training data: seeded angles, target V = C(theta*), label = canonical(theta*),
     # every pair verified with the decomposer's FP64 cost at the label
     t0 = time.perf_counter()
-    Xtr, Ytr, _, _, vcost = generate_pairs(cb, n_train, seed)
+    Xtr, Ytr, _, Vtr, vcost = generate_pairs(cb, n_train, seed)
     out["train_pairs"] = n_train
     out["train_pairs_max_verify_cost"] = float(np.max(np.abs(vcost)))
     out["datagen_s"] = time.perf_counter() - t0
@@ -77,9 +78,24 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
     losses = ws.train(Xtr, Ytr, epochs=epochs, batch=1024, lr=lr, seed=seed, log=log)
     out["train_s"] = time.perf_counter() - t0
     out["train_loss_first_last"] = [losses[0], losses[-1]]
-    del Xtr, Ytr
+    if ft_epochs > 0:
+        # second stage: the decomposition cost itself as the loss (phase-blind)
+        t0 = time.perf_counter()
+        Utr = np.conj(np.transpose(Vtr, (0, 2, 1)))
+        fc = ws.finetune_cost(Xtr, Utr, gates, 4, epochs=ft_epochs, lr=ft_lr, seed=seed, log=log)
+        out["finetune_s"] = time.perf_counter() - t0
+        out["finetune_cost_first_last"] = [fc[0], fc[-1]]
+        del Utr
+    del Xtr, Ytr, Vtr
 
     _, _, _, Vr, _ = generate_pairs(cb, n_reach, seed + 1)
+    # compile the circuit-specialised kernels (both precisions) outside the timed legs
+    t0 = time.perf_counter()
+    Uw = np.conj(np.transpose(Vr[:1], (0, 2, 1))).copy()
+    xw = np.array([initial_guess(P, seed)])
+    for mixed in (False, True):
+        _solve(ctx, Uw, xw, tol, mixed=mixed)
+    out["kernel_warmup_s"] = time.perf_counter() - t0
     Vh = np.array([haar_unitary(16, seed + 10_000 + i) for i in range(n_haar)]) if n_haar else None
 
     results = {}
@@ -147,7 +163,10 @@ if __name__ == "__main__":
     ap.add_argument("--haar", type=int, default=1024)
     ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--plain", type=int, default=128, help="targets timed on the one-at-a-time plain path")
+    ap.add_argument("--ft-epochs", type=int, default=0, help="epochs of training on the decomposition cost")
+    ap.add_argument("--ft-lr", type=float, default=1e-4)
+    ap.add_argument("--lr", type=float, default=1e-3)
     a = ap.parse_args()
-    res = run(n_train=a.train, n_reach=a.reach, n_haar=a.haar, epochs=a.epochs, n_plain=a.plain,
-              log=lambda *m: print(*m, file=sys.stderr, flush=True))
+    res = run(n_train=a.train, n_reach=a.reach, n_haar=a.haar, epochs=a.epochs, n_plain=a.plain, lr=a.lr,
+              ft_epochs=a.ft_epochs, ft_lr=a.ft_lr, log=lambda *m: print(*m, file=sys.stderr, flush=True))
     print(json.dumps(res))

@@ -281,6 +281,56 @@ class WarmStart:
                 log(f"epoch {ep}: loss {losses[-1]:.4f}")
         return losses
 
+    def finetune_cost(self, X: np.ndarray, Umtx: np.ndarray, gates, qbit_num: int, epochs=5, batch=1024, lr=1e-4,
+                      seed=0, log=None, variant=None):
+        """Train on the decomposition cost itself: for each target the loss is the
+        phase-blind cost 1 - |Tr(C(theta) Umtx)|/d at the predicted angles
+        theta = atan2(sin, cos), its gradient comes from the library's batched adjoint
+        cost+gradient (one target per element), chained through the sin/cos output
+        coding, and the network takes one Adam step per batch.  Returns the mean
+        training cost per epoch."""
+        import torch
+        n = X.shape[0]
+        P = self.P
+        ctx = nat.Context(self.ctx.device)
+        ctx.set_circuit(qbit_num, gates)
+        ctx.set_cost_function(nat.COST_ABS_TRACE if variant is None else variant)
+        Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
+        Ud = torch.from_numpy(np.ascontiguousarray(Umtx, dtype=np.complex128)).cuda()
+        Y = torch.empty((batch, 2 * P), dtype=torch.float32, device="cuda")
+        f = torch.empty(batch, dtype=torch.float64, device="cuda")
+        g = torch.empty((batch, P), dtype=torch.float64, device="cuda")
+        rng = np.random.default_rng(seed)
+        costs = []
+        for ep in range(epochs):
+            perm = torch.from_numpy(rng.permutation(n).astype(np.int32)).cuda()
+            Xs = Xd[perm.long()].contiguous()
+            torch.cuda.synchronize()
+            tot, nb = 0.0, 0
+            for s0 in range(0, n - batch + 1, batch):
+                idx = perm[s0:s0 + batch].contiguous()
+                xb = Xs[s0:s0 + batch]
+                self.net.forward_dev(xb.data_ptr(), batch, Y.data_ptr())
+                self.ctx.synchronize()
+                sn, cs = Y[:, :P].double(), Y[:, P:].double()
+                th = torch.atan2(sn, cs).contiguous()
+                torch.cuda.synchronize()  # th is written on torch's stream, read on the context's
+                ctx.check(ctx._L.sqh_optimization_problem_combined_targets_dev(
+                    ctx.h, Ud.data_ptr(), Ud.shape[0], idx.data_ptr(), th.data_ptr(), batch, nat.PREC_F64,
+                    f.data_ptr(), g.data_ptr()))
+                ctx.synchronize()
+                r2 = (sn * sn + cs * cs).clamp_min(1e-12)
+                dth = g / batch
+                dy = torch.cat([dth * cs / r2, -dth * sn / r2], dim=1).float().contiguous()
+                torch.cuda.synchronize()
+                self.net.backward_step_dev(xb.data_ptr(), dy.data_ptr(), batch, lr)
+                tot += float(f.mean())
+                nb += 1
+            costs.append(tot / max(nb, 1))
+            if log:
+                log(f"cost epoch {ep}: mean phase-blind cost {costs[-1]:.4f}")
+        return costs
+
     def predict(self, X: np.ndarray) -> np.ndarray:
         Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
         Yd = torch.empty((X.shape[0], 2 * self.P), dtype=torch.float32, device="cuda")

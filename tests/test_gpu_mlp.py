@@ -74,3 +74,28 @@ def test_training_lowers_loss():
     last = net.train_step_dev(xd.data_ptr(), td.data_ptr(), 512, 1e-3)
     say(f"  loss {first:.4f} -> {last:.4f}")
     assert last < 0.2 * first
+
+
+def test_backward_step_with_mse_gradient_equals_train_step():
+    """An Adam step from the caller-supplied output gradient equals the MSE training
+    step when the supplied gradient is the MSE gradient 2 (y - t) / (B out)."""
+    from squander_hip import _native as nat
+    from squander_hip.mlp import WarmStartMLP
+    ctx = nat.Context(0)
+    a = WarmStartMLP(ctx, 64, 128, 2, 40, seed=3)
+    b = WarmStartMLP(ctx, 64, 128, 2, 40, seed=3)
+    rng = np.random.default_rng(0)
+    B = 96
+    x = torch.from_numpy(rng.standard_normal((B, 64)).astype(np.float32)).cuda()
+    t = torch.from_numpy(rng.standard_normal((B, 40)).astype(np.float32)).cuda()
+    y = torch.empty((B, 40), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    b.forward_dev(x.data_ptr(), B, y.data_ptr())
+    ctx.synchronize()
+    dy = (2.0 * (y - t) / (B * 40)).contiguous()
+    torch.cuda.synchronize()
+    a.train_step_dev(x.data_ptr(), t.data_ptr(), B, 1e-3)
+    b.backward_step_dev(x.data_ptr(), dy.data_ptr(), B, 1e-3)
+    ctx.synchronize()
+    pa, pb = a.get_params(), b.get_params()
+    assert np.max(np.abs(pa - pb)) <= 1e-6 * max(1.0, np.max(np.abs(pa)))

@@ -189,3 +189,32 @@ def test_agents_rejects_other_costs():
     ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT)
     with pytest.raises(nat.SqhError):
         agents(ctx, np.zeros(W.param_count(gates)), 1)
+
+
+def test_combined_targets_matches_oracle(oracle):
+    """Per-element targets: element b against targets[index[b]], cost and gradient."""
+    import torch
+    say("combined cost+grad with per-element targets")
+    for n, variant in ((3, nat.COST_ABS_TRACE), (4, nat.COST_FROBENIUS), (6, nat.COST_HILBERT_SCHMIDT)):
+        gates = W.ansatz_layers(n, W.alternating_pairs(n, 5))
+        P = W.param_count(gates)
+        T = np.array([W.haar_unitary(1 << n, 60 + i) for i in range(5)])
+        idx = np.array([3, 0, 4, 4, 1, 2, 0], dtype=np.int32)
+        ps = W.random_params(P, 61, batch=len(idx))
+        for jit in (False, True):
+            ctx = nat.Context(0)
+            ctx.set_circuit(n, gates)
+            ctx.set_cost_function(variant)
+            ctx.set_jit(jit)
+            Td, Id, Pd = (torch.from_numpy(a).cuda() for a in (T, idx, ps))
+            f = torch.empty(len(idx), dtype=torch.float64, device="cuda")
+            g = torch.empty((len(idx), P), dtype=torch.float64, device="cuda")
+            torch.cuda.synchronize()
+            ctx.check(ctx._L.sqh_optimization_problem_combined_targets_dev(
+                ctx.h, Td.data_ptr(), len(T), Id.data_ptr(), Pd.data_ptr(), len(idx), nat.PREC_F64, f.data_ptr(),
+                g.data_ptr()))
+            ctx.synchronize()
+            for b, i in enumerate(idx):
+                fo, go = oracle.optimization_problem_combined(gates, T[i], n, ps[b], variant=variant)
+                assert abs(f[b].item() - fo) < 1e-12
+                assert np.max(np.abs(g[b].cpu().numpy() - go)) < 1e-12
