@@ -25,6 +25,8 @@ FP64_PEAK_TFLOPS = 78.6
 # script (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors from a 256 MiB device copy
 # in the same runs): profiles/k_tile_forward_k_stream_gate_k_block_mfma4_8q_b512.md
 TILE_FORWARD_8Q_B512_TRAFFIC = (11910.7 * 2.0 + 2048.0) * 1024
+# the same for the circuit-specialised kernel: profiles/sqh_jit_kernel_8q_b512.md
+JIT_FORWARD_8Q_B512_TRAFFIC = (24229.0 * 2.0 + 2048.0) * 1024
 
 
 def log(*a):
@@ -339,7 +341,8 @@ def main():
         "err_vs_fp64_oracle": err64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": TILE_FORWARD_8Q_B512_TRAFFIC if (B == 512 and args.generic) else None,
+                     "traffic": (None if B != 512 else
+                                 TILE_FORWARD_8Q_B512_TRAFFIC if args.generic else JIT_FORWARD_8Q_B512_TRAFFIC),
                      "kernel": ("k_tile_forward<8,double,1>" if args.generic else
                                 "sqh_jit_kernel (k_tile_forward specialised to the circuit, diagonal mode)"),
                      "kernel_avg_ms": k_avg_s * 1e3, "algorithmic_flops_per_launch": flops_eval * B},
