@@ -43,6 +43,7 @@ struct Group {
     const Layout* lays = nullptr;
     const Slot* slots = nullptr;
     const ProgRef* run_prog = nullptr;
+    int ops_cap = 0, lays_cap = 0;  // largest program of the group (LDS staging)
     DevBuf tgt64, tgt32, d_ops, d_lays, d_slots;
     ~Group() {
         for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots}) b->release();
@@ -95,6 +96,8 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             t.bi.run_target = bb.run_target.as<int>();
             t.bi.tgt_stride = (long long)gr.rows * gr.cols;
             t.bi.run_prog = gr.run_prog;
+            t.stage_ops = gr.ops_cap;
+            t.stage_lays = gr.lays_cap;
             HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns, kern, c->stream, t.bi));
             if (!grouped && jit_wants(c, 2)) {
                 const int rc = jit_launch(c, t, 2, c->stream);
@@ -286,6 +289,8 @@ extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targ
     g.wpb = waves_per_batch(N, g.m_pad);
     g.ops = c->d_ops.as<Op>() + kOpPad;
     g.nops = (int)pr.ops.size();
+    g.ops_cap = g.nops;
+    g.lays_cap = (int)pr.layouts.size();
     g.lays = c->d_lays.as<Layout>();
     g.slots = pr.slots.empty() ? nullptr : c->d_slots.as<Slot>();
     std::vector<const double*> tp(n_targets);
@@ -361,7 +366,10 @@ extern "C" int sqh_decompose_problems(sqh_ctx* c, const sqh_problem* problems, i
             pg.lay_off = (int)lays.size();
             pg.slot_off = (int)slots.size();
             pg.ns = (int)p.slots.size();
-            pg.pad0 = pg.pad1 = pg.pad2 = 0;
+            pg.nlays = (int)p.layouts.size();
+            pg.pad1 = pg.pad2 = 0;
+            g.ops_cap = std::max(g.ops_cap, pg.nops);
+            g.lays_cap = std::max(g.lays_cap, pg.nlays);
             ops.insert(ops.end(), p.ops.begin(), p.ops.end());
             ops.insert(ops.end(), kOpPad, nop);
             lays.insert(lays.end(), p.layouts.begin(), p.layouts.end());

@@ -277,7 +277,9 @@ __device__ __forceinline__ int kslot(const Op op) { return op.kind <= OP_CU ? op
 
 // Software-pipelined walk over a program: the op record two ahead and the kernel
 // of the next op are loaded while the current op computes.
-template <typename R>
+// LDSOPS: the op records come from LDS (a wavefront's staged copy), so each one is
+// made wave-uniform with readfirstlane before it drives the dispatch.
+template <typename R, bool LDSOPS = false>
 struct OpStream {
     const Op* ops;
     const cpx<R>* kb;
@@ -291,7 +293,17 @@ struct OpStream {
         load(kslot(cur), k);
     }
     // ops[-kOpPad .. nops + kOpPad) are readable (no-op padding on both sides).
-    __device__ __forceinline__ Op at(int p) const { return ops[p]; }
+    __device__ __forceinline__ Op at(int p) const {
+        Op o = ops[p];
+        if constexpr (LDSOPS) {
+            uint32_t w[2];
+            __builtin_memcpy(w, &o, sizeof(w));
+            w[0] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[0]);
+            w[1] = (uint32_t)__builtin_amdgcn_readfirstlane((int)w[1]);
+            __builtin_memcpy(&o, w, sizeof(w));
+        }
+        return o;
+    }
     __device__ __forceinline__ void load(int slot, cpx<R> out[4]) const {
         const cpx<R>* kp = kb + 4 * slot;
         out[0] = kp[0];
@@ -418,6 +430,32 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
     }
 }
 
+// One DPP move of a double (two 32-bit halves); rows / banks outside the masks read 0.
+template <int CTRL, int ROWS = 0xf, int BANKS = 0xf>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROWS, BANKS, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROWS, BANKS, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over all 64 lanes of the wavefront with DPP row operations (no LDS round
+// trips): pairs within quads, half rows, rows, then rows 0+1 into 1 and 2+3 into 3
+// by row broadcasts, and 0..2 into 3; the total lands in lane 63 and is read back
+// into every lane.
+__device__ __forceinline__ double wave_sum(double v) {
+    v += dpp_f64<0xb1>(v);         // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4e>(v);         // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);        // row_half_mirror
+    v += dpp_f64<0x140>(v);        // row_mirror
+    v += dpp_f64<0x142, 0xa>(v);   // row_bcast:15 -> rows 1, 3
+    v += dpp_f64<0x143, 0xc>(v);   // row_bcast:31 -> rows 2, 3
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Segmented butterfly sum over the lanes that share a batch element.
 template <int CW, typename T>
 __device__ __forceinline__ T seg_sum(T x, int lim) {
@@ -436,13 +474,17 @@ __device__ __forceinline__ T seg_sum(T x, int lim) {
 // finish kernel evaluates.  This replaces the reference's per-parameter derivative
 // matrices (Gates_block.cpp:1012-1150) with O(G d m) work.
 // ---------------------------------------------------------------------------
-template <int N, typename R, bool UNI>
+// STAGE (with UNI): the wavefront first copies its program (ops with their padding,
+// layouts) and its element's kernel table into LDS (dynamic shared memory, sized by
+// the host for the largest program of the launch: ops_cap ops, lays_cap layouts),
+// so the op walk waits on LDS instead of on scalar loads from L2/HBM.
+template <int N, typename R, bool UNI, bool STAGE = false>
 __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in, int in_stride, int m, int m_pad,
                                                   int B, const Op* __restrict__ ops, int nops,
                                                   const Layout* __restrict__ lays, const cpx<R>* __restrict__ kern,
                                                   long long kern_bstride, int trace_off, int ns,
                                                   double* __restrict__ diag, double* __restrict__ mpart,
-                                                  BatchIndex bi) {
+                                                  BatchIndex bi, int ops_cap, int lays_cap) {
     using G = Geo<N>;
     constexpr int CW = G::CW;
     __shared__ cpx<R> lds[G::LDS];
@@ -453,14 +495,37 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     if ((long long)blockIdx.x * CW / m_pad >= B) return;  // whole wavefront past the live elements
     const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x, blockIdx.x);
     if (bi.count) in += (long long)bi.run_target[bi.elem_run[li.b]] * bi.tgt_stride;
+    int nlays = lays_cap;
     if constexpr (UNI) {
         if (bi.run_prog) {  // this wavefront's run has its own circuit
             const ProgRef pg = bi.run_prog[bi.elem_run[li.b]];
             ops += pg.op_off;
             nops = pg.nops;
             lays += pg.lay_off;
+            nlays = pg.nlays;
         }
     }
+    const cpx<R>* kb = kern + (long long)li.b * kern_bstride;
+    if constexpr (STAGE && UNI) {
+        extern __shared__ __align__(16) unsigned char stage_raw[];
+        cpx<R>* s_kern = reinterpret_cast<cpx<R>*>(stage_raw);
+        Layout* s_lays = reinterpret_cast<Layout*>(s_kern + 4 * (size_t)ns);
+        Op* s_ops = reinterpret_cast<Op*>(s_lays + lays_cap);
+        const int lane = threadIdx.x;
+        for (int i = lane; i < 4 * ns; i += 64) s_kern[i] = kb[i];
+        {
+            const int4* src = reinterpret_cast<const int4*>(lays);
+            int4* dst = reinterpret_cast<int4*>(s_lays);
+            const int n4 = nlays * (int)(sizeof(Layout) / sizeof(int4));
+            for (int i = lane; i < n4; i += 64) dst[i] = src[i];
+        }
+        for (int i = lane; i < nops + 2 * kOpPad; i += 64) s_ops[i] = ops[i - kOpPad];
+        __syncthreads();
+        ops = s_ops + kOpPad;
+        lays = s_lays;
+        kb = s_kern;
+    }
+    (void)ops_cap;
     cpx<R> x[G::NR], y[G::NR];
     {
         const int so = sub_off<N>(&lays[0], li.s);
@@ -470,10 +535,9 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
             x[i] = li.valid ? in[(long long)row * in_stride + li.col] : mk<R>(0, 0);
         }
     }
-    const cpx<R>* kb = kern + (long long)li.b * kern_bstride;
     int cur = 0;
     {
-        OpStream<R> os(ops, kb, nops, 0, 1);
+        OpStream<R, STAGE && UNI> os(ops, kb, nops, 0, 1);
         for (int p = 0; p < nops; ++p) {
             cpx<R> kn[4];
             Op after;
@@ -515,7 +579,7 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     const int wib = m_pad >= CW ? (int)((item0 % m_pad) / CW) : 0;
     double* mp = mpart + ((long long)li.b * wpb + wib) * (long long)ns * 8;
 
-    OpStream<R> os(ops, kb, nops, nops - 1, -1);
+    OpStream<R, STAGE && UNI> os(ops, kb, nops, nops - 1, -1);
     for (int p = nops - 1; p >= 0; --p) {
         cpx<R> kn[4];
         Op after;
@@ -555,8 +619,13 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
             double r[8];
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                r[2 * e] = seg_sum<CW>((double)acc[e].re, lim);
-                r[2 * e + 1] = seg_sum<CW>((double)acc[e].im, lim);
+                if constexpr (UNI) {  // one element per wavefront: a full-wave sum
+                    r[2 * e] = wave_sum((double)acc[e].re);
+                    r[2 * e + 1] = wave_sum((double)acc[e].im);
+                } else {
+                    r[2 * e] = seg_sum<CW>((double)acc[e].re, lim);
+                    r[2 * e + 1] = seg_sum<CW>((double)acc[e].im, lim);
+                }
             }
             if (leader) {
                 double* d = mp + (long long)op.idx * 8;

@@ -70,7 +70,7 @@ struct alignas(16) Slot {
 // same number of qubits: offsets into the launch's concatenated op, layout and slot
 // arrays (each program's ops padded by kOpPad no-ops on both sides).
 struct ProgRef {
-    int32_t op_off, nops, lay_off, slot_off, ns, pad0, pad1, pad2;
+    int32_t op_off, nops, lay_off, slot_off, ns, nlays, pad1, pad2;
 };
 
 // Optional indirection for batched optimisation: launch element j is run

@@ -187,7 +187,10 @@ struct Gen {
     }
 
     // Straight-line k_tile_grad: forward, diagonal and selector, adjoint sweep.
+    bool uni_ = false;
+
     std::string grad_kernel(bool uni, bool f32) {
+        uni_ = uni;
         prologue(uni, f32, "sqh_jit_kernel");
         o << "  cpx<R> x[G::NR], y[G::NR];\n";
         load("x", "in", "a.in_stride");
@@ -238,8 +241,10 @@ struct Gen {
                   << "    pairs_corr<" << int(op.t) << ">(y, x, acc, msk, " << mb << ");\n"
                   << "    pairs_apply<" << int(op.t) << ", " << mb << ", false>(y, kt, msk);\n"
                   << "    double r[8];\n"
-                  << "    for (int e = 0; e < 4; ++e) { r[2 * e] = seg_sum<G::CW>((double)acc[e].re, lim); "
-                     "r[2 * e + 1] = seg_sum<G::CW>((double)acc[e].im, lim); }\n"
+                  << (uni_ ? "    for (int e = 0; e < 4; ++e) { r[2 * e] = wave_sum((double)acc[e].re); "
+                             "r[2 * e + 1] = wave_sum((double)acc[e].im); }\n"
+                           : "    for (int e = 0; e < 4; ++e) { r[2 * e] = seg_sum<G::CW>((double)acc[e].re, lim); "
+                             "r[2 * e + 1] = seg_sum<G::CW>((double)acc[e].im, lim); }\n")
                   << "    if (leader) { double* d = mp + " << 8 * op.idx << "; for (int e = 0; e < 8; ++e) d[e] = r[e]; }\n"
                   << "  }\n";
             }

@@ -33,6 +33,8 @@ struct TileLaunch {
     const Slot* slots = nullptr;
     const double* params = nullptr;
     BatchIndex bi;             // optional element -> run -> target indirection
+    int stage_ops = 0;         // > 0: gradient kernel stages the program in LDS (largest op count)
+    int stage_lays = 0;        //      ... and up to this many layouts
 };
 
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st);

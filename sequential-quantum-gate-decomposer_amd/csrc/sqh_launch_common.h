@@ -87,12 +87,29 @@ hipError_t launch_forward_n(const TileLaunch& t, hipStream_t st) {
     return t.mode == 0 ? launch_forward_nm<N, R, 0>(t, st, waves) : launch_forward_nm<N, R, 1>(t, st, waves);
 }
 
+// LDS bytes of a staged program (kernel table, layouts, padded ops).
+template <typename R>
+inline size_t stage_bytes(const TileLaunch& t) {
+    return sizeof(cpx<R>) * 4 * (size_t)t.ns + sizeof(Layout) * (size_t)t.stage_lays +
+           sizeof(Op) * ((size_t)t.stage_ops + 2 * kOpPad);
+}
+constexpr size_t kStageLdsBytes = 48 * 1024;
+
+template <int N, typename R, bool UNI, bool STAGE>
+hipError_t launch_grad_nus(const TileLaunch& t, hipStream_t st, unsigned grid) {
+    const size_t shm = STAGE ? stage_bytes<R>(t) : 0;
+    hipLaunchKernelGGL((k_tile_grad<N, R, UNI, STAGE>), dim3(grid), dim3(64), shm, st, (const cpx<R>*)t.in,
+                       t.in_stride, t.m, t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride,
+                       t.trace_off, t.ns, t.diag, t.mpart, t.bi, t.stage_ops, t.stage_lays);
+    return hipGetLastError();
+}
+
 template <int N, typename R, bool UNI>
 hipError_t launch_grad_nu(const TileLaunch& t, hipStream_t st, unsigned grid) {
-    hipLaunchKernelGGL((k_tile_grad<N, R, UNI>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in, t.in_stride, t.m,
-                       t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride, t.trace_off,
-                       t.ns, t.diag, t.mpart, t.bi);
-    return hipGetLastError();
+    if constexpr (UNI) {
+        if (t.stage_ops > 0 && stage_bytes<R>(t) <= kStageLdsBytes) return launch_grad_nus<N, R, true, true>(t, st, grid);
+    }
+    return launch_grad_nus<N, R, UNI, false>(t, st, grid);
 }
 
 template <int N, typename R>
