@@ -45,22 +45,25 @@ struct Group {
     const ProgRef* run_prog = nullptr;
     int ops_cap = 0, lays_cap = 0;  // largest program of the group (LDS staging)
     DevBuf tgt64, tgt32, d_ops, d_lays, d_slots;
+    DevBuf kern64, kern32, diag, mpart;  // the group's own evaluation scratch
     ~Group() {
-        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots}) b->release();
+        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots, &kern64, &kern32, &diag, &mpart}) b->release();
     }
 };
 
 struct BatchBufs {
-    DevBuf run_target, run_group, run_P, run_prog, x0, states, vecs, Z, fe, ge, xt, lists, counts, kern64, kern32,
-        diag, mpart;
+    DevBuf run_target, run_group, run_P, run_prog, x0, states, vecs, Z, fe, ge, xt, lists, counts;
     ~BatchBufs() {
         for (DevBuf* b : {&run_target, &run_group, &run_P, &run_prog, &x0, &states, &vecs, &Z, &fe, &ge, &xt, &lists,
-                          &counts, &kern64, &kern32, &diag, &mpart})
+                          &counts})
             b->release();
     }
 };
 
-// One lockstep round: advance all runs, then evaluate the requested points.
+// One lockstep round: advance all runs, then evaluate the requested points.  With
+// several qubit-count groups the groups' evaluation chains are independent (disjoint
+// runs, their own scratch), so the round forks them onto side streams after the
+// advance and joins them before the next round's advance.
 int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const BfgsOpts& o, int B, bool mixed,
                   bool grouped) {
     const int G = (int)groups.size();
@@ -68,8 +71,14 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
     HIPTRY(c, launch_bfgs_advance(bb.states.p, bb.vecs.as<double>(), bb.Z.as<double>(), bb.fe.as<double>(),
                                   bb.ge.as<double>(), bb.xt.as<double>(), bb.lists.as<int>(), bb.counts.as<int>(),
                                   grouped ? bb.run_group.as<int>() : nullptr, o, B, c->stream));
+    const bool fork = G > 1;
+    if (fork) {
+        HIPTRY(c, hipEventRecord(c->side_ev[0], c->stream));
+        for (int g = 0; g < G; ++g) HIPTRY(c, hipStreamWaitEvent(c->side[g], c->side_ev[0], 0));
+    }
     for (int g = 0; g < G; ++g) {
         Group& gr = groups[g];
+        hipStream_t st = fork ? c->side[g] : c->stream;
         for (int p = 0; p < (mixed ? 2 : 1); ++p) {
             const bool f32 = p == 1;
             TileLaunch t;
@@ -80,7 +89,7 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             t.lays = gr.lays;
             t.slots = gr.slots;
             t.ns = gr.ns;
-            void* kern = f32 ? bb.kern32.p : bb.kern64.p;
+            void* kern = f32 ? gr.kern32.p : gr.kern64.p;
             t.kern = kern;
             t.in = f32 ? gr.tgt32.p : gr.tgt64.p;
             t.in_stride = gr.cols;
@@ -89,8 +98,8 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             t.B = gr.B;
             t.kern_bstride = (long long)gr.ns * 4;
             t.trace_off = uses_offset(c->variant) && !grouped ? c->trace_offset : 0;
-            t.diag = bb.diag.as<double>();
-            t.mpart = bb.mpart.as<double>();
+            t.diag = gr.diag.as<double>();
+            t.mpart = gr.mpart.as<double>();
             t.bi.count = bb.counts.as<int>() + 4 + 2 * g + p;
             t.bi.elem_run = bb.lists.as<int>() + (size_t)(2 * g + p) * B;
             t.bi.run_target = bb.run_target.as<int>();
@@ -98,15 +107,21 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             t.bi.run_prog = gr.run_prog;
             t.stage_ops = gr.ops_cap;
             t.stage_lays = gr.lays_cap;
-            HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns, kern, c->stream, t.bi));
+            HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns, kern, st, t.bi));
             if (!grouped && jit_wants(c, 2)) {
-                const int rc = jit_launch(c, t, 2, c->stream);
+                const int rc = jit_launch(c, t, 2, st);
                 if (rc) return rc;
             } else {
-                HIPTRY(c, launch_tile_grad(t, c->stream));
+                HIPTRY(c, launch_tile_grad(t, st));
             }
             HIPTRY(c, launch_finish(t.diag, gr.cols, gr.B, c->variant, bb.fe.as<double>(), t.mpart, gr.wpb, gr.ns,
-                                    gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), c->stream, t.bi));
+                                    gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), st, t.bi));
+        }
+    }
+    if (fork) {
+        for (int g = 0; g < G; ++g) {
+            HIPTRY(c, hipEventRecord(c->side_ev[1 + g], c->side[g]));
+            HIPTRY(c, hipStreamWaitEvent(c->stream, c->side_ev[1 + g], 0));
         }
     }
     return SQH_OK;
@@ -122,11 +137,24 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
     const bool grouped = run_group != nullptr;
     const bool mixed = opts->mixed_precision != 0;
     const int G = (int)groups.size();
-    size_t kern_elems = 1, diag_elems = 1, mpart_elems = 1;
     for (Group& g : groups) {
-        kern_elems = std::max(kern_elems, (size_t)g.B * std::max(g.ns, 1) * 4);
-        diag_elems = std::max(diag_elems, (size_t)g.B * g.cols * 2);
-        mpart_elems = std::max(mpart_elems, (size_t)g.B * g.wpb * std::max(g.ns, 1) * 8);
+        const size_t kern_elems = (size_t)g.B * std::max(g.ns, 1) * 4;
+        HIPTRY(c, g.kern64.ensure(kern_elems * 16));
+        HIPTRY(c, g.kern32.ensure(kern_elems * 8));
+        HIPTRY(c, g.diag.ensure((size_t)g.B * g.cols * 2 * sizeof(double)));
+        HIPTRY(c, g.mpart.ensure((size_t)g.B * g.wpb * std::max(g.ns, 1) * 8 * sizeof(double)));
+    }
+    if (G > 1) {  // side streams and fork/join events, kept by the context
+        while ((int)c->side.size() < G) {
+            hipStream_t st = nullptr;
+            HIPTRY(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+            c->side.push_back(st);
+        }
+        while ((int)c->side_ev.size() < G + 1) {
+            hipEvent_t e = nullptr;
+            HIPTRY(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            c->side_ev.push_back(e);
+        }
     }
     BatchBufs bb;
     HIPTRY(c, bb.run_target.ensure((size_t)B * sizeof(int)));
@@ -139,10 +167,6 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
     HIPTRY(c, bb.xt.ensure((size_t)B * Ps * sizeof(double)));
     HIPTRY(c, bb.lists.ensure((size_t)2 * G * B * sizeof(int)));
     HIPTRY(c, bb.counts.ensure((size_t)(4 + 2 * G) * sizeof(int)));
-    HIPTRY(c, bb.kern64.ensure(kern_elems * 16));
-    HIPTRY(c, bb.kern32.ensure(kern_elems * 8));
-    HIPTRY(c, bb.diag.ensure(diag_elems * sizeof(double)));
-    HIPTRY(c, bb.mpart.ensure(mpart_elems * sizeof(double)));
     HIPTRY(c, hipMemcpyAsync(bb.run_target.p, run_target, (size_t)B * sizeof(int), hipMemcpyHostToDevice, c->stream));
     if (grouped) {
         HIPTRY(c, bb.run_group.ensure((size_t)B * sizeof(int)));

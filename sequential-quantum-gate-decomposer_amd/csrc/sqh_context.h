@@ -50,6 +50,10 @@ void jit_release(sqh_ctx* c);
 struct sqh_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Side streams of the batched optimiser: one per qubit-count group, forked from
+    // and joined back into `stream` inside each captured round (default priority).
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> side_ev;
     std::mutex mu;
     std::string err;
 
@@ -90,6 +94,8 @@ struct sqh_ctx {
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
                                &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx})
             b->release();
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };

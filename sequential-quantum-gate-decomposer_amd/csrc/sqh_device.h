@@ -456,9 +456,23 @@ __device__ __forceinline__ double wave_sum(double v) {
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Segmented butterfly sum over the lanes that share a batch element.
+// Segmented butterfly sum over the lanes that share a batch element: lanes
+// lane ^ mk for mk < lim (columns of the element in the wavefront) and mk >= CW (the
+// lanes of one column).  A wavefront of whole columns (CW = 64) with segments of up
+// to 16 lanes, or one segment of 64, is summed with DPP row operations instead of
+// LDS permutes (segments are aligned, so quad / half-row / row sums are exactly them).
 template <int CW, typename T>
 __device__ __forceinline__ T seg_sum(T x, int lim) {
+    if constexpr (CW == 64 && sizeof(T) == 8) {
+        if (lim == 64) return wave_sum(x);
+        if (lim <= 16) {
+            if (lim >= 2) x += dpp_f64<0xb1>(x);   // quad_perm [1,0,3,2]
+            if (lim >= 4) x += dpp_f64<0x4e>(x);   // quad_perm [2,3,0,1]
+            if (lim >= 8) x += dpp_f64<0x141>(x);  // row_half_mirror
+            if (lim >= 16) x += dpp_f64<0x140>(x); // row_mirror
+            return x;
+        }
+    }
 #pragma unroll
     for (int mk = 1; mk < 64; mk <<= 1) {
         if (mk < lim || mk >= CW) x += __shfl_xor(x, mk, 64);

@@ -122,6 +122,7 @@ __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
     __syncthreads();
     for (int row = tid; row < P; row += nt) {
         double z1 = Z[(size_t)(P - 1) * P + row];
+#pragma unroll 8
         for (int c = P - 1; c > 0; --c) {
             const double cv = rc[c], sv = rs[c];
             const double z0 = Z[(size_t)(c - 1) * P + row];
@@ -241,6 +242,7 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
             __syncthreads();
             for (int row = tid; row < P; row += nt) {
                 double s = 0.0;
+#pragma unroll 8
                 for (int c = 0; c < P; ++c) s += Z[(size_t)c * P + row] * ztg[c];
                 d[row] = -s;
             }
