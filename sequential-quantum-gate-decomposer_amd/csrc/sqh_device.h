@@ -187,6 +187,51 @@ __device__ __forceinline__ void pairs_apply(cpx<R> (&v)[NR], const cpx<R> k[4], 
     }
 }
 
+// Kernel forms whose zero entries are known from the gate type (the terms they would
+// contribute are exact zeros): U3 has a real (0,0) entry; RY / CRY are real; RZ is
+// diagonal; RX has a real diagonal and an imaginary off-diagonal.  K^dagger and K^T
+// keep the form.
+enum KForm : int { KF_GEN = 0, KF_U3 = 1, KF_REAL = 2, KF_DIAG = 3, KF_RX = 4 };
+
+template <int FORM, typename R>
+__device__ __forceinline__ void form_mac(const cpx<R> k[4], cpx<R> a, cpx<R> b, cpx<R>& r0, cpx<R>& r1) {
+    if constexpr (FORM == KF_U3) {
+        r0.re = fma(k[0].re, a.re, fma(k[1].re, b.re, -k[1].im * b.im));
+        r0.im = fma(k[0].re, a.im, fma(k[1].re, b.im, k[1].im * b.re));
+        r1 = mac2(k[2], a, k[3], b);
+    } else if constexpr (FORM == KF_REAL) {
+        r0.re = fma(k[0].re, a.re, k[1].re * b.re);
+        r0.im = fma(k[0].re, a.im, k[1].re * b.im);
+        r1.re = fma(k[2].re, a.re, k[3].re * b.re);
+        r1.im = fma(k[2].re, a.im, k[3].re * b.im);
+    } else if constexpr (FORM == KF_DIAG) {
+        r0.re = fma(k[0].re, a.re, -k[0].im * a.im);
+        r0.im = fma(k[0].re, a.im, k[0].im * a.re);
+        r1.re = fma(k[3].re, b.re, -k[3].im * b.im);
+        r1.im = fma(k[3].re, b.im, k[3].im * b.re);
+    } else if constexpr (FORM == KF_RX) {
+        r0.re = fma(k[0].re, a.re, -k[1].im * b.im);
+        r0.im = fma(k[0].re, a.im, k[1].im * b.re);
+        r1.re = fma(k[3].re, b.re, -k[2].im * a.im);
+        r1.im = fma(k[3].re, b.im, k[2].im * a.re);
+    } else {
+        r0 = mac2(k[0], a, k[1], b);
+        r1 = mac2(k[2], a, k[3], b);
+    }
+}
+
+// pairs_apply with a kernel of known form.
+template <int FORM, int T, bool MASKED, typename R, int NR>
+__device__ __forceinline__ void pairs_apply_f(cpx<R> (&v)[NR], const cpx<R> k[4], uint32_t mask) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        if (i & (1 << T)) continue;
+        const int j = i | (1 << T);
+        if (MASKED && !((mask >> i) & 1u)) continue;
+        form_mac<FORM, R>(k, v[i], v[j], v[i], v[j]);
+    }
+}
+
 template <int T, typename R, int NR>
 __device__ __forceinline__ void pairs_swap(cpx<R> (&v)[NR], uint32_t mask) {
 #pragma unroll

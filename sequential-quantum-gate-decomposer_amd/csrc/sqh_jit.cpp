@@ -94,16 +94,28 @@ struct Gen {
         o << "    __syncthreads(); }\n";
     }
 
+    // Form of a parametric op's kernel (which entries are exact zeros).
+    int form(const Op& op) const {
+        switch (pr.slots[op.idx].type) {
+        case GT_U3: return 1;                 // KF_U3
+        case GT_RY: case GT_CRY: return 2;    // KF_REAL
+        case GT_RZ: return 3;                 // KF_DIAG
+        case GT_RX: return 4;                 // KF_RX
+        default: return 0;
+        }
+    }
+
     // Forward execution of op p (not a remap) on tile v; a parametric op uses k<p>.
     void gate(const char* v, int p) {
         const Op& op = pr.ops[p];
         const int t = op.t;
         switch (op.kind) {
         case OP_U:
-            o << "  pairs_apply<" << t << ", false, false>(" << v << ", k" << p << ", 0u);\n";
+            o << "  pairs_apply_f<" << form(op) << ", " << t << ", false>(" << v << ", k" << p << ", 0u);\n";
             break;
         case OP_CU:
-            o << "  pairs_apply<" << t << ", true, false>(" << v << ", k" << p << ", " << mask_expr(N, op) << ");\n";
+            o << "  pairs_apply_f<" << form(op) << ", " << t << ", true>(" << v << ", k" << p << ", "
+              << mask_expr(N, op) << ");\n";
             break;
         case OP_X:
             o << "  pairs_swap<" << t << ">(" << v << ", " << mask_expr(N, op) << ");\n";
@@ -237,9 +249,9 @@ struct Gen {
                   << "    cpx<R> kt[4] = {k[0], k[2], k[1], k[3]};\n"
                   << "    cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};\n"
                   << "    const unsigned msk = " << mk << ";\n"
-                  << "    pairs_apply<" << int(op.t) << ", " << mb << ", false>(x, kd, msk);\n"
+                  << "    pairs_apply_f<" << form(op) << ", " << int(op.t) << ", " << mb << ">(x, kd, msk);\n"
                   << "    pairs_corr<" << int(op.t) << ">(y, x, acc, msk, " << mb << ");\n"
-                  << "    pairs_apply<" << int(op.t) << ", " << mb << ", false>(y, kt, msk);\n"
+                  << "    pairs_apply_f<" << form(op) << ", " << int(op.t) << ", " << mb << ">(y, kt, msk);\n"
                   << "    double r[8];\n"
                   << (uni_ ? "    for (int e = 0; e < 4; ++e) { r[2 * e] = wave_sum((double)acc[e].re); "
                              "r[2 * e + 1] = wave_sum((double)acc[e].im); }\n"

@@ -52,7 +52,7 @@ def plain_single(ctx, Umtx, x0, tol):
 
 
 def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=2026, log=print,
-        hidden=1024, n_hidden=3, n_plain=128, lr=1e-3, ft_epochs=0, ft_lr=1e-4, jit=True):
+        hidden=1024, n_hidden=3, n_plain=128, lr=1e-3, ft_epochs=0, ft_lr=1e-4, jit=True, ft_stream_steps=0):
     """Returns a dict with the three legs per target set:
     plain_single (the reference flow on the GPU, measured on the first n_plain targets),
     batched_fp64 (all targets in one batched FP64 run from the same random starts) and
@@ -86,6 +86,11 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
         out["finetune_s"] = time.perf_counter() - t0
         out["finetune_cost_first_last"] = [fc[0], fc[-1]]
         del Utr
+    if ft_stream_steps > 0:
+        t0 = time.perf_counter()
+        fc = ws.finetune_cost_stream(gates, 4, steps=ft_stream_steps, lr=ft_lr, seed=seed + 5, log=log)
+        out["stream_finetune_s"] = time.perf_counter() - t0
+        out["stream_finetune_cost"] = fc
     del Xtr, Ytr, Vtr
 
     _, _, _, Vr, _ = generate_pairs(cb, n_reach, seed + 1)
@@ -165,8 +170,10 @@ if __name__ == "__main__":
     ap.add_argument("--plain", type=int, default=128, help="targets timed on the one-at-a-time plain path")
     ap.add_argument("--ft-epochs", type=int, default=0, help="epochs of training on the decomposition cost")
     ap.add_argument("--ft-lr", type=float, default=1e-4)
+    ap.add_argument("--ft-stream", type=int, default=0, help="steps of cost training on fresh targets")
     ap.add_argument("--lr", type=float, default=1e-3)
     a = ap.parse_args()
     res = run(n_train=a.train, n_reach=a.reach, n_haar=a.haar, epochs=a.epochs, n_plain=a.plain, lr=a.lr,
-              ft_epochs=a.ft_epochs, ft_lr=a.ft_lr, log=lambda *m: print(*m, file=sys.stderr, flush=True))
+              ft_epochs=a.ft_epochs, ft_lr=a.ft_lr, ft_stream_steps=a.ft_stream,
+              log=lambda *m: print(*m, file=sys.stderr, flush=True))
     print(json.dumps(res))

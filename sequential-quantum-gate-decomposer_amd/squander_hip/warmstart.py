@@ -331,6 +331,65 @@ class WarmStart:
                 log(f"cost epoch {ep}: mean phase-blind cost {costs[-1]:.4f}")
         return costs
 
+    def finetune_cost_stream(self, gates, qbit_num: int, steps=2000, batch=1024, lr=1e-4, seed=0, log=None,
+                             log_every=200):
+        """Cost training on a stream of fresh targets: every step draws new angles
+        theta* on the GPU, builds V = C(theta*) with the library's batched apply_to,
+        takes the phase-normalised features (the rule of target_features) and the
+        target Umtx = V^dagger, and makes one Adam step on the phase-blind cost at the
+        predicted angles.  No target is seen twice.  Returns the mean cost per log
+        window."""
+        import torch
+        P = self.P
+        d = 1 << qbit_num
+        ctx = nat.Context(self.ctx.device)
+        ctx.set_circuit(qbit_num, gates)
+        ctx.set_cost_function(nat.COST_ABS_TRACE)
+        gen = torch.Generator(device="cuda")
+        gen.manual_seed(int(seed))
+        eye = torch.eye(d, dtype=torch.complex128, device="cuda").expand(batch, d, d).contiguous()
+        idx = torch.arange(batch, dtype=torch.int32, device="cuda")
+        Y = torch.empty((batch, 2 * P), dtype=torch.float32, device="cuda")
+        f = torch.empty(batch, dtype=torch.float64, device="cuda")
+        g = torch.empty((batch, P), dtype=torch.float64, device="cuda")
+        out, win = [], []
+        ar = torch.arange(batch, device="cuda")
+        for step in range(steps):
+            th = torch.rand((batch, P), generator=gen, dtype=torch.float64, device="cuda") * (2.0 * np.pi)
+            V = eye.clone()
+            torch.cuda.synchronize()
+            ctx.check(ctx._L.sqh_apply_to_dev(ctx.h, th.data_ptr(), P, V.data_ptr(), d, d, d, d * d, batch, 0))
+            ctx.synchronize()
+            col = V[:, :, 0]
+            k = torch.argmax(col.abs(), dim=1)
+            ph = col[ar, k]
+            Vn = V * (ph.conj() / ph.abs())[:, None, None]
+            X = torch.cat([Vn.real.reshape(batch, -1), Vn.imag.reshape(batch, -1)], dim=1).float().contiguous()
+            U = V.conj().transpose(1, 2).contiguous()
+            torch.cuda.synchronize()
+            self.net.forward_dev(X.data_ptr(), batch, Y.data_ptr())
+            self.ctx.synchronize()
+            sn, cs = Y[:, :P].double(), Y[:, P:].double()
+            tp = torch.atan2(sn, cs).contiguous()
+            torch.cuda.synchronize()
+            ctx.check(ctx._L.sqh_optimization_problem_combined_targets_dev(
+                ctx.h, U.data_ptr(), batch, idx.data_ptr(), tp.data_ptr(), batch, nat.PREC_F64, f.data_ptr(),
+                g.data_ptr()))
+            ctx.synchronize()
+            r2 = (sn * sn + cs * cs).clamp_min(1e-12)
+            dth = g / batch
+            dy = torch.cat([dth * cs / r2, -dth * sn / r2], dim=1).float().contiguous()
+            torch.cuda.synchronize()
+            self.net.backward_step_dev(X.data_ptr(), dy.data_ptr(), batch, lr)
+            win.append(float(f.mean()))
+            if (step + 1) % log_every == 0 or step + 1 == steps:
+                out.append(float(np.mean(win)))
+                win = []
+                if log:
+                    log(f"stream step {step + 1}: mean phase-blind cost {out[-1]:.4f}")
+        self.ctx.synchronize()
+        return out
+
     def predict(self, X: np.ndarray) -> np.ndarray:
         Xd = torch.from_numpy(np.ascontiguousarray(X, dtype=np.float32)).cuda()
         Yd = torch.empty((X.shape[0], 2 * self.P), dtype=torch.float32, device="cuda")
