@@ -60,6 +60,37 @@ struct BatchBufs {
     }
 };
 
+// The evaluation launch of group g in precision p (f32 when p == 1).
+TileLaunch group_launch(sqh_ctx* c, BatchBufs& bb, Group& gr, int g, int p, int B, bool grouped) {
+    const bool f32 = p == 1;
+    TileLaunch t;
+    t.N = gr.N;
+    t.f32 = f32;
+    t.ops = gr.ops;
+    t.nops = gr.nops;
+    t.lays = gr.lays;
+    t.slots = gr.slots;
+    t.ns = gr.ns;
+    t.kern = f32 ? gr.kern32.p : gr.kern64.p;
+    t.in = f32 ? gr.tgt32.p : gr.tgt64.p;
+    t.in_stride = gr.cols;
+    t.m = gr.cols;
+    t.m_pad = gr.m_pad;
+    t.B = gr.B;
+    t.kern_bstride = (long long)gr.ns * 4;
+    t.trace_off = uses_offset(c->variant) && !grouped ? c->trace_offset : 0;
+    t.diag = gr.diag.as<double>();
+    t.mpart = gr.mpart.as<double>();
+    t.bi.count = bb.counts.as<int>() + 4 + 2 * g + p;
+    t.bi.elem_run = bb.lists.as<int>() + (size_t)(2 * g + p) * B;
+    t.bi.run_target = bb.run_target.as<int>();
+    t.bi.tgt_stride = (long long)gr.rows * gr.cols;
+    t.bi.run_prog = gr.run_prog;
+    t.stage_ops = gr.ops_cap;
+    t.stage_lays = gr.lays_cap;
+    return t;
+}
+
 // One lockstep round: advance all runs, then evaluate the requested points.  With
 // several qubit-count groups the groups' evaluation chains are independent (disjoint
 // runs, their own scratch), so the round forks them onto side streams after the
@@ -80,34 +111,9 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
         Group& gr = groups[g];
         hipStream_t st = fork ? c->side[g] : c->stream;
         for (int p = 0; p < (mixed ? 2 : 1); ++p) {
-            const bool f32 = p == 1;
-            TileLaunch t;
-            t.N = gr.N;
-            t.f32 = f32;
-            t.ops = gr.ops;
-            t.nops = gr.nops;
-            t.lays = gr.lays;
-            t.slots = gr.slots;
-            t.ns = gr.ns;
-            void* kern = f32 ? gr.kern32.p : gr.kern64.p;
-            t.kern = kern;
-            t.in = f32 ? gr.tgt32.p : gr.tgt64.p;
-            t.in_stride = gr.cols;
-            t.m = gr.cols;
-            t.m_pad = gr.m_pad;
-            t.B = gr.B;
-            t.kern_bstride = (long long)gr.ns * 4;
-            t.trace_off = uses_offset(c->variant) && !grouped ? c->trace_offset : 0;
-            t.diag = gr.diag.as<double>();
-            t.mpart = gr.mpart.as<double>();
-            t.bi.count = bb.counts.as<int>() + 4 + 2 * g + p;
-            t.bi.elem_run = bb.lists.as<int>() + (size_t)(2 * g + p) * B;
-            t.bi.run_target = bb.run_target.as<int>();
-            t.bi.tgt_stride = (long long)gr.rows * gr.cols;
-            t.bi.run_prog = gr.run_prog;
-            t.stage_ops = gr.ops_cap;
-            t.stage_lays = gr.lays_cap;
-            HIPTRY(c, launch_build_kernels(f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns, kern, st, t.bi));
+            const TileLaunch t = group_launch(c, bb, gr, g, p, B, grouped);
+            HIPTRY(c, launch_build_kernels(t.f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns,
+                                           const_cast<void*>(t.kern), st, t.bi));
             if (!grouped && jit_wants(c, 2)) {
                 const int rc = jit_launch(c, t, 2, st);
                 if (rc) return rc;
@@ -198,7 +204,7 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
         // compile the run-time kernels outside the capture (a module load is not a
         // stream operation)
         for (int p = 0; p < (mixed ? 2 : 1); ++p) {
-            const int rc = jit_prepare(c, groups[0].N, groups[0].m_pad, 2, p == 1);
+            const int rc = jit_prepare(c, group_launch(c, bb, groups[0], 0, p, B, grouped), 2);
             if (rc) return rc;
         }
     }

@@ -135,7 +135,7 @@ int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s);
 // Whether the context uses a run-time kernel of this kind for its program.
 bool jit_wants(sqh_ctx* c, int kind);
 // Compiles the kernel jit_launch would use for these shapes (before a graph capture).
-int jit_prepare(sqh_ctx* c, int N, int m_pad, int kind, bool f32);
+int jit_prepare(sqh_ctx* c, const TileLaunch& t, int kind);
 double jit_compile_ms(sqh_ctx* c, int* compiles);
 
 hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,

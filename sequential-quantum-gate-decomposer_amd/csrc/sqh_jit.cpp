@@ -59,11 +59,38 @@ std::string sub_expr(int N, const Layout& l) {
     return o.str();
 }
 
+// Launch shape compiled into a kernel as constants (so lane and batch indexing
+// divide by literals and hold no scalar registers for the sizes).
+struct JitShape {
+    int m = 0, m_pad = 0, in_stride = 0, out_stride = 0, trace_off = 0, ns = 0;
+    long long in_bstride = 0, out_bstride = 0, kern_bstride = 0;
+    bool operator==(const JitShape& b) const {
+        return m == b.m && m_pad == b.m_pad && in_stride == b.in_stride && out_stride == b.out_stride &&
+               trace_off == b.trace_off && ns == b.ns && in_bstride == b.in_bstride && out_bstride == b.out_bstride &&
+               kern_bstride == b.kern_bstride;
+    }
+};
+
+JitShape shape_of(const TileLaunch& t) {
+    JitShape s;
+    s.m = t.m;
+    s.m_pad = t.m_pad;
+    s.in_stride = t.in_stride;
+    s.out_stride = t.out_stride;
+    s.trace_off = t.trace_off;
+    s.ns = t.ns;
+    s.in_bstride = t.in_bstride;
+    s.out_bstride = t.out_bstride;
+    s.kern_bstride = t.kern_bstride;
+    return s;
+}
+
 struct Gen {
     const Program& pr;
     int N, NR;
+    JitShape sh;
     std::ostringstream o;
-    explicit Gen(const Program& p) : pr(p), N(p.N), NR(1 << geo_k(p.N)) {}
+    Gen(const Program& p, const JitShape& s) : pr(p), N(p.N), NR(1 << geo_k(p.N)), sh(s) {}
 
     // Kernel-table loads run one parametric op ahead of their use: op p's kernel is
     // loaded into k<p> while the previous parametric op computes, and a scheduling
@@ -160,38 +187,42 @@ struct Gen {
     void prologue(bool uni, bool f32, const char* name) {
         o << "#include \"sqh_device.h\"\nusing namespace sqh;\ntypedef " << (f32 ? "float" : "double")
           << " R;\nconstexpr int N = " << N << ";\ntypedef Geo<N> G;\n"
+          << "constexpr int SM = " << sh.m << ", SMPAD = " << sh.m_pad << ", SINS = " << sh.in_stride
+          << ", SOUTS = " << sh.out_stride << ", STR = " << sh.trace_off << ", SNS = " << sh.ns << ";\n"
+          << "constexpr long long SINB = " << sh.in_bstride << "LL, SOUTB = " << sh.out_bstride
+          << "LL, SKB = " << sh.kern_bstride << "LL;\n"
           << "extern \"C\" __global__ __launch_bounds__(64) void " << name << "(JitArgs a) {\n"
           << "  __shared__ cpx<R> lds[G::LDS];\n"
           << "  int B = a.B;\n"
           << "  if (a.count) { B = *a.count; if (B <= 0) return; }\n"
-          << "  if ((long long)blockIdx.x * G::CW / a.m_pad >= B) return;\n"
+          << "  if ((long long)blockIdx.x * G::CW / SMPAD >= B) return;\n"
           << "  const LaneInfo li = lane_info<N, " << (uni ? "true" : "false")
-          << ">(a.m, a.m_pad, B, threadIdx.x, blockIdx.x);\n"
+          << ">(SM, SMPAD, B, threadIdx.x, blockIdx.x);\n"
           << "  const int s = li.s, cc = li.cc;\n"
           << "  (void)s; (void)cc; (void)lds;\n"
-          << "  const cpx<R>* in = (const cpx<R>*)a.in + (long long)li.b * a.in_bstride;\n"
+          << "  const cpx<R>* in = (const cpx<R>*)a.in + (long long)li.b * SINB;\n"
           << "  if (a.count) in = (const cpx<R>*)a.in + (long long)a.run_target[a.elem_run[li.b]] * a.tgt_stride;\n"
-          << "  const cpx<R>* kb = (const cpx<R>*)a.kern + (long long)li.b * a.kern_bstride;\n";
+          << "  const cpx<R>* kb = (const cpx<R>*)a.kern + (long long)li.b * SKB;\n";
     }
 
     // Straight-line k_tile_forward (mode 0: store; 1: diagonal entries).
     std::string forward_kernel(bool uni, bool f32, int mode) {
         prologue(uni, f32, "sqh_jit_kernel");
         o << "  cpx<R> v[G::NR];\n";
-        load("v", "in", "a.in_stride");
+        load("v", "in", "SINS");
         const int cur = forward("v");
         const Layout& lf = pr.layouts[cur];
         o << "  const int sf = " << sub_expr(N, lf) << ";\n";
         if (mode == 0) {
-            o << "  cpx<R>* dst = (cpx<R>*)a.out + (long long)li.b * a.out_bstride;\n";
+            o << "  cpx<R>* dst = (cpx<R>*)a.out + (long long)li.b * SOUTB;\n";
             for (int i = 0; i < NR; ++i)
-                o << "  if (li.valid) dst[(long long)(sf + " << lf.reg_off[i] << ") * a.out_stride + li.col] = v[" << i
+                o << "  if (li.valid) dst[(long long)(sf + " << lf.reg_off[i] << ") * SOUTS + li.col] = v[" << i
                   << "];\n";
         } else {
-            o << "  const int tr = li.col + a.trace_off;\n";
+            o << "  const int tr = li.col + STR;\n";
             for (int i = 0; i < NR; ++i)
-                o << "  if (li.valid && sf + " << lf.reg_off[i] << " == tr) { a.diag[2 * ((long long)li.b * a.m + li.col)] = "
-                  << "(double)v[" << i << "].re; a.diag[2 * ((long long)li.b * a.m + li.col) + 1] = (double)v[" << i
+                o << "  if (li.valid && sf + " << lf.reg_off[i] << " == tr) { a.diag[2 * ((long long)li.b * SM + li.col)] = "
+                  << "(double)v[" << i << "].re; a.diag[2 * ((long long)li.b * SM + li.col) + 1] = (double)v[" << i
                   << "].im; }\n";
         }
         o << "}\n";
@@ -205,23 +236,23 @@ struct Gen {
         uni_ = uni;
         prologue(uni, f32, "sqh_jit_kernel");
         o << "  cpx<R> x[G::NR], y[G::NR];\n";
-        load("x", "in", "a.in_stride");
+        load("x", "in", "SINS");
         const int cur = forward("x");
         const Layout& lf = pr.layouts[cur];
-        o << "  { const int sf = " << sub_expr(N, lf) << ";\n    const int tr = li.col + a.trace_off;\n";
+        o << "  { const int sf = " << sub_expr(N, lf) << ";\n    const int tr = li.col + STR;\n";
         for (int i = 0; i < NR; ++i) {
             o << "    { const bool hit = li.valid && sf + " << lf.reg_off[i] << " == tr;\n"
-              << "      if (hit) { a.diag[2 * ((long long)li.b * a.m + li.col)] = (double)x[" << i
-              << "].re; a.diag[2 * ((long long)li.b * a.m + li.col) + 1] = (double)x[" << i << "].im; }\n"
+              << "      if (hit) { a.diag[2 * ((long long)li.b * SM + li.col)] = (double)x[" << i
+              << "].re; a.diag[2 * ((long long)li.b * SM + li.col) + 1] = (double)x[" << i << "].im; }\n"
               << "      y[" << i << "] = mk<R>(hit ? (R)1 : (R)0, (R)0); }\n";
         }
         o << "  }\n"
-          << "  const int lim = a.m_pad < G::CW ? a.m_pad : G::CW;\n"
+          << "  const int lim = SMPAD < G::CW ? SMPAD : G::CW;\n"
           << "  const bool leader = li.bvalid && (li.s == 0) && (li.cc % lim == 0);\n"
-          << "  const int wpb = a.m_pad >= G::CW ? a.m_pad / G::CW : 1;\n"
+          << "  const int wpb = SMPAD >= G::CW ? SMPAD / G::CW : 1;\n"
           << "  const long long item0 = (long long)blockIdx.x * G::CW + li.cc;\n"
-          << "  const int wib = a.m_pad >= G::CW ? (int)((item0 % a.m_pad) / G::CW) : 0;\n"
-          << "  double* mp = a.mpart + ((long long)li.b * wpb + wib) * (long long)a.ns * 8;\n";
+          << "  const int wib = SMPAD >= G::CW ? (int)((item0 % SMPAD) / G::CW) : 0;\n"
+          << "  double* mp = a.mpart + ((long long)li.b * wpb + wib) * (long long)SNS * 8;\n";
         std::vector<int> par = order_fwd();
         std::reverse(par.begin(), par.end());
         size_t next = 0;
@@ -275,6 +306,7 @@ struct JitEntry {
     long long prog_version;
     int kind;  // 0: forward store, 1: forward diagonal, 2: gradient
     bool f32, uni;
+    JitShape shape;
     hipModule_t mod = nullptr;
     hipFunction_t fn = nullptr;
 };
@@ -296,8 +328,8 @@ void jit_release(sqh_ctx* c) {
 
 namespace {
 
-std::string jit_source(const Program& pr, int kind, bool f32, bool uni) {
-    Gen g(pr);
+std::string jit_source(const Program& pr, int kind, bool f32, bool uni, const JitShape& sh) {
+    Gen g(pr, sh);
     return kind == 2 ? g.grad_kernel(uni, f32) : g.forward_kernel(uni, f32, kind);
 }
 
@@ -335,11 +367,12 @@ bool jit_compile(const std::string& src, const std::string& arch, std::vector<ch
 
 // Compiles (once) and returns the kernel for the current program; nullptr with the
 // context's error set on failure.
-static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni) {
+static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni, const JitShape& sh) {
     if (!c->jit_cache) c->jit_cache = new JitCache();
     JitCache& jc = *c->jit_cache;
     for (auto& e : jc.entries)
-        if (e.prog_version == c->prog_version && e.kind == kind && e.f32 == f32 && e.uni == uni) return e.fn;
+        if (e.prog_version == c->prog_version && e.kind == kind && e.f32 == f32 && e.uni == uni && e.shape == sh)
+            return e.fn;
     const auto t0 = std::chrono::steady_clock::now();
     hipDeviceProp_t prop;
     std::string arch = "gfx950";
@@ -349,7 +382,7 @@ static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni) {
     }
     std::vector<char> code;
     std::string log;
-    if (!jit_compile(jit_source(c->prog, kind, f32, uni), arch, code, log)) {
+    if (!jit_compile(jit_source(c->prog, kind, f32, uni, sh), arch, code, log)) {
         fail(c, SQH_ERR_DEVICE, "run-time kernel compilation failed: %.400s", log.c_str());
         return nullptr;
     }
@@ -358,6 +391,7 @@ static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni) {
     e.kind = kind;
     e.f32 = f32;
     e.uni = uni;
+    e.shape = sh;
     hipError_t he = hipModuleLoadData(&e.mod, code.data());
     if (he == hipSuccess) he = hipModuleGetFunction(&e.fn, e.mod, "sqh_jit_kernel");
     if (he != hipSuccess) {
@@ -375,7 +409,7 @@ int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
     const int cw = tile_columns_per_wave(t.N);
     const bool uni = t.N > 4 && t.m_pad % cw == 0;
     if (t.bi.run_prog) return fail(c, SQH_ERR_ARG, "run-time kernels serve one circuit");
-    hipFunction_t fn = jit_get(c, kind, t.f32, uni);
+    hipFunction_t fn = jit_get(c, kind, t.f32, uni, shape_of(t));
     if (!fn) return SQH_ERR_DEVICE;
     JitArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -415,10 +449,10 @@ bool jit_wants(sqh_ctx* c, int kind) {
     return kind != 2 || c->prog.ops.size() <= kJitMaxGradOps;
 }
 
-int jit_prepare(sqh_ctx* c, int N, int m_pad, int kind, bool f32) {
-    const int cw = tile_columns_per_wave(N);
-    const bool uni = N > 4 && m_pad % cw == 0;
-    return jit_get(c, kind, f32, uni) ? SQH_OK : SQH_ERR_DEVICE;
+int jit_prepare(sqh_ctx* c, const TileLaunch& t, int kind) {
+    const int cw = tile_columns_per_wave(t.N);
+    const bool uni = t.N > 4 && t.m_pad % cw == 0;
+    return jit_get(c, kind, t.f32, uni, shape_of(t)) ? SQH_OK : SQH_ERR_DEVICE;
 }
 
 double jit_compile_ms(sqh_ctx* c, int* compiles) {
@@ -446,7 +480,15 @@ extern "C" int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_
         if (log && log_cap > 0) std::snprintf(log, (size_t)log_cap, "%s", err.c_str());
         return SQH_ERR_GATE;
     }
-    const std::string src = jit_source(pr, kind, is_f32 != 0, uniform != 0);
+    JitShape sh;  // a representative launch: square target, one parameter vector per element
+    const int d = 1 << pr.N;
+    sh.m = d;
+    sh.m_pad = d;
+    sh.in_stride = sh.out_stride = d;
+    sh.ns = (int)pr.slots.size();
+    sh.kern_bstride = 4LL * sh.ns;
+    sh.out_bstride = (long long)d * d;
+    const std::string src = jit_source(pr, kind, is_f32 != 0, uniform != 0, sh);
     if (source_bytes) *source_bytes = (long long)src.size();
     std::vector<char> code;
     std::string lg;
