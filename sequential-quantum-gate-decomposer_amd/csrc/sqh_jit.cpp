@@ -441,8 +441,9 @@ int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
 }
 
 // Gradient kernels of long programs compile slowly (the straight-line adjoint sweep
-// of the 128-gate 8-qubit circuit takes about a minute); those keep the generic kernel.
-constexpr size_t kJitMaxGradOps = 96;
+// of the 128-gate, 142-op 8-qubit circuit takes about 10 s on the host); longer
+// programs keep the generic gradient kernel.
+constexpr size_t kJitMaxGradOps = 192;
 
 bool jit_wants(sqh_ctx* c, int kind) {
     if (!c->jit || c->prog.N > kMaxTileQubits) return false;

@@ -112,3 +112,23 @@ def test_jit_decompose_batch_matches_oracle_driver(oracle):
         assert abs(oracle.optimization_problem(gates, T[r], n, res["x"][r]) - res["f"][r]) < 1e-12
         assert (fo < 1e-8) == (res["f"][r] < 1e-8) or abs(fo - res["f"][r]) < 1e-10
     assert ctx.jit_stats()[1] >= 1
+
+
+def test_jit_gradient_8q_ansatz_matches_oracle(oracle):
+    """The 264-parameter 8-qubit circuit: specialised cost+gradient against the oracle."""
+    say("jit 8q ansatz gradient")
+    n = 8
+    gates = W.ansatz_8q_40(2026)
+    P = W.param_count(gates)
+    U = W.haar_unitary(256, 2026).conj().T.copy()
+    ctx = _ctx(n, gates, U, True)
+    ps = W.random_params(P, 5, batch=2)
+    f = np.empty(2)
+    g = np.empty((2, P))
+    ctx.check(ctx._L.sqh_optimization_problem_combined_batch(ctx.h, ps.ctypes.data, 2, nat.PREC_F64, f.ctypes.data,
+                                                             g.ctypes.data))
+    for b in range(2):
+        fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b])
+        assert abs(f[b] - fo) < 1e-12
+        assert np.max(np.abs(g[b] - go)) < 1e-12
+    say(f"  compile {ctx.jit_stats()[0]:.0f} ms")
