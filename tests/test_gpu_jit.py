@@ -132,3 +132,30 @@ def test_jit_gradient_8q_ansatz_matches_oracle(oracle):
         assert abs(f[b] - fo) < 1e-12
         assert np.max(np.abs(g[b] - go)) < 1e-12
     say(f"  compile {ctx.jit_stats()[0]:.0f} ms")
+
+
+def test_jit_rectangular_trace_offset_matches_oracle(oracle):
+    """The reference's 8-qubit 256 x 160 target with trace offset 80
+    (test_optmization_problem_combined.py shape), specialised kernels."""
+    say("jit 8q 256x160 offset 80")
+    n = 8
+    gates = W.ansatz_layers(n, W.random_adjacent_pairs(n, 12, 42))
+    P = W.param_count(gates)
+    rng = np.random.default_rng(42)
+    U = W.haar_unitary(256, 43)[:, :160].copy()
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    ctx.set_trace_offset(80)
+    ctx.set_jit(True)
+    ps = rng.uniform(0, 2 * np.pi, (3, P))
+    f = np.empty(3)
+    g = np.empty((3, P))
+    ctx.check(ctx._L.sqh_optimization_problem_combined_batch(ctx.h, ps.ctypes.data, 3, nat.PREC_F64, f.ctypes.data,
+                                                             g.ctypes.data))
+    c = np.empty(3)
+    ctx.check(ctx._L.sqh_optimization_problem_batch(ctx.h, ps.ctypes.data, 3, nat.PREC_F64, c.ctypes.data))
+    for b in range(3):
+        fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b], trace_offset=80)
+        assert abs(f[b] - fo) < 1e-12 and abs(c[b] - fo) < 1e-12
+        assert np.max(np.abs(g[b] - go)) < 1e-12
