@@ -257,3 +257,42 @@ def test_fused_streaming_matches_oracle(oracle, n):
         for b in range(B):
             ref = oracle.apply_to(gates, ps[b] if per_b else ps[0], X[b])
             assert rel_fro(got[b], ref) < 1e-12
+
+
+def test_streaming_float32_faster_than_float64():
+    """Counterpart of the reference's tests/gates/test_float32_performance.py:23-116
+    (U3 in float32 >= 2x faster than float64, CNOT >= 0.25x, 12 qubits x 256 columns):
+    on the GPU the unfused per-gate pass is HBM-bound, so complex64 moves half the
+    bytes.  Bounds are lenient (1.4x, 0.25x) for a shared device."""
+    import time
+    import torch
+    from squander_hip import Context
+    say("streaming f32 vs f64, 12 qubits x 256 columns")
+    n, m, B = 12, 256, 16
+    d = 1 << n
+    res = {}
+    for name, gate in (("U3", [[W.GATE_U3, 5, -1, 0]]), ("CNOT", [[W.GATE_CNOT, 5, 7, 0]])):
+        gates = np.array(gate * 8, dtype=np.int32)
+        gates[:, 3] = np.arange(8) * (3 if name == "U3" else 0)
+        P = W.param_count(gates)
+        ctx = Context(0)
+        ctx.set_circuit(n, gates)
+        p = torch.from_numpy(W.random_params(max(P, 1), 3)).cuda()
+        times = {}
+        for f32 in (False, True):
+            dt = torch.complex64 if f32 else torch.complex128
+            M = torch.zeros((B, d, m), dtype=dt, device="cuda")
+            torch.cuda.synchronize()
+            call = lambda: ctx.check(ctx._L.sqh_apply_to_stream_dev(ctx.h, p.data_ptr(), 0, M.data_ptr(), d, m, m,
+                                                                    d * m, B, int(f32)))
+            call()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                call()
+            ctx.synchronize()
+            times[f32] = (time.perf_counter() - t0) / 5
+        res[name] = times[False] / times[True]
+        say(f"  {name}: f64 {times[False] * 1e3:.3f} ms, f32 {times[True] * 1e3:.3f} ms, speedup {res[name]:.2f}")
+    assert res["U3"] >= 1.4
+    assert res["CNOT"] >= 0.25
