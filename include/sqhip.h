@@ -58,7 +58,11 @@ extern "C" {
 /* Reference cost_function_type values (Optimization_Interface.h:43-45);
  * SQH_COST_ABS_TRACE = 1 - |Tr|/m has no reference counterpart. */
 #define SQH_COST_FROBENIUS 0
+#define SQH_COST_FROBENIUS_CORRECTION1 1   /* FROBENIUS_NORM_CORRECTION1 */
+#define SQH_COST_FROBENIUS_CORRECTION2 2   /* FROBENIUS_NORM_CORRECTION2 */
 #define SQH_COST_HILBERT_SCHMIDT 3
+#define SQH_COST_HILBERT_SCHMIDT_CORRECTION1 4
+#define SQH_COST_HILBERT_SCHMIDT_CORRECTION2 5
 #define SQH_COST_INFIDELITY 9
 #define SQH_COST_ABS_TRACE 100
 
@@ -95,6 +99,11 @@ int sqh_get_parameter_num(const sqh_ctx* ctx);
 int sqh_set_target(sqh_ctx* ctx, const double* U, int rows, int cols, int stride);
 int sqh_set_trace_offset(sqh_ctx* ctx, int trace_offset);       /* set_Trace_Offset */
 int sqh_set_cost_function(sqh_ctx* ctx, int variant);           /* set_Cost_Function_Variant */
+/* Weights of the correction variants (Optimization_Interface.h:83-89 members
+ * prev_cost_fnv_val, correction1_scale, correction2_scale; defaults 1, 1/1.7, 1/2):
+ * the single / double bit-flip traces enter with sqrt(prev_cost) * scale1 / scale2
+ * (Optimization_Interface.cpp:676-716).  prev_cost must be >= 0. */
+int sqh_set_correction_weights(sqh_ctx* ctx, double prev_cost, double scale1, double scale2);
 
 /* ---- Gates_block::apply_to -------------------------------------------------- */
 /* In place on a host matrix. is_f32: complex64 matrix + float32 parameters. */

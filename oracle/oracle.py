@@ -24,6 +24,7 @@ RX/RY/RZ/CRY angle and for U3's theta (U3.cpp:49-51, RX.cpp:24).
 from __future__ import annotations
 
 import ctypes
+import itertools
 import os
 import subprocess
 
@@ -38,6 +39,13 @@ PARAM_NUM = {U3: 3, RX: 1, RY: 1, RZ: 1, CRY: 1, CNOT: 0, CZ: 0}
 
 # Reference cost_function_type values (Optimization_Interface.h:43-45) + ABS_TRACE.
 COST_FROBENIUS, COST_HS, COST_INFIDELITY, COST_ABS_TRACE = 0, 3, 9, 100
+COST_FROBENIUS_C1, COST_FROBENIUS_C2, COST_HS_C1, COST_HS_C2 = 1, 2, 4, 5
+# Optimization_Interface.cpp:74-76: prev_cost_fnv_val, correction1_scale, correction2_scale
+DEFAULT_CORRECTION = (1.0, 1.0 / 1.7, 1.0 / 2.0)
+
+
+class _C128(ctypes.Structure):
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
 
 
 class _Gate(ctypes.Structure):
@@ -49,7 +57,8 @@ class _Problem(ctypes.Structure):
     _fields_ = [("gates", ctypes.POINTER(_Gate)), ("ng", ctypes.c_int), ("P", ctypes.c_int),
                 ("qbit_num", ctypes.c_int), ("target", ctypes.c_void_p),
                 ("rows", ctypes.c_int), ("cols", ctypes.c_int), ("stride", ctypes.c_int),
-                ("variant", ctypes.c_int), ("trace_offset", ctypes.c_int)]
+                ("variant", ctypes.c_int), ("trace_offset", ctypes.c_int),
+                ("prev_cost", ctypes.c_double), ("scale1", ctypes.c_double), ("scale2", ctypes.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -97,6 +106,7 @@ class Oracle:
 
     def __init__(self):
         self.lib = ctypes.CDLL(build())
+        self.correction = DEFAULT_CORRECTION
         L = self.lib
         dp = ctypes.POINTER(ctypes.c_double)
         L.or_apply_circuit.argtypes = [ctypes.POINTER(_Gate), ctypes.c_int, dp, ctypes.c_void_p,
@@ -107,6 +117,12 @@ class Oracle:
         L.or_deriv_kernel.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_void_p]
         L.or_cost.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_cost.restype = ctypes.c_double
+        L.or_cost_corr.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.or_cost_corr.restype = ctypes.c_double
+        L.or_trace_flips.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int]
+        L.or_trace_flips.restype = _C128
         pp = ctypes.POINTER(_Problem)
         L.or_optimization_problem.argtypes = [pp, dp]
         L.or_optimization_problem.restype = ctypes.c_double
@@ -151,8 +167,9 @@ class Oracle:
     def _problem(self, gates, target, qbit_num, variant, trace_offset):
         garr, ng = self._gates(gates)
         target = np.ascontiguousarray(target, dtype=np.complex128)
+        prev, s1, s2 = self.correction
         pr = _Problem(garr, ng, param_count(gates), qbit_num, target.ctypes.data,
-                      target.shape[0], target.shape[1], target.shape[1], variant, trace_offset)
+                      target.shape[0], target.shape[1], target.shape[1], variant, trace_offset, prev, s1, s2)
         return pr, (garr, target)
 
     # -- gate level --------------------------------------------------------
@@ -193,8 +210,20 @@ class Oracle:
         return out
 
     def cost(self, M, variant=COST_FROBENIUS, trace_offset=0):
+        """Cost of a transformed matrix (rows = 2^qbit_num); the correction variants use
+        self.correction = (prev_cost, scale1, scale2)."""
         M = np.ascontiguousarray(M, dtype=np.complex128)
-        return self.lib.or_cost(M.ctypes.data, M.shape[1], M.shape[1], variant, trace_offset)
+        n = int(M.shape[0]).bit_length() - 1
+        prev, s1, s2 = self.correction
+        r = np.sqrt(prev)
+        return self.lib.or_cost_corr(M.ctypes.data, n, M.shape[1], M.shape[1], variant, trace_offset, r * s1, r * s2)
+
+    def trace_flips(self, M, bits, trace_offset=0):
+        """sum_c sum_{|mask| = bits} M[(c + trace_offset) ^ mask, c] as a complex number."""
+        M = np.ascontiguousarray(M, dtype=np.complex128)
+        n = int(M.shape[0]).bit_length() - 1
+        t = self.lib.or_trace_flips(M.ctypes.data, n, M.shape[1], M.shape[1], trace_offset, bits)
+        return complex(t.re, t.im)
 
     def optimization_problem(self, gates, target, qbit_num, params, variant=COST_FROBENIUS, trace_offset=0):
         pr, keep = self._problem(gates, target, qbit_num, variant, trace_offset)
@@ -346,8 +375,39 @@ def brute_trace(M, trace_offset=0):
     return np.sum(M[np.arange(m) + trace_offset, np.arange(m)])
 
 
-def brute_cost(M, variant=COST_FROBENIUS, trace_offset=0):
+def brute_flip_trace(M, bits, trace_offset=0):
+    """Bit-flip trace from operators instead of index arithmetic: the X string on a
+    mask (a Kronecker product of Pauli X and identities, qubit 0 the least significant
+    index bit) times the shift that puts column c on row c + trace_offset selects the
+    entries M[(c + offset) ^ mask, c]; their sum is <S, M> summed over the masks."""
+    d, m = M.shape
+    n = d.bit_length() - 1
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    shift = np.zeros((d, m), dtype=complex)
+    shift[np.arange(m) + trace_offset, np.arange(m)] = 1.0
+    total = 0.0j
+    for qs in itertools.combinations(range(n), bits):
+        op = np.ones((1, 1), dtype=complex)
+        for q in reversed(range(n)):  # kron(A_{n-1}, ..., A_0): qubit 0 last
+            op = np.kron(op, X if q in qs else np.eye(2))
+        total += np.sum((op @ shift) * M)
+    return total
+
+
+def brute_cost(M, variant=COST_FROBENIUS, trace_offset=0, correction=DEFAULT_CORRECTION):
     m = M.shape[1]
+    prev, s1, s2 = correction
+    w1, w2 = np.sqrt(prev) * s1, np.sqrt(prev) * s2
+    if variant in (COST_FROBENIUS_C1, COST_FROBENIUS_C2):
+        c = 1.0 - (brute_trace(M, trace_offset).real + w1 * brute_flip_trace(M, 1, trace_offset).real) / m
+        if variant == COST_FROBENIUS_C2:
+            c -= w2 * brute_flip_trace(M, 2, trace_offset).real / m
+        return c
+    if variant in (COST_HS_C1, COST_HS_C2):
+        s = abs(brute_trace(M, 0)) ** 2 + w1 * abs(brute_flip_trace(M, 1)) ** 2
+        if variant == COST_HS_C2:
+            s += w2 * abs(brute_flip_trace(M, 2)) ** 2
+        return 1.0 - s / m ** 2
     if variant == COST_FROBENIUS:
         return 1.0 - brute_trace(M, trace_offset).real / m
     if variant == COST_HS:

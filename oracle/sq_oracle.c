@@ -284,6 +284,71 @@ double or_cost(const or_c128* M, int cols, int stride, int variant, int trace_of
     }
 }
 
+/* The bit-flip sums of get_cost_function_with_correction[2]
+ * (N_Qubit_Decomposition_Cost_Function.cpp:191-252, 291-380; loops over the qubit
+ * (pair) outside, columns inside) and get_trace_with_correction[2] (:564-664, the
+ * same sums with trace_offset 0, real and imaginary parts). */
+or_c128 or_trace_flips(const or_c128* M, int qbit_num, int cols, int stride, int trace_offset, int bits) {
+    or_c128 t = {0.0, 0.0};
+    for (int q = 0; q < qbit_num; q++) {
+        if (bits == 1) {
+            const int mask = 1 << q;
+            for (int c = 0; c < cols; c++) {
+                const or_c128 v = M[(long)((c + trace_offset) ^ mask) * stride + c];
+                t.re += v.re; t.im += v.im;
+            }
+        } else {
+            for (int q2 = q + 1; q2 < qbit_num; q2++) {
+                const int mask = (1 << q) + (1 << q2);
+                for (int c = 0; c < cols; c++) {
+                    const or_c128 v = M[(long)((c + trace_offset) ^ mask) * stride + c];
+                    t.re += v.re; t.im += v.im;
+                }
+            }
+        }
+    }
+    return t;
+}
+
+/* calculate_cost_function (Optimization_Interface.cpp:676-716): FROBENIUS_CORRECTION1/2
+ * = get_cost_function - sqrt(prev) * (scale1 * flips1 + scale2 * flips2) / m;
+ * HILBERT_SCHMIDT_TEST_CORRECTION1/2 = 1 - (|T|^2 + sqrt(prev) * (scale1 |T1|^2 +
+ * scale2 |T2|^2)) / m^2 on the plain trace T and the offset-free flip traces. */
+double or_cost_corr(const or_c128* M, int qbit_num, int cols, int stride, int variant, int trace_offset,
+                    double w1, double w2) {
+    const double m = (double)cols;
+    switch (variant) {
+    case OR_COST_FROBENIUS_C1:
+    case OR_COST_FROBENIUS_C2: {
+        const or_c128 t0 = or_trace_offset(M, cols, stride, trace_offset);
+        const or_c128 t1 = or_trace_flips(M, qbit_num, cols, stride, trace_offset, 1);
+        double c = (1.0 - t0.re / m) - w1 * (t1.re / m);
+        if (variant == OR_COST_FROBENIUS_C2) {
+            const or_c128 t2 = or_trace_flips(M, qbit_num, cols, stride, trace_offset, 2);
+            c -= w2 * (t2.re / m);
+        }
+        return c;
+    }
+    case OR_COST_HS_C1:
+    case OR_COST_HS_C2: {
+        const or_c128 t0 = or_trace_plain(M, cols, stride);
+        const or_c128 t1 = or_trace_flips(M, qbit_num, cols, stride, 0, 1);
+        const double d = 1.0 / m;
+        double s = t0.re * t0.re + t0.im * t0.im + w1 * (t1.re * t1.re + t1.im * t1.im);
+        if (variant == OR_COST_HS_C2) {
+            const or_c128 t2 = or_trace_flips(M, qbit_num, cols, stride, 0, 2);
+            s += w2 * (t2.re * t2.re + t2.im * t2.im);
+        }
+        return 1.0 - d * d * s;
+    }
+    default:
+        return or_cost(M, cols, stride, variant, trace_offset);
+    }
+}
+
+static double pr_w1(const or_problem* pr) { return sqrt(pr->prev_cost) * pr->scale1; }
+static double pr_w2(const or_problem* pr) { return sqrt(pr->prev_cost) * pr->scale2; }
+
 /* Optimization_Interface::optimization_problem (Optimization_Interface.cpp:634-668):
  * copy the target, apply the circuit, reduce the cost. */
 double or_optimization_problem(const or_problem* pr, const double* params) {
@@ -293,7 +358,8 @@ double or_optimization_problem(const or_problem* pr, const double* params) {
     for (int r = 0; r < pr->rows; r++)
         memcpy(M + (size_t)r * pr->cols, pr->target + (size_t)r * pr->stride, sizeof(or_c128) * pr->cols);
     or_apply_circuit(pr->gates, pr->ng, params, M, pr->rows, pr->cols, pr->cols);
-    double c = or_cost(M, pr->cols, pr->cols, pr->variant, pr->trace_offset);
+    double c = or_cost_corr(M, pr->qbit_num, pr->cols, pr->cols, pr->variant, pr->trace_offset, pr_w1(pr),
+                            pr_w2(pr));
     free(M);
     return c;
 }
@@ -310,17 +376,46 @@ int or_optimization_problem_combined(const or_problem* pr, const double* params,
         memcpy(M + (size_t)r * cols, pr->target + (size_t)r * pr->stride, sizeof(or_c128) * cols);
     or_apply_derivate(pr->gates, pr->ng, params, M, rows, cols, cols, D);
     or_apply_circuit(pr->gates, pr->ng, params, M, rows, cols, cols);
-    *f0 = or_cost(M, cols, cols, pr->variant, pr->trace_offset);
+    const double w1 = pr_w1(pr), w2 = pr_w2(pr);
+    *f0 = or_cost_corr(M, pr->qbit_num, cols, cols, pr->variant, pr->trace_offset, w1, w2);
     const double m = (double)cols;
     or_c128 t = (pr->variant == OR_COST_FROBENIUS || pr->variant == OR_COST_ABS_TRACE)
                     ? or_trace_offset(M, cols, cols, pr->trace_offset)
                     : or_trace_plain(M, cols, cols);
+    /* offset-free flip traces of C.U for the Hilbert-Schmidt corrections */
+    const or_c128 t1 = or_trace_flips(M, pr->qbit_num, cols, cols, 0, 1);
+    const or_c128 t2 = or_trace_flips(M, pr->qbit_num, cols, cols, 0, 2);
     for (int k = 0; k < P; k++) {
         const or_c128* Dk = D + (size_t)k * msz;
         switch (pr->variant) {
         case OR_COST_FROBENIUS:
             grad[k] = (1.0 - or_trace_offset(Dk, cols, cols, pr->trace_offset).re / m) - 1.0;
             break;
+        /* Optimization_Interface.cpp:1396-1405: the derivative's correction cost minus 1 */
+        case OR_COST_FROBENIUS_C1:
+        case OR_COST_FROBENIUS_C2: {
+            const or_c128 d0 = or_trace_offset(Dk, cols, cols, pr->trace_offset);
+            const or_c128 d1 = or_trace_flips(Dk, pr->qbit_num, cols, cols, pr->trace_offset, 1);
+            double g = (1.0 - d0.re / m) - w1 * (d1.re / m);
+            if (pr->variant == OR_COST_FROBENIUS_C2)
+                g -= w2 * (or_trace_flips(Dk, pr->qbit_num, cols, cols, pr->trace_offset, 2).re / m);
+            grad[k] = g - 1.0;
+            break;
+        }
+        /* Optimization_Interface.cpp:1412-1423 */
+        case OR_COST_HS_C1:
+        case OR_COST_HS_C2: {
+            const or_c128 d0 = or_trace_plain(Dk, cols, cols);
+            const or_c128 d1 = or_trace_flips(Dk, pr->qbit_num, cols, cols, 0, 1);
+            const double d = 1.0 / m;
+            double s = t.re * d0.re + t.im * d0.im + w1 * (t1.re * d1.re + t1.im * d1.im);
+            if (pr->variant == OR_COST_HS_C2) {
+                const or_c128 d2 = or_trace_flips(Dk, pr->qbit_num, cols, cols, 0, 2);
+                s += w2 * (t2.re * d2.re + t2.im * d2.im);
+            }
+            grad[k] = -2.0 * d * d * s;
+            break;
+        }
         case OR_COST_HS: {
             or_c128 tk = or_trace_plain(Dk, cols, cols);
             const double d = 1.0 / m;

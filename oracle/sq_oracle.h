@@ -32,7 +32,9 @@ enum {
  * (Optimization_Interface.h:43-45).  OR_COST_ABS_TRACE (1 - |Tr|/m) has no
  * counterpart in the reference and gets an id outside its range. */
 enum {
-    OR_COST_FROBENIUS = 0, OR_COST_HS = 3, OR_COST_INFIDELITY = 9, OR_COST_ABS_TRACE = 100
+    OR_COST_FROBENIUS = 0, OR_COST_FROBENIUS_C1 = 1, OR_COST_FROBENIUS_C2 = 2,
+    OR_COST_HS = 3, OR_COST_HS_C1 = 4, OR_COST_HS_C2 = 5,
+    OR_COST_INFIDELITY = 9, OR_COST_ABS_TRACE = 100
 };
 
 typedef struct { int type, target, control, param_start; } or_gate;
@@ -64,12 +66,21 @@ int or_apply_derivate(const or_gate* gates, int ng, const double* params,
 or_c128 or_trace_offset(const or_c128* M, int cols, int stride, int trace_offset);
 or_c128 or_trace_plain(const or_c128* M, int cols, int stride);
 double or_cost(const or_c128* M, int cols, int stride, int variant, int trace_offset);
+/* Bit-flip traces of the correction terms: sum over columns c and over the masks of
+ * `bits` (1 or 2) set bits below 2^qbit_num of M[(c + trace_offset) ^ mask, c]. */
+or_c128 or_trace_flips(const or_c128* M, int qbit_num, int cols, int stride, int trace_offset, int bits);
+/* Every variant, the correction ones with w1 = sqrt(prev_cost) * scale1 and
+ * w2 = sqrt(prev_cost) * scale2. */
+double or_cost_corr(const or_c128* M, int qbit_num, int cols, int stride, int variant, int trace_offset,
+                    double w1, double w2);
 
 /* A decomposition problem: target (rows x cols), circuit, cost variant. */
 typedef struct {
     const or_gate* gates; int ng; int P; int qbit_num;
     const or_c128* target; int rows, cols, stride;
     int variant, trace_offset;
+    /* Optimization_Interface's prev_cost_fnv_val, correction1_scale, correction2_scale */
+    double prev_cost, scale1, scale2;
 } or_problem;
 
 double or_optimization_problem(const or_problem* pr, const double* params);

@@ -4,6 +4,7 @@
 #include "sqh_context.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -36,10 +37,26 @@ int waves_per_batch(int N, int m_pad) {
     return m_pad >= cw ? m_pad / cw : 1;
 }
 
-bool uses_offset(int variant) { return variant == CV_FROBENIUS || variant == CV_ABS_TRACE; }
+// The FROBENIUS family reads the offset trace (N_Qubit_Decomposition_Cost_Function.cpp:
+// 137-158, 191-380); the Hilbert-Schmidt family and INFIDELITY the plain one (:482-664).
+bool uses_offset(int variant) {
+    return variant == CV_FROBENIUS || variant == CV_FROBENIUS_C1 || variant == CV_FROBENIUS_C2 ||
+           variant == CV_ABS_TRACE;
+}
 
 static bool valid_variant(int v) {
-    return v == CV_FROBENIUS || v == CV_HS || v == CV_INFIDELITY || v == CV_ABS_TRACE;
+    return v == CV_FROBENIUS || v == CV_FROBENIUS_C1 || v == CV_FROBENIUS_C2 || v == CV_HS || v == CV_HS_C1 ||
+           v == CV_HS_C2 || v == CV_INFIDELITY || v == CV_ABS_TRACE;
+}
+
+TraceSpec trace_spec(sqh_ctx* c, int B, int m) {
+    TraceSpec ts;
+    ts.nch = correction_channels(c->variant);
+    ts.chstride = (long long)B * m;
+    const double r = std::sqrt(c->prev_cost);
+    ts.w1 = ts.nch >= 1 ? r * c->corr_scale1 : 0.0;
+    ts.w2 = ts.nch >= 2 ? r * c->corr_scale2 : 0.0;
+    return ts;
 }
 
 int require_circuit(sqh_ctx* c) {
@@ -135,8 +152,10 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
     const bool f32 = precision == SQH_PREC_F32;
     const int N = c->prog.N, m = c->cols, ns = (int)c->prog.slots.size();
     const int off = uses_offset(c->variant) ? c->trace_offset : 0;
+    const TraceSpec ts = trace_spec(c, B, m);
+    const bool hs_corr = ts.nch > 0 && !uses_offset(c->variant);
     if ((rc = build_kernels(c, d_params, B, f32))) return rc;
-    HIPTRY(c, c->d_diag.ensure((size_t)B * m * 2 * sizeof(double)));
+    HIPTRY(c, c->d_diag.ensure((size_t)(ts.nch + 1) * B * m * 2 * sizeof(double)));
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
     if (N <= kMaxTileQubits) {
         TileLaunch t = base_launch(c, f32);
@@ -149,10 +168,23 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
         t.kern_bstride = (long long)ns * 4;
         t.trace_off = off;
         t.diag = c->d_diag.as<double>();
+        t.ts = ts;
         const int wpb = waves_per_batch(N, t.m_pad);
         if (d_grad) {
             HIPTRY(c, c->d_mpart.ensure((size_t)B * wpb * std::max(ns, 1) * 8 * sizeof(double)));
             t.mpart = c->d_mpart.as<double>();
+            if (hs_corr) {
+                // |t_k|^2 terms: a forward pass gives the channel traces, whose conjugates
+                // (weighted) seed the adjoint sweep.
+                HIPTRY(c, c->d_seed.ensure((size_t)B * 6 * sizeof(double)));
+                TileLaunch tf = t;
+                tf.mode = 1;
+                tf.mpart = nullptr;
+                HIPTRY(c, launch_tile_forward(tf, c->stream));
+                HIPTRY(c, launch_finish(t.diag, m, B, c->variant, nullptr, nullptr, 0, ns, t.slots, d_params,
+                                        c->prog.P, nullptr, c->stream, BatchIndex(), ts, c->d_seed.as<double>()));
+                t.ts.seed = c->d_seed.as<double>();
+            }
             {
                 TimedLaunch tl(c);
                 if (jit_wants(c, 2)) {
@@ -162,7 +194,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
                 }
             }
             HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, t.mpart, wpb, ns, t.slots, d_params, c->prog.P,
-                                    d_grad, c->stream));
+                                    d_grad, c->stream, BatchIndex(), ts));
         } else {
             t.mode = 1;
             {
@@ -174,7 +206,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
                 }
             }
             HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, nullptr, 0, ns, t.slots, d_params, c->prog.P,
-                                    nullptr, c->stream));
+                                    nullptr, c->stream, BatchIndex(), ts));
         }
         return SQH_OK;
     }
@@ -185,9 +217,9 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
     HIPTRY(c, c->d_mat.ensure((size_t)B * msz * esz));
     HIPTRY(c, launch_broadcast_copy(f32, U, c->rows, m, m, c->d_mat.p, msz, B, c->stream));
     if ((rc = fused_apply(c, c->d_mat.p, c->rows, m, m, msz, B, f32, (long long)ns * 4))) return rc;
-    HIPTRY(c, launch_extract_diag(f32, c->d_mat.p, msz, m, m, B, off, c->d_diag.as<double>(), c->stream));
+    HIPTRY(c, launch_extract_diag(f32, c->d_mat.p, msz, m, m, B, off, c->d_diag.as<double>(), N, ts, c->stream));
     HIPTRY(c, launch_finish(c->d_diag.as<double>(), m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(),
-                            d_params, c->prog.P, nullptr, c->stream));
+                            d_params, c->prog.P, nullptr, c->stream, BatchIndex(), ts));
     return SQH_OK;
 }
 
@@ -370,6 +402,17 @@ int sqh_set_trace_offset(sqh_ctx* c, int trace_offset) {
     return SQH_OK;
 }
 
+int sqh_set_correction_weights(sqh_ctx* c, double prev_cost, double scale1, double scale2) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!(prev_cost >= 0.0) || !std::isfinite(prev_cost) || !std::isfinite(scale1) || !std::isfinite(scale2))
+        return fail(c, SQH_ERR_ARG, "correction weights must be finite and prev_cost non-negative");
+    c->prev_cost = prev_cost;
+    c->corr_scale1 = scale1;
+    c->corr_scale2 = scale2;
+    return SQH_OK;
+}
+
 int sqh_set_cost_function(sqh_ctx* c, int variant) {
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -506,6 +549,8 @@ int sqh_optimization_problem_combined_targets_dev(sqh_ctx* c, const double* d_ta
         return fail(c, SQH_ERR_ARG, "need targets, indices, a batch and output buffers");
     if (N > kMaxTileQubits) return fail(c, SQH_ERR_ARG, "gradient evaluation supports up to %d qubits", kMaxTileQubits);
     if (precision != SQH_PREC_F64 && precision != SQH_PREC_F32) return fail(c, SQH_ERR_ARG, "bad precision");
+    if (correction_channels(c->variant) > 0)
+        return fail(c, SQH_ERR_ARG, "per-target evaluation does not support the correction cost variants");
     const bool f32 = precision == SQH_PREC_F32;
     const int d = 1 << N;
     const long long msz = (long long)d * d;

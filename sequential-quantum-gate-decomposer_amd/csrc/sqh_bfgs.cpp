@@ -80,6 +80,7 @@ TileLaunch group_launch(sqh_ctx* c, BatchBufs& bb, Group& gr, int g, int p, int 
     t.kern_bstride = (long long)gr.ns * 4;
     t.trace_off = uses_offset(c->variant) && !grouped ? c->trace_offset : 0;
     t.diag = gr.diag.as<double>();
+    t.ts = trace_spec(c, gr.B, gr.cols);
     t.mpart = gr.mpart.as<double>();
     t.bi.count = bb.counts.as<int>() + 4 + 2 * g + p;
     t.bi.elem_run = bb.lists.as<int>() + (size_t)(2 * g + p) * B;
@@ -121,7 +122,7 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
                 HIPTRY(c, launch_tile_grad(t, st));
             }
             HIPTRY(c, launch_finish(t.diag, gr.cols, gr.B, c->variant, bb.fe.as<double>(), t.mpart, gr.wpb, gr.ns,
-                                    gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), st, t.bi));
+                                    gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), st, t.bi, t.ts));
         }
     }
     if (fork) {
@@ -147,7 +148,7 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
         const size_t kern_elems = (size_t)g.B * std::max(g.ns, 1) * 4;
         HIPTRY(c, g.kern64.ensure(kern_elems * 16));
         HIPTRY(c, g.kern32.ensure(kern_elems * 8));
-        HIPTRY(c, g.diag.ensure((size_t)g.B * g.cols * 2 * sizeof(double)));
+        HIPTRY(c, g.diag.ensure((size_t)(correction_channels(c->variant) + 1) * g.B * g.cols * 2 * sizeof(double)));
         HIPTRY(c, g.mpart.ensure((size_t)g.B * g.wpb * std::max(g.ns, 1) * 8 * sizeof(double)));
     }
     if (G > 1) {  // side streams and fork/join events, kept by the context
@@ -263,6 +264,10 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
 int check_opts(sqh_ctx* c, const sqh_decompose_options* opts) {
     if (opts->max_inner_iterations <= 0 || opts->max_outer_iterations <= 0)
         return fail(c, SQH_ERR_ARG, "iteration limits must be positive");
+    // The Hilbert-Schmidt corrections need each point's traces before its gradient
+    // sweep (a second pass per round); the batched optimiser does not run it.
+    if (c->variant == CV_HS_C1 || c->variant == CV_HS_C2)
+        return fail(c, SQH_ERR_ARG, "batched optimisation does not support the Hilbert-Schmidt correction variants");
     return SQH_OK;
 }
 

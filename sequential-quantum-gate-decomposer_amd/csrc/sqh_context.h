@@ -68,8 +68,11 @@ struct sqh_ctx {
     sqh::DevBuf d_U64, d_U32;
     int trace_offset = 0;
     int variant = sqh::CV_FROBENIUS;
+    // correction-variant weights (Optimization_Interface.cpp:74-76 defaults)
+    double prev_cost = 1.0, corr_scale1 = 1.0 / 1.7, corr_scale2 = 1.0 / 2.0;
 
     sqh::DevBuf d_params, d_kern, d_diag, d_mpart, d_res, d_mat, d_out;
+    sqh::DevBuf d_seed;  // [B][3] complex adjoint seed weights of the HS correction variants
     sqh::DevBuf d_idx;  // [count][0..B) identity element list of the per-target evaluation
     int idx_B = -1;
 
@@ -92,7 +95,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx, &d_seed})
             b->release();
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);
@@ -115,6 +118,9 @@ int fail(sqh_ctx* c, int code, const char* fmt, ...);
 int m_pad_for(int N, int m);
 int waves_per_batch(int N, int m_pad);
 bool uses_offset(int variant);
+// Trace channels of the context's cost variant for a launch over B elements of m
+// columns (channel stride B * m), with the constant seed weights.
+TraceSpec trace_spec(sqh_ctx* c, int B, int m);
 int require_circuit(sqh_ctx* c);
 int require_target(sqh_ctx* c);
 int build_kernels(sqh_ctx* c, const double* d_params, int B, bool f32);

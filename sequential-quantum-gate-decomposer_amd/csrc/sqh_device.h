@@ -396,6 +396,85 @@ __device__ __forceinline__ LaneInfo lane_info(int m, int m_pad, int B, int lane,
     return li;
 }
 
+// Trace channels of column li.col from the NR rows this lane holds (row so +
+// lay->reg_off[i] in register i), tr = li.col + trace offset.  Without extra
+// channels only the lane holding row tr writes; with them each lane sums its rows
+// by popcount(row ^ tr), the column's L lanes (lane bits above log2 CW) combine
+// their sums, and the sub-0 lane writes channels 0..nch.
+template <int N, typename R>
+__device__ __forceinline__ void emit_trace(const cpx<R>* v, const Layout* lay, int so, const LaneInfo& li, int tr,
+                                           int m, double* __restrict__ diag, const TraceSpec& ts) {
+    using G = Geo<N>;
+    const long long at = (long long)li.b * m + li.col;
+    if (ts.nch == 0) {
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const int row = so + lay->reg_off[i];
+            if (li.valid && row == tr) {
+                diag[2 * at] = (double)v[i].re;
+                diag[2 * at + 1] = (double)v[i].im;
+            }
+        }
+        return;
+    }
+    double a[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+#pragma unroll
+    for (int i = 0; i < G::NR; ++i) {
+        const int pc = __popc((unsigned)((so + lay->reg_off[i]) ^ tr));
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (li.valid && pc == k && k <= ts.nch) {
+                a[k][0] += (double)v[i].re;
+                a[k][1] += (double)v[i].im;
+            }
+    }
+#pragma unroll
+    for (int o = G::CW; o < 64; o <<= 1)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            a[k][0] += __shfl_xor(a[k][0], o, 64);
+            a[k][1] += __shfl_xor(a[k][1], o, 64);
+        }
+    if (li.valid && li.s == 0)
+        for (int k = 0; k <= ts.nch; ++k) {
+            diag[2 * (k * ts.chstride + at)] = a[k][0];
+            diag[2 * (k * ts.chstride + at) + 1] = a[k][1];
+        }
+}
+
+// Adjoint seed of column li.col: weight w[popcount(row ^ tr)] on the rows within
+// nch bit flips of tr (w = (1, w1, w2), or the element's weights from ts.seed).
+template <int N, typename R>
+__device__ __forceinline__ void seed_trace(cpx<R>* y, const Layout* lay, int so, const LaneInfo& li, int tr,
+                                           const TraceSpec& ts) {
+    using G = Geo<N>;
+    if (ts.nch == 0) {
+#pragma unroll
+        for (int i = 0; i < G::NR; ++i) {
+            const bool hit = li.valid && so + lay->reg_off[i] == tr;
+            y[i] = mk<R>(hit ? (R)1 : (R)0, (R)0);
+        }
+        return;
+    }
+    double w[3][2] = {{1.0, 0.0}, {ts.w1, 0.0}, {ts.w2, 0.0}};
+    if (ts.seed) {
+        const double* sw = ts.seed + 6 * (long long)li.b;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            w[k][0] = sw[2 * k];
+            w[k][1] = sw[2 * k + 1];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < G::NR; ++i) {
+        const int pc = __popc((unsigned)((so + lay->reg_off[i]) ^ tr));
+        y[i] = mk<R>((R)0, (R)0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (li.valid && pc == k && k <= ts.nch) y[i] = mk<R>((R)w[k][0], (R)w[k][1]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Forward tile kernel: apply the whole program, then either store the slab
 // (Gates_block::apply_to) or emit the diagonal entries the trace needs.
@@ -410,7 +489,8 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
                                                            int B, const Op* __restrict__ ops, int nops,
                                                            const Layout* __restrict__ lays,
                                                            const cpx<R>* __restrict__ kern, long long kern_bstride,
-                                                           int trace_off, double* __restrict__ diag, int ns) {
+                                                           int trace_off, double* __restrict__ diag, int ns,
+                                                           TraceSpec ts) {
     using G = Geo<N>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -463,15 +543,7 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
             if (li.valid) dst[(long long)row * out_stride + li.col] = v[i];
         }
     } else {
-        const int tr = li.col + trace_off;
-#pragma unroll
-        for (int i = 0; i < G::NR; ++i) {
-            const int row = so + lays[cur].reg_off[i];
-            if (li.valid && row == tr) {
-                diag[2 * ((long long)li.b * m + li.col)] = (double)v[i].re;
-                diag[2 * ((long long)li.b * m + li.col) + 1] = (double)v[i].im;
-            }
-        }
+        emit_trace<N, R>(v, &lays[cur], so, li, li.col + trace_off, m, diag, ts);
     }
 }
 
@@ -543,7 +615,7 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
                                                   const Layout* __restrict__ lays, const cpx<R>* __restrict__ kern,
                                                   long long kern_bstride, int trace_off, int ns,
                                                   double* __restrict__ diag, double* __restrict__ mpart,
-                                                  BatchIndex bi, int ops_cap, int lays_cap) {
+                                                  BatchIndex bi, int ops_cap, int lays_cap, TraceSpec ts) {
     using G = Geo<N>;
     constexpr int CW = G::CW;
     __shared__ cpx<R> lds[G::LDS];
@@ -615,20 +687,12 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
             for (int e = 0; e < 4; ++e) os.k[e] = kn[e];
         }
     }
-    // Diagonal of C.U and the selector E under the final layout.
+    // Trace channels of C.U and the selector (adjoint seed) under the final layout.
     {
         const int so = sub_off<N>(&lays[cur], li.s);
         const int tr = li.col + trace_off;
-#pragma unroll
-        for (int i = 0; i < G::NR; ++i) {
-            const int row = so + lays[cur].reg_off[i];
-            const bool hit = li.valid && row == tr;
-            if (hit) {
-                diag[2 * ((long long)li.b * m + li.col)] = (double)x[i].re;
-                diag[2 * ((long long)li.b * m + li.col) + 1] = (double)x[i].im;
-            }
-            y[i] = mk<R>(hit ? (R)1 : (R)0, (R)0);
-        }
+        emit_trace<N, R>(x, &lays[cur], so, li, tr, m, diag, ts);
+        seed_trace<N, R>(y, &lays[cur], so, li, tr, ts);
     }
     // Which lanes write the per-wave partials.
     const int lim = m_pad < CW ? m_pad : CW;
@@ -844,27 +908,66 @@ __device__ inline double grad_of(int variant, double tre, double tim, double kre
     }
 }
 
+// The correction variants (Optimization_Interface.cpp:676-716, 1391-1420) from the
+// channel traces t[k] = (re, im): FROBENIUS_C* = 1 - (Re t0 + w1 Re t1 + w2 Re t2)/m
+// (get_cost_function_with_correction[2]: N_Qubit_Decomposition_Cost_Function.cpp:191-380),
+// HS_C* = 1 - (|t0|^2 + w1 |t1|^2 + w2 |t2|^2)/m^2 (get_trace_with_correction[2]: :564-664),
+// w_k = sqrt(previous cost) * correction_k scale, w2 = 0 for the C1 variants.
+__device__ inline double cost_corr(int variant, const double t[3][2], double w1, double w2, double m) {
+    if (variant == CV_FROBENIUS_C1 || variant == CV_FROBENIUS_C2)
+        return 1.0 - (t[0][0] + w1 * t[1][0] + w2 * t[2][0]) / m;
+    const double d = 1.0 / m;
+    return 1.0 - d * d *
+                     (t[0][0] * t[0][0] + t[0][1] * t[0][1] + w1 * (t[1][0] * t[1][0] + t[1][1] * t[1][1]) +
+                      w2 * (t[2][0] * t[2][0] + t[2][1] * t[2][1]));
+}
+
+// Gradient of a correction variant from kre + i kim = sum_ab D_ab M_ab, where M was
+// built with the seed weights: (1, w1, w2) for FROBENIUS_C* (linear in C.U), and
+// (conj t0, w1 conj t1, w2 conj t2) for HS_C*, so Re(sum D M) is the weighted sum of
+// Re(conj(t_k) dt_k) of the reference's formula.
+__device__ inline double grad_corr(int variant, double kre, double m) {
+    if (variant == CV_FROBENIUS_C1 || variant == CV_FROBENIUS_C2) return -kre / m;
+    return -2.0 / (m * m) * kre;
+}
+
 #ifdef SQH_DEFINE_MISC_KERNELS
-// One workgroup per batch element: trace = sum of the diagonal entries, cost, and
-// (if mpart) the gradient from the per-gate correlations.
+// One workgroup per batch element: trace channels = sums of the per-column entries,
+// cost, and (if mpart) the gradient from the per-gate correlations; with seed_out,
+// also the HS_C* adjoint seed weights of the element.
 __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag, int m, int variant,
                                                 double* __restrict__ costs, const double* __restrict__ mpart,
                                                 int wpb, int ns, const Slot* __restrict__ slots,
                                                 const double* __restrict__ params, int P,
-                                                double* __restrict__ grad, BatchIndex bi) {
+                                                double* __restrict__ grad, BatchIndex bi, TraceSpec ts,
+                                                double* __restrict__ seed_out) {
     __shared__ double sh[8];
     const int b = blockIdx.x;
     if (bi.count && b >= *bi.count) return;
     // Results go to the run's slots; params/costs/grad are indexed by run.
     const int r = bi.count ? bi.elem_run[b] : b;
-    double re = 0.0, im = 0.0;
-    for (int c = threadIdx.x; c < m; c += blockDim.x) {
-        re += diag[2 * ((long long)b * m + c)];
-        im += diag[2 * ((long long)b * m + c) + 1];
+    double t[3][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    for (int k = 0; k <= ts.nch; ++k) {
+        const double* dk = diag + 2 * (k * ts.chstride + (long long)b * m);
+        double re = 0.0, im = 0.0;
+        for (int c = threadIdx.x; c < m; c += blockDim.x) {
+            re += dk[2 * c];
+            im += dk[2 * c + 1];
+        }
+        t[k][0] = block_sum(re, sh);
+        t[k][1] = block_sum(im, sh);
     }
-    const double tre = block_sum(re, sh);
-    const double tim = block_sum(im, sh);
-    if (threadIdx.x == 0 && costs) costs[r] = cost_of(variant, tre, tim, (double)m);
+    const double tre = t[0][0], tim = t[0][1];
+    const bool corr = ts.nch > 0;
+    if (threadIdx.x == 0 && costs)
+        costs[r] = corr ? cost_corr(variant, t, ts.w1, ts.w2, (double)m) : cost_of(variant, tre, tim, (double)m);
+    if (threadIdx.x == 0 && seed_out) {
+        const double w[3] = {1.0, ts.w1, ts.w2};
+        for (int k = 0; k < 3; ++k) {
+            seed_out[6 * (long long)b + 2 * k] = w[k] * t[k][0];
+            seed_out[6 * (long long)b + 2 * k + 1] = -w[k] * t[k][1];
+        }
+    }
     if (!mpart) return;
     int nsr = ns;  // ns: stride of the correlation table
     if (bi.run_prog) {
@@ -890,7 +993,8 @@ __global__ __launch_bounds__(256) void k_finish(const double* __restrict__ diag,
                 kre += d[e].re * M[2 * e] - d[e].im * M[2 * e + 1];
                 kim += d[e].re * M[2 * e + 1] + d[e].im * M[2 * e];
             }
-            grad[(long long)r * P + sl.param_start + q] = grad_of(variant, tre, tim, kre, kim, (double)m);
+            grad[(long long)r * P + sl.param_start + q] =
+                corr ? grad_corr(variant, kre, (double)m) : grad_of(variant, tre, tim, kre, kim, (double)m);
         }
     }
 }
@@ -938,15 +1042,41 @@ __global__ __launch_bounds__(256) void k_stream_gate(cpx<R>* __restrict__ mats, 
     *c = mac2(k2, x0, k3, x1);
 }
 
+// Trace channels of column c of each matrix (rows = 2^n): channel k sums
+// M[(c + off) ^ mask, c] over the masks of k set bits below 2^n, in the order of
+// get_cost_function_with_correction2 (N_Qubit_Decomposition_Cost_Function.cpp:291-380).
 template <typename R>
 __global__ void k_extract_diag(const cpx<R>* __restrict__ mats, long long bstride, int stride, int m, int B,
-                               int trace_off, double* __restrict__ diag) {
+                               int trace_off, double* __restrict__ diag, int n, TraceSpec ts) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (long long)B * m) return;
     const int b = (int)(i / m), c = (int)(i % m);
-    const cpx<R> v = mats[(long long)b * bstride + (long long)(c + trace_off) * stride + c];
+    const cpx<R>* M = mats + (long long)b * bstride;
+    const int tr = c + trace_off;
+    const cpx<R> v = M[(long long)tr * stride + c];
     diag[2 * i] = (double)v.re;
     diag[2 * i + 1] = (double)v.im;
+    if (ts.nch >= 1) {
+        double re = 0.0, im = 0.0;
+        for (int q = 0; q < n; ++q) {
+            const cpx<R> u = M[(long long)(tr ^ (1 << q)) * stride + c];
+            re += (double)u.re;
+            im += (double)u.im;
+        }
+        diag[2 * (ts.chstride + i)] = re;
+        diag[2 * (ts.chstride + i) + 1] = im;
+    }
+    if (ts.nch >= 2) {
+        double re = 0.0, im = 0.0;
+        for (int q = 0; q < n - 1; ++q)
+            for (int q2 = q + 1; q2 < n; ++q2) {
+                const cpx<R> u = M[(long long)(tr ^ (1 << q) ^ (1 << q2)) * stride + c];
+                re += (double)u.re;
+                im += (double)u.im;
+            }
+        diag[2 * (2 * ts.chstride + i)] = re;
+        diag[2 * (2 * ts.chstride + i) + 1] = im;
+    }
 }
 
 template <typename R>

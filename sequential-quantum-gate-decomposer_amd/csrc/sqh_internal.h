@@ -28,8 +28,16 @@ enum GateType : int32_t {
 // Reference cost_function_type values (Optimization_Interface.h:43-45); ABS_TRACE
 // (1 - |Tr|/m) has no reference counterpart.
 enum CostVariant : int32_t {
-    CV_FROBENIUS = 0, CV_HS = 3, CV_INFIDELITY = 9, CV_ABS_TRACE = 100
+    CV_FROBENIUS = 0, CV_FROBENIUS_C1 = 1, CV_FROBENIUS_C2 = 2, CV_HS = 3, CV_HS_C1 = 4, CV_HS_C2 = 5,
+    CV_INFIDELITY = 9, CV_ABS_TRACE = 100
 };
+
+// Extra trace channels a cost variant reads beyond the diagonal: the single (1) and
+// double (2) bit-flip sums of the arXiv:2210.09191 corrections.
+constexpr int correction_channels(int variant) {
+    return (variant == CV_FROBENIUS_C1 || variant == CV_HS_C1) ? 1
+           : (variant == CV_FROBENIUS_C2 || variant == CV_HS_C2) ? 2 : 0;
+}
 
 enum OpKind : uint8_t {
     OP_U = 0,      // uncontrolled 2x2 from the kernel table (slot = idx)
@@ -83,6 +91,20 @@ struct BatchIndex {
     const int* run_target = nullptr;
     long long tgt_stride = 0;
     const ProgRef* run_prog = nullptr;  // per run; requires one element per wavefront
+};
+
+// Trace channels and gradient seed of a launch.  Channel k (0 <= k <= nch) of
+// column c sums M[(c + off) ^ mask, c] over the masks of k set bits below 2^N
+// (k = 0: the trace entry itself); diag holds channel k at offset k * chstride
+// complex entries.  The gradient kernel's adjoint seed weights row r of column c
+// by w[popcount(r ^ (c + off))]: w = (1, w1, w2) when seed is null, else the
+// element's three complex weights seed[6 b ... 6 b + 5].
+struct TraceSpec {
+    int nch = 0;
+    int pad = 0;
+    long long chstride = 0;
+    const double* seed = nullptr;
+    double w1 = 0.0, w2 = 0.0;
 };
 
 // Arguments of a run-time compiled tile kernel (one struct, passed by value).

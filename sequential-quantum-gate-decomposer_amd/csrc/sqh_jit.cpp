@@ -447,6 +447,8 @@ constexpr size_t kJitMaxGradOps = 192;
 
 bool jit_wants(sqh_ctx* c, int kind) {
     if (!c->jit || c->prog.N > kMaxTileQubits) return false;
+    // the generated epilogue reads the diagonal only
+    if (kind != 0 && correction_channels(c->variant) > 0) return false;
     return kind != 2 || c->prog.ops.size() <= kJitMaxGradOps;
 }
 

@@ -43,9 +43,9 @@ hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, co
 
 hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,
                          int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st,
-                         BatchIndex bi) {
+                         BatchIndex bi, const TraceSpec& ts, double* seed_out) {
     hipLaunchKernelGGL(k_finish, dim3(B), dim3(256), 0, st, diag, m, variant, costs, mpart, wpb, ns, slots, params, P,
-                       grad, bi);
+                       grad, bi, ts, seed_out);
     return hipGetLastError();
 }
 
@@ -66,14 +66,14 @@ hipError_t launch_stream_gate(bool f32, void* mats, long long bstride, int strid
 }
 
 hipError_t launch_extract_diag(bool f32, const void* mats, long long bstride, int stride, int m, int B,
-                               int trace_off, double* diag, hipStream_t st) {
+                               int trace_off, double* diag, int nq, const TraceSpec& ts, hipStream_t st) {
     const long long n = (long long)B * m;
     if (f32)
         hipLaunchKernelGGL(k_extract_diag<float>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                           (const cpx<float>*)mats, bstride, stride, m, B, trace_off, diag);
+                           (const cpx<float>*)mats, bstride, stride, m, B, trace_off, diag, nq, ts);
     else
         hipLaunchKernelGGL(k_extract_diag<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st,
-                           (const cpx<double>*)mats, bstride, stride, m, B, trace_off, diag);
+                           (const cpx<double>*)mats, bstride, stride, m, B, trace_off, diag, nq, ts);
     return hipGetLastError();
 }
 

@@ -28,7 +28,8 @@ struct TileLaunch {
     long long kern_bstride = 0;  // complex entries between batch kernel tables (0: shared)
     int trace_off = 0;
     int ns = 0;                  // parametric gate count
-    double* diag = nullptr;      // [B][m] complex diagonal entries
+    double* diag = nullptr;      // [nch + 1][B][m] complex trace-channel entries
+    TraceSpec ts;                // trace channels and gradient seed (default: diagonal only)
     double* mpart = nullptr;     // [B][wpb][ns][8] gradient correlations
     const Slot* slots = nullptr;
     const double* params = nullptr;
@@ -46,12 +47,13 @@ hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, co
                                 hipStream_t st, BatchIndex bi = BatchIndex());
 hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,
                          int ns, const Slot* slots, const double* params, int P, double* grad, hipStream_t st,
-                         BatchIndex bi = BatchIndex());
+                         BatchIndex bi = BatchIndex(), const TraceSpec& ts = TraceSpec(),
+                         double* seed_out = nullptr);
 hipError_t launch_stream_gate(bool f32, void* mats, long long bstride, int stride, int rows, int m, int B, int kind,
                               int target, int control, const void* kern, long long kern_bstride, int slot,
                               hipStream_t st);
 hipError_t launch_extract_diag(bool f32, const void* mats, long long bstride, int stride, int m, int B,
-                               int trace_off, double* diag, hipStream_t st);
+                               int trace_off, double* diag, int nq, const TraceSpec& ts, hipStream_t st);
 hipError_t launch_broadcast_copy(bool f32, const void* src, int rows, int m, int sstride, void* dst,
                                  long long dbstride, int B, hipStream_t st);
 hipError_t launch_convert(bool to_f32, const void* src, void* dst, long long nreal, hipStream_t st);

@@ -41,7 +41,7 @@ hipError_t launch_forward_nmu(const TileLaunch& t, hipStream_t st, unsigned wave
     hipLaunchKernelGGL((k_tile_forward<N, R, MODE, UNI, WPG>), dim3(grid), dim3(64 * WPG), shm, st,
                        (const cpx<R>*)t.in, t.in_bstride, t.in_stride, (cpx<R>*)t.out, t.out_bstride, t.out_stride,
                        t.m, t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride,
-                       t.trace_off, t.diag, t.ns);
+                       t.trace_off, t.diag, t.ns, t.ts);
     return hipGetLastError();
 }
 
@@ -100,7 +100,7 @@ hipError_t launch_grad_nus(const TileLaunch& t, hipStream_t st, unsigned grid) {
     const size_t shm = STAGE ? stage_bytes<R>(t) : 0;
     hipLaunchKernelGGL((k_tile_grad<N, R, UNI, STAGE>), dim3(grid), dim3(64), shm, st, (const cpx<R>*)t.in,
                        t.in_stride, t.m, t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride,
-                       t.trace_off, t.ns, t.diag, t.mpart, t.bi, t.stage_ops, t.stage_lays);
+                       t.trace_off, t.ns, t.diag, t.mpart, t.bi, t.stage_ops, t.stage_lays, t.ts);
     return hipGetLastError();
 }
 

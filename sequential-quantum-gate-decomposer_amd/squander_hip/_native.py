@@ -26,13 +26,15 @@ GATE_NAMES = {GATE_U3: "U3", GATE_RX: "RX", GATE_RY: "RY", GATE_RZ: "RZ", GATE_C
 
 # Cost variants: the reference's cost_function_type values + ABS_TRACE.
 COST_FROBENIUS, COST_HILBERT_SCHMIDT, COST_INFIDELITY, COST_ABS_TRACE = 0, 3, 9, 100
+COST_FROBENIUS_CORRECTION1, COST_FROBENIUS_CORRECTION2 = 1, 2
+COST_HILBERT_SCHMIDT_CORRECTION1, COST_HILBERT_SCHMIDT_CORRECTION2 = 4, 5
 PREC_F64, PREC_F32 = 0, 1
 
 # Exported symbols (checked by the CPU test suite against include/sqhip.h).
 EXPORTS = [
     "sqh_create", "sqh_destroy", "sqh_last_error", "sqh_get_stream", "sqh_synchronize", "sqh_version",
     "sqh_set_circuit", "sqh_get_parameter_num", "sqh_set_target", "sqh_set_trace_offset",
-    "sqh_set_cost_function", "sqh_apply_to", "sqh_apply_to_dev", "sqh_apply_to_stream_dev",
+    "sqh_set_cost_function", "sqh_set_correction_weights", "sqh_apply_to", "sqh_apply_to_dev", "sqh_apply_to_stream_dev",
     "sqh_apply_derivate_to", "sqh_apply_derivate_to_dev", "sqh_optimization_problem_batch",
     "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
@@ -117,6 +119,7 @@ def lib():
             "sqh_set_target": ([vp, vp, i, i, i], i),
             "sqh_set_trace_offset": ([vp, i], i),
             "sqh_set_cost_function": ([vp, i], i),
+            "sqh_set_correction_weights": ([vp, ctypes.c_double, ctypes.c_double, ctypes.c_double], i),
             "sqh_apply_to": ([vp, vp, vp, i, i, i, i], i),
             "sqh_apply_to_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
             "sqh_apply_to_stream_dev": ([vp, vp, i, vp, i, i, i, ll, i, i], i),
@@ -239,6 +242,9 @@ class Context:
 
     def set_cost_function(self, variant):
         self.check(self._L.sqh_set_cost_function(self.h, int(variant)))
+
+    def set_correction_weights(self, prev_cost, scale1, scale2):
+        self.check(self._L.sqh_set_correction_weights(self.h, float(prev_cost), float(scale1), float(scale2)))
 
     def set_jit(self, enable=True):
         """Use run-time specialised kernels for this context's circuit."""

@@ -24,7 +24,12 @@ import numpy as np
 from . import _native as nat
 from .circuit import Circuit
 
-_COST_NAMES = {"FROBENIUS_NORM": nat.COST_FROBENIUS, "HILBERT_SCHMIDT_TEST": nat.COST_HILBERT_SCHMIDT,
+_COST_NAMES = {"FROBENIUS_NORM": nat.COST_FROBENIUS,
+               "FROBENIUS_NORM_CORRECTION1": nat.COST_FROBENIUS_CORRECTION1,
+               "FROBENIUS_NORM_CORRECTION2": nat.COST_FROBENIUS_CORRECTION2,
+               "HILBERT_SCHMIDT_TEST": nat.COST_HILBERT_SCHMIDT,
+               "HILBERT_SCHMIDT_TEST_CORRECTION1": nat.COST_HILBERT_SCHMIDT_CORRECTION1,
+               "HILBERT_SCHMIDT_TEST_CORRECTION2": nat.COST_HILBERT_SCHMIDT_CORRECTION2,
                "INFIDELITY": nat.COST_INFIDELITY, "ABS_TRACE": nat.COST_ABS_TRACE}
 
 
@@ -84,6 +89,12 @@ class N_Qubit_Decomposition_custom:
             costfnc = _COST_NAMES[costfnc.upper()]
         self._ctx.set_cost_function(int(costfnc))
         self.cost_variant = int(costfnc)
+
+    def set_Correction_Weights(self, prev_cost=1.0, scale1=1.0 / 1.7, scale2=1.0 / 2.0):
+        """Weights of the correction variants: Optimization_Interface's prev_cost_fnv_val,
+        correction1_scale and correction2_scale (Optimization_Interface.cpp:74-76; the
+        reference sets prev_cost_fnv_val only from its ADAM engines, ADAM.cpp:201)."""
+        self._ctx.set_correction_weights(prev_cost, scale1, scale2)
 
     def set_Optimization_Tolerance(self, tol):
         self.optimization_tolerance = float(tol)
