@@ -145,39 +145,46 @@ def ten_qubit_leg(steps, o):
 
 def derivate_leg(gates, P, U):
     """apply_derivate_to of the 8-qubit ansatz on a 256 x 256 matrix: 264 derivative
-    matrices of 1 MiB written to HBM per call."""
+    matrices of 1 MiB written to HBM per call.  The forward store kernel runs over 264
+    elements sharing the input, element k with gate k's kernel replaced by its
+    derivative: one circuit pass per parameter (the reference's replay count), with
+    the generic and the circuit-specialised kernel."""
     import torch
 
     from squander_hip import Context
     from squander_hip import workloads as W
     d = 256
-    ctx = Context(0)
-    ctx.set_circuit(8, gates)
     p = torch.from_numpy(W.random_params(P, 3)).cuda()
     M = torch.from_numpy(U.copy()).cuda()
     out = torch.empty((P, d, d), dtype=torch.complex128, device="cuda")
-    call = lambda: ctx.check(ctx._L.sqh_apply_derivate_to_dev(ctx.h, p.data_ptr(), M.data_ptr(), d, d, d,
-                                                              out.data_ptr()))
-    call()
-    ctx.synchronize()
-    ctx.set_kernel_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(5):
-        call()
-    ctx.synchronize()
-    dt = (time.perf_counter() - t0) / 5
-    kms, nl = ctx.kernel_timing()
-    ctx.set_kernel_timing(False)
     wbytes = P * d * d * 16
-    k_s = kms / max(nl, 1) / 1e3
     flops_eval, _ = gate_work(gates, d, d, 16)
-    # every derivative matrix is one replay of the circuit with one gate replaced by
-    # its derivative kernel: P x (forward flops) per call, against 264 MiB written
+    # one pass of the circuit per parameter
     flops = P * flops_eval
-    return {"ms": dt * 1e3, "kernel_avg_ms": k_s * 1e3, "bound": "mfma (FP64)",
-            "algorithmic_flops": flops, "achieved_TFLOPs": flops / k_s / 1e12,
-            "frac_of_fp64_peak": flops / k_s / 1e12 / FP64_PEAK_TFLOPS, "bytes_written": wbytes,
-            "write_GBs": wbytes / k_s / 1e9, "kernel": "k_tile_deriv<8,double>"}
+    res = {"bound": "mfma (FP64)", "algorithmic_flops": flops, "bytes_written": wbytes}
+    for name, jit in (("generic", False), ("specialised", True)):
+        ctx = Context(0)
+        ctx.set_circuit(8, gates)
+        ctx.set_jit(jit)
+        call = lambda: ctx.check(ctx._L.sqh_apply_derivate_to_dev(ctx.h, p.data_ptr(), M.data_ptr(), d, d, d,
+                                                                  out.data_ptr()))
+        call()
+        ctx.synchronize()
+        ctx.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(5):
+            call()
+        ctx.synchronize()
+        dt = (time.perf_counter() - t0) / 5
+        kms, nl = ctx.kernel_timing()
+        ctx.set_kernel_timing(False)
+        k_s = kms / max(nl, 1) / 1e3
+        res[name] = {"ms": dt * 1e3, "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": flops / k_s / 1e12,
+                     "frac_of_fp64_peak": flops / k_s / 1e12 / FP64_PEAK_TFLOPS, "write_GBs": wbytes / k_s / 1e9,
+                     "kernel": ("sqh_jit_kernel (forward store, per-parameter kernel tables)" if jit
+                                else "k_tile_forward<8,double,0> (per-parameter kernel tables)")}
+        log(f"apply_derivate_to 8q {name}: {k_s * 1e3:.3f} ms")
+    return res
 
 
 def uneven_leg():

@@ -231,8 +231,9 @@ int sqh_agents(sqh_ctx* ctx, double* x, const sqh_agents_options* opts, unsigned
 
 /* Run-time specialised tile kernels.  With enable = 1 the context compiles its
  * circuit, on first use, into straight-line forward and gradient kernels (hiprtc,
- * cached per circuit) and uses them for apply_to, the batched costs and gradients and
- * sqh_decompose_batch; results equal the generic kernels' to rounding.  No
+ * cached per circuit) and uses them for apply_to, apply_derivate_to, the batched
+ * costs and gradients and sqh_decompose_batch; results equal the generic kernels' to
+ * rounding.  No
  * counterpart in the reference (which has one generic loop).  sqh_get_jit_stats
  * reports the compile time spent so far. */
 int sqh_set_jit(sqh_ctx* ctx, int enable);

@@ -258,6 +258,10 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
     return stream_apply(c, d_inout, rows, cols, stride, bstride, B, f32, kb);
 }
 
+// Gates_block::apply_derivate_to: the forward store kernel over P elements that share
+// the input, element k running the circuit with parameter k's gate replaced by its
+// derivative kernel (per-element kernel tables; a controlled gate's control-0 block
+// zeroed).  Output k is at d_out + k * rows * cols, row stride cols.
 int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, int cols, int stride, void* d_out) {
     int rc = require_circuit(c);
     if (rc) return rc;
@@ -267,17 +271,31 @@ int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, in
     if (N > kMaxTileQubits)
         return fail(c, SQH_ERR_ARG, "apply_derivate_to supports up to %d qubits", kMaxTileQubits);
     if (c->prog.slots.empty()) return SQH_OK;
+    const int ns = (int)c->prog.slots.size(), P = c->prog.P;
+    bool controlled = false;
+    for (const Op& op : c->prog.ops) controlled |= op.kind == OP_CU;
     if ((rc = build_kernels(c, d_params, 1, false))) return rc;
+    HIPTRY(c, c->d_dtab.ensure((size_t)P * ns * 4 * 16));
+    HIPTRY(c, c->d_zslot.ensure((size_t)P * sizeof(int)));
+    HIPTRY(c, launch_deriv_tables(c->d_kern.p, c->d_slots.as<Slot>(), ns, d_params, P, c->d_dtab.p,
+                                  c->d_zslot.as<int>(), c->stream));
     TileLaunch t = base_launch(c, false);
+    t.kern = c->d_dtab.p;
+    t.kern_bstride = (long long)ns * 4;
     t.in = d_in;
+    t.in_bstride = 0;
     t.in_stride = stride;
     t.out = d_out;
     t.out_bstride = (long long)rows * cols;
+    t.out_stride = cols;
     t.m = cols;
     t.m_pad = m_pad_for(N, cols);
-    t.params = d_params;
+    t.B = P;
+    t.mode = 0;
+    t.zslot = controlled ? c->d_zslot.as<int>() : nullptr;
     TimedLaunch tl(c);
-    HIPTRY(c, launch_tile_deriv(t, c->stream));
+    if (jit_wants(c, 0)) return jit_launch(c, t, 0, c->stream);
+    HIPTRY(c, launch_tile_forward(t, c->stream));
     return SQH_OK;
 }
 

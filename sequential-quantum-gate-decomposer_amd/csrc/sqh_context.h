@@ -73,6 +73,7 @@ struct sqh_ctx {
 
     sqh::DevBuf d_params, d_kern, d_diag, d_mpart, d_res, d_mat, d_out;
     sqh::DevBuf d_seed;  // [B][3] complex adjoint seed weights of the HS correction variants
+    sqh::DevBuf d_dtab, d_zslot;  // apply_derivate_to: per-parameter kernel tables, derivative slots
     sqh::DevBuf d_idx;  // [count][0..B) identity element list of the per-target evaluation
     int idx_B = -1;
 
@@ -95,7 +96,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx, &d_seed})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx, &d_seed, &d_dtab, &d_zslot})
             b->release();
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);

@@ -257,6 +257,22 @@ __device__ __forceinline__ void pairs_negate(cpx<R> (&v)[NR], uint32_t mask) {
     }
 }
 
+// Derivative of a controlled gate: its control-0 pairs (mask bit clear) become zero
+// when z (apply_kernel_to_input.cpp:93-97).
+template <int T, typename R, int NR>
+__device__ __forceinline__ void pairs_zero_off(cpx<R> (&v)[NR], uint32_t mask, bool z) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i) {
+        if (i & (1 << T)) continue;
+        const int j = i | (1 << T);
+        if ((mask >> i) & 1u) continue;
+        if (z) {
+            v[i] = mk<R>(0, 0);
+            v[j] = mk<R>(0, 0);
+        }
+    }
+}
+
 // Correlation of the adjoint tile Y with the state X on the active pairs of target T:
 // acc[a*2+b] += sum Y[r_a] X[r_b].
 template <int T, typename R, int NR>
@@ -490,7 +506,7 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
                                                            const Layout* __restrict__ lays,
                                                            const cpx<R>* __restrict__ kern, long long kern_bstride,
                                                            int trace_off, double* __restrict__ diag, int ns,
-                                                           TraceSpec ts) {
+                                                           TraceSpec ts, const int* __restrict__ zslot) {
     using G = Geo<N>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -516,6 +532,8 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
         kb = ktab;
     }
     int cur = 0;
+    // derivative mode of one controlled slot per element (apply_derivate_to tables)
+    const int zs = zslot ? zslot[li.b] : -1;
     OpStream<R> os(ops, kb, nops, 0, 1);
     for (int p = 0; p < nops; ++p) {
         cpx<R> kn[4];
@@ -528,6 +546,16 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
             cur = n;
         } else {
             exec_gate<N, false, R>(v, op, os.k, li.s);
+            if (op.kind == OP_CU && zslot) {
+                const uint32_t msk = ctrl_mask<N>(op.cmode, op.c, li.s);
+                switch (op.t) {
+                case 0: if constexpr (G::K > 0) pairs_zero_off<0>(v, msk, zs == op.idx); break;
+                case 1: if constexpr (G::K > 1) pairs_zero_off<1>(v, msk, zs == op.idx); break;
+                case 2: if constexpr (G::K > 2) pairs_zero_off<2>(v, msk, zs == op.idx); break;
+                case 3: if constexpr (G::K > 3) pairs_zero_off<3>(v, msk, zs == op.idx); break;
+                default: break;
+                }
+            }
         }
         os.cur = os.nxt;
         os.nxt = after;
@@ -763,90 +791,6 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
     }
 }
 
-// Materialised derivative matrices (Gates_block::apply_derivate_to,
-// Gates_block.cpp:1012-1150): each wavefront takes one parametric gate and one
-// column slab, runs the prefix, and for each parameter of the gate applies the
-// derivative kernel (control-0 rows zeroed) and the suffix, then stores.
-// ---------------------------------------------------------------------------
-template <int N, typename R>
-__global__ __launch_bounds__(64) void k_tile_deriv(const cpx<R>* __restrict__ in, int in_stride, int m,
-                                                   int m_pad, const Op* __restrict__ ops, int nops,
-                                                   const Layout* __restrict__ lays,
-                                                   const cpx<R>* __restrict__ kern, const Slot* __restrict__ slots,
-                                                   const double* __restrict__ params, cpx<R>* __restrict__ out,
-                                                   long long out_mstride) {
-    using G = Geo<N>;
-    __shared__ cpx<R> lds[G::LDS];
-    const int wpb = m_pad >= G::CW ? m_pad / G::CW : 1;
-    const int slot = blockIdx.x / wpb;
-    const int wave = blockIdx.x % wpb;
-    // lane_info with the slab index of this wave (batch of one).
-    LaneInfo li;
-    li.s = threadIdx.x / G::CW;
-    li.cc = threadIdx.x % G::CW;
-    li.b = 0;
-    li.col = wave * G::CW + li.cc;
-    li.valid = li.col < m;
-    li.bvalid = true;
-    cpx<R> x[G::NR], z[G::NR];
-    {
-        const int so = sub_off<N>(&lays[0], li.s);
-#pragma unroll
-        for (int i = 0; i < G::NR; ++i) {
-            const int row = so + lays[0].reg_off[i];
-            x[i] = li.valid ? in[(long long)row * in_stride + li.col] : mk<R>(0, 0);
-        }
-    }
-    const Slot sl = slots[slot];
-    int cur = 0;
-    for (int p = 0; p < sl.op; ++p) {
-        const Op op = ops[p];
-        if (op.kind == OP_REMAP) {
-            const int o = op.idx >> 16, n = op.idx & 0xffff;
-            remap<N, R>(x, &lays[o], &lays[n], li.s, li.cc, lds);
-            cur = n;
-            continue;
-        }
-        cpx<R> k[4];
-        if (op.kind <= OP_CU) load_k(kern + 4 * op.idx, k);
-        exec_gate<N, false, R>(x, op, k, li.s);
-    }
-    const int lay_at = cur;
-    const Op dop = ops[sl.op];
-    for (int q = 0; q < sl.nparam; ++q) {
-#pragma unroll
-        for (int i = 0; i < G::NR; ++i) z[i] = x[i];
-        cur = lay_at;
-        {
-            cpx<double> kd[4];
-            dev_deriv_kernel(sl.type, params + sl.param_start, q, kd);
-            cpx<R> k[4] = {cvt<R>(kd[0]), cvt<R>(kd[1]), cvt<R>(kd[2]), cvt<R>(kd[3])};
-            Op o2 = dop;
-            o2.kind = OP_CU;  // derivative mode always honours the (possibly absent) control
-            exec_gate<N, true, R>(z, o2, k, li.s);
-        }
-        for (int p = sl.op + 1; p < nops; ++p) {
-            const Op op = ops[p];
-            if (op.kind == OP_REMAP) {
-                const int o = op.idx >> 16, n = op.idx & 0xffff;
-                remap<N, R>(z, &lays[o], &lays[n], li.s, li.cc, lds);
-                cur = n;
-                continue;
-            }
-            cpx<R> k[4];
-            if (op.kind <= OP_CU) load_k(kern + 4 * op.idx, k);
-            exec_gate<N, false, R>(z, op, k, li.s);
-        }
-        cpx<R>* dst = out + (long long)(sl.param_start + q) * out_mstride;
-        const int so = sub_off<N>(&lays[cur], li.s);
-#pragma unroll
-        for (int i = 0; i < G::NR; ++i) {
-            const int row = so + lays[cur].reg_off[i];
-            if (li.valid) dst[(long long)row * m + li.col] = z[i];
-        }
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Small kernels: kernel table, finish (cost + gradient), streaming gates.
 // ---------------------------------------------------------------------------
@@ -932,6 +876,32 @@ __device__ inline double grad_corr(int variant, double kre, double m) {
 }
 
 #ifdef SQH_DEFINE_MISC_KERNELS
+// Per-parameter kernel tables of apply_derivate_to (one workgroup per parameter k,
+// threads over the slots): the slot holding parameter k gets its derivative kernel
+// (Gate::apply_derivate_to, Gate.cpp:645-706), every other slot its forward kernel.
+__global__ __launch_bounds__(64) void k_deriv_tables(const cpx<double>* __restrict__ base,
+                                                     const Slot* __restrict__ slots, int ns,
+                                                     const double* __restrict__ params, int P,
+                                                     cpx<double>* __restrict__ tables, int* __restrict__ zslot) {
+    const int k = blockIdx.x;
+    if (k >= P) return;
+    cpx<double>* tk = tables + (long long)k * ns * 4;
+    for (int j = threadIdx.x; j < ns; j += blockDim.x) {
+        const Slot sl = slots[j];
+        const int q = k - sl.param_start;
+        if (q >= 0 && q < sl.nparam) {
+            cpx<double> kd[4];
+            dev_deriv_kernel(sl.type, params + sl.param_start, q, kd);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tk[4 * j + e] = kd[e];
+            zslot[k] = sl.type == GT_CRY ? j : -1;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tk[4 * j + e] = base[4 * j + e];
+        }
+    }
+}
+
 // One workgroup per batch element: trace channels = sums of the per-column entries,
 // cost, and (if mpart) the gradient from the per-gate correlations; with seed_out,
 // also the HS_C* adjoint seed weights of the element.

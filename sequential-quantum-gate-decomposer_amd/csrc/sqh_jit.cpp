@@ -143,6 +143,9 @@ struct Gen {
         case OP_CU:
             o << "  pairs_apply_f<" << form(op) << ", " << t << ", true>(" << v << ", k" << p << ", "
               << mask_expr(N, op) << ");\n";
+            if (zero_cu_)  // apply_derivate_to tables: this element differentiates this gate
+                o << "  pairs_zero_off<" << t << ">(" << v << ", " << mask_expr(N, op) << ", zs == " << op.idx
+                  << ");\n";
             break;
         case OP_X:
             o << "  pairs_swap<" << t << ">(" << v << ", " << mask_expr(N, op) << ");\n";
@@ -205,9 +208,16 @@ struct Gen {
           << "  const cpx<R>* kb = (const cpx<R>*)a.kern + (long long)li.b * SKB;\n";
     }
 
-    // Straight-line k_tile_forward (mode 0: store; 1: diagonal entries).
+    // Straight-line k_tile_forward (mode 0: store; 1: diagonal entries).  The store
+    // kernel also serves apply_derivate_to (per-element kernel tables, a.zslot set).
+    bool zero_cu_ = false;
+
     std::string forward_kernel(bool uni, bool f32, int mode) {
         prologue(uni, f32, "sqh_jit_kernel");
+        if (mode == 0)
+            for (const Op& op : pr.ops)
+                if (op.kind == OP_CU) zero_cu_ = true;
+        if (zero_cu_) o << "  const int zs = a.zslot ? a.zslot[li.b] : -1;\n";
         o << "  cpx<R> v[G::NR];\n";
         load("v", "in", "SINS");
         const int cur = forward("v");
@@ -432,6 +442,7 @@ int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
     a.elem_run = t.bi.elem_run;
     a.run_target = t.bi.run_target;
     a.tgt_stride = t.bi.tgt_stride;
+    a.zslot = t.zslot;
     const long long items = (long long)t.B * t.m_pad;
     const unsigned grid = (unsigned)((items + cw - 1) / cw);
     if (grid == 0) return SQH_OK;
@@ -447,8 +458,8 @@ constexpr size_t kJitMaxGradOps = 192;
 
 bool jit_wants(sqh_ctx* c, int kind) {
     if (!c->jit || c->prog.N > kMaxTileQubits) return false;
-    // the generated epilogue reads the diagonal only
-    if (kind != 0 && correction_channels(c->variant) > 0) return false;
+    // the generated epilogues read the diagonal only
+    if ((kind == 1 || kind == 2) && correction_channels(c->variant) > 0) return false;
     return kind != 2 || c->prog.ops.size() <= kJitMaxGradOps;
 }
 

@@ -9,8 +9,6 @@ hipError_t launch_tile_forward_double(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_forward_float(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_grad_double(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_grad_float(const TileLaunch& t, hipStream_t st);
-hipError_t launch_tile_deriv_double(const TileLaunch& t, hipStream_t st);
-hipError_t launch_tile_deriv_float(const TileLaunch& t, hipStream_t st);
 
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st) {
     return t.f32 ? launch_tile_forward_float(t, st) : launch_tile_forward_double(t, st);
@@ -18,10 +16,6 @@ hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st) {
 hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st) {
     return t.f32 ? launch_tile_grad_float(t, st) : launch_tile_grad_double(t, st);
 }
-hipError_t launch_tile_deriv(const TileLaunch& t, hipStream_t st) {
-    return t.f32 ? launch_tile_deriv_float(t, st) : launch_tile_deriv_double(t, st);
-}
-
 int tile_columns_per_wave(int N) {
     if (N < 1 || N > kMaxTileQubits) return 0;
     const int K = N < 4 ? N : 4;
@@ -38,6 +32,14 @@ hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, co
     else
         hipLaunchKernelGGL(k_build_kernels<double>, dim3(blocks_for(n, 256)), dim3(256), 0, st, params, P, B, slots,
                            ns, (cpx<double>*)kern, bi);
+    return hipGetLastError();
+}
+
+hipError_t launch_deriv_tables(const void* base, const Slot* slots, int ns, const double* params, int P,
+                               void* tables, int* zslot, hipStream_t st) {
+    if (P <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_deriv_tables, dim3(P), dim3(64), 0, st, (const cpx<double>*)base, slots, ns, params, P,
+                       (cpx<double>*)tables, zslot);
     return hipGetLastError();
 }
 

@@ -32,17 +32,23 @@ struct TileLaunch {
     TraceSpec ts;                // trace channels and gradient seed (default: diagonal only)
     double* mpart = nullptr;     // [B][wpb][ns][8] gradient correlations
     const Slot* slots = nullptr;
-    const double* params = nullptr;
     BatchIndex bi;             // optional element -> run -> target indirection
+    const int* zslot = nullptr;  // forward store: per element, the controlled slot applied in
+                                 // derivative mode (control-0 block zeroed), or -1
     int stage_ops = 0;         // > 0: gradient kernel stages the program in LDS (largest op count)
     int stage_lays = 0;        //      ... and up to this many layouts
 };
 
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st);
-hipError_t launch_tile_deriv(const TileLaunch& t, hipStream_t st);
 int tile_columns_per_wave(int N);
 
+// Per-parameter kernel tables of apply_derivate_to: table k (ns entries) is the
+// circuit's table base with the entry of parameter k's gate replaced by the gate's
+// derivative kernel w.r.t. that parameter; zslot[k] is that gate's slot when it is
+// controlled (its control-0 block must be zeroed), else -1.
+hipError_t launch_deriv_tables(const void* base, const Slot* slots, int ns, const double* params, int P,
+                               void* tables, int* zslot, hipStream_t st);
 hipError_t launch_build_kernels(bool f32, const double* params, int P, int B, const Slot* slots, int ns, void* kern,
                                 hipStream_t st, BatchIndex bi = BatchIndex());
 hipError_t launch_finish(const double* diag, int m, int B, int variant, double* costs, const double* mpart, int wpb,

@@ -41,7 +41,7 @@ hipError_t launch_forward_nmu(const TileLaunch& t, hipStream_t st, unsigned wave
     hipLaunchKernelGGL((k_tile_forward<N, R, MODE, UNI, WPG>), dim3(grid), dim3(64 * WPG), shm, st,
                        (const cpx<R>*)t.in, t.in_bstride, t.in_stride, (cpx<R>*)t.out, t.out_bstride, t.out_stride,
                        t.m, t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride,
-                       t.trace_off, t.diag, t.ns, t.ts);
+                       t.trace_off, t.diag, t.ns, t.ts, t.zslot);
     return hipGetLastError();
 }
 
@@ -124,15 +124,5 @@ hipError_t launch_grad_n(const TileLaunch& t, hipStream_t st) {
     return launch_grad_nu<N, R, false>(t, st, grid);
 }
 
-template <int N, typename R>
-hipError_t launch_deriv_n(const TileLaunch& t, hipStream_t st) {
-    using G = Geo<N>;
-    const int wpb = t.m_pad >= G::CW ? t.m_pad / G::CW : 1;
-    const unsigned grid = (unsigned)(wpb * t.ns);
-    hipLaunchKernelGGL((k_tile_deriv<N, R>), dim3(grid), dim3(64), 0, st, (const cpx<R>*)t.in, t.in_stride, t.m,
-                       t.m_pad, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.slots, t.params,
-                       (cpx<R>*)t.out, t.out_bstride);
-    return hipGetLastError();
-}
 
 }  // namespace sqh

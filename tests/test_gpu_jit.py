@@ -159,3 +159,29 @@ def test_jit_rectangular_trace_offset_matches_oracle(oracle):
         fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b], trace_offset=80)
         assert abs(f[b] - fo) < 1e-12 and abs(c[b] - fo) < 1e-12
         assert np.max(np.abs(g[b] - go)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 8, 10])
+def test_jit_apply_derivate_to_matches_oracle(oracle, n):
+    """Circuit-specialised derivative matrices (Gates_block::apply_derivate_to) against
+    the oracle's derivative matrices, square and rectangular inputs (1e-12 relative)."""
+    say(f"jit apply_derivate_to n={n}")
+    gates = W.mixed_gate_circuit(n, 6 * n + 6, 1200 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    p = W.random_params(P, 1300 + n)
+    rng = np.random.default_rng(1400 + n)
+    for m in sorted({d, max(1, d // 2 + 3)}):
+        X = rng.standard_normal((d, m)) + 1j * rng.standard_normal((d, m))
+        ref = oracle.apply_derivate_to(gates, p, X)
+        for jit in (True, False):
+            ctx = nat.Context(0)
+            ctx.set_circuit(n, gates)
+            ctx.set_jit(jit)
+            out = np.empty((P, d, m), dtype=np.complex128)
+            ctx.check(ctx._L.sqh_apply_derivate_to(ctx.h, p.ctypes.data, X.ctypes.data, d, m, m, out.ctypes.data))
+            for k in range(P):
+                err = np.linalg.norm(out[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)
+                assert err < 1e-12, (jit, m, k, err)
+            if jit:
+                assert ctx.jit_stats()[1] == 1

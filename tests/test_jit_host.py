@@ -7,7 +7,8 @@ from squander_hip import _native as nat
 from squander_hip import workloads as W
 
 
-@pytest.mark.parametrize("n,kind,f32", [(3, 1, False), (6, 1, False), (6, 0, True), (3, 2, False), (6, 2, True)])
+@pytest.mark.parametrize("n,kind,f32", [(3, 1, False), (6, 1, False), (6, 0, True), (3, 2, False), (6, 2, True),
+                                        (3, 0, False)])
 def test_generated_kernel_compiles(n, kind, f32):
     gates = W.mixed_gate_circuit(n, 18, 40 + n)
     ok, log, code, src = nat.jit_compile_check(n, gates, kind=kind, is_f32=f32, uniform=n > 4)
@@ -18,3 +19,4 @@ def test_generated_kernel_compiles(n, kind, f32):
 def test_compile_check_rejects_bad_circuit():
     ok, log, _, _ = nat.jit_compile_check(2, [[nat.GATE_U3, 3, -1, 0]], kind=1)
     assert not ok and log
+
