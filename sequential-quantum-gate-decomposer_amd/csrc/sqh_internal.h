@@ -114,7 +114,7 @@ struct JitArgs {
     int in_stride, m, m_pad, B;
     void* out;
     long long out_bstride;
-    int out_stride, trace_off, ns, pad0;
+    int out_stride, trace_off, ns, pad0;  // pad0 = 0 (the gradient kernel's opaque table offset)
     const void* kern;
     long long kern_bstride;
     double* diag;
