@@ -140,6 +140,25 @@ def ten_qubit_leg(steps, o):
                      "peak_TFLOPs": peak, "frac": tf / peak,
                      "err_vs_fp64_oracle": float(np.abs(costs[:2].cpu().numpy() - ref).max())}
         log(f"10q {name}: {res[name]['evals_per_s']:.0f} evals/s, {tf:.1f} TFLOP/s")
+    # complex128 cost + gradient over the batch (specialised adjoint kernel)
+    grad = torch.empty((B, P), dtype=torch.float64, device="cuda")
+    gstep = lambda: ctx.check(ctx._L.sqh_optimization_problem_combined_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64,
+                                                                                costs.data_ptr(), grad.data_ptr()))
+    gstep()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    for _ in range(3):
+        gstep()
+    ctx.synchronize()
+    kms, nl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    k_s = kms / max(nl, 1) / 1e3
+    gflops = flops * B * (3.0 + 8.0 / 7.0)
+    _, gr = o.optimization_problem_combined(gates, U, n, params[0].cpu().numpy())
+    res["c128_gradient"] = {"kernel_avg_ms": k_s * 1e3, "grads_per_s": B / k_s, "achieved_TFLOPs": gflops / k_s / 1e12,
+                            "frac_of_fp64_peak": gflops / k_s / 1e12 / FP64_PEAK_TFLOPS,
+                            "err_grad_vs_oracle": float(np.abs(grad[0].cpu().numpy() - gr).max())}
+    log(f"10q c128 cost+gradient: {k_s * 1e3:.2f} ms")
     return res
 
 
@@ -461,6 +480,39 @@ def main():
     fr, gr = o.optimization_problem_combined(gates, U, n, p1.cpu().numpy()[0])
     out["grad_8q_single"] = {"ms": (t1 - t0) / 20 * 1e3, "err_grad_vs_oracle": float(np.abs(g0.cpu().numpy() - gr).max()),
                              "err_cost_vs_oracle": float(abs(f0.item() - fr))}
+
+    # batched cost + gradient (optimization_problem_combined over the batch), 8 qubits
+    # B = 512, specialised adjoint kernel
+    G = torch.empty((B, P), dtype=torch.float64, device=dev)
+    fb = torch.empty(B, dtype=torch.float64, device=dev)
+
+    def grad_batch():
+        ctx.check(L.sqh_optimization_problem_combined_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64, fb.data_ptr(),
+                                                                 G.data_ptr()))
+
+    grad_batch()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        grad_batch()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    gkms, gnl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    gk_s = gkms / max(gnl, 1) / 1e3
+    fr1, gr1 = o.optimization_problem_combined(gates, U, n, params[1].cpu().numpy())
+    # adjoint sweep: forward pass, G^dagger on the state, G^T on the adjoint and the
+    # pair correlations (32 flop per pair against 28 for a gate): (3 + 8/7) x forward
+    gflops = flops_eval * B * (3.0 + 8.0 / 7.0)
+    out["grad_8q_b512"] = {"ms": (t1 - t0) / 5 * 1e3, "kernel_avg_ms": gk_s * 1e3,
+                           "grads_per_s": B / ((t1 - t0) / 5), "algorithmic_flops": gflops,
+                           "achieved_TFLOPs": gflops / gk_s / 1e12,
+                           "frac_of_fp64_peak": gflops / gk_s / 1e12 / FP64_PEAK_TFLOPS,
+                           "err_grad_vs_oracle": float(np.abs(G[1].cpu().numpy() - gr1).max()),
+                           "kernel": "sqh_jit_kernel (adjoint gradient specialised to the circuit)"}
+    log(f"8q B=512 cost+gradient: {gk_s * 1e3:.3f} ms, {gflops / gk_s / 1e12:.1f} TFLOP/s")
+    del G
 
     out["ten_qubit_b64"] = ten_qubit_leg(args.steps, o)
     out["apply_derivate_8q"] = derivate_leg(gates, P, U)
