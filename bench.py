@@ -162,6 +162,47 @@ def ten_qubit_leg(steps, o):
     return res
 
 
+def gradient_leg(ctx, params, gates, U, o, reps=5):
+    """Batched cost + gradient (optimization_problem_combined over the batch) of the
+    8-qubit ansatz with the circuit-specialised adjoint kernel."""
+    import numpy as np
+    import torch
+
+    from squander_hip import PREC_F64
+    n, d = 8, 256
+    B, P = params.shape
+    L = ctx._L
+    flops_eval, _ = gate_work(gates, d, d, 16)
+    G = torch.empty((B, P), dtype=torch.float64, device="cuda")
+    fb = torch.empty(B, dtype=torch.float64, device="cuda")
+
+    def grad_batch():
+        ctx.check(L.sqh_optimization_problem_combined_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64, fb.data_ptr(),
+                                                                 G.data_ptr()))
+
+    grad_batch()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grad_batch()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    gkms, gnl = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    gk_s = gkms / max(gnl, 1) / 1e3
+    _, gr1 = o.optimization_problem_combined(gates, U, n, params[1].cpu().numpy())
+    # adjoint sweep: forward pass, G^dagger on the state, G^T on the adjoint and the
+    # pair correlations (32 flop per pair against 28 for a gate): (3 + 8/7) x forward
+    gflops = flops_eval * B * (3.0 + 8.0 / 7.0)
+    log(f"8q B={B} cost+gradient: {gk_s * 1e3:.3f} ms, {gflops / gk_s / 1e12:.1f} TFLOP/s")
+    return {"ms": (t1 - t0) / reps * 1e3, "kernel_avg_ms": gk_s * 1e3, "grads_per_s": B / ((t1 - t0) / reps),
+            "algorithmic_flops": gflops, "achieved_TFLOPs": gflops / gk_s / 1e12,
+            "frac_of_fp64_peak": gflops / gk_s / 1e12 / FP64_PEAK_TFLOPS,
+            "err_grad_vs_oracle": float(np.abs(G[1].cpu().numpy() - gr1).max()),
+            "kernel": "sqh_jit_kernel (adjoint gradient specialised to the circuit)"}
+
+
 def derivate_leg(gates, P, U):
     """apply_derivate_to of the 8-qubit ansatz on a 256 x 256 matrix: 264 derivative
     matrices of 1 MiB written to HBM per call.  The forward store kernel runs over 264
@@ -280,6 +321,8 @@ def main():
     ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--only-main", action="store_true", help="time the headline step only (for profiling)")
+    ap.add_argument("--only-gradient", action="store_true",
+                    help="time the batched 8-qubit cost + gradient only (for profiling the adjoint kernel)")
     ap.add_argument("--skip-solves", action="store_true", help="skip the decomposition legs (4q solves, uneven batch)")
     ap.add_argument("--generic", action="store_true", help="headline with the generic (not circuit-specialised) kernel")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -314,6 +357,12 @@ def main():
     params = torch.from_numpy(W.random_params(P, 7, batch=B)).to(dev)
     costs = torch.empty(B, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    if args.only_gradient:
+        res = gradient_leg(ctx, params, gates, U, orc.Oracle(), reps=args.steps)
+        print(json.dumps({"metric": "cost+gradient evals/s @8q batch 512 (FP64)", "value": res["grads_per_s"],
+                          "unit": "gradients/s", "higher_is_better": True, "dtype": "f64", "gradient": res}),
+              flush=True)
+        return
 
     def step(prec=PREC_F64):
         ctx.check(L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, prec, costs.data_ptr()))
@@ -481,39 +530,7 @@ def main():
     out["grad_8q_single"] = {"ms": (t1 - t0) / 20 * 1e3, "err_grad_vs_oracle": float(np.abs(g0.cpu().numpy() - gr).max()),
                              "err_cost_vs_oracle": float(abs(f0.item() - fr))}
 
-    # batched cost + gradient (optimization_problem_combined over the batch), 8 qubits
-    # B = 512, specialised adjoint kernel
-    G = torch.empty((B, P), dtype=torch.float64, device=dev)
-    fb = torch.empty(B, dtype=torch.float64, device=dev)
-
-    def grad_batch():
-        ctx.check(L.sqh_optimization_problem_combined_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64, fb.data_ptr(),
-                                                                 G.data_ptr()))
-
-    grad_batch()
-    ctx.synchronize()
-    ctx.set_kernel_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(5):
-        grad_batch()
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    gkms, gnl = ctx.kernel_timing()
-    ctx.set_kernel_timing(False)
-    gk_s = gkms / max(gnl, 1) / 1e3
-    fr1, gr1 = o.optimization_problem_combined(gates, U, n, params[1].cpu().numpy())
-    # adjoint sweep: forward pass, G^dagger on the state, G^T on the adjoint and the
-    # pair correlations (32 flop per pair against 28 for a gate): (3 + 8/7) x forward
-    gflops = flops_eval * B * (3.0 + 8.0 / 7.0)
-    out["grad_8q_b512"] = {"ms": (t1 - t0) / 5 * 1e3, "kernel_avg_ms": gk_s * 1e3,
-                           "grads_per_s": B / ((t1 - t0) / 5), "algorithmic_flops": gflops,
-                           "achieved_TFLOPs": gflops / gk_s / 1e12,
-                           "frac_of_fp64_peak": gflops / gk_s / 1e12 / FP64_PEAK_TFLOPS,
-                           "err_grad_vs_oracle": float(np.abs(G[1].cpu().numpy() - gr1).max()),
-                           "kernel": "sqh_jit_kernel (adjoint gradient specialised to the circuit)"}
-    log(f"8q B=512 cost+gradient: {gk_s * 1e3:.3f} ms, {gflops / gk_s / 1e12:.1f} TFLOP/s")
-    del G
-
+    out["grad_8q_b512"] = gradient_leg(ctx, params, gates, U, o)
     out["ten_qubit_b64"] = ten_qubit_leg(args.steps, o)
     out["apply_derivate_8q"] = derivate_leg(gates, P, U)
     if not args.skip_solves:
