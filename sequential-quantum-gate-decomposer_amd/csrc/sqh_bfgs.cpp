@@ -46,8 +46,10 @@ struct Group {
     int ops_cap = 0, lays_cap = 0;  // largest program of the group (LDS staging)
     DevBuf tgt64, tgt32, d_ops, d_lays, d_slots;
     DevBuf kern64, kern32, diag, mpart;  // the group's own evaluation scratch
+    DevBuf seed;                         // Hilbert-Schmidt corrections: per-element seed weights
     ~Group() {
-        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots, &kern64, &kern32, &diag, &mpart}) b->release();
+        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots, &kern64, &kern32, &diag, &mpart, &seed})
+            b->release();
     }
 };
 
@@ -118,6 +120,17 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             if (!grouped && jit_wants(c, 2)) {
                 const int rc = jit_launch(c, t, 2, st);
                 if (rc) return rc;
+            } else if (t.ts.nch > 0 && !uses_offset(c->variant)) {
+                // Hilbert-Schmidt corrections: the points' trace channels first, whose
+                // weighted conjugates seed the adjoint sweep
+                TileLaunch tf = t;
+                tf.mpart = nullptr;
+                HIPTRY(c, launch_tile_grad(tf, st));
+                HIPTRY(c, launch_finish(t.diag, gr.cols, gr.B, c->variant, nullptr, nullptr, 0, gr.ns, gr.slots,
+                                        bb.xt.as<double>(), o.P, nullptr, st, t.bi, t.ts, gr.seed.as<double>()));
+                TileLaunch ts = t;
+                ts.ts.seed = gr.seed.as<double>();
+                HIPTRY(c, launch_tile_grad(ts, st));
             } else {
                 HIPTRY(c, launch_tile_grad(t, st));
             }
@@ -149,6 +162,7 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
         HIPTRY(c, g.kern64.ensure(kern_elems * 16));
         HIPTRY(c, g.kern32.ensure(kern_elems * 8));
         HIPTRY(c, g.diag.ensure((size_t)(correction_channels(c->variant) + 1) * g.B * g.cols * 2 * sizeof(double)));
+        HIPTRY(c, g.seed.ensure((size_t)g.B * 6 * sizeof(double)));
         HIPTRY(c, g.mpart.ensure((size_t)g.B * g.wpb * std::max(g.ns, 1) * 8 * sizeof(double)));
     }
     if (G > 1) {  // side streams and fork/join events, kept by the context
@@ -264,10 +278,6 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
 int check_opts(sqh_ctx* c, const sqh_decompose_options* opts) {
     if (opts->max_inner_iterations <= 0 || opts->max_outer_iterations <= 0)
         return fail(c, SQH_ERR_ARG, "iteration limits must be positive");
-    // The Hilbert-Schmidt corrections need each point's traces before its gradient
-    // sweep (a second pass per round); the batched optimiser does not run it.
-    if (c->variant == CV_HS_C1 || c->variant == CV_HS_C2)
-        return fail(c, SQH_ERR_ARG, "batched optimisation does not support the Hilbert-Schmidt correction variants");
     return SQH_OK;
 }
 

@@ -633,6 +633,7 @@ __device__ __forceinline__ T seg_sum(T x, int lim) {
 // finish kernel evaluates.  This replaces the reference's per-parameter derivative
 // matrices (Gates_block.cpp:1012-1150) with O(G d m) work.
 // ---------------------------------------------------------------------------
+// With mpart null the kernel stops after the forward pass and the trace channels.
 // STAGE (with UNI): the wavefront first copies its program (ops with their padding,
 // layouts) and its element's kernel table into LDS (dynamic shared memory, sized by
 // the host for the largest program of the launch: ops_cap ops, lays_cap layouts),
@@ -720,6 +721,7 @@ __global__ __launch_bounds__(64) void k_tile_grad(const cpx<R>* __restrict__ in,
         const int so = sub_off<N>(&lays[cur], li.s);
         const int tr = li.col + trace_off;
         emit_trace<N, R>(x, &lays[cur], so, li, tr, m, diag, ts);
+        if (!mpart) return;  // forward pass only: the trace channels (seed weights come next)
         seed_trace<N, R>(y, &lays[cur], so, li, tr, ts);
     }
     // Which lanes write the per-wave partials.

@@ -131,8 +131,11 @@ def test_reachable_target_zero_at_solution(oracle):
         assert np.abs(g).max() < 1e-12
 
 
-@pytest.mark.parametrize("variant", [nat.COST_FROBENIUS_CORRECTION1, nat.COST_FROBENIUS_CORRECTION2])
-def test_batched_optimiser_frobenius_corrections(oracle, variant):
+@pytest.mark.parametrize("variant", CORR)
+def test_batched_optimiser_corrections(oracle, variant):
+    """Every correction variant in the batched optimiser (the Hilbert-Schmidt ones with
+    a forward pass for the seed weights each round): reported costs equal the oracle's
+    cost of the returned parameters."""
     say(f"decompose_batch correction variant={variant}")
     n = 3
     gates = W.ansatz_layers(n, W.alternating_pairs(n, 4))
@@ -151,16 +154,26 @@ def test_batched_optimiser_frobenius_corrections(oracle, variant):
         assert abs(f_check - res["f"][r]) < 1e-12
 
 
-def test_batched_optimiser_rejects_hs_corrections():
-    n = 2
-    gates = W.ansatz_layers(n, [0])
+def test_batched_optimiser_hs_correction_mixed_precision(oracle):
+    """Hilbert-Schmidt correction with the mixed-precision optimiser (complex64 and
+    complex128 evaluations in one round share the seed buffer)."""
+    say("decompose_batch HS correction 2, mixed precision")
+    n = 3
+    gates = W.ansatz_layers(n, W.alternating_pairs(n, 4))
     P = W.param_count(gates)
+    rng = np.random.default_rng(53)
+    T = np.array([oracle.apply_to(gates, rng.uniform(0, 2 * np.pi, P), np.eye(8)).conj().T for _ in range(3)])
+    x0 = np.array([initial_guess(P, s, "RANDOM") for s in range(11, 14)])
     ctx = nat.Context(0)
     ctx.set_circuit(n, gates)
-    ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT_CORRECTION1)
-    T = np.eye(4, dtype=np.complex128)[None]
-    with pytest.raises(nat.SqhError, match="correction"):
-        decompose_batch(ctx, T, np.zeros(1, dtype=np.int32), np.zeros((1, P)), default_options(n))
+    ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT_CORRECTION2)
+    res = decompose_batch(ctx, T, np.arange(3, dtype=np.int32), x0,
+                          default_options(n, optimization_tolerance=1e-8, max_outer_iterations=2,
+                                          mixed_precision=1))
+    for r in range(3):
+        f_check = oracle.optimization_problem(gates, T[r], n, res["x"][r], nat.COST_HILBERT_SCHMIDT_CORRECTION2)
+        say(f"  run {r}: f={res['f'][r]:.3e} evals={res['evals'][r]}")
+        assert abs(f_check - res["f"][r]) < 1e-12
 
 
 def test_correction_weights_validated():
