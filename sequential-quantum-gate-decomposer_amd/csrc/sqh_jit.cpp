@@ -465,8 +465,8 @@ int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
 }
 
 // Gradient kernels of long programs compile slowly (the straight-line adjoint sweep
-// of the 128-gate, 142-op 8-qubit circuit takes about 10 s on the host); longer
-// programs keep the generic gradient kernel.
+// of the 142-op 8-qubit circuit takes about 12 s on the host, the 214-op 10-qubit one
+// about 17 s); longer programs keep the generic gradient kernel.
 constexpr size_t kJitMaxGradOps = 256;
 
 bool jit_wants(sqh_ctx* c, int kind) {
