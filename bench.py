@@ -247,6 +247,44 @@ def derivate_leg(gates, P, U):
     return res
 
 
+def agents_leg(o):
+    """AGENTS (gradient-free population search, AGENTS.cpp:323-600) on the 8-qubit
+    ansatz: 64 agents, each iteration's population costs in one batched launch.  The
+    CPU oracle follows the identical trajectory (same generator consumption), timed on
+    a bounded sample of iterations on one host thread."""
+    import numpy as np
+
+    from squander_hip import Context, agents, agents_options
+    from squander_hip import workloads as W
+    n = 8
+    gates = W.ansatz_8q_40(2026)
+    P = W.param_count(gates)
+    theta = W.random_params(P, 31)
+    T = o.apply_to(gates, theta, np.eye(256)).conj().T.copy()
+    x0 = W.random_params(P, 32)
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(T)
+    res = {"workload": "8-qubit ansatz (264 angles), reachable target, 64 agents"}
+    k_cpu, k_gpu = 12, 200
+    agents(ctx, x0, 5, agents_options(max_iterations=2))  # warm-up
+    t0 = time.perf_counter()
+    fg, xg, itg, evg = agents(ctx, x0, 5, agents_options(max_iterations=k_gpu, optimization_tolerance=1e-8))
+    tg = time.perf_counter() - t0
+    fs, xs, its, evs = agents(ctx, x0, 5, agents_options(max_iterations=k_cpu, optimization_tolerance=1e-8))
+    t0 = time.perf_counter()
+    fo, xo, ito, evo = o.agents(gates, T, n, 5, x0, agent_num=64, max_iterations=k_cpu, tolerance=1e-8)
+    tc = time.perf_counter() - t0
+    res["gpu"] = {"iterations": itg, "evaluations": evg, "seconds": tg, "iterations_per_s": itg / tg,
+                  "evals_per_s": evg / tg, "final_cost": fg}
+    res["cpu"] = {"iterations": ito, "evaluations": evo, "seconds": tc, "iterations_per_s": ito / tc,
+                  "evals_per_s": evo / tc, "cores": 1, "kind": "port",
+                  "sample": f"the first {k_cpu} iterations of the same run"}
+    res["same_trajectory"] = bool(its == ito and evs == evo and abs(fs - fo) < 1e-10)
+    log(f"AGENTS 8q: GPU {itg / tg:.1f} it/s, CPU {ito / tc:.2f} it/s, same trajectory {res['same_trajectory']}")
+    return res
+
+
 def uneven_leg():
     from squander_hip import uneven_bench
     res, _, _ = uneven_bench.run(count=256, seed=2028, tol=1e-8, mixed=True, log=log)
@@ -534,6 +572,7 @@ def main():
     out["ten_qubit_b64"] = ten_qubit_leg(args.steps, o)
     out["apply_derivate_8q"] = derivate_leg(gates, P, U)
     if not args.skip_solves:
+        out["agents_8q"] = agents_leg(o)
         out["uneven_256"] = uneven_leg()
         out["solves_4q"] = solves_leg()
     if not args.skip_cpu:
