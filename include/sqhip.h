@@ -287,6 +287,12 @@ int sqh_get_kernel_timing(sqh_ctx* ctx, double* total_ms, int* launches);
  * No counterpart in the reference: it exists for tooling and the CPU test suite. */
 int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* ops_out, int ops_cap,
                         void* layouts_out, int layouts_cap, int* n_ops, int* n_layouts);
+/* The same program's parametric-gate records (16 bytes each: gate type, first
+ * parameter, parameter count, op index), in the order the ops' slot indices use
+ * (the compiler may run gates on disjoint qubits out of list order).  No counterpart
+ * in the reference (tooling and the CPU test suite). */
+int sqh_inspect_slots(int qbit_num, const sqh_gate* gates, int n_gates, void* slots_out, int slots_cap,
+                      int* n_slots);
 
 #ifdef __cplusplus
 }

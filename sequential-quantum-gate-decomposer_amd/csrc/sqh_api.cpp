@@ -100,8 +100,8 @@ TileLaunch base_launch(sqh_ctx* c, bool f32) {
 // [B][ns][4] kernel tables (kern_bstride = 0 when the parameters are shared).
 int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
                  long long kern_bstride) {
-    int slot = 0;
-    for (const GateRec& g : c->gates) {
+    for (size_t gi = 0; gi < c->gates.size(); ++gi) {
+        const GateRec& g = c->gates[gi];
         int kind, control = -1;
         switch (g.type) {
         case GT_CNOT: kind = OP_X; control = g.control; break;
@@ -109,7 +109,7 @@ int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lo
         case GT_CRY: kind = OP_CU; control = g.control; break;
         default: kind = OP_U; break;
         }
-        const int s = (kind == OP_U || kind == OP_CU) ? slot++ : 0;
+        const int s = (kind == OP_U || kind == OP_CU) ? c->prog.gate_slot[gi] : 0;
         TimedLaunch tl(c);
         HIPTRY(c, launch_stream_gate(f32, mats, bstride, stride, rows, cols, B, kind, g.target, control, c->d_kern.p,
                                      kern_bstride, s, c->stream));
@@ -371,7 +371,7 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
     HIPTRY(c, hipMemcpy(c->d_lays.p, p.layouts.data(), p.layouts.size() * sizeof(Layout), hipMemcpyHostToDevice));
     if (!p.slots.empty())
         HIPTRY(c, hipMemcpy(c->d_slots.p, p.slots.data(), p.slots.size() * sizeof(Slot), hipMemcpyHostToDevice));
-    FusionPlan fp = plan_fusion(qbit_num, g);
+    FusionPlan fp = plan_fusion(qbit_num, g, p.gate_slot);
     static_assert(sizeof(FusedRec) == 8, "fused record layout");
     HIPTRY(c, c->d_fgates.ensure(std::max<size_t>(fp.gates.size(), 1) * sizeof(FusedRec)));
     HIPTRY(c, c->d_ffirst.ensure(fp.first.size() * sizeof(int)));
@@ -738,6 +738,20 @@ int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* 
     if (layouts_out && layouts_cap > 0)
         std::memcpy(layouts_out, p.layouts.data(),
                     sizeof(Layout) * std::min<size_t>(p.layouts.size(), (size_t)layouts_cap));
+    return SQH_OK;
+}
+
+int sqh_inspect_slots(int qbit_num, const sqh_gate* gates, int n_gates, void* slots_out, int slots_cap,
+                      int* n_slots) {
+    if (n_gates < 0 || (n_gates > 0 && !gates)) return SQH_ERR_ARG;
+    std::vector<GateRec> g(n_gates);
+    for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
+    Program p;
+    std::string err;
+    if (!compile_program(qbit_num, g, p, err)) return SQH_ERR_GATE;
+    if (n_slots) *n_slots = (int)p.slots.size();
+    if (slots_out && slots_cap > 0)
+        std::memcpy(slots_out, p.slots.data(), sizeof(Slot) * std::min<size_t>(p.slots.size(), (size_t)slots_cap));
     return SQH_OK;
 }
 

@@ -19,6 +19,7 @@ struct Program {
     std::vector<Op> ops;
     std::vector<Layout> layouts;
     std::vector<Slot> slots;
+    std::vector<int> gate_slot;  // per gate of the input list: its slot (kernel-table index), or -1
     int remaps = 0;
 };
 
@@ -36,7 +37,8 @@ struct FusionPlan {
     int k = 0;                          // qubits per block
 };
 
-FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates);
+// gate_slot: the kernel-table index of each parametric gate (Program::gate_slot).
+FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates, const std::vector<int>& gate_slot);
 
 // Validates the gate list (types, qubit indices, parameter ranges) the way the
 // reference's gate constructors and Gates_block::add_gate do (U3.cpp:41-46,

@@ -38,7 +38,7 @@ EXPORTS = [
     "sqh_apply_derivate_to", "sqh_apply_derivate_to_dev", "sqh_optimization_problem_batch",
     "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
-    "sqh_inspect_program", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
+    "sqh_inspect_program", "sqh_inspect_slots", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents",
@@ -154,6 +154,7 @@ def lib():
                                        ctypes.POINTER(ll), ctypes.POINTER(ll)], i),
             "sqh_get_jit_stats": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
+            "sqh_inspect_slots": ([i, ctypes.POINTER(_Gate), i, vp, i, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_program": ([i, ctypes.POINTER(_Gate), i, vp, i, vp, i, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)], i),
         }
@@ -189,6 +190,22 @@ def inspect_program(qbit_num, gates):
     if rc:
         raise SqhError(rc, "program compilation failed")
     return ops[:nops.value], lays[:nl.value]
+
+
+def inspect_slots(qbit_num, gates):
+    """The program's parametric-gate records in slot order: int32 array [n, 4] of
+    (gate type, first parameter, parameter count, op index)."""
+    L = lib()
+    garr, ng = gate_array(gates)
+    n = ctypes.c_int(0)
+    rc = L.sqh_inspect_slots(qbit_num, garr, ng, None, 0, ctypes.byref(n))
+    if rc:
+        raise SqhError(rc, "program compilation failed")
+    sl = np.zeros((max(n.value, 1), 4), dtype=np.int32)
+    rc = L.sqh_inspect_slots(qbit_num, garr, ng, sl.ctypes.data, n.value, ctypes.byref(n))
+    if rc:
+        raise SqhError(rc, "program compilation failed")
+    return sl[:n.value]
 
 
 class Context:

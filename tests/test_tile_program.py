@@ -15,11 +15,12 @@ from squander_hip import _native as nat
 from squander_hip import workloads as W
 
 
-def kernels_for(oracle, gates, params):
+def kernels_for(oracle, n, gates, params):
+    """Kernel table in the program's slot order (the compiler may run gates on
+    disjoint qubits out of list order)."""
     ks = []
-    for t, tg, c, ps in gates:
-        if nat.PARAM_NUM[int(t)]:
-            ks.append(oracle.gate_kernel(int(t), params[ps:ps + nat.PARAM_NUM[int(t)]]).reshape(-1))
+    for t, ps, npar, _ in nat.inspect_slots(n, gates):
+        ks.append(oracle.gate_kernel(int(t), params[ps:ps + npar]).reshape(-1))
     return ks
 
 
@@ -32,7 +33,7 @@ def test_forward_program_matches_oracle(oracle, n):
     d = 1 << n
     m = min(d, 48)
     X = np.random.default_rng(7).standard_normal((d, m)) + 1j * np.random.default_rng(8).standard_normal((d, m))
-    out, _ = te.run_forward(n, ops, lays, kernels_for(oracle, gates, p), X)
+    out, _ = te.run_forward(n, ops, lays, kernels_for(oracle, n, gates, p), X)
     assert rel_fro(out, oracle.apply_to(gates, p, X)) < 1e-13
 
 
@@ -57,13 +58,12 @@ def test_adjoint_gradient_matches_oracle(oracle, n, off, variant):
     U = W.haar_unitary(d, 50 + n)[:, :m].copy()
     ops, lays = te.decode(*nat.inspect_program(n, gates))
     toff = off if variant in (orc.COST_FROBENIUS, orc.COST_ABS_TRACE) else 0
-    trace, Ms = te.run_adjoint(n, ops, lays, kernels_for(oracle, gates, p), U, toff)
+    trace, Ms = te.run_adjoint(n, ops, lays, kernels_for(oracle, n, gates, p), U, toff)
     # gradient from the correlations, as the finish kernel forms it
     grad = np.zeros(P)
-    slot = 0
-    for t, tg, c, ps in gates:
-        for q in range(nat.PARAM_NUM[int(t)]):
-            D = oracle.deriv_kernel(int(t), p[ps:ps + nat.PARAM_NUM[int(t)]], q).reshape(-1)
+    for slot, (t, ps, npar, _) in enumerate(nat.inspect_slots(n, gates)):
+        for q in range(npar):
+            D = oracle.deriv_kernel(int(t), p[ps:ps + npar], q).reshape(-1)
             tk = np.sum(D * Ms[slot])
             if variant == orc.COST_FROBENIUS:
                 grad[ps + q] = -tk.real / m
@@ -73,8 +73,6 @@ def test_adjoint_gradient_matches_oracle(oracle, n, off, variant):
                 grad[ps + q] = -2.0 / m / (m + 1) * (trace.real * tk.real + trace.imag * tk.imag)
             else:
                 grad[ps + q] = -(trace.real * tk.real + trace.imag * tk.imag) / (abs(trace) * m)
-        if nat.PARAM_NUM[int(t)]:
-            slot += 1
     f_ref, g_ref = oracle.optimization_problem_combined(gates, U, n, p, variant, off)
     assert abs(orc.brute_cost(oracle.apply_to(gates, p, U), variant, off) - f_ref) < 1e-13
     assert np.abs(grad - g_ref).max() < 1e-12
