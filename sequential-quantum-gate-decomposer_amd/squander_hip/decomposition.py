@@ -58,6 +58,10 @@ class N_Qubit_Decomposition_custom:
         self.max_iterations = None
         self._ctx = nat.Context(0)
         self._ctx.set_target(U)
+        # Circuit-specialised kernels (compiled on first use per circuit: about 0.6 s
+        # for the 8-qubit cost kernel, 12 s for its gradient kernel); a config key of
+        # this build, off by default so single calls keep their latency.
+        self._ctx.set_jit(bool(self.config.get("specialised_kernels", False)))
 
     # -- configuration -------------------------------------------------------
     def set_Gate_Structure(self, circuit: Circuit):

@@ -185,3 +185,21 @@ def test_jit_apply_derivate_to_matches_oracle(oracle, n):
                 assert err < 1e-12, (jit, m, k, err)
             if jit:
                 assert ctx.jit_stats()[1] == 1
+
+
+def test_decomposition_config_enables_specialised_kernels(oracle):
+    """N_Qubit_Decomposition_custom(config={"specialised_kernels": True}) evaluates
+    through the run-time compiled kernels, with the oracle's results."""
+    from squander_hip import Circuit, N_Qubit_Decomposition_custom
+    say("decomposition class with specialised kernels")
+    n = 5
+    gates = W.mixed_gate_circuit(n, 24, 1500)
+    P = W.param_count(gates)
+    U = W.haar_unitary(32, 1501).conj().T.copy()
+    p = W.random_params(P, 1502)
+    dec = N_Qubit_Decomposition_custom(U, config={"specialised_kernels": True})
+    dec.set_Gate_Structure(Circuit.from_gate_array(n, gates))
+    f, g = dec.Optimization_Problem_Combined(p)
+    fo, go = oracle.optimization_problem_combined(gates, U, n, p)
+    assert abs(f - fo) < 1e-12 and np.abs(g - go).max() < 1e-12
+    assert dec._ctx.jit_stats()[1] >= 1
