@@ -21,6 +21,12 @@ __device__ inline double wg_sum(double v, double* sh) {
     return t;
 }
 
+// Sum over the wavefront, lane 0's result broadcast (so a branch on it is uniform).
+__device__ inline double wave_total(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return __shfl(v, 0, 64);
+}
+
 __device__ inline double wg_min(double v, double* sh) {
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
@@ -77,8 +83,8 @@ __global__ void k_bfgs_reset_counts(int* __restrict__ counts, int n_groups) {
 // consecutive threads on consecutive rows.  The rotation angles of the sweep depend
 // only on the vector being folded (Z^T g), so one thread computes all of them first
 // and every thread then applies the whole sequence to its own rows without barriers.
-__device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int P, int Ps, double& norm_Z,
-                            double* sh, double* zs, double* rc, double* rs) {
+__device__ bool bfgs_update(double* __restrict__ Z, double* __restrict__ V, int P, int Ps, double& norm_Z,
+                            double* sh, double* zs, double* rc, double* rs, double* pd) {
     const int tid = threadIdx.x, nt = blockDim.x;
     double* x = V + RV_X * Ps;
     double* g = V + RV_G * Ps;
@@ -86,7 +92,8 @@ __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
     double* g0 = V + RV_G0 * Ps;
     double* delta = V + RV_DELTA * Ps;
     double* gamma = V + RV_GAMMA * Ps;
-    const double* ztg = V + RV_ZTG * Ps;
+    double* ztg = V + RV_ZTG * Ps;
+    double* d = V + RV_D * Ps;
     double nd = 0.0, t0 = 0.0, dtg = 0.0;
     for (int k = tid; k < P; k += nt) {
         const double dk = x[k] - x0[k], gk = g[k] - g0[k];
@@ -99,7 +106,7 @@ __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
     nd = wg_sum(nd, sh);
     t0 = wg_sum(t0, sh);
     dtg = wg_sum(dtg, sh);
-    if (dtg < fabs(t0) * 0.1) return;
+    if (dtg < fabs(t0) * 0.1) return false;
     dtg = fmax(kNumPrecision, dtg);
     // Givens rotations folding Z^T g into its first entry (Powell 1987, Eq. 1.7):
     // angles first (sequential in c), then the column pairs (c-1, c) of every row.
@@ -147,28 +154,51 @@ __device__ void bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
     const double sq = sqrt(dtg);
     for (int row = tid; row < P; row += nt) Z[row] = fabs(delta[row]) > kNumPrecision ? delta[row] / sq : 0.0;
     __syncthreads();
-    for (int c = 1 + tid; c < P; c += nt) {
+    // Column c (c >= 1) per wavefront, its rows over the lanes (contiguous, coalesced).
+    const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+    for (int c = 1 + wave; c < P; c += nw) {
         const double* zc = Z + (size_t)c * P;
         double s = 0.0;
-        for (int row = 0; row < P; ++row) s += gamma[row] * zc[row];
-        zs[c] = s / dtg;
+        for (int row = lane; row < P; row += 64) s += gamma[row] * zc[row];
+        s = wave_total(s);
+        if (lane == 0) zs[c] = s / dtg;
     }
+    for (int i = tid; i < nw * P; i += nt) pd[i] = 0.0;
     __syncthreads();
-    for (int c = 1 + tid; c < P; c += nt) {
-        const double gz = zs[c];
+    // The column updates, and in the same pass the next direction's products with the
+    // new Z (PH_DIRECTION: Z^T g per column, and each column's share of Z (Z^T g),
+    // summed over the wavefronts' partials afterwards).
+    for (int c = wave; c < P; c += nw) {
         double* zc = Z + (size_t)c * P;
-        double nz = 0.0;
-        for (int row = 0; row < P; ++row) {
-            const double v = zc[row] - gz * delta[row];
-            zc[row] = v;
-            nz += v * v;
+        if (c > 0) {
+            const double gz = zs[c];
+            double nz = 0.0;
+            for (int row = lane; row < P; row += 64) {
+                const double v = zc[row] - gz * delta[row];
+                zc[row] = v;
+                nz += v * v;
+            }
+            nz = wave_total(nz);
+            if (nz < norm_Z && nz > kNumPrecision) {
+                const double sc = sqrt(norm_Z / nz);
+                for (int row = lane; row < P; row += 64) zc[row] *= sc;
+            }
         }
-        if (nz < norm_Z && nz > kNumPrecision) {
-            const double sc = sqrt(norm_Z / nz);
-            for (int row = 0; row < P; ++row) zc[row] *= sc;
-        }
+        // each lane reads back only the rows it wrote (cache hits)
+        double tg = 0.0;
+        for (int row = lane; row < P; row += 64) tg += zc[row] * g[row];
+        tg = wave_total(tg);
+        if (lane == 0) ztg[c] = tg;
+        for (int row = lane; row < P; row += 64) pd[wave * P + row] += zc[row] * tg;
     }
     __syncthreads();
+    for (int row = tid; row < P; row += nt) {
+        double sum = 0.0;
+        for (int w = 0; w < nw; ++w) sum += pd[w * P + row];
+        d[row] = -sum;
+    }
+    __syncthreads();
+    return true;
 }
 
 __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restrict__ states,
@@ -179,13 +209,17 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                                                                const int* __restrict__ run_group, BfgsOpts o,
                                                                int B) {
     __shared__ double sh[8];
-    __shared__ double zs[kBfgsMaxP];
-    __shared__ double rot_c[kBfgsMaxP];
-    __shared__ double rot_s[kBfgsMaxP];
+    // dynamic LDS (sized by Ps): zs, rot_c, rot_s [Ps] and the wavefronts' partial
+    // directions [kBfgsThreads / 64][Ps]
+    extern __shared__ double bfgs_lds[];
+    const int Ps = o.P;  // stride of the per-run vectors
+    double* zs = bfgs_lds;
+    double* rot_c = zs + Ps;
+    double* rot_s = rot_c + Ps;
+    double* pd = rot_s + Ps;
     const int r = blockIdx.x;
     if (r >= B) return;
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int Ps = o.P;  // stride of the per-run vectors
     RunState S = states[r];
     if (S.phase == PH_FINISHED) return;
     const int P = S.P;
@@ -202,6 +236,7 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
     double* xr = xt + (size_t)r * Ps;
     const double* gr = ge + (size_t)r * Ps;
     bool request = false;
+    bool dir_ready = false;  // the update already formed d = -Z Z^T g
     for (int guard = 0; guard < 16 && !request && S.phase != PH_FINISHED; ++guard) {
         switch (S.phase) {
         case PH_OUTER_START: {
@@ -232,21 +267,28 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                 S.phase = PH_BFGS_END;
                 break;
             }
-            // d = -Z Z^T g (BFGS_Powell.cpp:396-439); Z column-major
-            for (int c = tid; c < P; c += nt) {
-                const double* zc = Z + (size_t)c * P;
-                double s = 0.0;
-                for (int row = 0; row < P; ++row) s += zc[row] * g[row];
-                ztg[c] = s;
-            }
-            __syncthreads();
-            for (int row = tid; row < P; row += nt) {
-                double s = 0.0;
+            // d = -Z Z^T g (BFGS_Powell.cpp:396-439); Z column-major: Z^T g with a
+            // column per wavefront (coalesced rows), then Z (Z^T g) with a row per
+            // thread -- unless the update just formed both
+            if (!dir_ready) {
+                const int lane = tid & 63, wave = tid >> 6, nw = nt >> 6;
+                for (int c = wave; c < P; c += nw) {
+                    const double* zc = Z + (size_t)c * P;
+                    double s = 0.0;
+                    for (int row = lane; row < P; row += 64) s += zc[row] * g[row];
+                    s = wave_total(s);
+                    if (lane == 0) ztg[c] = s;
+                }
+                __syncthreads();
+                for (int row = tid; row < P; row += nt) {
+                    double s = 0.0;
 #pragma unroll 8
-                for (int c = 0; c < P; ++c) s += Z[(size_t)c * P + row] * ztg[c];
-                d[row] = -s;
+                    for (int c = 0; c < P; ++c) s += Z[(size_t)c * P + row] * ztg[c];
+                    d[row] = -s;
+                }
+                __syncthreads();
             }
-            __syncthreads();
+            dir_ready = false;
             double dg = 0.0, gn = 0.0, ms = INFINITY;
             for (int k = tid; k < P; k += nt) {
                 dg += d[k] * g[k];
@@ -356,7 +398,7 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                 S.phase = PH_BFGS_END;
                 break;
             }
-            bfgs_update(Z, V, P, Ps, S.norm_Z, sh, zs, rot_c, rot_s);
+            dir_ready = bfgs_update(Z, V, P, Ps, S.norm_Z, sh, zs, rot_c, rot_s, pd);
             S.phase = PH_DIRECTION;
             break;
         }
@@ -426,8 +468,9 @@ hipError_t launch_bfgs_reset_counts(int* counts, int n_groups, hipStream_t s) {
 hipError_t launch_bfgs_advance(void* st, double* vecs, double* Z, const double* fe, const double* ge, double* xt,
                                int* lists, int* counts, const int* run_group, const BfgsOpts& o, int B,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_bfgs_advance, dim3(B), dim3(kBfgsThreads), 0, s, (RunState*)st, vecs, Z, fe, ge, xt, lists,
-                       counts, run_group, o, B);
+    const size_t shm = sizeof(double) * (size_t)o.P * (3 + kBfgsThreads / 64);
+    hipLaunchKernelGGL(k_bfgs_advance, dim3(B), dim3(kBfgsThreads), shm, s, (RunState*)st, vecs, Z, fe, ge, xt,
+                       lists, counts, run_group, o, B);
     return hipGetLastError();
 }
 
