@@ -37,6 +37,32 @@ def test_forward_program_matches_oracle(oracle, n):
     assert rel_fro(out, oracle.apply_to(gates, p, X)) < 1e-13
 
 
+@pytest.mark.parametrize("n,seed", [(5, 1), (6, 2), (8, 3), (10, 4)])
+def test_schedule_keeps_each_qubits_gate_order(n, seed):
+    """The compiler may run gates on disjoint qubits out of list order, never two gates
+    that share a qubit: along every qubit, the parametric gates' slot order follows
+    the list order."""
+    gates = W.mixed_gate_circuit(n, 8 * n, seed=900 + seed)
+    slots = nat.inspect_slots(n, gates)
+    index_of = {}
+    for i, (t, tg, c, ps) in enumerate(gates):
+        if nat.PARAM_NUM[int(t)]:
+            index_of[int(ps)] = i
+    order = [index_of[int(ps)] for _, ps, _, _ in slots]
+    assert sorted(order) == sorted(index_of.values())
+    for q in range(n):
+        on_q = [i for i in order if q in (int(gates[i][1]), int(gates[i][2]))]
+        assert on_q == sorted(on_q)
+
+
+def test_schedule_remap_counts_of_the_benchmark_circuits():
+    """Dependency-order scheduling: 5 remaps for the 8-qubit 40-layer ansatz and 7 for
+    the 10-qubit 60-layer one (gate order needs 14 and 24)."""
+    for n, gates, limit in ((8, W.ansatz_8q_40(2026), 5), (10, W.ansatz_10q_60(2027), 7)):
+        ops, _ = te.decode(*nat.inspect_program(n, gates))
+        assert sum(1 for o in ops if o[0] == te.OP_REMAP) <= limit
+
+
 def test_program_remaps_only_when_needed():
     # 8-qubit ansatz: every op target must be a register qubit; remaps are far fewer than gates.
     gates = W.ansatz_8q_40()
