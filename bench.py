@@ -54,15 +54,30 @@ def gate_work(gates, d, m, esz):
     return fl, by
 
 
-def cpu_baseline(gates, U, n, P, seconds=10.0):
-    """The CPU oracle's batched cost, threaded over batch elements on the host cores."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+def host_info():
+    """Core count, CPU model and the SIMD extensions the oracle can use."""
+    info = {"nproc": os.cpu_count()}
+    try:
+        with open("/proc/cpuinfo") as f:
+            txt = f.read()
+        model = [l.split(":", 1)[1].strip() for l in txt.splitlines() if l.startswith("model name")]
+        flags = [l.split(":", 1)[1].split() for l in txt.splitlines() if l.startswith("flags")]
+        info["model"] = model[0] if model else None
+        info["simd"] = sorted(f for f in (flags[0] if flags else []) if f in ("avx2", "fma", "avx512f", "avx512dq"))
+    except OSError:
+        pass
+    return info
+
+
+def cpu_baseline(gates, U, n, P, o, seconds=10.0):
+    """The CPU oracle (the restated C path, -O3 -mavx2 -mfma, OpenMP over the batch) on
+    the host cores: the 8-qubit batched cost on all cores (the reported value) and on
+    one core, the 8-qubit single-vector cost + gradient latency (P derivative
+    matrices, as the reference's combined evaluation forms them) and the 10-qubit
+    batched cost."""
     import numpy as np
 
-    import oracle as orc
     from squander_hip import workloads as W
-
-    o = orc.Oracle()
     threads = min(os.cpu_count() or 1, 64)
     ps = W.random_params(P, 99, batch=64)
     done, t0 = 0, time.perf_counter()
@@ -70,9 +85,26 @@ def cpu_baseline(gates, U, n, P, seconds=10.0):
         o.optimization_problem_batched(gates, U, n, ps, nthreads=threads)
         done += ps.shape[0]
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "cost evals/s", "cores": threads, "kind": "port",
-            "sample": f"{done} cost evals of the 8q/40-layer ansatz (batches of 64, OpenMP over the batch, "
-                      f"{dt:.1f} s)"}
+    res = {"value": done / dt, "unit": "cost evals/s", "cores": threads, "kind": "port",
+           "sample": f"{done} cost evals of the 8q/40-layer ansatz (batches of 64, OpenMP over the batch, "
+                     f"{dt:.1f} s)", "host": host_info()}
+    done1, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < 3.0:
+        o.optimization_problem_batched(gates, U, n, ps[:8], nthreads=1)
+        done1 += 8
+    res["one_core"] = {"value": done1 / (time.perf_counter() - t0), "unit": "cost evals/s", "cores": 1}
+    t0 = time.perf_counter()
+    o.optimization_problem_combined(gates, U, n, ps[0])
+    res["c2_cost_grad_latency_s"] = {"value": time.perf_counter() - t0, "cores": 1,
+                                     "sample": "one 264-parameter cost + gradient from the derivative matrices"}
+    g10 = W.ansatz_10q_60(2027)
+    U10 = W.haar_unitary(1024, 2027).conj().T.copy()
+    p10 = W.random_params(W.param_count(g10), 8, batch=threads)
+    t0 = time.perf_counter()
+    o.optimization_problem_batched(g10, U10, 10, p10, nthreads=threads)
+    res["c3_10q"] = {"value": threads / (time.perf_counter() - t0), "unit": "cost evals/s", "cores": threads,
+                     "sample": f"one batch of {threads} 10-qubit evaluations"}
+    return res
 
 
 def generic_leg(gates, U, params, costs, steps, flops):
@@ -244,6 +276,15 @@ def derivate_leg(gates, P, U):
                      "kernel": ("sqh_jit_kernel (forward store, per-parameter kernel tables)" if jit
                                 else "k_tile_forward<8,double,0> (per-parameter kernel tables)")}
         log(f"apply_derivate_to 8q {name}: {k_s * 1e3:.3f} ms")
+    # error of the 264 derivative matrices (the last call's) against the FP64 oracle
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+
+    import oracle as orc
+    ref = orc.Oracle().apply_derivate_to(gates, p.cpu().numpy(), U)
+    got = out.cpu().numpy()
+    res["err_rel_fro_vs_fp64_oracle"] = max(float(np.linalg.norm(got[k] - ref[k]) / np.linalg.norm(ref[k]))
+                                            for k in range(P))
     return res
 
 
@@ -499,6 +540,18 @@ def main():
                   "frac_of_unpacked_peak": flops_eval * B / k32_s / 1e12 / FP64_PEAK_TFLOPS}
     log(f"c64: {out['c64']['value']:.0f} evals/s, err {err32:.2e}")
 
+    def apply_err(fn):
+        """Max relative Frobenius error of two matrices applied once against the oracle."""
+        m2 = torch.from_numpy(np.broadcast_to(U, (2, d, d)).copy()).to(dev)
+        ctx.check(fn(ctx.h, params.data_ptr(), P, m2.data_ptr(), d, d, d, d * d, 2, 0))
+        ctx.synchronize()
+        got = m2.cpu().numpy()
+        err = 0.0
+        for b in range(2):
+            ref = o.apply_to(gates, params[b].cpu().numpy(), U)
+            err = max(err, float(np.linalg.norm(got[b] - ref) / np.linalg.norm(ref)))
+        return err
+
     # unfused streaming path: one HBM pass per gate over B matrices of 1 MiB
     mats = torch.from_numpy(np.broadcast_to(U, (B, d, d)).copy()).to(dev)
     torch.cuda.synchronize()
@@ -522,6 +575,7 @@ def main():
     out["unfused_stream"] = {"achieved_GBs": gbs, "peak_GBs": HBM_PEAK_GBS, "frac": gbs / HBM_PEAK_GBS,
                              "evals_per_s": B * reps / (t1 - t0), "kernel": "k_stream_gate<double>",
                              "launches": snl, "bytes_per_pass": bytes_pass * B,
+                             "err_rel_fro_vs_fp64_oracle": apply_err(L.sqh_apply_to_stream_dev),
                              "note": "algorithmic bytes over summed per-gate kernel time"}
     log(f"unfused stream: {gbs:.0f} GB/s")
 
@@ -543,6 +597,7 @@ def main():
     out["fused_stream"] = {"evals_per_s": B * reps / (t1 - t0), "blocks_per_pass": fnl // reps,
                            "achieved_GBs": blk_bytes * fnl / (fkms / 1e3) / 1e9,
                            "algorithmic_TFLOPs": flops_eval * B * reps / (fkms / 1e3) / 1e12,
+                           "err_rel_fro_vs_fp64_oracle": apply_err(L.sqh_apply_to_fused_dev),
                            "kernel": "k_block_mfma4 (v_mfma_f64_16x16x4f64)"}
     log(f"fused stream: {out['fused_stream']['evals_per_s']:.0f} evals/s")
     del mats
@@ -577,7 +632,7 @@ def main():
         out["solves_4q"] = solves_leg()
     if not args.skip_cpu:
         log("cpu baseline (oracle) ...")
-        out["cpu_baseline"] = cpu_baseline(gates, U, n, P)
+        out["cpu_baseline"] = cpu_baseline(gates, U, n, P, o)
         out["c1_3q_plain_solves"] = c1_leg(o)
     print(json.dumps(out), flush=True)
 
