@@ -229,6 +229,18 @@ typedef struct sqh_agents_options {
 int sqh_agents(sqh_ctx* ctx, double* x, const sqh_agents_options* opts, unsigned int seed, double* f_out,
                long long* iters_out, long long* evals_out);
 
+/* n_runs independent AGENTS runs on the context's circuit (FROBENIUS cost, its trace
+ * offset, up to 10 qubits), run k against the k-th complex128 rows x cols target of
+ * targets (host, contiguous, rows = 2^qbit_num), from the guess x + k*P with its
+ * generator seeded with seeds[k].  Each run's results (x + k*P, f_out[k], iters_out[k],
+ * evals_out[k]) equal those of sqh_agents on that target and seed; the runs advance in
+ * lockstep and every stage's evaluations of all live runs are one launch.  The
+ * reference runs one optimisation per target (AGENTS.cpp:42-870 per call); this entry
+ * point batches those calls. */
+int sqh_agents_batch(sqh_ctx* ctx, const double* targets, int n_runs, int rows, int cols, double* x,
+                     const sqh_agents_options* opts, const unsigned int* seeds, double* f_out, long long* iters_out,
+                     long long* evals_out);
+
 /* Run-time specialised tile kernels.  With enable = 1 the context compiles its
  * circuit, on first use, into straight-line forward and gradient kernels (hiprtc,
  * cached per circuit) and uses them for apply_to, apply_derivate_to, the batched

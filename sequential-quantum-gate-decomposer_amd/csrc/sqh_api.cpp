@@ -223,6 +223,47 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
     return SQH_OK;
 }
 
+int eval_targets(sqh_ctx* c, const double* d_params, int B, const void* d_targets, int rows, int cols,
+                 const int* d_index, double* d_costs) {
+    int rc = require_circuit(c);
+    if (rc) return rc;
+    const int N = c->prog.N, m = cols, ns = (int)c->prog.slots.size();
+    if (N > kMaxTileQubits) return fail(c, SQH_ERR_ARG, "per-target costs support up to %d qubits", kMaxTileQubits);
+    if (B <= 0) return fail(c, SQH_ERR_ARG, "batch size must be positive");
+    if (c->variant == CV_HS_C1 || c->variant == CV_HS_C2)
+        return fail(c, SQH_ERR_ARG, "per-target costs do not take the corrected HS cost");
+    const int off = uses_offset(c->variant) ? c->trace_offset : 0;
+    if (rows != (1 << N) || cols <= 0 || cols + off > rows)
+        return fail(c, SQH_ERR_ARG, "targets must be %d x m with m + trace offset <= %d", 1 << N, 1 << N);
+    const TraceSpec ts = trace_spec(c, B, m);
+    if ((rc = build_kernels(c, d_params, B, false))) return rc;
+    HIPTRY(c, c->d_diag.ensure((size_t)(ts.nch + 1) * B * m * 2 * sizeof(double)));
+    TileLaunch t = base_launch(c, false);
+    t.in = d_targets;
+    t.in_bstride = (long long)rows * m;
+    t.in_stride = m;
+    t.in_index = d_index;
+    t.m = m;
+    t.m_pad = m_pad_for(N, m);
+    t.B = B;
+    t.kern_bstride = (long long)ns * 4;
+    t.trace_off = off;
+    t.diag = c->d_diag.as<double>();
+    t.ts = ts;
+    t.mode = 1;
+    {
+        TimedLaunch tl(c);
+        if (jit_wants(c, 1)) {
+            if ((rc = jit_launch(c, t, 1, c->stream))) return rc;
+        } else {
+            HIPTRY(c, launch_tile_forward(t, c->stream));
+        }
+    }
+    HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, nullptr, 0, ns, t.slots, d_params, c->prog.P, nullptr,
+                            c->stream, BatchIndex(), ts));
+    return SQH_OK;
+}
+
 int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,
               long long bstride, int B, bool f32, int path) {
     int rc = require_circuit(c);

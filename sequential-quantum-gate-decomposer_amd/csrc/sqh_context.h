@@ -129,6 +129,11 @@ TileLaunch base_launch(sqh_ctx* c, bool f32);
 int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
                  long long kern_bstride);
 int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad);
+// FP64 costs of B parameter vectors, element b against the d_index[b]-th of the
+// complex128 rows x cols targets at d_targets (contiguous, rows = 2^N); the context's
+// circuit, cost variant and trace offset.  N <= 10.
+int eval_targets(sqh_ctx* c, const double* d_params, int B, const void* d_targets, int rows, int cols,
+                 const int* d_index, double* d_costs);
 // path: 0 register tile (streaming above the tile limit), 1 one pass per gate,
 // 2 fused dense blocks.
 int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, int rows, int cols, int stride,

@@ -506,7 +506,8 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
                                                            const Layout* __restrict__ lays,
                                                            const cpx<R>* __restrict__ kern, long long kern_bstride,
                                                            int trace_off, double* __restrict__ diag, int ns,
-                                                           TraceSpec ts, const int* __restrict__ zslot) {
+                                                           TraceSpec ts, const int* __restrict__ zslot,
+                                                           const int* __restrict__ in_index) {
     using G = Geo<N>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cpx<R>* smem = reinterpret_cast<cpx<R>*>(smem_raw);
@@ -516,7 +517,8 @@ __global__ __launch_bounds__(64 * WPG) void k_tile_forward(const cpx<R>* in, lon
     const LaneInfo li = lane_info<N, UNI>(m, m_pad, B, threadIdx.x & 63, (long long)blockIdx.x * WPG + wave);
     cpx<R> v[G::NR];
     {
-        const cpx<R>* src = in + (long long)li.b * in_bstride;
+        // input of element b: in + b * in_bstride, or in + in_index[b] * in_bstride
+        const cpx<R>* src = in + (long long)(in_index ? in_index[li.b] : li.b) * in_bstride;
         const int so = sub_off<N>(&lays[0], li.s);
 #pragma unroll
         for (int i = 0; i < G::NR; ++i) {

@@ -123,7 +123,8 @@ struct JitArgs {
     const int* elem_run;
     const int* run_target;
     long long tgt_stride;
-    const int* zslot;  // per element: the controlled slot whose control-0 block is zeroed, or -1
+    const int* zslot;     // per element: the controlled slot whose control-0 block is zeroed, or -1
+    const int* in_index;  // forward kernels: element b reads input in_index[b] (null: input b)
 };
 
 constexpr int kMaxTileQubits = 10;   // one column fits one wavefront up to 2^10 rows

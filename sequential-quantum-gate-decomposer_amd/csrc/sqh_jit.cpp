@@ -206,6 +206,7 @@ struct Gen {
           << "  (void)s; (void)cc; (void)lds;\n"
           << "  const cpx<R>* in = (const cpx<R>*)a.in + (long long)li.b * SINB;\n"
           << "  if (a.count) in = (const cpx<R>*)a.in + (long long)a.run_target[a.elem_run[li.b]] * a.tgt_stride;\n"
+          << "  if (a.in_index) in = (const cpx<R>*)a.in + (long long)a.in_index[li.b] * SINB;\n"
           << "  const cpx<R>* kb = (const cpx<R>*)a.kern + (long long)li.b * SKB;\n";
     }
 
@@ -456,6 +457,7 @@ int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
     a.run_target = t.bi.run_target;
     a.tgt_stride = t.bi.tgt_stride;
     a.zslot = t.zslot;
+    a.in_index = t.in_index;
     const long long items = (long long)t.B * t.m_pad;
     const unsigned grid = (unsigned)((items + cw - 1) / cw);
     if (grid == 0) return SQH_OK;

@@ -33,6 +33,7 @@ struct TileLaunch {
     double* mpart = nullptr;     // [B][wpb][ns][8] gradient correlations
     const Slot* slots = nullptr;
     BatchIndex bi;             // optional element -> run -> target indirection
+    const int* in_index = nullptr;  // forward: element b reads input in_index[b] (null: input b)
     const int* zslot = nullptr;  // forward store: per element, the controlled slot applied in
                                  // derivative mode (control-0 block zeroed), or -1
     int stage_ops = 0;         // > 0: gradient kernel stages the program in LDS (largest op count)

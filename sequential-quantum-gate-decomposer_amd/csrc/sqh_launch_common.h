@@ -41,7 +41,7 @@ hipError_t launch_forward_nmu(const TileLaunch& t, hipStream_t st, unsigned wave
     hipLaunchKernelGGL((k_tile_forward<N, R, MODE, UNI, WPG>), dim3(grid), dim3(64 * WPG), shm, st,
                        (const cpx<R>*)t.in, t.in_bstride, t.in_stride, (cpx<R>*)t.out, t.out_bstride, t.out_stride,
                        t.m, t.m_pad, t.B, t.ops, t.nops, t.lays, (const cpx<R>*)t.kern, t.kern_bstride,
-                       t.trace_off, t.diag, t.ns, t.ts, t.zslot);
+                       t.trace_off, t.diag, t.ns, t.ts, t.zslot, t.in_index);
     return hipGetLastError();
 }
 

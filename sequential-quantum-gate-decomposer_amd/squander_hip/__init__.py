@@ -10,8 +10,8 @@ from ._native import (COST_ABS_TRACE, COST_FROBENIUS, COST_HILBERT_SCHMIDT, COST
 from .circuit import Circuit
 from .decomposition import N_Qubit_Decomposition_custom
 from . import workloads
-from .optimizer import agents, agents_options, decompose_batch, decompose_problems, default_options, initial_guess, mt19937_uniform_2pi
+from .optimizer import agents, agents_batch, agents_options, decompose_batch, decompose_problems, default_options, initial_guess, mt19937_uniform_2pi
 
-__all__ = ["Circuit", "N_Qubit_Decomposition_custom", "Context", "SqhError", "inspect_program", "lib", "workloads", "agents", "agents_options", "decompose_batch", "decompose_problems", "default_options", "initial_guess",
+__all__ = ["Circuit", "N_Qubit_Decomposition_custom", "Context", "SqhError", "inspect_program", "lib", "workloads", "agents", "agents_batch", "agents_options", "decompose_batch", "decompose_problems", "default_options", "initial_guess",
            "mt19937_uniform_2pi",
            "COST_FROBENIUS", "COST_HILBERT_SCHMIDT", "COST_INFIDELITY", "COST_ABS_TRACE", "PREC_F64", "PREC_F32"]

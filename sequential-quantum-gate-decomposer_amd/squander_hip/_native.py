@@ -41,7 +41,7 @@ EXPORTS = [
     "sqh_inspect_program", "sqh_inspect_slots", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
-    "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents",
+    "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
@@ -147,6 +147,7 @@ def lib():
                                         vp, ctypes.POINTER(ll)], i),
             "sqh_agents": ([vp, vp, ctypes.POINTER(AgentsOptions), ctypes.c_uint, d, ctypes.POINTER(ll),
                             ctypes.POINTER(ll)], i),
+            "sqh_agents_batch": ([vp, vp, i, i, i, vp, ctypes.POINTER(AgentsOptions), vp, vp, vp, vp], i),
             "sqh_set_jit": ([vp, i], i),
             "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
             "sqh_mlp_backward_step_dev": ([vp, vp, vp, i, ctypes.c_float], i),

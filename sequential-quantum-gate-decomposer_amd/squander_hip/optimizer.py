@@ -130,9 +130,36 @@ def agents(ctx: nat.Context, x0, seed: int, options: nat.AgentsOptions | None = 
     """AGENTS on the context's circuit and target from the guess x0; returns
     (current_minimum, optimized parameters, iterations, evaluations)."""
     x = np.array(x0, dtype=np.float64, copy=True)
+    if x.shape != (ctx.parameter_num(),):
+        raise ValueError("x0 must hold parameter_num values")
     f = ctypes.c_double(0.0)
     it, ev = ctypes.c_longlong(0), ctypes.c_longlong(0)
     o = options if options is not None else agents_options()
     ctx.check(ctx._L.sqh_agents(ctx.h, x.ctypes.data, ctypes.byref(o), seed, ctypes.byref(f), ctypes.byref(it),
                                 ctypes.byref(ev)))
     return f.value, x, int(it.value), int(ev.value)
+
+
+def agents_batch(ctx: nat.Context, targets, x0s, seeds, options: nat.AgentsOptions | None = None):
+    """Independent AGENTS runs, run k against targets[k] from the guess x0s[k] with
+    seed seeds[k], advanced in lockstep with one launch per stage; returns
+    (current_minimum[K], parameters[K, P], iterations[K], evaluations[K]).  Each run
+    equals agents() on a context holding that target."""
+    T = np.ascontiguousarray(targets, dtype=np.complex128)
+    if T.ndim != 3:
+        raise ValueError("targets must be K x rows x cols")
+    K, rows, cols = T.shape
+    x = np.array(x0s, dtype=np.float64, copy=True)
+    if x.ndim != 2 or x.shape != (K, ctx.parameter_num()):
+        raise ValueError("x0s must be K x parameter_num")
+    x = np.ascontiguousarray(x)
+    sd = np.ascontiguousarray(seeds, dtype=np.uint32)
+    if sd.shape != (K,):
+        raise ValueError("seeds must hold one seed per target")
+    f = np.empty(K)
+    it = np.empty(K, dtype=np.int64)
+    ev = np.empty(K, dtype=np.int64)
+    o = options if options is not None else agents_options()
+    ctx.check(ctx._L.sqh_agents_batch(ctx.h, T.ctypes.data, K, rows, cols, x.ctypes.data, ctypes.byref(o),
+                                      sd.ctypes.data, f.ctypes.data, it.ctypes.data, ev.ctypes.data))
+    return f, x, it, ev

@@ -180,6 +180,54 @@ def test_agents_matches_oracle(oracle, n, layers, seed):
     assert np.max(np.abs(d)) < 1e-6
 
 
+@pytest.mark.parametrize("n,layers,jit", [(3, 4, False), (4, 4, True), (8, 2, False)])
+def test_agents_batch_equals_single_runs(n, layers, jit):
+    """Runs against several targets advanced in lockstep (one launch per stage) give
+    exactly the results of one sqh_agents call per target: the same parameters,
+    minimum, iteration and evaluation counts.  Also with a trace offset."""
+    from squander_hip import agents, agents_batch, agents_options
+    gates = W.ansatz_layers(n, W.alternating_pairs(n, layers))
+    P = W.param_count(gates)
+    K = 5
+    d = 1 << n
+    off = 1 if n == 4 else 0
+    m = d - off
+    say(f"AGENTS batch n={n} layers={layers} jit={jit} K={K} offset={off}")
+    Ts = np.array([T[:, :m] for T in reachable_targets(n, gates, K, 700 + n)])
+    x0 = np.random.default_rng(n + 900).uniform(0, 2 * np.pi, (K, P))
+    seeds = np.array([11 + 7 * k for k in range(K)], dtype=np.uint32)
+    # the runs stop at different iterations (lifetime 7 exercises the period steps)
+    opts = agents_options(agent_num=16, max_iterations=40 + 15 * n, optimization_tolerance=1e-8, agent_lifetime=7,
+                          agent_randomization_rate=0.6)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_trace_offset(off)
+    ctx.set_jit(jit)
+    fb, xb, itb, evb = agents_batch(ctx, Ts, x0, seeds, opts)
+    for k in range(K):
+        ctx.set_target(Ts[k])
+        fs, xs, its, evs = agents(ctx, x0[k], int(seeds[k]), opts)
+        say(f"  run {k}: f={fs:.3e} it={its} evals={evs}")
+        assert (its, evs) == (itb[k], evb[k])
+        assert fs == fb[k]
+        assert np.array_equal(xs, xb[k])
+
+
+def test_agents_batch_rejects_bad_shapes():
+    from squander_hip import agents_batch
+    gates = W.ansatz_layers(3, [0, 1])
+    P = W.param_count(gates)
+    ctx = nat.Context(0)
+    ctx.set_circuit(3, gates)
+    with pytest.raises(nat.SqhError):  # 4 rows for a 3-qubit circuit
+        agents_batch(ctx, np.zeros((2, 4, 4), complex), np.zeros((2, P)), [1, 2])
+    ctx.set_trace_offset(2)
+    with pytest.raises(nat.SqhError):  # 7 columns + offset 2 > 8 rows
+        agents_batch(ctx, np.zeros((2, 8, 7), complex), np.zeros((2, P)), [1, 2])
+    with pytest.raises(ValueError):
+        agents_batch(ctx, np.zeros((2, 8, 6), complex), np.zeros((2, P + 1)), [1, 2])
+
+
 def test_agents_rejects_other_costs():
     from squander_hip import agents
     gates = W.ansatz_layers(2, [0])
