@@ -559,10 +559,11 @@ def main():
                   "frac_of_unpacked_peak": flops_eval * B / k32_s / 1e12 / FP64_PEAK_TFLOPS}
     log(f"c64: {out['c64']['value']:.0f} evals/s, err {err32:.2e}")
 
-    def apply_err(fn):
+    def apply_err(fn, f32=False):
         """Max relative Frobenius error of two matrices applied once against the oracle."""
-        m2 = torch.from_numpy(np.broadcast_to(U, (2, d, d)).copy()).to(dev)
-        ctx.check(fn(ctx.h, params.data_ptr(), P, m2.data_ptr(), d, d, d, d * d, 2, 0))
+        m2 = torch.from_numpy(np.array(np.broadcast_to(U, (2, d, d)), dtype=np.complex64 if f32 else np.complex128,
+                                       order="C")).to(dev)
+        ctx.check(fn(ctx.h, params.data_ptr(), P, m2.data_ptr(), d, d, d, d * d, 2, int(f32)))
         ctx.synchronize()
         got = m2.cpu().numpy()
         err = 0.0
@@ -617,9 +618,34 @@ def main():
                            "achieved_GBs": blk_bytes * fnl / (fkms / 1e3) / 1e9,
                            "algorithmic_TFLOPs": flops_eval * B * reps / (fkms / 1e3) / 1e12,
                            "err_rel_fro_vs_fp64_oracle": apply_err(L.sqh_apply_to_fused_dev),
-                           "kernel": "k_block_mfma4 (v_mfma_f64_16x16x4f64)"}
+                           "kernel": "k_block_mfma4<double> (v_mfma_f64_16x16x4f64)"}
     log(f"fused stream: {out['fused_stream']['evals_per_s']:.0f} evals/s")
     del mats
+
+    # the same in complex64: 16x16 blocks on v_mfma_f32_16x16x4f32, FP32 accumulation
+    mats32 = torch.from_numpy(np.array(np.broadcast_to(U, (B, d, d)), dtype=np.complex64, order="C")).to(dev)
+    assert mats32.is_contiguous() and mats32.stride(0) == d * d
+
+    def fused_step32():
+        ctx.check(L.sqh_apply_to_fused_dev(ctx.h, params.data_ptr(), P, mats32.data_ptr(), d, d, d, d * d, B, 1))
+
+    fused_step32()
+    ctx.synchronize()
+    ctx.set_kernel_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fused_step32()
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    fkms32, fnl32 = ctx.kernel_timing()
+    ctx.set_kernel_timing(False)
+    out["fused_stream_c64"] = {"evals_per_s": B * reps / (t1 - t0), "blocks_per_pass": fnl32 // reps,
+                               "achieved_GBs": blk_bytes / 2 * fnl32 / (fkms32 / 1e3) / 1e9,
+                               "err_rel_fro_vs_fp64_oracle": apply_err(L.sqh_apply_to_fused_dev, f32=True),
+                               "kernel": "k_block_mfma4<float> (v_mfma_f32_16x16x4f32)"}
+    log(f"fused stream c64: {out['fused_stream_c64']['evals_per_s']:.0f} evals/s, "
+        f"{out['fused_stream_c64']['achieved_GBs']:.0f} GB/s, err {out['fused_stream_c64']['err_rel_fro_vs_fp64_oracle']:.1e}")
+    del mats32
 
     # single-vector cost + gradient latency (8 qubits, 264 angles)
     f0 = torch.empty(1, dtype=torch.float64, device=dev)

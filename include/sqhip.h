@@ -117,7 +117,8 @@ int sqh_apply_to_stream_dev(sqh_ctx* ctx, const double* d_params, int pbstride, 
                             int stride, long long bstride, int B, int is_f32);
 
 /* The same result with runs of gates fused into dense 16x16 blocks on <= 4 qubits
- * (one HBM pass per block, FP64 blocks on the matrix cores); needs >= 4 qubits. */
+ * (one HBM pass per block on the matrix cores: FP64 MFMA for complex128, FP32 MFMA
+ * with FP32 accumulation for complex64); needs >= 4 qubits. */
 int sqh_apply_to_fused_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
                            int stride, long long bstride, int B, int is_f32);
 /* apply_large_kernel_to_input (squander/src-cpp/gates/kernels/apply_large_kernel_to_input.cpp:82-102):

@@ -3,10 +3,11 @@
 // index bit i <-> the i-th smallest involved qubit), and the fusion of gate runs into
 // such blocks.
 //
-// k = 4 in FP64 runs on the matrix cores: a wavefront computes a 16 x 16 output tile
-// Y = F X for 16 columns with 16 v_mfma_f64_16x16x4 (four real products, K = 16 in
-// steps of 4); F stays in registers for the whole wavefront.  Other k (and complex64)
-// use one thread per (group, column) with F in LDS.
+// k = 4 runs on the matrix cores: a wavefront computes a 16 x 16 output tile Y = F X
+// for 16 columns with 16 MFMAs (four real products, K = 16 in steps of 4):
+// v_mfma_f64_16x16x4f64 for complex128, v_mfma_f32_16x16x4f32 (FP32 accumulation) for
+// complex64; F stays in registers for the whole wavefront.  Other k use one thread per
+// (group, column) with F in LDS.
 #include <hip/hip_runtime.h>
 
 #include "sqh_device.h"
@@ -15,6 +16,28 @@
 namespace sqh {
 
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// 16x16x4 MFMA of one precision.  Operands: lane l holds A[l % 16][l / 16] and
+// B[l / 16][l % 16]; result item r of lane l is D[row(r, l)][l % 16].
+template <typename R>
+struct Mfma16;
+template <>
+struct Mfma16<double> {
+    typedef f64x4 V;
+    static __device__ __forceinline__ V op(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int lane) { return 4 * r + (lane >> 4); }
+};
+template <>
+struct Mfma16<float> {
+    typedef f32x4 V;
+    static __device__ __forceinline__ V op(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int lane) { return 4 * (lane >> 4) + r; }
+};
 
 struct BlockQubits {
     int q[5];   // sorted ascending
@@ -74,10 +97,12 @@ __global__ __launch_bounds__(256) void k_block_fma(cpx<R>* __restrict__ mats, lo
     }
 }
 
-// k = 4, FP64, matrix cores.  Wavefront -> (matrix b, group g, 16-column tile).
-__global__ __launch_bounds__(256) void k_block_mfma4(cpx<double>* __restrict__ mats, long long bstride, int stride,
-                                                     int rows, int m, int B, const cpx<double>* __restrict__ F,
+// k = 4 on the matrix cores.  Wavefront -> (matrix b, group g, 16-column tile).
+template <typename R>
+__global__ __launch_bounds__(256) void k_block_mfma4(cpx<R>* __restrict__ mats, long long bstride, int stride,
+                                                     int rows, int m, int B, const cpx<R>* __restrict__ F,
                                                      long long fbstride, BlockQubits bq) {
+    typedef Mfma16<R> MF;
     const int lane = threadIdx.x & 63;
     const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long long groups = rows >> 4;
@@ -88,43 +113,42 @@ __global__ __launch_bounds__(256) void k_block_mfma4(cpx<double>* __restrict__ m
     const long long rem = wave % per_b;
     const long long g = rem / tiles;
     const int c0 = (int)(rem % tiles) * 16;
-    const cpx<double>* Fb = F + (long long)b * fbstride;
+    const cpx<R>* Fb = F + (long long)b * fbstride;
     // A operand of step s: A[i][k] = F[lane%16][4s + lane/16]
-    double ar[4], ai[4], ain[4];
+    R ar[4], ai[4], ain[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        const cpx<double> f = Fb[(lane & 15) * 16 + 4 * s + (lane >> 4)];
+        const cpx<R> f = Fb[(lane & 15) * 16 + 4 * s + (lane >> 4)];
         ar[s] = f.re;
         ai[s] = f.im;
         ain[s] = -f.im;
     }
-    cpx<double>* M = mats + (long long)b * bstride;
+    cpx<R>* M = mats + (long long)b * bstride;
     const long long base = block_base(g, bq);
     const int col = c0 + (lane & 15);
     const bool cok = col < m;
     // B operand of step s: B[k][n] = X[4s + lane/16][c0 + lane%16]
-    double xr[4], xi[4];
+    R xr[4], xi[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         const int i = 4 * s + (lane >> 4);
-        const cpx<double> v = cok ? M[(base + block_off(i, bq)) * stride + col] : mk(0.0, 0.0);
+        const cpx<R> v = cok ? M[(base + block_off(i, bq)) * stride + col] : mk<R>(0, 0);
         xr[s] = v.re;
         xi[s] = v.im;
     }
-    f64x4 yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
+    typename MF::V yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s], xr[s], yr, 0, 0, 0);
-        yr = __builtin_amdgcn_mfma_f64_16x16x4f64(ain[s], xi[s], yr, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[s], xi[s], yi, 0, 0, 0);
-        yi = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[s], xr[s], yi, 0, 0, 0);
+        yr = MF::op(ar[s], xr[s], yr);
+        yr = MF::op(ain[s], xi[s], yr);
+        yi = MF::op(ar[s], xi[s], yi);
+        yi = MF::op(ai[s], xr[s], yi);
     }
-    // D[i][n]: i = 4*r + lane/16, n = lane % 16 (v_mfma_f64_16x16x4f64 output layout)
     if (cok) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = 4 * r + (lane >> 4);
-            M[(base + block_off(i, bq)) * stride + col] = mk(yr[r], yi[r]);
+            const int i = MF::row(r, lane);
+            M[(base + block_off(i, bq)) * stride + col] = mk<R>(yr[r], yi[r]);
         }
     }
 }
@@ -198,17 +222,21 @@ static hipError_t launch_fma(void* mats, long long bstride, int stride, int rows
 }
 
 // Dense block on the k sorted qubits of every matrix.  F holds one block, or one per
-// matrix (fbstride = 4^k).  use_mfma selects the matrix-core kernel for k = 4 FP64.
+// matrix (fbstride = 4^k).  use_mfma selects the matrix-core kernel for k = 4.
 hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,
                               int B, const void* F, long long fbstride, const int* qubits, int k, hipStream_t s) {
     if (k < 1 || k > 5 || (1 << k) > rows) return hipErrorInvalidValue;
     BlockQubits bq;
     bq.k = k;
     for (int j = 0; j < 5; ++j) bq.q[j] = j < k ? qubits[j] : 0;
-    if (!f32 && use_mfma && k == 4) {
+    if (use_mfma && k == 4) {
         const long long waves = (long long)(rows >> 4) * ((m + 15) / 16) * B;
-        hipLaunchKernelGGL(k_block_mfma4, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, (cpx<double>*)mats,
-                           bstride, stride, rows, m, B, (const cpx<double>*)F, fbstride, bq);
+        if (f32)
+            hipLaunchKernelGGL(k_block_mfma4<float>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
+                               (cpx<float>*)mats, bstride, stride, rows, m, B, (const cpx<float>*)F, fbstride, bq);
+        else
+            hipLaunchKernelGGL(k_block_mfma4<double>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
+                               (cpx<double>*)mats, bstride, stride, rows, m, B, (const cpx<double>*)F, fbstride, bq);
         return hipGetLastError();
     }
     // The FMA kernel stages one block per workgroup: per-matrix blocks need each
