@@ -31,7 +31,10 @@ Layout make_layout(int N, int K, const std::vector<int>& pos) {
             if ((i >> j) & 1) off |= 1 << pos[j];
         l.reg_off[i] = off;
     }
-    for (int j = 0; j < N - K; ++j) l.sub_pos[j] = pos[K + j];
+    // Lane-bit positions exist for the tile kernels' range (N - K <= 8 entries); above
+    // it the layouts are never read (the streaming paths apply the gates by slot).
+    const int nsub = std::min(N - K, (int)(sizeof(l.sub_pos) / sizeof(l.sub_pos[0])));
+    for (int j = 0; j < nsub; ++j) l.sub_pos[j] = pos[K + j];
     return l;
 }
 

@@ -52,7 +52,7 @@ def test_apply_to_matches_oracle(oracle, n):
     assert rel_fro(Z32, ref) < 1e-5
 
 
-@pytest.mark.parametrize("n", [11, 12])
+@pytest.mark.parametrize("n", [11, 12, 13, 14, 16])
 def test_apply_to_streaming_beyond_tile(oracle, n):
     say(f"apply_to streaming n={n}")
     gates = W.mixed_gate_circuit(n, 24, seed=600 + n)
@@ -232,7 +232,7 @@ def test_large_kernel_matches_dense_reference(n, k):
         assert err < (1e-5 if f32 else 1e-12), (f32, err)
 
 
-@pytest.mark.parametrize("n", [4, 6, 8, 10, 12])
+@pytest.mark.parametrize("n", [4, 6, 8, 10, 12, 14])
 def test_fused_streaming_matches_oracle(oracle, n):
     import torch
     from squander_hip import Context
@@ -260,6 +260,44 @@ def test_fused_streaming_matches_oracle(oracle, n):
                 ref = oracle.apply_to(gates, ps[b] if per_b else ps[0], X[b])
                 err = rel_fro(got[b], ref)
                 assert err < tol, (f32, per_b, b, err)
+
+
+def test_block_launch_shapes_give_identical_results(oracle):
+    """Every launch shape of the 4-qubit matrix-core block kernel (tiles per wavefront x
+    threads per workgroup) computes the same bits, including a column count that is not a
+    multiple of the tile group; the tuner picks one of them."""
+    import torch
+    from squander_hip import Context
+    n, m, B = 7, 75, 3
+    d = 1 << n
+    gates = W.mixed_gate_circuit(n, 40, seed=1500)
+    P = W.param_count(gates)
+    ps = W.random_params(P, 15, batch=B)
+    X = np.random.default_rng(15).standard_normal((B, d, m)) + 1j * np.random.default_rng(16).standard_normal((B, d, m))
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    pd = torch.from_numpy(ps).cuda()
+    for f32 in (False, True):
+        say(f"block launch shapes f32={f32}")
+        dt = np.complex64 if f32 else np.complex128
+        outs = {}
+        for cfg in ((1, 256), (2, 256), (4, 256), (1, 64), (2, 64), (4, 64), (0, 0)):
+            ctx.set_block_config(*cfg)
+            Xd = torch.from_numpy(X.astype(dt)).cuda()
+            torch.cuda.synchronize()
+            ctx.check(ctx._L.sqh_apply_to_fused_dev(ctx.h, pd.data_ptr(), P, Xd.data_ptr(), d, m, m, d * m, B,
+                                                    int(f32)))
+            ctx.synchronize()
+            outs[cfg] = Xd.cpu().numpy()
+        for cfg, got in outs.items():
+            assert np.array_equal(got, outs[(1, 256)]), cfg
+        for b in range(B):
+            assert rel_fro(outs[(1, 256)][b], oracle.apply_to(gates, ps[b], X[b])) < (2e-5 if f32 else 1e-12)
+        tiles, threads, ms = ctx.block_config(d, m, B, f32)
+        say(f"  tuned: {tiles} tiles x {threads} threads ({ms:.1f} ms of tuning so far)")
+        assert tiles in (1, 2, 4) and threads in (64, 256)
+    with pytest.raises(Exception):
+        ctx.set_block_config(3, 256)
 
 
 def test_streaming_float32_faster_than_float64():

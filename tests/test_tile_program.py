@@ -102,3 +102,17 @@ def test_adjoint_gradient_matches_oracle(oracle, n, off, variant):
     f_ref, g_ref = oracle.optimization_problem_combined(gates, U, n, p, variant, off)
     assert abs(orc.brute_cost(oracle.apply_to(gates, p, U), variant, off) - f_ref) < 1e-13
     assert np.abs(grad - g_ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("n", [11, 12, 13, 16, 20, 24])
+def test_compile_beyond_the_tile_range(n):
+    """Circuits above the register tile (streaming paths) compile: one slot per
+    parametric gate in gate order, one op per gate plus the remaps.  (A layout holds lane
+    positions for the tile range only; 13 qubits and up once wrote past it.)"""
+    gates = W.mixed_gate_circuit(n, 6 * n, seed=1700 + n)
+    ops, lays = te.decode(*nat.inspect_program(n, gates))
+    nremap = sum(1 for o in ops if o[0] == te.OP_REMAP)
+    assert len(ops) - nremap == len(gates)
+    slots = nat.inspect_slots(n, gates)
+    par = [int(g[3]) for g in gates if nat.PARAM_NUM[int(g[0])]]
+    assert [int(ps) for _, ps, _, _ in slots] == par
