@@ -128,6 +128,20 @@ int sqh_apply_to_fused_dev(sqh_ctx* ctx, const double* d_params, int pbstride, v
 int sqh_apply_large_kernel_dev(sqh_ctx* ctx, const void* d_block, const int* qubits, int k, void* d_inout, int rows,
                                int cols, int stride, long long bstride, int B, int is_f32);
 
+/* Launch shape of the 4-qubit matrix-core block kernel (sqh_apply_to_fused_dev,
+ * sqh_apply_large_kernel_dev with k = 4, the streaming cost path): sixteen-column tiles
+ * per wavefront (1, 2, 4) and threads per workgroup (64, 256).  With tuning on (the
+ * default) the first call of each (precision, rows, cols, B) shape times every
+ * candidate on a scratch batch of that shape and keeps the fastest for the context;
+ * sqh_set_block_config forces one shape (0, 0 returns to automatic);
+ * sqh_get_block_config reports the shape used for a given call shape (tuning it if
+ * needed) and the tuning time spent so far.  The shape never changes results.  No
+ * counterpart in the reference (its AVX kernels have fixed blocking). */
+int sqh_set_autotune(sqh_ctx* ctx, int enable);
+int sqh_set_block_config(sqh_ctx* ctx, int tiles_per_wave, int threads);
+int sqh_get_block_config(sqh_ctx* ctx, int is_f32, int rows, int cols, int B, int* tiles_per_wave, int* threads,
+                         double* tune_ms);
+
 /* ---- Gates_block::apply_derivate_to / apply_to_combined ---------------------- */
 /* out receives P dense rows x cols complex128 matrices: out[k] = (dC/dp_k) . in. */
 int sqh_apply_derivate_to(sqh_ctx* ctx, const double* params, const double* in, int rows, int cols, int stride,

@@ -130,6 +130,9 @@ int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lon
     HIPTRY(c, c->d_fused.ensure((size_t)Bk * nb * 256 * esz));
     HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, c->d_kern.p, kern_bstride, Bk,
                                  c->d_fused.p, c->stream));
+    BlockCfg cfg;
+    int rc = block_config(c, f32, rows, cols, B, &cfg);
+    if (rc) return rc;
     for (int blk = 0; blk < nb; ++blk) {
         const char* F = static_cast<const char*>(c->d_fused.p) + (size_t)blk * 256 * esz;
         const int D = 1 << fp.k;
@@ -138,7 +141,7 @@ int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lon
         (void)D;
         TimedLaunch tl(c);
         HIPTRY(c, launch_block_apply(f32, true, mats, bstride, stride, rows, cols, B, F,
-                                     Bk > 1 ? (long long)nb * 256 : 0, fp.qubits[blk].data(), fp.k, c->stream));
+                                     Bk > 1 ? (long long)nb * 256 : 0, fp.qubits[blk].data(), fp.k, cfg, c->stream));
     }
     return SQH_OK;
 }
@@ -543,8 +546,13 @@ int sqh_apply_large_kernel_dev(sqh_ctx* c, const void* d_block, const int* qubit
         if (qubits[j] < 0 || qubits[j] >= n || (j > 0 && qubits[j] <= qubits[j - 1]))
             return fail(c, SQH_ERR_ARG, "involved qubits must be distinct, sorted and below %d", n);
     HIPTRY(c, hipSetDevice(c->device));
+    BlockCfg cfg;
+    if (k == 4) {
+        int rc = block_config(c, is_f32 != 0, rows, cols, B, &cfg);
+        if (rc) return rc;
+    }
     HIPTRY(c, launch_block_apply(is_f32 != 0, true, d_inout, bstride, stride, rows, cols, B, d_block, 0, qubits, k,
-                                 c->stream));
+                                 cfg, c->stream));
     return SQH_OK;
 }
 

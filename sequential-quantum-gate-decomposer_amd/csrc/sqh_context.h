@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -85,6 +87,13 @@ struct sqh_ctx {
     bool jit = false;
     sqh::JitCache* jit_cache = nullptr;
 
+    // Launch shape of the 4-qubit matrix-core block kernel: forced (sqh_set_block_config)
+    // or picked per (precision, rows, columns, batch) by the run-time tuner.
+    bool autotune = true;
+    sqh::BlockCfg block_force;
+    std::map<std::array<long long, 4>, sqh::BlockCfg> block_tuned;
+    double tune_ms = 0.0;
+
     // Kernel timers: HIP events around the dominant kernel of each call.
     bool timing = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -151,7 +160,12 @@ int jit_prepare(sqh_ctx* c, const TileLaunch& t, int kind);
 double jit_compile_ms(sqh_ctx* c, int* compiles);
 
 hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,
-                              int B, const void* F, long long fbstride, const int* qubits, int k, hipStream_t s);
+                              int B, const void* F, long long fbstride, const int* qubits, int k, BlockCfg cfg,
+                              hipStream_t s);
+// Launch shape of the 4-qubit block kernel for B matrices of rows x m: the forced one,
+// the tuned one for this shape, or (tuning on, not under graph capture) tuned now by
+// timing every candidate on a scratch batch of the same shape (sqh_tune.cpp).
+int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out);
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
                               long long kern_bstride, int Bk, void* out, hipStream_t s);
 size_t fused_gate_bytes();

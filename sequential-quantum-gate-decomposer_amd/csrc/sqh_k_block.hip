@@ -97,22 +97,24 @@ __global__ __launch_bounds__(256) void k_block_fma(cpx<R>* __restrict__ mats, lo
     }
 }
 
-// k = 4 on the matrix cores.  Wavefront -> (matrix b, group g, 16-column tile).
-template <typename R>
+// k = 4 on the matrix cores.  Wavefront -> (matrix b, group g, T consecutive 16-column
+// tiles); the wavefront loads all its tiles before the first MFMA (the stores of one
+// tile cannot be reordered before the loads of the next) and reuses F for each.
+template <typename R, int T>
 __global__ __launch_bounds__(256) void k_block_mfma4(cpx<R>* __restrict__ mats, long long bstride, int stride,
                                                      int rows, int m, int B, const cpx<R>* __restrict__ F,
                                                      long long fbstride, BlockQubits bq) {
     typedef Mfma16<R> MF;
     const int lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const long long groups = rows >> 4;
-    const long long tiles = (m + 15) / 16;
-    const long long per_b = groups * tiles;
+    const long long tgroups = ((m + 15) / 16 + T - 1) / T;
+    const long long per_b = groups * tgroups;
     if (wave >= per_b * B) return;
     const int b = (int)(wave / per_b);
     const long long rem = wave % per_b;
-    const long long g = rem / tiles;
-    const int c0 = (int)(rem % tiles) * 16;
+    const long long g = rem / tgroups;
+    const int c0 = (int)(rem % tgroups) * 16 * T;
     const cpx<R>* Fb = F + (long long)b * fbstride;
     // A operand of step s: A[i][k] = F[lane%16][4s + lane/16]
     R ar[4], ai[4], ain[4];
@@ -125,30 +127,39 @@ __global__ __launch_bounds__(256) void k_block_mfma4(cpx<R>* __restrict__ mats, 
     }
     cpx<R>* M = mats + (long long)b * bstride;
     const long long base = block_base(g, bq);
-    const int col = c0 + (lane & 15);
-    const bool cok = col < m;
-    // B operand of step s: B[k][n] = X[4s + lane/16][c0 + lane%16]
-    R xr[4], xi[4];
+    long long roff[4];  // rows of the B operand: X[4s + lane/16][.]
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int i = 4 * s + (lane >> 4);
-        const cpx<R> v = cok ? M[(base + block_off(i, bq)) * stride + col] : mk<R>(0, 0);
-        xr[s] = v.re;
-        xi[s] = v.im;
+    for (int s = 0; s < 4; ++s) roff[s] = (base + block_off(4 * s + (lane >> 4), bq)) * stride;
+    // B operand of step s for tile t: B[k][n] = X[4s + lane/16][c0 + 16t + lane%16]
+    R xr[T][4], xi[T][4];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int col = c0 + 16 * t + (lane & 15);
+        const bool cok = col < m;
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> v = cok ? M[roff[s] + col] : mk<R>(0, 0);
+            xr[t][s] = v.re;
+            xi[t][s] = v.im;
+        }
     }
-    typename MF::V yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        yr = MF::op(ar[s], xr[s], yr);
-        yr = MF::op(ain[s], xi[s], yr);
-        yi = MF::op(ar[s], xi[s], yi);
-        yi = MF::op(ai[s], xr[s], yi);
-    }
-    if (cok) {
+    for (int t = 0; t < T; ++t) {
+        typename MF::V yr = {0, 0, 0, 0}, yi = {0, 0, 0, 0};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            const int i = MF::row(r, lane);
-            M[(base + block_off(i, bq)) * stride + col] = mk<R>(yr[r], yi[r]);
+        for (int s = 0; s < 4; ++s) {
+            yr = MF::op(ar[s], xr[t][s], yr);
+            yr = MF::op(ain[s], xi[t][s], yr);
+            yi = MF::op(ar[s], xi[t][s], yi);
+            yi = MF::op(ai[s], xr[t][s], yi);
+        }
+        const int col = c0 + 16 * t + (lane & 15);
+        if (col < m) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = MF::row(r, lane);
+                M[(base + block_off(i, bq)) * stride + col] = mk<R>(yr[r], yi[r]);
+            }
         }
     }
 }
@@ -223,21 +234,42 @@ static hipError_t launch_fma(void* mats, long long bstride, int stride, int rows
 
 // Dense block on the k sorted qubits of every matrix.  F holds one block, or one per
 // matrix (fbstride = 4^k).  use_mfma selects the matrix-core kernel for k = 4.
+template <typename R, int T>
+static hipError_t launch_mfma4_t(void* mats, long long bstride, int stride, int rows, int m, int B, const void* F,
+                                 long long fbstride, const BlockQubits& bq, int threads, hipStream_t s) {
+    const long long waves = (long long)(rows >> 4) * (((m + 15) / 16 + T - 1) / T) * B;
+    const int wpb = threads / 64;
+    hipLaunchKernelGGL((k_block_mfma4<R, T>), dim3((unsigned)((waves + wpb - 1) / wpb)), dim3(threads), 0, s,
+                       (cpx<R>*)mats, bstride, stride, rows, m, B, (const cpx<R>*)F, fbstride, bq);
+    return hipGetLastError();
+}
+
+template <typename R>
+static hipError_t launch_mfma4(void* mats, long long bstride, int stride, int rows, int m, int B, const void* F,
+                               long long fbstride, const BlockQubits& bq, BlockCfg cfg, hipStream_t s) {
+    switch (cfg.tiles) {
+    case 1: return launch_mfma4_t<R, 1>(mats, bstride, stride, rows, m, B, F, fbstride, bq, cfg.threads, s);
+    case 2: return launch_mfma4_t<R, 2>(mats, bstride, stride, rows, m, B, F, fbstride, bq, cfg.threads, s);
+    case 4: return launch_mfma4_t<R, 4>(mats, bstride, stride, rows, m, B, F, fbstride, bq, cfg.threads, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool block_cfg_valid(BlockCfg cfg) {
+    return (cfg.tiles == 1 || cfg.tiles == 2 || cfg.tiles == 4) && (cfg.threads == 64 || cfg.threads == 256);
+}
+
 hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bstride, int stride, int rows, int m,
-                              int B, const void* F, long long fbstride, const int* qubits, int k, hipStream_t s) {
+                              int B, const void* F, long long fbstride, const int* qubits, int k, BlockCfg cfg,
+                              hipStream_t s) {
     if (k < 1 || k > 5 || (1 << k) > rows) return hipErrorInvalidValue;
     BlockQubits bq;
     bq.k = k;
     for (int j = 0; j < 5; ++j) bq.q[j] = j < k ? qubits[j] : 0;
     if (use_mfma && k == 4) {
-        const long long waves = (long long)(rows >> 4) * ((m + 15) / 16) * B;
-        if (f32)
-            hipLaunchKernelGGL(k_block_mfma4<float>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
-                               (cpx<float>*)mats, bstride, stride, rows, m, B, (const cpx<float>*)F, fbstride, bq);
-        else
-            hipLaunchKernelGGL(k_block_mfma4<double>, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s,
-                               (cpx<double>*)mats, bstride, stride, rows, m, B, (const cpx<double>*)F, fbstride, bq);
-        return hipGetLastError();
+        if (!block_cfg_valid(cfg)) return hipErrorInvalidValue;
+        return f32 ? launch_mfma4<float>(mats, bstride, stride, rows, m, B, F, fbstride, bq, cfg, s)
+                   : launch_mfma4<double>(mats, bstride, stride, rows, m, B, F, fbstride, bq, cfg, s);
     }
     // The FMA kernel stages one block per workgroup: per-matrix blocks need each
     // workgroup inside one matrix.

@@ -6,6 +6,13 @@
 
 namespace sqh {
 
+// Launch shape of the 4-qubit matrix-core block kernel: 16-column tiles per wavefront
+// (1, 2 or 4) and threads per workgroup (64 or 256).
+struct BlockCfg {
+    int tiles = 0, threads = 0;
+};
+bool block_cfg_valid(BlockCfg cfg);
+
 // Arguments of one register-tile launch.  Matrices are interleaved complex
 // (complex128, or complex64 when f32), row-major with a row stride in elements.
 struct TileLaunch {

@@ -42,6 +42,7 @@ EXPORTS = [
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
+    "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
@@ -148,6 +149,10 @@ def lib():
             "sqh_agents": ([vp, vp, ctypes.POINTER(AgentsOptions), ctypes.c_uint, d, ctypes.POINTER(ll),
                             ctypes.POINTER(ll)], i),
             "sqh_agents_batch": ([vp, vp, i, i, i, vp, ctypes.POINTER(AgentsOptions), vp, vp, vp, vp], i),
+            "sqh_set_autotune": ([vp, i], i),
+            "sqh_set_block_config": ([vp, i, i], i),
+            "sqh_get_block_config": ([vp, i, i, i, i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                      d], i),
             "sqh_set_jit": ([vp, i], i),
             "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
             "sqh_mlp_backward_step_dev": ([vp, vp, vp, i, ctypes.c_float], i),
@@ -272,6 +277,20 @@ class Context:
         ms, n = ctypes.c_double(0.0), ctypes.c_int(0)
         self.check(self._L.sqh_get_jit_stats(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def set_autotune(self, enable=True):
+        self.check(self._L.sqh_set_autotune(self.h, int(bool(enable))))
+
+    def set_block_config(self, tiles_per_wave=0, threads=0):
+        """Force the 4-qubit block kernel's launch shape; (0, 0) returns to automatic."""
+        self.check(self._L.sqh_set_block_config(self.h, int(tiles_per_wave), int(threads)))
+
+    def block_config(self, rows, cols, batch, f32=False):
+        """(tiles per wavefront, threads per workgroup, tuning ms so far) for a call shape."""
+        t, th, ms = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_double(0.0)
+        self.check(self._L.sqh_get_block_config(self.h, int(bool(f32)), int(rows), int(cols), int(batch),
+                                                ctypes.byref(t), ctypes.byref(th), ctypes.byref(ms)))
+        return t.value, th.value, ms.value
 
     def set_kernel_timing(self, enable=True):
         self.check(self._L.sqh_set_kernel_timing(self.h, 1 if enable else 0))
