@@ -74,6 +74,35 @@ def test_start_decomposition_matches_oracle_driver(oracle, seed):
     assert (fo < 1e-10) == (err < 1e-10) or abs(err - fo) < 1e-10
 
 
+def test_decompose_ibm_challenge_fixture(oracle):
+    """The reference's data fixture (4th IBM Challenge unitary, data/Umtx.mat, 4 qubits,
+    tests/decomposition/test_IBM.py:47-110) decomposed on the GPU through the Python
+    mirror with the phase-invariant HS cost: the run from config random_seed 1 reaches the
+    oracle driver's result from the same seed, and the phase-corrected error the
+    reference test computes stays below its 1e-3 bar."""
+    import os
+    say("IBM challenge fixture, 4 qubits, HS cost")
+    U = np.load(os.path.join(os.path.dirname(__file__), "golden", "ibm_challenge_umtx.npy"))
+    gates = W.ansatz_layers(4, [0, 1, 2] * 4)
+    dec = N_Qubit_Decomposition_custom(U.conj().T, config={"random_seed": 1, "max_outer_iterations": 8,
+                                                           "optimization_tolerance": 1e-8})
+    dec.set_Cost_Function_Variant(nat.COST_HILBERT_SCHMIDT)
+    dec.set_Gate_Structure(Circuit.from_gate_array(4, gates))
+    dec.Start_Decomposition()
+    x = dec.get_Optimized_Parameters()
+    f = dec.get_Decomposition_Error()
+    fo, xo, _ = oracle.decompose_custom(gates, U.conj().T, 4, seed=1, max_outer_iterations=8,
+                                        optimization_tolerance=1e-8, variant=orc.COST_HS)
+    say(f"  gpu cost {f:.3e}, oracle {fo:.3e}")
+    assert abs(f - oracle.optimization_problem(gates, U.conj().T, 4, x, variant=orc.COST_HS)) < 1e-12
+    assert abs(f) < 1e-7 and abs(fo) < 1e-7
+    C = oracle.apply_to(gates, x, np.eye(16))
+    prod = U @ C.conj().T
+    prod = prod * np.exp(-1j * np.angle(prod[0, 0]))
+    err = np.real(np.trace(np.eye(16) * 2 - prod - prod.conj().T)) / 2
+    assert err < 1e-3
+
+
 def test_mixed_precision_batch_4q(oracle):
     say("mixed-precision batch 4q")
     n = 4
