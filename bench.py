@@ -361,19 +361,24 @@ def solves_leg():
     return res
 
 
-def c1_leg(o, seconds=10.0):
+def c1_leg(o, layers=12, seconds=6.0):
     """The plain decomposition of 3-qubit targets on the host cores (the oracle's
-    restated BFGS driver, one target per thread) beside the GPU's batched driver."""
+    restated BFGS driver, one target per thread) beside the GPU's batched driver, with
+    the phase-free cost 1 - |Tr(U^dagger V)|/8 and stop at 1e-8.  At 12 CNOT layers
+    (the configured structure) neither finds exact solutions from random starts (a
+    generic 3-qubit unitary needs >= 14 CNOTs, and the reachable targets sit in a
+    landscape of local minima near 1e-3 that scipy's BFGS finds too); at 16 layers
+    every target solves, which is where solves/s is measured."""
     import numpy as np
     from concurrent.futures import ThreadPoolExecutor
 
-    from squander_hip import Context, decompose_batch, default_options, initial_guess
+    import oracle as orc
+    from squander_hip import COST_ABS_TRACE, Context, decompose_batch, default_options, initial_guess
     from squander_hip import workloads as W
     n = 3
-    gates = W.ansatz_3q(12)
+    gates = W.ansatz_3q(layers)
     P = W.param_count(gates)
-    # half Haar targets (a generic 3-qubit unitary needs >= 14 CNOTs, so these rarely
-    # solve at 12), half reachable targets built from seeded angles
+    # half Haar targets, half reachable targets built from seeded angles
     Ts = [W.haar_unitary(8, 5000 + i).conj().T.copy() for i in range(128)]
     Ts += [o.apply_to(gates, W.random_params(P, 6000 + i), np.eye(8)).conj().T.copy() for i in range(128)]
     kind = np.array([0] * 128 + [1] * 128)
@@ -385,8 +390,8 @@ def c1_leg(o, seconds=10.0):
     with ThreadPoolExecutor(threads) as ex:
         while time.perf_counter() - t0 < seconds and done < len(Ts):
             batch = [int(i) for i in order[done:min(done + threads, len(Ts))]]
-            fs = list(ex.map(lambda i: o.decompose_custom(gates, Ts[i], n, 11 + i, optimization_tolerance=1e-8)[0],
-                             batch))
+            fs = list(ex.map(lambda i: o.decompose_custom(gates, Ts[i], n, 11 + i, optimization_tolerance=1e-8,
+                                                          variant=orc.COST_ABS_TRACE)[0], batch))
             for i, f in zip(batch, fs):
                 seen[kind[i]] += 1
                 ok[kind[i]] += int(f <= 1e-8)
@@ -394,12 +399,14 @@ def c1_leg(o, seconds=10.0):
     cpu_dt = time.perf_counter() - t0
     ctx = Context(0)
     ctx.set_circuit(n, gates)
+    ctx.set_cost_function(COST_ABS_TRACE)
     x0 = np.array([initial_guess(P, 11 + i) for i in range(len(Ts))])
     t0 = time.perf_counter()
     res = decompose_batch(ctx, np.array(Ts), np.arange(len(Ts), dtype=np.int32), x0,
                           default_options(n, optimization_tolerance=1e-8))
     gpu_dt = time.perf_counter() - t0
-    return {"workload": "3-qubit Haar targets, U3 layer + 12 alternating CNOT layers, random starts, cost <= 1e-8",
+    return {"workload": f"3 qubits, U3 layer + {layers} alternating CNOT layers ({P} angles), 128 Haar + 128 "
+                        f"reachable targets, random starts, 1 - |Tr|/8 <= 1e-8",
             "cpu": {"targets": done, "seconds": cpu_dt, "decompositions_per_s": done / cpu_dt,
                     "solves_per_s": (ok[0] + ok[1]) / cpu_dt,
                     "success_reachable": ok[1] / max(seen[1], 1), "success_haar": ok[0] / max(seen[0], 1),
@@ -408,6 +415,9 @@ def c1_leg(o, seconds=10.0):
                             "solves_per_s": float(np.sum(res["f"] <= 1e-8)) / gpu_dt,
                             "success_reachable": float(np.mean(res["f"][kind == 1] <= 1e-8)),
                             "success_haar": float(np.mean(res["f"][kind == 0] <= 1e-8)),
+                            "max_fp64_cost_of_solves": float(max((o.optimization_problem(
+                                gates, Ts[i], n, res["x"][i], variant=orc.COST_ABS_TRACE)
+                                for i in range(len(Ts)) if res["f"][i] <= 1e-8), default=float("nan"))),
                             "same_start_points_as_cpu": True}}
 
 
@@ -678,7 +688,13 @@ def main():
     if not args.skip_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(gates, U, n, P, o)
-        out["c1_3q_plain_solves"] = c1_leg(o)
+        out["c1_3q_plain_solves"] = c1_leg(o, 12)
+        out["c1_3q_16_layers"] = c1_leg(o, 16)
+        log(f"C1 3q: 12 layers GPU {out['c1_3q_plain_solves']['gpu_batched']['decompositions_per_s']:.0f} dec/s; "
+            f"16 layers GPU {out['c1_3q_16_layers']['gpu_batched']['solves_per_s']:.0f} solves/s "
+            f"(success {out['c1_3q_16_layers']['gpu_batched']['success_reachable']:.2f} reachable / "
+            f"{out['c1_3q_16_layers']['gpu_batched']['success_haar']:.2f} Haar), CPU "
+            f"{out['c1_3q_16_layers']['cpu']['solves_per_s']:.0f} solves/s")
     print(json.dumps(out), flush=True)
 
 

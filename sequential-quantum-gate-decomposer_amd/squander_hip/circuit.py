@@ -10,6 +10,8 @@ P new matrices.
 """
 from __future__ import annotations
 
+import threading
+
 import numpy as np
 
 from . import _native as nat
@@ -26,6 +28,7 @@ class Circuit:
         self._P = 0
         self._ctx = None
         self._ctx_dirty = True
+        self._ctx_lock = threading.Lock()   # first use from several threads at once
 
     # -- building --------------------------------------------------------
     def _add(self, gtype, target, control=-1):
@@ -98,13 +101,14 @@ class Circuit:
 
     # -- execution -----------------------------------------------------------
     def _context(self):
-        if self._ctx is None:
-            self._ctx = nat.Context(0)
-            self._ctx_dirty = True
-        if self._ctx_dirty:
-            self._ctx.set_circuit(self.qbit_num, self.gate_array())
-            self._ctx_dirty = False
-        return self._ctx
+        with self._ctx_lock:
+            if self._ctx is None:
+                self._ctx = nat.Context(0)
+                self._ctx_dirty = True
+            if self._ctx_dirty:
+                self._ctx.set_circuit(self.qbit_num, self.gate_array())
+                self._ctx_dirty = False
+            return self._ctx
 
     def _params(self, parameters_mtx, is_f32):
         p = np.asarray(parameters_mtx)
