@@ -345,19 +345,42 @@ def agents_leg(o):
     return res
 
 
-def uneven_leg():
+def uneven_leg(o):
+    """C4, with each reported final cost checked against the FP64 oracle's cost of the
+    returned parameters (the acceptance is FP64 on the GPU; the descent is complex64)."""
     from squander_hip import uneven_bench
-    res, _, _ = uneven_bench.run(count=256, seed=2028, tol=1e-8, mixed=True, log=log)
+    res, probs, raw = uneven_bench.run(count=256, seed=2028, tol=1e-8, mixed=True, log=log)
     res["workload"] = ("256 decompositions, qubits uniform 3..6, U3 layer + 4..20 random adjacent CNOT layers, "
                        "half Haar / half ansatz-reachable targets, complex64 descent + FP64 acceptance")
+    err = 0.0
+    for i, (n, g, U, _, _) in enumerate(probs):
+        err = max(err, abs(float(raw["f"][i]) - o.optimization_problem(g, U, n, raw["x"][i])))
+    res["max_err_reported_cost_vs_fp64_oracle"] = err
+    log(f"uneven: max |reported cost - FP64 oracle cost| = {err:.1e}")
     return res
 
 
-def solves_leg():
+def solves_leg(o):
+    """C5, with every reported final cost of the batched and learned legs checked against
+    the FP64 oracle's cost of the returned parameters."""
     from squander_hip import solve_bench
-    res = solve_bench.run(n_train=200_000, n_reach=1024, n_haar=128, epochs=20, tol=1e-8, n_plain=64, log=log)
+    from squander_hip.workloads import ansatz_4q_12
+    keep = {}
+    res = solve_bench.run(n_train=200_000, n_reach=1024, n_haar=128, epochs=20, tol=1e-8, n_plain=64, log=log,
+                          keep=keep)
     res["workload"] = ("4 qubits, U3 layer + 12 alternating CNOT layers (84 angles); warm-start network trained on "
                        "200k generated pairs for 20 epochs (full-size evaluation: 4096 reachable + 1024 Haar)")
+    gates = ansatz_4q_12()
+    for name, kept in keep.items():
+        for leg in ("batched", "learned"):
+            f, x = kept[leg]
+            err = max(abs(float(f[i]) - o.optimization_problem(gates, kept["Umtx"][i], 4, x[i])) for i in range(len(f)))
+            res["results"][name][("batched_fp64" if leg == "batched" else "learned_mixed")][
+                "max_err_reported_cost_vs_fp64_oracle"] = err
+    log("4q solves: max |reported cost - FP64 oracle cost| = "
+        + ", ".join(f"{k} {v['batched_fp64']['max_err_reported_cost_vs_fp64_oracle']:.1e} / "
+                    f"{v['learned_mixed']['max_err_reported_cost_vs_fp64_oracle']:.1e}"
+                    for k, v in res["results"].items()))
     return res
 
 
@@ -683,8 +706,8 @@ def main():
     out["apply_derivate_8q"] = derivate_leg(gates, P, U)
     if not args.skip_solves:
         out["agents_8q"] = agents_leg(o)
-        out["uneven_256"] = uneven_leg()
-        out["solves_4q"] = solves_leg()
+        out["uneven_256"] = uneven_leg(o)
+        out["solves_4q"] = solves_leg(o)
     if not args.skip_cpu:
         log("cpu baseline (oracle) ...")
         out["cpu_baseline"] = cpu_baseline(gates, U, n, P, o)

@@ -52,11 +52,14 @@ def plain_single(ctx, Umtx, x0, tol):
 
 
 def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=2026, log=print,
-        hidden=1024, n_hidden=3, n_plain=128, lr=1e-3, ft_epochs=0, ft_lr=1e-4, jit=True, ft_stream_steps=0):
+        hidden=1024, n_hidden=3, n_plain=128, lr=1e-3, ft_epochs=0, ft_lr=1e-4, jit=True, ft_stream_steps=0,
+        keep=None):
     """Returns a dict with the three legs per target set:
     plain_single (the reference flow on the GPU, measured on the first n_plain targets),
     batched_fp64 (all targets in one batched FP64 run from the same random starts) and
-    learned_mixed (network start, batched mixed precision, random-start fallback)."""
+    learned_mixed (network start, batched mixed precision, random-start fallback).
+    With a dict as keep, keep[target set] receives the targets (Umtx), the reported
+    costs and the parameters of the batched and learned legs, for an outside check."""
     ctx = nat.Context(0)
     ctx.set_jit(jit)
     gates = ansatz_4q_12()
@@ -154,6 +157,8 @@ def run(n_train=200_000, n_reach=4096, n_haar=1024, epochs=20, tol=1e-8, seed=20
             f"rand {r['start_cost_median']['random']:.3f} (phase-free pred "
             f"{r['start_cost_median']['predicted_phase_free']:.3f} rand {r['start_cost_median']['random_phase_free']:.3f})")
         results[name] = r
+        if keep is not None:
+            keep[name] = {"Umtx": Umtx, "batched": (res_b["f"], res_b["x"]), "learned": (res_l["f"], res_l["x"])}
     out["results"] = results
     return out
 
