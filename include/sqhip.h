@@ -320,6 +320,13 @@ int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* 
  * in the reference (tooling and the CPU test suite). */
 int sqh_inspect_slots(int qbit_num, const sqh_gate* gates, int n_gates, void* slots_out, int slots_cap,
                       int* n_slots);
+/* The dense-block fusion plan of the streaming path (sqh_apply_to_fused_dev): block b
+ * runs gate-list entries src[block_first[b] .. block_first[b+1]) in that order on the
+ * sorted qubits block_qubits[4b .. 4b+3].  Buffers may be NULL to query *n_blocks; sizes:
+ * block_first n_blocks + 1, block_qubits 4 n_blocks, src n_gates.  No counterpart in the
+ * reference (tooling and the CPU test suite). */
+int sqh_inspect_fusion(int qbit_num, const sqh_gate* gates, int n_gates, int* block_first, int* block_qubits,
+                       int blocks_cap, int* src, int src_cap, int* n_blocks);
 
 #ifdef __cplusplus
 }

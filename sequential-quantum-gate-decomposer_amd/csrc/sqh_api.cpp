@@ -790,6 +790,24 @@ int sqh_inspect_program(int qbit_num, const sqh_gate* gates, int n_gates, void* 
     return SQH_OK;
 }
 
+int sqh_inspect_fusion(int qbit_num, const sqh_gate* gates, int n_gates, int* block_first, int* block_qubits,
+                       int blocks_cap, int* src, int src_cap, int* n_blocks) {
+    if (n_gates < 0 || (n_gates > 0 && !gates)) return SQH_ERR_ARG;
+    std::vector<GateRec> g(n_gates);
+    for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
+    Program p;
+    std::string err;
+    if (!compile_program(qbit_num, g, p, err)) return SQH_ERR_GATE;
+    const FusionPlan fp = plan_fusion(qbit_num, g, p.gate_slot);
+    const int nb = (int)fp.qubits.size();
+    if (n_blocks) *n_blocks = nb;
+    for (int b = 0; b <= nb && block_first && b <= blocks_cap; ++b) block_first[b] = fp.first[b];
+    for (int b = 0; b < nb && block_qubits && b < blocks_cap; ++b)
+        for (int j = 0; j < 4; ++j) block_qubits[4 * b + j] = j < fp.k ? fp.qubits[b][j] : -1;
+    for (int i = 0; i < (int)fp.src.size() && src && i < src_cap; ++i) src[i] = fp.src[i];
+    return SQH_OK;
+}
+
 int sqh_inspect_slots(int qbit_num, const sqh_gate* gates, int n_gates, void* slots_out, int slots_cap,
                       int* n_slots) {
     if (n_gates < 0 || (n_gates > 0 && !gates)) return SQH_ERR_ARG;

@@ -80,14 +80,22 @@ Dag build_dag(const std::vector<GateRec>& g) {
     return d;
 }
 
+// Register set R holds every qubit of gate g (a dense block on R can absorb it).
+bool holds(const GateRec& g, uint32_t R) {
+    if (!((R >> g.target) & 1u)) return false;
+    return !controlled(g.type) || ((R >> g.control) & 1u);
+}
+
 // Runs, with register set R, every gate that is or becomes runnable (lowest index
-// first); frontier holds the runnable-by-dependencies gates R does not serve.  With
-// order null the state is restored afterwards (a trial); returns the gates run.
+// first) and that R takes (fits: serves for the tile program, holds for dense blocks);
+// frontier holds the runnable-by-dependencies gates R does not take.  With order null
+// the state is restored afterwards (a trial); returns the gates run.
+template <typename Fits>
 int closure(const std::vector<GateRec>& g, const Dag& d, std::vector<int>& pend, std::vector<int>& frontier,
-            uint32_t R, std::vector<int>* order) {
+            uint32_t R, std::vector<int>* order, Fits fits) {
     std::priority_queue<int, std::vector<int>, std::greater<int>> heap;
     std::vector<int> rest, touched;
-    for (int i : frontier) (serves(g[i], R) ? (void)heap.push(i) : rest.push_back(i));
+    for (int i : frontier) (fits(g[i], R) ? (void)heap.push(i) : rest.push_back(i));
     int n = 0;
     while (!heap.empty()) {
         const int i = heap.top();
@@ -97,7 +105,7 @@ int closure(const std::vector<GateRec>& g, const Dag& d, std::vector<int>& pend,
         for (int s : d.succ[i]) {
             if (s < 0) continue;
             touched.push_back(s);
-            if (--pend[s] == 0) (serves(g[s], R) ? (void)heap.push(s) : rest.push_back(s));
+            if (--pend[s] == 0) (fits(g[s], R) ? (void)heap.push(s) : rest.push_back(s));
         }
     }
     if (order) {
@@ -125,7 +133,7 @@ void schedule(int N, int K, const std::vector<GateRec>& g, std::vector<int>& ord
         int best_n = -1;
         for (uint32_t R = 0; R < (1u << N); ++R) {
             if (__builtin_popcount(R) != K) continue;
-            const int n = closure(g, d, pend, frontier, R, nullptr);
+            const int n = closure(g, d, pend, frontier, R, nullptr, serves);
             if (n > best_n) {
                 best_n = n;
                 best = R;
@@ -133,7 +141,39 @@ void schedule(int N, int K, const std::vector<GateRec>& g, std::vector<int>& ord
         }
         seg_start.push_back((int)order.size());
         sets.push_back(best);
-        closure(g, d, pend, frontier, best, &order);
+        closure(g, d, pend, frontier, best, &order, serves);
+    }
+}
+
+// Dense-block partition: repeatedly the 4-qubit set under which the most gates run
+// (each gate with all its qubits in the set, in dependency order), as the tile
+// schedule picks register sets.  blocks[k] lists the gates of the k-th block in run
+// order; sets[k] is its qubit set.
+void schedule_blocks(int N, const std::vector<GateRec>& g, std::vector<std::vector<int>>& blocks,
+                     std::vector<uint32_t>& sets) {
+    const int G = (int)g.size();
+    Dag d = build_dag(g);
+    std::vector<int> pend = d.npred, frontier;
+    for (int i = 0; i < G; ++i)
+        if (pend[i] == 0) frontier.push_back(i);
+    blocks.clear();
+    sets.clear();
+    int placed = 0;
+    while (placed < G) {
+        uint32_t best = 0;
+        int best_n = -1;
+        for (uint32_t R = 0; R < (1u << N); ++R) {
+            if (__builtin_popcount(R) != 4) continue;
+            const int n = closure(g, d, pend, frontier, R, nullptr, holds);
+            if (n > best_n) {
+                best_n = n;
+                best = R;
+            }
+        }
+        blocks.emplace_back();
+        sets.push_back(best);
+        closure(g, d, pend, frontier, best, &blocks.back(), holds);
+        placed += (int)blocks.back().size();
     }
 }
 
@@ -304,12 +344,8 @@ FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates, const std::vect
     FusionPlan fp;
     const int kmax = N < 4 ? N : 4;
     fp.k = kmax;
-    std::vector<int> cur;   // qubits of the open block
-    std::vector<size_t> members;
-    const std::vector<int>& slot_of = gate_slot;
-    auto close = [&]() {
-        if (members.empty()) return;
-        // pad to kmax qubits with the lowest unused ones
+    // one block: its qubits (padded to kmax with the lowest unused ones) and gates in order
+    auto emit = [&](std::vector<int> cur, const std::vector<int>& members) {
         for (int q = 0; q < N && (int)cur.size() < kmax; ++q)
             if (std::find(cur.begin(), cur.end(), q) == cur.end()) cur.push_back(q);
         std::sort(cur.begin(), cur.end());
@@ -318,13 +354,13 @@ FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates, const std::vect
         fp.first.push_back((int)fp.gates.size());
         fp.qubits.push_back(qs);
         auto local = [&](int q) { return (int)(std::find(cur.begin(), cur.end(), q) - cur.begin()); };
-        for (size_t gi : members) {
+        for (int gi : members) {
             const GateRec& g = gates[gi];
             FusedRec r;
             r.pad = 0;
             r.t = (int8_t)local(g.target);
             r.c = -1;
-            r.slot = slot_of[gi];
+            r.slot = gate_slot[gi];
             switch (g.type) {
             case GT_CNOT: r.kind = OP_X; r.c = (int8_t)local(g.control); break;
             case GT_CZ: r.kind = OP_Z; r.c = (int8_t)local(g.control); break;
@@ -332,25 +368,42 @@ FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates, const std::vect
             default: r.kind = OP_U; break;
             }
             fp.gates.push_back(r);
+            fp.src.push_back(gi);
         }
-        cur.clear();
-        members.clear();
     };
-    for (size_t i = 0; i < gates.size(); ++i) {
-        const GateRec& g = gates[i];
-        std::vector<int> q = {g.target};
-        if (g.type == GT_CNOT || g.type == GT_CZ || g.type == GT_CRY) q.push_back(g.control);
-        std::vector<int> uni = cur;
-        for (int x : q)
-            if (std::find(uni.begin(), uni.end(), x) == uni.end()) uni.push_back(x);
-        if ((int)uni.size() > kmax) {
-            close();
-            uni = q;
+    if (N >= 4 && N <= kFuseScheduleMaxQubits) {
+        // gates on disjoint qubits commute: pack each block with every gate that can
+        // run inside its qubit set (8-qubit 40-layer ansatz: 6 blocks where list order
+        // needs 15; 10-qubit 60-layer: 9 where it needs 26)
+        std::vector<std::vector<int>> blocks;
+        std::vector<uint32_t> sets;
+        schedule_blocks(N, gates, blocks, sets);
+        for (size_t k = 0; k < blocks.size(); ++k) {
+            std::vector<int> cur;
+            for (int q = 0; q < N; ++q)
+                if ((sets[k] >> q) & 1u) cur.push_back(q);
+            emit(cur, blocks[k]);
         }
-        cur = uni;
-        members.push_back(i);
+    } else {
+        // list order: grow a block while the union of touched qubits stays <= kmax
+        std::vector<int> cur, members;
+        for (size_t i = 0; i < gates.size(); ++i) {
+            const GateRec& g = gates[i];
+            std::vector<int> q = {g.target};
+            if (controlled(g.type)) q.push_back(g.control);
+            std::vector<int> uni = cur;
+            for (int x : q)
+                if (std::find(uni.begin(), uni.end(), x) == uni.end()) uni.push_back(x);
+            if ((int)uni.size() > kmax) {
+                if (!members.empty()) emit(cur, members);
+                members.clear();
+                uni = q;
+            }
+            cur = uni;
+            members.push_back((int)i);
+        }
+        if (!members.empty()) emit(cur, members);
     }
-    close();
     fp.first.push_back((int)fp.gates.size());
     return fp;
 }

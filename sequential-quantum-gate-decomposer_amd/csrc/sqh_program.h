@@ -32,10 +32,15 @@ struct FusedRec {
 
 struct FusionPlan {
     std::vector<FusedRec> gates;
+    std::vector<int> src;               // gate-list index of each entry of gates
     std::vector<int> first;             // block b covers gates[first[b] .. first[b+1])
     std::vector<std::array<int, 4>> qubits;  // sorted qubits of each block
     int k = 0;                          // qubits per block
 };
+
+// Up to this many qubits the blocks are packed by dependency scheduling (a search over
+// the C(N, 4) qubit sets per block); above it, in gate-list order.
+constexpr int kFuseScheduleMaxQubits = 16;
 
 // gate_slot: the kernel-table index of each parametric gate (Program::gate_slot).
 FusionPlan plan_fusion(int N, const std::vector<GateRec>& gates, const std::vector<int>& gate_slot);

@@ -38,7 +38,7 @@ EXPORTS = [
     "sqh_apply_derivate_to", "sqh_apply_derivate_to_dev", "sqh_optimization_problem_batch",
     "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
-    "sqh_inspect_program", "sqh_inspect_slots", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
+    "sqh_inspect_program", "sqh_inspect_slots", "sqh_inspect_fusion", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
@@ -161,6 +161,7 @@ def lib():
             "sqh_get_jit_stats": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_slots": ([i, ctypes.POINTER(_Gate), i, vp, i, ctypes.POINTER(ctypes.c_int)], i),
+            "sqh_inspect_fusion": ([i, ctypes.POINTER(_Gate), i, vp, vp, i, vp, i, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_program": ([i, ctypes.POINTER(_Gate), i, vp, i, vp, i, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int)], i),
         }
@@ -212,6 +213,26 @@ def inspect_slots(qbit_num, gates):
     if rc:
         raise SqhError(rc, "program compilation failed")
     return sl[:n.value]
+
+
+def inspect_fusion(qbit_num, gates):
+    """The streaming path's dense-block plan: a list of (sorted qubits, gate-list
+    indices in run order), one entry per block."""
+    L = lib()
+    garr, ng = gate_array(gates)
+    nb = ctypes.c_int(0)
+    rc = L.sqh_inspect_fusion(qbit_num, garr, ng, None, None, 0, None, 0, ctypes.byref(nb))
+    if rc:
+        raise SqhError(rc, "program compilation failed")
+    first = np.zeros(nb.value + 1, dtype=np.int32)
+    qs = np.zeros((max(nb.value, 1), 4), dtype=np.int32)
+    src = np.zeros(max(ng, 1), dtype=np.int32)
+    rc = L.sqh_inspect_fusion(qbit_num, garr, ng, first.ctypes.data, qs.ctypes.data, nb.value, src.ctypes.data, ng,
+                              ctypes.byref(nb))
+    if rc:
+        raise SqhError(rc, "program compilation failed")
+    return [(tuple(int(q) for q in qs[b] if q >= 0), [int(x) for x in src[first[b]:first[b + 1]]])
+            for b in range(nb.value)]
 
 
 class Context:

@@ -116,3 +116,53 @@ def test_compile_beyond_the_tile_range(n):
     slots = nat.inspect_slots(n, gates)
     par = [int(g[3]) for g in gates if nat.PARAM_NUM[int(g[0])]]
     assert [int(ps) for _, ps, _, _ in slots] == par
+
+
+def _embed(n, qubits, F):
+    """Dense 2^n matrix of the block F acting on the sorted qubits (local bit j = qubits[j])."""
+    d = 1 << n
+    full = np.zeros((d, d), dtype=complex)
+    mask = sum(1 << q for q in qubits)
+    for r in range(d):
+        li = sum(((r >> q) & 1) << j for j, q in enumerate(qubits))
+        base = r & ~mask
+        for lo in range(1 << len(qubits)):
+            ro = base | sum(((lo >> j) & 1) << q for j, q in enumerate(qubits))
+            full[ro, r] += F[lo, li]
+    return full
+
+
+@pytest.mark.parametrize("n,which", [(4, "mixed"), (6, "mixed"), (8, "ansatz"), (9, "mixed")])
+def test_fusion_plan_is_a_legal_reordering(oracle, n, which):
+    """The streaming path's 4-qubit blocks: every gate once, each block holds its gates'
+    qubits, every qubit's gates keep their list order, and the product of the blocks
+    (each built by the oracle from its gates) is the circuit."""
+    gates = W.ansatz_8q_40(2026) if which == "ansatz" else W.mixed_gate_circuit(n, 7 * n, seed=2700 + n)
+    P = W.param_count(gates)
+    p = np.random.default_rng(n).uniform(0, 2 * np.pi, P)
+    plan = nat.inspect_fusion(n, gates)
+    order = [i for _, src in plan for i in src]
+    assert sorted(order) == list(range(len(gates)))
+    for q in range(n):
+        on_q = [i for i in order if q in (int(gates[i][1]), int(gates[i][2]))]
+        assert on_q == sorted(on_q)
+    C = np.eye(1 << n, dtype=complex)
+    for qubits, src in plan:
+        assert len(qubits) == 4
+        local = {q: j for j, q in enumerate(qubits)}
+        sub = []
+        for i in src:
+            t, tg, c, ps = (int(v) for v in gates[i])
+            ctrl = t in (W.GATE_CNOT, W.GATE_CZ, W.GATE_CRY)
+            assert tg in local and (not ctrl or c in local)
+            sub.append((t, local[tg], local[c] if ctrl else -1, ps))
+        F = oracle.apply_to(np.array(sub, dtype=np.int32).reshape(-1, 4), p, np.eye(16, dtype=complex))
+        C = _embed(n, list(qubits), F) @ C
+    assert rel_fro(C, oracle.apply_to(gates, p, np.eye(1 << n, dtype=complex))) < 1e-12
+
+
+def test_fusion_plan_block_counts():
+    """Dependency packing: 6 blocks for the 8-qubit 40-layer ansatz and 9 for the
+    10-qubit 60-layer one (gate-list order needs 15 and 26)."""
+    assert len(nat.inspect_fusion(8, W.ansatz_8q_40(2026))) <= 6
+    assert len(nat.inspect_fusion(10, W.ansatz_10q_60(2027))) <= 9
