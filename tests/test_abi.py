@@ -98,6 +98,17 @@ def test_no_gpu_fails_loudly():
         nat.Context(0)
 
 
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    """Without the built HIP library the package raises instead of computing anything
+    another way (there is no CPU fallback)."""
+    monkeypatch.setattr(nat, "LIB_PATH", str(tmp_path / "libsqhip.so"))
+    monkeypatch.setattr(nat, "_lib", None)
+    with pytest.raises(ImportError, match="not built"):
+        nat.lib()
+    with pytest.raises(ImportError):
+        Circuit.from_gate_array(2, W.ansatz_layers(2, [0])).get_Matrix(np.zeros(W.param_count(W.ansatz_layers(2, [0]))))
+
+
 def test_initial_guess_matches_oracle_mt19937(oracle):
     from squander_hip import mt19937_uniform_2pi
     for seed in (1, 42, 2026):
