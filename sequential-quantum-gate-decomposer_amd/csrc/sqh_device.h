@@ -423,13 +423,21 @@ __device__ __forceinline__ void emit_trace(const cpx<R>* v, const Layout* lay, i
     using G = Geo<N>;
     const long long at = (long long)li.b * m + li.col;
     if (ts.nch == 0) {
+        // at most one register holds row tr: pick it with selects and store once
+        // (a conditional store per register invites the compiler to sink earlier work
+        // into the branches)
+        R dre = 0, dim = 0;
+        bool hit = false;
 #pragma unroll
         for (int i = 0; i < G::NR; ++i) {
-            const int row = so + lay->reg_off[i];
-            if (li.valid && row == tr) {
-                diag[2 * at] = (double)v[i].re;
-                diag[2 * at + 1] = (double)v[i].im;
-            }
+            const bool m = so + lay->reg_off[i] == tr;
+            dre = m ? v[i].re : dre;
+            dim = m ? v[i].im : dim;
+            hit = hit || m;
+        }
+        if (li.valid && hit) {
+            diag[2 * at] = (double)dre;
+            diag[2 * at + 1] = (double)dim;
         }
         return;
     }

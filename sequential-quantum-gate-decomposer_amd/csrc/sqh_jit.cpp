@@ -257,11 +257,15 @@ struct Gen {
                 o << "  if (li.valid) dst[(long long)(sf + " << lf.reg_off[i] << ") * SOUTS + li.col] = v[" << i
                   << "];\n";
         } else {
-            o << "  const int tr = li.col + STR;\n";
+            // the lane's diagonal entry (at most one register matches) picked with selects
+            // and stored once: conditional stores per register let the compiler sink the
+            // last gates into sixteen branches and spill scalar registers there
+            o << "  const int tr = li.col + STR;\n  R dre = 0, dim = 0; bool hit = false;\n";
             for (int i = 0; i < NR; ++i)
-                o << "  if (li.valid && sf + " << lf.reg_off[i] << " == tr) { a.diag[2 * ((long long)li.b * SM + li.col)] = "
-                  << "(double)v[" << i << "].re; a.diag[2 * ((long long)li.b * SM + li.col) + 1] = (double)v[" << i
-                  << "].im; }\n";
+                o << "  { const bool m = sf + " << lf.reg_off[i] << " == tr; dre = m ? v[" << i << "].re : dre; dim = m ? v["
+                  << i << "].im : dim; hit = hit || m; }\n";
+            o << "  if (li.valid && hit) { a.diag[2 * ((long long)li.b * SM + li.col)] = (double)dre; "
+                 "a.diag[2 * ((long long)li.b * SM + li.col) + 1] = (double)dim; }\n";
         }
         o << "}\n";
         return o.str();
@@ -280,13 +284,16 @@ struct Gen {
         load("x", "in", "SINS");
         const int cur = forward("x");
         const Layout& lf = pr.layouts[cur];
-        o << "  { const int sf = " << sub_expr(N, lf) << ";\n    const int tr = li.col + STR;\n";
+        // diagonal entry by selects and one store (as in the forward kernel)
+        o << "  { const int sf = " << sub_expr(N, lf) << ";\n    const int tr = li.col + STR;\n"
+          << "    R dre = 0, dim = 0; bool any = false;\n";
         for (int i = 0; i < NR; ++i) {
             o << "    { const bool hit = li.valid && sf + " << lf.reg_off[i] << " == tr;\n"
-              << "      if (hit) { a.diag[2 * ((long long)li.b * SM + li.col)] = (double)x[" << i
-              << "].re; a.diag[2 * ((long long)li.b * SM + li.col) + 1] = (double)x[" << i << "].im; }\n"
+              << "      dre = hit ? x[" << i << "].re : dre; dim = hit ? x[" << i << "].im : dim; any = any || hit;\n"
               << "      y[" << i << "] = mk<R>(hit ? (R)1 : (R)0, (R)0); }\n";
         }
+        o << "    if (any) { a.diag[2 * ((long long)li.b * SM + li.col)] = (double)dre; "
+             "a.diag[2 * ((long long)li.b * SM + li.col) + 1] = (double)dim; }\n";
         o << "  }\n"
           << "  const int lim = SMPAD < G::CW ? SMPAD : G::CW;\n"
           << "  const bool leader = li.bvalid && (li.s == 0) && (li.cc % lim == 0);\n"
