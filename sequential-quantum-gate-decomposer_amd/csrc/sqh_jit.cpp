@@ -112,29 +112,30 @@ struct Gen {
     // A remap permutes rows within each column, so with rp_ > 1 the wavefront's columns
     // go through the buffer in rp_ groups (an LDS buffer 1/rp_ the slab's size).
     int rp_ = 1;
+    // LDS slot of (row, column group cq): row * LW + (row / 16) * LPAD + cq (LW = CW / RP
+    // columns, one padding slot per 16 rows to spread rows over the banks).  A row is a
+    // lane part (sub-index bits) plus a register offset on disjoint bits, so the slot is
+    // the lane's base plus a constant the compiler folds into the LDS instruction.
+    int lw() const { return (64 >> (N - geo_k(N))) / rp_; }
+    int slot_const(int off) const { return off * lw() + (off >> 4); }
     void remap(const char* v, int from, int to, const char* sv = "s", const char* cv = "cc") {
         if (N <= geo_k(N)) return;
         const Layout& a = pr.layouts[from];
         const Layout& b = pr.layouts[to];
-        o << "  { const int so = " << sub_expr(N, a, sv) << ", sn = " << sub_expr(N, b, sv) << ";\n";
-        if (rp_ == 1) {
-            for (int i = 0; i < NR; ++i)
-                o << "    lds[lds_idx<N>(so + " << a.reg_off[i] << ", " << cv << ")] = " << v << "[" << i << "];\n";
-            o << "    __syncthreads();\n";
-            for (int i = 0; i < NR; ++i)
-                o << "    " << v << "[" << i << "] = lds[lds_idx<N>(sn + " << b.reg_off[i] << ", " << cv << ")];\n";
-            o << "    __syncthreads(); }\n";
-            return;
-        }
-        o << "    const int cq = " << cv << " % (G::CW / RP), ch = " << cv << " / (G::CW / RP);\n";
+        o << "  { const int so = " << sub_expr(N, a, sv) << ", sn = " << sub_expr(N, b, sv) << ";\n"
+          << "    const int cq = " << cv << " % LW, ch = " << cv << " / LW; (void)ch;\n"
+          << "    const int bo = ldsb(so, cq), bn = ldsb(sn, cq);\n";
         for (int pass = 0; pass < rp_; ++pass) {
-            o << "    if (ch == " << pass << ") {\n";
+            if (rp_ > 1) o << "    if (ch == " << pass << ") {\n";
             for (int i = 0; i < NR; ++i)
-                o << "      lds[ldsq(so + " << a.reg_off[i] << ", cq)] = " << v << "[" << i << "];\n";
-            o << "    }\n    __syncthreads();\n    if (ch == " << pass << ") {\n";
+                o << "      lds[bo + " << slot_const(a.reg_off[i]) << "] = " << v << "[" << i << "];\n";
+            if (rp_ > 1) o << "    }\n";
+            o << "    __syncthreads();\n";
+            if (rp_ > 1) o << "    if (ch == " << pass << ") {\n";
             for (int i = 0; i < NR; ++i)
-                o << "      " << v << "[" << i << "] = lds[ldsq(sn + " << b.reg_off[i] << ", cq)];\n";
-            o << "    }\n    __syncthreads();\n";
+                o << "      " << v << "[" << i << "] = lds[bn + " << slot_const(b.reg_off[i]) << "];\n";
+            if (rp_ > 1) o << "    }\n";
+            o << "    __syncthreads();\n";
         }
         o << "  }\n";
     }
@@ -212,10 +213,11 @@ struct Gen {
           << ", SOUTS = " << sh.out_stride << ", STR = " << sh.trace_off << ", SNS = " << sh.ns << ";\n"
           << "constexpr long long SINB = " << sh.in_bstride << "LL, SOUTB = " << sh.out_bstride
           << "LL, SKB = " << sh.kern_bstride << "LL;\n"
-          << "constexpr int RP = " << rp_ << ";\n"
-          << "__device__ __forceinline__ int ldsq(int row, int cq) { return (row ^ ((row >> 4) & 15)) * (G::CW / RP) + cq; }\n"
+          << "constexpr int RP = " << rp_ << ", LW = G::CW / RP, LPAD = 1;\n"
+          << "constexpr int LDSN = (N > G::K) ? G::D * LW + (G::D / 16) * LPAD : 1;\n"
+          << "__device__ __forceinline__ int ldsb(int row, int cq) { return row * LW + (row >> 4) * LPAD + cq; }\n"
           << "extern \"C\" __global__ __launch_bounds__(64) void " << name << "(JitArgs a) {\n"
-          << "  __shared__ cpx<R> lds[G::LDS / RP];\n"
+          << "  __shared__ cpx<R> lds[LDSN];\n"
           << "  int B = a.B;\n"
           << "  if (a.count) { B = *a.count; if (B <= 0) return; }\n"
           << "  if ((long long)blockIdx.x * G::CW / SMPAD >= B) return;\n"
