@@ -26,7 +26,7 @@ FP64_PEAK_TFLOPS = 78.6
 # in the same runs): profiles/k_tile_forward_k_stream_gate_k_block_mfma4_8q_b512.md
 TILE_FORWARD_8Q_B512_TRAFFIC = (11922.6 * 2.0 + 2048.0) * 1024
 # the same for the circuit-specialised kernel: profiles/sqh_jit_kernel_8q_b512.md
-JIT_FORWARD_8Q_B512_TRAFFIC = (24118.5 * 2.0 + 2048.0) * 1024
+JIT_FORWARD_8Q_B512_TRAFFIC = (24042.5 * 2.0 + 2048.0) * 1024
 
 
 def log(*a):
