@@ -262,6 +262,29 @@ def test_fused_streaming_matches_oracle(oracle, n):
                 assert err < tol, (f32, per_b, b, err)
 
 
+def test_fused_streaming_list_order_plan_17_qubits(oracle):
+    """Above 16 qubits the fused path uses the list-order block plan."""
+    import torch
+    from squander_hip import Context
+    n, m, B = 17, 4, 2
+    say(f"fused streaming n={n} (list-order plan)")
+    gates = W.mixed_gate_circuit(n, 40, seed=3000)
+    P = W.param_count(gates)
+    d = 1 << n
+    ps = W.random_params(P, 31, batch=B)
+    X = np.random.default_rng(31).standard_normal((B, d, m)) + 1j * np.random.default_rng(32).standard_normal((B, d, m))
+    ctx = Context(0)
+    ctx.set_circuit(n, gates)
+    pd = torch.from_numpy(ps).cuda()
+    Xd = torch.from_numpy(X.copy()).cuda()
+    torch.cuda.synchronize()
+    ctx.check(ctx._L.sqh_apply_to_fused_dev(ctx.h, pd.data_ptr(), P, Xd.data_ptr(), d, m, m, d * m, B, 0))
+    ctx.synchronize()
+    got = Xd.cpu().numpy()
+    for b in range(B):
+        assert rel_fro(got[b], oracle.apply_to(gates, ps[b], X[b])) < 1e-12
+
+
 def test_block_launch_shapes_give_identical_results(oracle):
     """Every launch shape of the 4-qubit matrix-core block kernel (tiles per wavefront x
     threads per workgroup) computes the same bits, including a column count that is not a

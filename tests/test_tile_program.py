@@ -166,3 +166,18 @@ def test_fusion_plan_block_counts():
     10-qubit 60-layer one (gate-list order needs 15 and 26)."""
     assert len(nat.inspect_fusion(8, W.ansatz_8q_40(2026))) <= 6
     assert len(nat.inspect_fusion(10, W.ansatz_10q_60(2027))) <= 9
+
+
+@pytest.mark.parametrize("n", [17, 20])
+def test_fusion_plan_in_list_order_above_sixteen_qubits(n):
+    """Above 16 qubits blocks grow in gate-list order: every gate once and in list
+    order, each block holding its gates' qubits."""
+    gates = W.mixed_gate_circuit(n, 4 * n, seed=2900 + n)
+    plan = nat.inspect_fusion(n, gates)
+    order = [i for _, src in plan for i in src]
+    assert order == list(range(len(gates)))
+    for qubits, src in plan:
+        assert len(qubits) == 4 and list(qubits) == sorted(qubits)
+        for i in src:
+            t, tg, c, _ = (int(v) for v in gates[i])
+            assert tg in qubits and (t not in (W.GATE_CNOT, W.GATE_CZ, W.GATE_CRY) or c in qubits)
