@@ -101,6 +101,36 @@ def agents_opts(**kw):
     return o
 
 
+class BfgsEvent(ctypes.Structure):
+    """or_bfgs_event (sq_oracle.h): one observed step of the optimiser."""
+    _fields_ = [("kind", ctypes.c_int), ("n", ctypes.c_long), ("iter", ctypes.c_long),
+                ("step", ctypes.c_double), ("f", ctypes.c_double), ("dg", ctypes.c_double),
+                ("f0", ctypes.c_double), ("dg0", ctypes.c_double), ("f_lowest", ctypes.c_double),
+                ("max_step", ctypes.c_double), ("exit_curvature", ctypes.c_int), ("exit_calls", ctypes.c_int),
+                ("dTg", ctypes.c_double), ("g0_dot_delta", ctypes.c_double), ("norm_Z", ctypes.c_double)] + \
+        [(name, ctypes.POINTER(ctypes.c_double)) for name in ("g", "d", "g0", "delta", "gamma", "Z", "Z_rot", "Ztg",
+                                                              "x")]
+
+
+EV_DIRECTION, EV_PROBE, EV_LS_END, EV_UPDATE, EV_SKIP, EV_START, EV_END = range(7)
+_HOOK = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(BfgsEvent))
+_FG = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                       ctypes.POINTER(ctypes.c_double))
+
+
+def _event_dict(ev: BfgsEvent) -> dict:
+    n = int(ev.n)
+    out = {k: getattr(ev, k) for k in ("kind", "iter", "step", "f", "dg", "f0", "dg0", "f_lowest", "max_step",
+                                       "exit_curvature", "exit_calls", "dTg", "g0_dot_delta", "norm_Z")}
+    for name in ("g", "d", "g0", "delta", "gamma", "Ztg", "x"):
+        ptr = getattr(ev, name)
+        out[name] = np.ctypeslib.as_array(ptr, shape=(n,)).copy() if ptr else None
+    for name in ("Z", "Z_rot"):
+        ptr = getattr(ev, name)
+        out[name] = np.ctypeslib.as_array(ptr, shape=(n, n)).copy() if ptr else None
+    return out
+
+
 class Oracle:
     """ctypes binding of sq_oracle.c."""
 
@@ -150,6 +180,10 @@ class Oracle:
         L.or_mt_next.restype = ctypes.c_uint
         L.or_mt_uniform_2pi.argtypes = [ctypes.c_void_p]
         L.or_mt_uniform_2pi.restype = ctypes.c_double
+        L.or_bfgs_set_hook.argtypes = [_HOOK, ctypes.c_void_p]
+        L.or_bfgs_powell_fn.argtypes = [_FG, ctypes.c_void_p, ctypes.c_long, dp, ctypes.c_long,
+                                        ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int)]
+        L.or_bfgs_powell_fn.restype = ctypes.c_double
 
     # -- helpers -----------------------------------------------------------
     @staticmethod
@@ -277,6 +311,33 @@ class Oracle:
                                          iteration_loops, max_inner_iterations, optimization_tolerance,
                                          convergence_threshold, self._dp(x), ctypes.byref(calls))
         return f, x, calls.value
+
+    def traced(self, call):
+        """Runs call() with the optimiser hook set; returns (call's result, events)."""
+        events = []
+        hook = _HOOK(lambda user, ev: events.append(_event_dict(ev.contents)))
+        self.lib.or_bfgs_set_hook(hook, None)
+        try:
+            res = call()
+        finally:
+            self.lib.or_bfgs_set_hook(_HOOK(), None)
+        return res, events
+
+    def bfgs_fn(self, fg, x0, max_calls=10000):
+        """BFGS_Powell on a Python objective fg(x) -> (f, g); returns (f, x, calls, status)."""
+        n = len(x0)
+
+        def cb(user, xp, fp, gp):
+            f, g = fg(np.ctypeslib.as_array(xp, shape=(n,)).copy())
+            fp[0] = float(f)
+            np.ctypeslib.as_array(gp, shape=(n,))[:] = g
+
+        fn = _FG(cb)
+        x = np.ascontiguousarray(x0, dtype=np.float64).copy()
+        calls = ctypes.c_long(0)
+        st = ctypes.c_int(0)
+        f = self.lib.or_bfgs_powell_fn(fn, None, n, self._dp(x), max_calls, ctypes.byref(calls), ctypes.byref(st))
+        return f, x, calls.value, st.value
 
     def agents(self, gates, target, qbit_num, seed, x0, variant=COST_FROBENIUS, **kw):
         """AGENTS from the solution guess x0 with mt19937(seed); returns

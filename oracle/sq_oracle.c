@@ -480,16 +480,29 @@ enum { ST_INITIAL = 0, ST_VARS_INIT = 1, ST_ZERO_STEP = 2, ST_MAX_ITER = 3,
        ST_NO_DESCENT = 4, ST_MINIMUM = 5 };
 
 typedef struct {
-    const or_problem* pr;
+    or_fg_fn fn;
+    void* user;
     long n, calls, max_calls;
     int status;
     double* Z;      /* n x n row-major */
     double* Ztg;    /* Z^T g */
 } bfgs_state;
 
+/* Optional observer of the optimiser's internal steps (test hook, per thread). */
+static _Thread_local or_bfgs_hook g_hook = 0;
+static _Thread_local void* g_hook_user = 0;
+
+void or_bfgs_set_hook(or_bfgs_hook hook, void* user) { g_hook = hook; g_hook_user = user; }
+
+static void emit(or_bfgs_event* ev) { if (g_hook) g_hook(g_hook_user, ev); }
+
 static void eval_fg(bfgs_state* s, const double* x, double* f, double* g) {
     s->calls++;
-    or_optimization_problem_combined(s->pr, x, f, g);
+    s->fn(s->user, x, f, g);
+}
+
+static void problem_fg(void* user, const double* x, double* f, double* g) {
+    or_optimization_problem_combined((const or_problem*)user, x, f, g);
 }
 
 /* Grad_Descend::get_Maximal_Line_Search_Step (grad_descend.cpp:396-425). */
@@ -513,6 +526,7 @@ static void line_search(bfgs_state* s, double* x, double* g, const double* dir, 
     memcpy(g_best, g, sizeof(double) * n);
     const long max_loops = 50;
     double step = fmin(1.0, maximal_step);
+    const double f0 = *f;
     double f_lowest = *f, f_highest = *f;
     double step_highest = maximal_step, step_lowest = 0.0;
     double step_best = 0.0, f_best = *f, dg_abs_best = fabs(dg0), dg_highest = 0.0;
@@ -526,6 +540,15 @@ static void line_search(bfgs_state* s, double* x, double* g, const double* dir, 
         if (*f < f_best || fabs(dg) < dg_abs_best) {
             step_best = step; f_best = *f; dg_abs_best = fabs(dg);
             memcpy(g_best, g, sizeof(double) * n);
+        }
+        if (g_hook) {
+            or_bfgs_event ev = {0};
+            ev.kind = OR_EV_PROBE; ev.n = n; ev.iter = it; ev.step = step; ev.f = *f; ev.dg = dg;
+            ev.f0 = f0; ev.dg0 = dg0; ev.f_lowest = f_lowest; ev.max_step = maximal_step;
+            ev.exit_calls = (s->calls == s->max_calls);
+            ev.exit_curvature = !ev.exit_calls && (*f < f_lowest + step * 0.1 * dg0) && (dg >= dg0 * 0.7);
+            ev.g = g; ev.d = dir;
+            emit(&ev);
         }
         if (s->calls == s->max_calls) break;
         if (*f < f_lowest + step * 0.1 * dg0) {
@@ -551,6 +574,12 @@ static void line_search(bfgs_state* s, double* x, double* g, const double* dir, 
         for (long k = 0; k < n; k++) x[k] = x0[k] + step * dir[k];
     }
     if (step == 0.0) s->status = ST_ZERO_STEP;
+    if (g_hook) {
+        or_bfgs_event ev = {0};
+        ev.kind = OR_EV_LS_END; ev.n = n; ev.step = step; ev.f = *f; ev.f0 = f0; ev.dg0 = dg0;
+        ev.max_step = maximal_step; ev.g = g; ev.d = dir; ev.g0 = g0;
+        emit(&ev);
+    }
     free(g_best);
 }
 
@@ -566,6 +595,11 @@ static double search_direction(bfgs_state* s, const double* g, double* d) {
     }
     double dg = 0.0;
     for (long k = 0; k < n; k++) dg += d[k] * g[k];
+    if (g_hook) {
+        or_bfgs_event ev = {0};
+        ev.kind = OR_EV_DIRECTION; ev.n = n; ev.dg = dg; ev.g = g; ev.d = d; ev.Z = s->Z; ev.Ztg = s->Ztg;
+        emit(&ev);
+    }
     return dg;
 }
 
@@ -584,7 +618,15 @@ static void bfgs_iteration(bfgs_state* s, const double* x, const double* g, cons
         gamma[k] = g[k] - g0[k];
         dTg += gamma[k] * delta[k];
     }
-    if (dTg < fabs(tmp) * 0.1) { free(delta); free(gamma); return; }
+    if (dTg < fabs(tmp) * 0.1) {
+        if (g_hook) {
+            or_bfgs_event ev = {0};
+            ev.kind = OR_EV_SKIP; ev.n = n; ev.dTg = dTg; ev.g0_dot_delta = tmp;
+            ev.delta = delta; ev.gamma = gamma; ev.g0 = g0; ev.Z = s->Z;
+            emit(&ev);
+        }
+        free(delta); free(gamma); return;
+    }
     dTg = fmax(num_precision, dTg);
 
     /* Givens rotations folding Z^T g into its first entry (Eq. 1.7 of Powell 1987). */
@@ -600,6 +642,11 @@ static void bfgs_iteration(bfgs_state* s, const double* x, const double* g, cons
             z[0] = cv * z[0] + sv * z[1];
             z[1] = t;
         }
+    }
+    double* Z_rot = 0;
+    if (g_hook) {
+        Z_rot = (double*)malloc(sizeof(double) * n * n);
+        memcpy(Z_rot, s->Z, sizeof(double) * n * n);
     }
     if (*norm_Z < 0.0) {
         *norm_Z = norm_delta / dTg;
@@ -628,15 +675,24 @@ static void bfgs_iteration(bfgs_state* s, const double* x, const double* g, cons
             for (long r = 0; r < n; r++) s->Z[r * n + c] *= sc;
         }
     }
+    if (g_hook) {
+        or_bfgs_event ev = {0};
+        ev.kind = OR_EV_UPDATE; ev.n = n; ev.dTg = dTg; ev.g0_dot_delta = 0.0;
+        for (long k = 0; k < n; k++) ev.g0_dot_delta += g0[k] * delta[k];
+        ev.norm_Z = *norm_Z; ev.delta = delta; ev.gamma = gamma; ev.g0 = g0; ev.Z = s->Z;
+        ev.Z_rot = Z_rot; ev.Ztg = s->Ztg;
+        emit(&ev);
+        free(Z_rot);
+    }
     free(gTZ); free(nz); free(delta); free(gamma);
 }
 
 /* BFGS_Powell::Optimize (BFGS_Powell.cpp:70-192) via Grad_Descend::Start_Optimization
  * (grad_descend.cpp:83-111). */
-double or_bfgs_powell(const or_problem* pr, double* x, long max_calls, long* calls, int* status) {
+double or_bfgs_powell_fn(or_fg_fn fn, void* user, long n, double* x, long max_calls, long* calls,
+                         int* status) {
     bfgs_state s;
-    const long n = pr->P;
-    s.pr = pr; s.n = n; s.calls = 0; s.max_calls = max_calls; s.status = ST_INITIAL;
+    s.fn = fn; s.user = user; s.n = n; s.calls = 0; s.max_calls = max_calls; s.status = ST_INITIAL;
     if (n <= 0) { if (calls) *calls = 0; if (status) *status = ST_INITIAL; return DBL_MAX; }
     s.Z = (double*)calloc((size_t)(n * n), sizeof(double));
     s.Ztg = (double*)calloc((size_t)n, sizeof(double));
@@ -647,6 +703,11 @@ double or_bfgs_powell(const or_problem* pr, double* x, long max_calls, long* cal
     double* x0 = (double*)calloc((size_t)n, sizeof(double));
     double* g0 = (double*)calloc((size_t)n, sizeof(double));
     double f, norm_Z = -1.0;
+    if (g_hook) {
+        or_bfgs_event ev = {0};
+        ev.kind = OR_EV_START; ev.n = n; ev.x = x;
+        emit(&ev);
+    }
     eval_fg(&s, x, &f, g);
     double fprev = fabs(f + f + 1.0);
     for (;;) {
@@ -665,10 +726,19 @@ double or_bfgs_powell(const or_problem* pr, double* x, long max_calls, long* cal
         if (s.status == ST_ZERO_STEP) break;
         bfgs_iteration(&s, x, g, x0, g0, &norm_Z);
     }
+    if (g_hook) {
+        or_bfgs_event ev = {0};
+        ev.kind = OR_EV_END; ev.n = n; ev.x = x; ev.f = f; ev.iter = s.status;
+        emit(&ev);
+    }
     if (calls) *calls = s.calls;
     if (status) *status = s.status;
     free(s.Z); free(s.Ztg); free(g); free(d); free(x0); free(g0);
     return f;
+}
+
+double or_bfgs_powell(const or_problem* pr, double* x, long max_calls, long* calls, int* status) {
+    return or_bfgs_powell_fn(problem_fg, (void*)pr, pr->P, x, max_calls, calls, status);
 }
 
 /* ------------------------------------------------------------------------- */
@@ -710,6 +780,8 @@ double or_mt_uniform_2pi(or_mt19937* g) {
     return u * (2.0 * M_PI - 0.0) + 0.0;
 }
 
+static unsigned int mt_uniform_range(or_mt19937* g, unsigned int range);
+
 /* Decomposition driver for a custom circuit optimised as one block:
  * initial guess (Decomposition_Base.cpp:415-446), outer loop with the 20-entry
  * convergence window (:492-675, 742-754), BFGS restarts with U[0,2pi)/100 jitter
@@ -721,6 +793,9 @@ double or_decompose_custom(const or_problem* pr, unsigned int seed, int initial_
     const int P = pr->P;
     or_mt19937 gen;
     or_mt_seed(&gen, seed);
+    /* The decomposition object's constructor draws its id from the same generator
+     * (Optimization_Interface.cpp:177-180: uniform_int_distribution<>(0, INT_MAX)). */
+    (void)mt_uniform_range(&gen, 2147483648u);
     double* opt = (double*)calloc((size_t)(P > 0 ? P : 1), sizeof(double));
     double* guess = (double*)calloc((size_t)(P > 0 ? P : 1), sizeof(double));
     for (int k = 0; k < P; k++) {
@@ -769,10 +844,10 @@ double or_decompose_custom(const or_problem* pr, unsigned int seed, int initial_
 
 /* ---- libstdc++ distributions ------------------------------------------------ */
 
-int or_mt_uniform_int(or_mt19937* g, int n) {
-    /* uniform_int_distribution::operator() with a 32-bit engine range: _S_nd<uint64>
-     * (Lemire, "Fast random integer generation in an interval", 2019). */
-    const unsigned int range = (unsigned int)n;
+/* uniform_int_distribution::operator() with a 32-bit engine range: _S_nd<uint64>
+ * (Lemire, "Fast random integer generation in an interval", 2019), for a range
+ * of 1 .. 2^31 values. */
+static unsigned int mt_uniform_range(or_mt19937* g, unsigned int range) {
     unsigned long long product = (unsigned long long)or_mt_next(g) * range;
     unsigned int low = (unsigned int)product;
     if (low < range) {
@@ -782,8 +857,10 @@ int or_mt_uniform_int(or_mt19937* g, int n) {
             low = (unsigned int)product;
         }
     }
-    return (int)(product >> 32);
+    return (unsigned int)(product >> 32);
 }
+
+int or_mt_uniform_int(or_mt19937* g, int n) { return (int)mt_uniform_range(g, (unsigned int)n); }
 
 double or_mt_canonical(or_mt19937* g) {
     const double r = 4294967296.0;

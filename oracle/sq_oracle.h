@@ -101,6 +101,38 @@ void or_optimization_problem_combined_batched(const or_problem* pr, const double
  * cost+gradient evaluations.  status codes mirror the reference's. */
 double or_bfgs_powell(const or_problem* pr, double* x, long max_calls, long* calls, int* status);
 
+/* The same optimiser on any objective: fn(user, x, &f, g) returns f(x) and its
+ * gradient (n entries). */
+typedef void (*or_fg_fn)(void* user, const double* x, double* f, double* g);
+double or_bfgs_powell_fn(or_fg_fn fn, void* user, long n, double* x, long max_calls, long* calls,
+                         int* status);
+
+/* Observer of the optimiser's internal steps, called on the calling thread (a test
+ * hook: the arrays are valid only during the call).
+ *   OR_EV_DIRECTION  d = -Z Z^T g was formed: g, d, Z (n x n row-major), Ztg = Z^T g, dg = d.g
+ *   OR_EV_PROBE      a line-search probe x0 + step*d: f, dg = d.g(x), f0, dg0, f_lowest
+ *                    before the Armijo test, max_step; exit_curvature = the probe ends the
+ *                    search by the Wolfe tests, exit_calls = the evaluation budget ended it
+ *   OR_EV_LS_END     the accepted step (the best probe), f, g at it, g0 at x0, d
+ *   OR_EV_UPDATE     Powell's update applied: delta, gamma, g0, dTg = delta.gamma (clamped
+ *                    below at 1.42e-14), g0_dot_delta, Z_rot = Z after the Givens sweep,
+ *                    Ztg = the rotated Z^T g0, Z = the updated factor, norm_Z
+ *   OR_EV_SKIP       the update was skipped (delta.gamma < 0.1 |g0.delta|)
+ *   OR_EV_START      a BFGS run starts at x
+ *   OR_EV_END        a BFGS run ended at x with cost f; iter holds its status        */
+enum { OR_EV_DIRECTION = 0, OR_EV_PROBE = 1, OR_EV_LS_END = 2, OR_EV_UPDATE = 3, OR_EV_SKIP = 4,
+       OR_EV_START = 5, OR_EV_END = 6 };
+typedef struct {
+    int kind;
+    long n, iter;
+    double step, f, dg, f0, dg0, f_lowest, max_step;
+    int exit_curvature, exit_calls;
+    double dTg, g0_dot_delta, norm_Z;
+    const double *g, *d, *g0, *delta, *gamma, *Z, *Z_rot, *Ztg, *x;
+} or_bfgs_event;
+typedef void (*or_bfgs_hook)(void* user, const or_bfgs_event* ev);
+void or_bfgs_set_hook(or_bfgs_hook hook, void* user);
+
 /* mt19937 + uniform_real_distribution(0, 2pi) as consumed by the reference's
  * decomposition driver (Decomposition_Base.cpp:32-42,426-436), libstdc++ semantics. */
 typedef struct { unsigned int mt[624]; int idx; } or_mt19937;
