@@ -186,6 +186,19 @@ typedef struct sqh_decompose_options {
     double switch_cost;               /* ... handing over to FP64 below this FP32 cost */
     int rounds_per_graph;             /* lockstep rounds per captured hipGraph (0: 16) */
     long long max_rounds;             /* safety cap on rounds (0: derived from the limits) */
+    /* BFGS iteration loops per outer iteration (BFGS.cpp:65-83,121-133; the reference
+     * runs 3 at 2 qubits, else 1, or config max_iteration_loops[_bfgs]); 0 means 1.
+     * After a loop improves the run's minimum the next loop starts from its end point
+     * plus the next restart-jitter vector. */
+    int iteration_loops;
+    /* Restart-jitter vectors per run (J), and the host array holding them: the
+     * reference's U[0,2pi)/100 draws in the order its generator yields them, J x P
+     * values per run, run after run (sqh_decompose_problems: problem i's J x P_i block
+     * after problem i-1's).  A run uses one vector per improvement; one that improves
+     * more than J times restarts its later loops without jitter.  NULL or J = 0: no
+     * jitter.  Read only when iteration_loops > 1. */
+    int restart_jitter_count;
+    const double* restart_jitter;
 } sqh_decompose_options;
 
 /* n_runs independent decompositions of the context's circuit, run r on target
@@ -193,7 +206,8 @@ typedef struct sqh_decompose_options {
  * its problem) from start point x0[r] (n_runs x P).  Each run is the reference's
  * driver for a custom circuit: BFGS_Powell with the Wolfe line search
  * (BFGS_Powell.cpp:70-386, grad_descend.cpp:127-272) restarted from the best point
- * (Decomposition_Base.cpp:375-709, BFGS.cpp:42-152, one iteration loop).  All runs
+ * (Decomposition_Base.cpp:375-709) with the engine's iteration loops
+ * (BFGS.cpp:42-152).  All runs
  * advance in lockstep on the GPU, one batched cost+gradient launch per round.
  * Outputs: best parameters, best FP64 cost, status of the last BFGS run (-1 if the
  * round cap stopped it), evaluations per run, and the number of rounds. */

@@ -54,10 +54,10 @@ struct Group {
 };
 
 struct BatchBufs {
-    DevBuf run_target, run_group, run_P, run_prog, x0, states, vecs, Z, fe, ge, xt, lists, counts;
+    DevBuf run_target, run_group, run_P, run_prog, x0, states, vecs, Z, fe, ge, xt, lists, counts, jitter;
     ~BatchBufs() {
         for (DevBuf* b : {&run_target, &run_group, &run_P, &run_prog, &x0, &states, &vecs, &Z, &fe, &ge, &xt, &lists,
-                          &counts})
+                          &counts, &jitter})
             b->release();
     }
 };
@@ -147,13 +147,24 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
     return SQH_OK;
 }
 
+int loops_of(const sqh_decompose_options* opts) { return opts->iteration_loops > 0 ? opts->iteration_loops : 1; }
+
+constexpr int kMaxJitter = 4096;  // restart-jitter vectors per run
+
+// Jitter vectors per run the optimiser reads (0 when it runs one loop).
+int jitter_count(const sqh_decompose_options* opts) {
+    return (loops_of(opts) > 1 && opts->restart_jitter) ? opts->restart_jitter_count : 0;
+}
+
 // Runs the optimiser over B runs in the given groups.  x0: B x Ps (row stride Ps);
+// jitter: B x J x Ps restart-jitter vectors (J = max_outer * loops) or empty;
 // run_target: index into the run's group targets; run_P / run_group: per run (null
 // for the single-circuit case).  Outputs follow sqh_decompose_batch, with x_out
 // rows of stride Ps.
 int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* run_target, const int* run_P,
-              const int* run_group, const double* x0, const sqh_decompose_options* opts, double* x_out,
-              int x_stride, double* f_out, int* status_out, long long* evals_out, long long* rounds_out) {
+              const int* run_group, const double* x0, const std::vector<double>& jitter,
+              const sqh_decompose_options* opts, double* x_out, int x_stride, double* f_out, int* status_out,
+              long long* evals_out, long long* rounds_out) {
     const bool grouped = run_group != nullptr;
     const bool mixed = opts->mixed_precision != 0;
     const int G = (int)groups.size();
@@ -208,11 +219,22 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
     o.conv_thr = opts->convergence_threshold;
     o.switch_cost = opts->switch_cost;
     o.P = Ps;
+    o.loops = loops_of(opts);
+    o.jitter_per_run = 0;
+    o.jitter = nullptr;
+    if (o.loops > 1 && !jitter.empty()) {
+        o.jitter_per_run = opts->restart_jitter_count;
+        HIPTRY(c, bb.jitter.ensure(jitter.size() * sizeof(double)));
+        HIPTRY(c, hipMemcpyAsync(bb.jitter.p, jitter.data(), jitter.size() * sizeof(double), hipMemcpyHostToDevice,
+                                 c->stream));
+        o.jitter = bb.jitter.as<double>();
+    }
 
     const int R = opts->rounds_per_graph > 0 ? opts->rounds_per_graph : 16;
     const long long max_rounds =
         opts->max_rounds > 0 ? opts->max_rounds
-                             : (long long)opts->max_outer_iterations * (opts->max_inner_iterations + 2) * (mixed ? 2 : 1);
+                             : (long long)opts->max_outer_iterations * loops_of(opts) *
+                                   (opts->max_inner_iterations + 2) * (mixed ? 2 : 1);
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (!grouped && jit_wants(c, 2)) {
@@ -278,7 +300,16 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
 int check_opts(sqh_ctx* c, const sqh_decompose_options* opts) {
     if (opts->max_inner_iterations <= 0 || opts->max_outer_iterations <= 0)
         return fail(c, SQH_ERR_ARG, "iteration limits must be positive");
+    if (opts->iteration_loops < 0) return fail(c, SQH_ERR_ARG, "iteration_loops must be >= 0");
+    if (opts->restart_jitter_count < 0 || opts->restart_jitter_count > kMaxJitter)
+        return fail(c, SQH_ERR_ARG, "restart_jitter_count must be 0..%d", kMaxJitter);
     return SQH_OK;
+}
+
+// The restart-jitter vectors of one run (J x P from src) at stride Ps in dst.
+void copy_jitter(std::vector<double>& dst, int r, int J, int Ps, const double* src, int P) {
+    for (int j = 0; j < J; ++j)
+        std::memcpy(&dst[((size_t)r * J + j) * Ps], src + (size_t)j * P, (size_t)P * sizeof(double));
 }
 
 // Uploads n_targets complex128 targets of a group and their complex64 copies.
@@ -341,8 +372,11 @@ extern "C" int sqh_decompose_batch(sqh_ctx* c, const double* targets, int n_targ
     std::vector<const double*> tp(n_targets);
     for (int i = 0; i < n_targets; ++i) tp[i] = targets + (size_t)i * rows * cols * 2;
     if ((rc = upload_targets(c, g, tp.data()))) return rc;
-    return run_batch(c, groups, n_runs, P, run_target, nullptr, nullptr, x0, opts, x_out, P, f_out, status_out,
-                     evals_out, rounds_out);
+    std::vector<double> jitter;
+    if (jitter_count(opts) > 0)
+        jitter.assign(opts->restart_jitter, opts->restart_jitter + (size_t)n_runs * jitter_count(opts) * P);
+    return run_batch(c, groups, n_runs, P, run_target, nullptr, nullptr, x0, jitter, opts, x_out, P, f_out,
+                     status_out, evals_out, rounds_out);
 }
 
 extern "C" int sqh_decompose_problems(sqh_ctx* c, const sqh_problem* problems, int n_problems,
@@ -444,14 +478,24 @@ extern "C" int sqh_decompose_problems(sqh_ctx* c, const sqh_problem* problems, i
         if ((rc = upload_targets(c, g, tp.data()))) return rc;
         ++gi;
     }
+    std::vector<double> jitter;
+    if (jitter_count(opts) > 0) {
+        const int J = jitter_count(opts);
+        jitter.assign((size_t)n_problems * J * Ps, 0.0);
+        size_t off = 0;
+        for (int i = 0; i < n_problems; ++i) {
+            copy_jitter(jitter, i, J, Ps, opts->restart_jitter + off, progs[i].P);
+            off += (size_t)J * progs[i].P;
+        }
+    }
     DevBuf d_refs;
     HIPTRY(c, d_refs.ensure((size_t)n_problems * sizeof(ProgRef)));
     HIPTRY(c, hipMemcpyAsync(d_refs.p, refs.data(), (size_t)n_problems * sizeof(ProgRef), hipMemcpyHostToDevice,
                              c->stream));
     for (Group& g : groups) g.run_prog = d_refs.as<ProgRef>();
     std::vector<double> xo((size_t)n_problems * Ps);
-    rc = run_batch(c, groups, n_problems, Ps, run_target.data(), run_P.data(), run_group.data(), x0.data(), opts,
-                   xo.data(), Ps, f_out, status_out, evals_out, rounds_out);
+    rc = run_batch(c, groups, n_problems, Ps, run_target.data(), run_P.data(), run_group.data(), x0.data(), jitter,
+                   opts, xo.data(), Ps, f_out, status_out, evals_out, rounds_out);
     if (rc == SQH_OK) {
         size_t o = 0;
         for (int i = 0; i < n_problems; ++i) {

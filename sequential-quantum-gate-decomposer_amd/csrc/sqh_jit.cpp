@@ -432,6 +432,21 @@ static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni, const Jit
     for (auto& e : jc.entries)
         if (e.prog_version == c->prog_version && e.kind == kind && e.f32 == f32 && e.uni == uni && e.shape == sh)
             return e.fn;
+    // Kernels of circuits the context no longer holds are unloaded once the stream
+    // that ran them is idle.
+    bool stale = false;
+    for (auto& e : jc.entries) stale = stale || e.prog_version != c->prog_version;
+    if (stale && hipStreamSynchronize(c->stream) == hipSuccess) {
+        std::vector<JitEntry> keep;
+        for (auto& e : jc.entries) {
+            if (e.prog_version == c->prog_version) {
+                keep.push_back(e);
+            } else if (e.mod) {
+                (void)hipModuleUnload(e.mod);
+            }
+        }
+        jc.entries.swap(keep);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     hipDeviceProp_t prop;
     std::string arch = "gfx950";

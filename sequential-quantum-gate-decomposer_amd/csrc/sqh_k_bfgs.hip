@@ -58,7 +58,8 @@ __global__ void k_bfgs_init(RunState* __restrict__ st, double* __restrict__ vecs
         S.ls_iter = 0;
         S.finished = 0;
         S.P = P;
-        S.pad1 = 0;
+        S.loop = 0;
+        S.jit_used = S.pad1 = S.pad2 = S.pad3 = 0;
         S.calls = S.total_calls = S.calls32 = 0;
         S.f = S.fprev = 0.0;
         S.norm_Z = -1.0;
@@ -239,10 +240,15 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
     bool dir_ready = false;  // the update already formed d = -Z Z^T g
     for (int guard = 0; guard < 16 && !request && S.phase != PH_FINISHED; ++guard) {
         switch (S.phase) {
-        case PH_OUTER_START: {
+        case PH_OUTER_START:
+        case PH_RUN_START: {
+            // an outer iteration starts from the best point, a further iteration loop
+            // from where the last run ended (jittered if it improved the minimum)
+            const bool outer = S.phase == PH_OUTER_START;
             for (int k = tid; k < P; k += nt) {
-                x[k] = best[k];
-                xr[k] = best[k];
+                const double v = outer ? best[k] : x[k];
+                x[k] = v;
+                xr[k] = v;
             }
             for (size_t i = tid; i < (size_t)P * P; i += nt) Z[i] = (i / P == i % P) ? 1.0 : 0.0;
             S.calls = 0;
@@ -413,10 +419,27 @@ __global__ __launch_bounds__(kBfgsThreads) void k_bfgs_advance(RunState* __restr
                 __syncthreads();
                 break;
             }
+            // BFGS.cpp:121-133: keep an improvement and jitter the guess by the next
+            // U[0,2pi)/100 draws (consumed even when no further loop follows)
             if (S.cur_min > S.f) {
                 S.cur_min = S.f;
-                for (int k = tid; k < P; k += nt) best[k] = x[k];
+                const bool more = S.loop + 1 < o.loops;
+                const double* jt = (o.jitter && S.jit_used < o.jitter_per_run)
+                                       ? o.jitter + ((size_t)r * o.jitter_per_run + S.jit_used) * Ps
+                                       : nullptr;
+                for (int k = tid; k < P; k += nt) {
+                    best[k] = x[k];
+                    if (more && jt) x[k] = x[k] + jt[k];
+                }
+                S.jit_used++;
             }
+            S.loop++;
+            if (S.loop < o.loops) {
+                S.phase = PH_RUN_START;
+                __syncthreads();
+                break;
+            }
+            S.loop = 0;
             double mean = 0.0;
             for (int i = kMinWindow - 1; i > 0; --i) {
                 S.minvec[i] = S.minvec[i - 1];

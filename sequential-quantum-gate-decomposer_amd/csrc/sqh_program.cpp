@@ -211,11 +211,22 @@ bool compile_program(int N, const std::vector<GateRec>& gates, Program& prog, st
         }
         P += np;
     }
+    // The gates' parameter ranges must tile [0, P) without overlap, as
+    // Gates_block::add_gate assigns them (Gates_block.cpp:2500-2525): a shared index
+    // would take only one gate's gradient term, an uncovered one none.
+    std::vector<uint8_t> cover((size_t)P, 0);
     for (size_t gi = 0; gi < gates.size(); ++gi) {
         const int np = param_num_of(gates[gi].type);
         if (np > 0 && gates[gi].param_start + np > P) {
             err = "parameter start index beyond the parameter count at gate " + std::to_string(gi);
             return false;
+        }
+        for (int k = 0; k < np; ++k) {
+            if (cover[(size_t)gates[gi].param_start + k]++) {
+                err = "parameter index " + std::to_string(gates[gi].param_start + k) + " shared by two gates (gate " +
+                      std::to_string(gi) + ")";
+                return false;
+            }
         }
     }
     prog.P = P;

@@ -53,7 +53,9 @@ class DecomposeOptions(ctypes.Structure):
     _fields_ = [("max_inner_iterations", ctypes.c_longlong), ("max_outer_iterations", ctypes.c_int),
                 ("optimization_tolerance", ctypes.c_double), ("convergence_threshold", ctypes.c_double),
                 ("mixed_precision", ctypes.c_int), ("switch_cost", ctypes.c_double),
-                ("rounds_per_graph", ctypes.c_int), ("max_rounds", ctypes.c_longlong)]
+                ("rounds_per_graph", ctypes.c_int), ("max_rounds", ctypes.c_longlong),
+                ("iteration_loops", ctypes.c_int), ("restart_jitter_count", ctypes.c_int),
+                ("restart_jitter", ctypes.c_void_p)]
 
 
 class SqhError(Exception):

@@ -56,6 +56,11 @@ class N_Qubit_Decomposition_custom:
         self.decomposition_error = None
         self.optimization_tolerance = float(self.config.get("optimization_tolerance", 1e-7))
         self.max_iterations = None
+        # N_Qubit_Decomposition_custom.cpp:63-75: BFGS with 4 outer iterations up to 5
+        # qubits, ADAM with one above (ADAM is not built here: Start_Decomposition
+        # then raises unless set_Optimizer("BFGS") was called).
+        self.optimizer = "BFGS" if n <= 5 else "ADAM"
+        self.max_outer_iterations = 4 if n <= 5 else 1
         self._ctx = nat.Context(0)
         self._ctx.set_target(U)
         # Circuit-specialised kernels (compiled on first use per circuit: about 0.6 s
@@ -99,6 +104,22 @@ class N_Qubit_Decomposition_custom:
         correction1_scale and correction2_scale (Optimization_Interface.cpp:74-76; the
         reference sets prev_cost_fnv_val only from its ADAM engines, ADAM.cpp:201)."""
         self._ctx.set_correction_weights(prev_cost, scale1, scale2)
+
+    def set_Optimizer(self, optimizer="BFGS"):
+        """Optimization_Interface::set_optimizer (Optimization_Interface.cpp:1620-1697,
+        names as in the wrapper, qgd_N_Qubit_Decompositions_Wrapper.cpp:1197-1260).  BFGS
+        sets the reference's BFGS limits (10000 evaluations per run, 1e8 outer
+        iterations); the other engines are not built here and make Start_Decomposition
+        raise."""
+        name = str(optimizer).upper()
+        known = ("BFGS", "ADAM", "GRAD_DESCEND", "ADAM_BATCHED", "COSINE", "GRAD_DESCEND_PARAMETER_SHIFT_RULE",
+                 "AGENTS", "AGENTS_COMBINED", "BFGS2", "BAYES_OPT", "BAYES_AGENTS")
+        if name not in known:
+            raise Exception(f"set_Optimizer: unknown optimizer {optimizer}")
+        self.optimizer = name
+        if name == "BFGS":
+            self.max_outer_iterations = 100000000
+            self.max_iterations = 10000
 
     def set_Optimization_Tolerance(self, tol):
         self.optimization_tolerance = float(tol)
@@ -183,26 +204,41 @@ class N_Qubit_Decomposition_custom:
             cfg.setdefault("max_inner_iterations", self.max_iterations)
         cfg.setdefault("optimization_tolerance", self.optimization_tolerance)
         keys = ("max_inner_iterations", "max_outer_iterations", "optimization_tolerance", "convergence_threshold",
-                "mixed_precision", "switch_cost", "rounds_per_graph", "max_rounds")
+                "mixed_precision", "switch_cost", "rounds_per_graph", "max_rounds", "iteration_loops")
         if "max_inner_iterations_bfgs" in cfg:
             cfg["max_inner_iterations"] = cfg["max_inner_iterations_bfgs"]
+        cfg.setdefault("max_outer_iterations", self.max_outer_iterations)
         return default_options(self.qbit_num, **{k: cfg[k] for k in keys if k in cfg})
 
     def Start_Decomposition(self, **kw):
         """Decompose Umtx with the gate structure (N_Qubit_Decomposition_custom::
         start_decomposition, N_Qubit_Decomposition_custom.cpp:116-190): initial guess from
         config ``random_seed`` (Decomposition_Base.cpp:415-446), BFGS restarted from the best
-        point, all on the GPU.  Extra keyword options: mixed_precision, switch_cost."""
-        from .optimizer import decompose_batch, initial_guess
+        point with the engine's iteration loops and restart jitter (BFGS.cpp:42-152: 3 loops
+        at 2 qubits, config max_iteration_loops[_bfgs]), all on the GPU.  Extra keyword
+        options: mixed_precision, switch_cost.  Above 5 qubits the reference runs ADAM
+        (N_Qubit_Decomposition_custom.cpp:63-75), which is not built here: the call raises
+        unless set_Optimizer("BFGS") chose BFGS."""
+        from .optimizer import MAX_JITTER_OUTER, decompose_batch, initial_guess, iteration_loops_for, restart_jitter
         if self.circuit is None:
             raise Exception("no gate structure set")
+        if self.optimizer != "BFGS":
+            raise Exception(f"Start_Decomposition: the {self.optimizer} optimizer (the reference's choice above 5 "
+                            f"qubits) is not built on the GPU; call set_Optimizer('BFGS') to decompose with BFGS")
         P = self.get_Parameter_Num()
+        seed = int(self.config.get("random_seed", np.random.SeedSequence().entropy & 0xFFFFFFFF))
         if self.optimized_parameters is not None and self.optimized_parameters.size == P:
             x0 = self.optimized_parameters.copy()
         else:
-            seed = int(self.config.get("random_seed", np.random.SeedSequence().entropy & 0xFFFFFFFF))
             x0 = initial_guess(P, seed, self.initial_guess)
-        res = decompose_batch(self._ctx, self.Umtx, np.zeros(1, dtype=np.int32), x0[None, :], self._options(**kw))
+        opts = self._options(**kw)
+        loops = iteration_loops_for(self.qbit_num, self.config)
+        opts.iteration_loops = loops
+        jitter = None
+        if loops > 1:
+            J = min(int(opts.max_outer_iterations), MAX_JITTER_OUTER) * loops
+            jitter = restart_jitter(P, seed, self.initial_guess, J)[None]
+        res = decompose_batch(self._ctx, self.Umtx, np.zeros(1, dtype=np.int32), x0[None, :], opts, jitter)
         self.optimized_parameters = res["x"][0].copy()
         self.num_iters += int(res["evals"][0]) * (P + 1)
         self.decomposition_error = float(res["f"][0])

@@ -56,6 +56,17 @@ def test_program_compiler_rejects_bad_gates():
         nat.inspect_program(3, [(nat.GATE_U3, 0, -1, 5)])          # parameters out of range
 
 
+def test_program_compiler_rejects_overlapping_parameter_ranges():
+    """Parameter ranges must tile [0, P) as Gates_block assigns them
+    (Gates_block.cpp:2500-2525): a shared index is rejected."""
+    with pytest.raises(nat.SqhError):   # RY reads index 2, U3 reads 0..2: 4 parameters, index 3 unread
+        nat.inspect_program(2, [(nat.GATE_U3, 0, -1, 0), (nat.GATE_RY, 1, -1, 2)])
+    with pytest.raises(nat.SqhError):
+        nat.inspect_program(2, [(nat.GATE_RX, 0, -1, 0), (nat.GATE_RY, 1, -1, 0)])
+    ops, _ = nat.inspect_program(2, [(nat.GATE_RY, 1, -1, 3), (nat.GATE_U3, 0, -1, 0)])  # any order is fine
+    assert len(ops) > 0
+
+
 def test_program_for_small_circuits_has_no_remaps():
     ops, lays = nat.inspect_program(4, W.ansatz_4q_12())
     assert len(lays) == 1 and len(ops) == len(W.ansatz_4q_12())

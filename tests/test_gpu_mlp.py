@@ -24,7 +24,7 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-30))
 
 
-@pytest.mark.parametrize("dims", [(64, 128, 2, 40), (512, 256, 3, 168)])
+@pytest.mark.parametrize("dims", [(64, 128, 2, 40), (512, 256, 3, 168), (512, 1024, 3, 168)])
 def test_forward_and_gradients_match_oracle(dims):
     say(f"mlp dims {dims}")
     ind, hid, nh, out = dims
