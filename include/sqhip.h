@@ -199,7 +199,18 @@ typedef struct sqh_decompose_options {
      * jitter.  Read only when iteration_loops > 1. */
     int restart_jitter_count;
     const double* restart_jitter;
+    /* Execution engine: SQH_ENGINE_AUTO picks the persistent solver when every run has
+     * at most 4 qubits and the cost variant has no correction channels, else lockstep
+     * rounds (the persistent solver serves up to 6 qubits when asked for);
+     * SQH_ENGINE_ROUNDS forces the lockstep rounds; SQH_ENGINE_PERSISTENT requires the
+     * persistent solver (an error where it does not apply).  Both run the same
+     * optimiser; results agree to rounding. */
+    int engine;
 } sqh_decompose_options;
+
+#define SQH_ENGINE_AUTO 0
+#define SQH_ENGINE_ROUNDS 1        /* every run advances one evaluation per lockstep round */
+#define SQH_ENGINE_PERSISTENT 2    /* one wavefront per run, the whole optimisation in one launch */
 
 /* n_runs independent decompositions of the context's circuit, run r on target
  * run_target[r] (targets: n_targets x rows x cols complex128, each the Umtx of

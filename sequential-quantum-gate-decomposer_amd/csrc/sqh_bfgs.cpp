@@ -17,6 +17,7 @@
 
 #include "sqh_bfgs_device.h"
 #include "sqh_context.h"
+#include "sqh_solve.h"
 
 namespace sqh {
 
@@ -149,11 +150,117 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
 
 int loops_of(const sqh_decompose_options* opts) { return opts->iteration_loops > 0 ? opts->iteration_loops : 1; }
 
+// The persistent solver (sqh_k_solve.hip) serves up to this many qubits, the cost
+// variants without correction channels, and runs capped at this many evaluations.
+constexpr int kSolveMaxQubits = 6;
+constexpr int kSolveAutoQubits = 4;
+constexpr long long kSolveMaxEvals = 1000000;
+constexpr size_t kSolveMaxLds = 64 * 1024;
+
+// Whether the persistent solver can run these groups ("" if so, else the reason).
+const char* persistent_blocker(sqh_ctx* c, const std::vector<Group>& groups, int Ps, long long max_evals) {
+    if (correction_channels(c->variant) > 0) return "the correction cost variants need the lockstep rounds";
+    if (max_evals > kSolveMaxEvals) return "the iteration limits allow more evaluations per run than it runs";
+    for (const Group& g : groups) {
+        if (g.N > kSolveMaxQubits) return "it serves up to 6 qubits";
+        if (solve_lds_bytes(g.N, Ps, std::max(g.ns, 1)) > kSolveMaxLds) return "the problem needs too much LDS";
+    }
+    return "";
+}
+
+// Every run to completion with one persistent launch per qubit-count group (the
+// groups' launches run side by side on forked streams).
+int solve_persistent(sqh_ctx* c, std::vector<Group>& groups, BatchBufs& bb, const BfgsOpts& o, int B,
+                     const int* run_group, bool grouped, long long max_evals) {
+    const int G = (int)groups.size();
+    std::vector<std::vector<int>> queue(G);
+    for (int r = 0; r < B; ++r) queue[run_group ? run_group[r] : 0].push_back(r);
+    std::vector<int> flat;
+    std::vector<size_t> first(G);
+    for (int g = 0; g < G; ++g) {
+        first[g] = flat.size();
+        flat.insert(flat.end(), queue[g].begin(), queue[g].end());
+    }
+    DevBuf d_queue, d_next;
+    HIPTRY(c, d_queue.ensure(flat.size() * sizeof(int)));
+    HIPTRY(c, d_next.ensure((size_t)G * sizeof(int)));
+    HIPTRY(c, hipMemcpyAsync(d_queue.p, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(c, hipMemsetAsync(d_next.p, 0, (size_t)G * sizeof(int), c->stream));
+    const bool fork = G > 1;
+    if (fork) {
+        HIPTRY(c, hipEventRecord(c->side_ev[0], c->stream));
+        for (int g = 0; g < G; ++g) HIPTRY(c, hipStreamWaitEvent(c->side[g], c->side_ev[0], 0));
+    }
+    for (int g = 0; g < G; ++g) {
+        Group& gr = groups[g];
+        if (queue[g].empty()) continue;
+        SolveArgs a;
+        a.next = d_next.as<int>() + g;
+        a.n_runs = (int)queue[g].size();
+        a.queue = d_queue.as<int>() + first[g];
+        a.ops = gr.ops;
+        a.nops = gr.nops;
+        a.lays = gr.lays;
+        a.slots = gr.slots;
+        a.ns = gr.ns;
+        a.ns_max = std::max(gr.ns, 1);
+        a.run_prog = gr.run_prog;
+        a.run_P = grouped ? bb.run_P.as<int>() : nullptr;
+        a.run_target = bb.run_target.as<int>();
+        a.tgt64 = gr.tgt64.p;
+        a.tgt32 = gr.tgt32.p;
+        a.tgt_stride = (long long)gr.rows * gr.cols;
+        a.m = gr.cols;
+        a.trace_off = uses_offset(c->variant) && !grouped ? c->trace_offset : 0;
+        a.variant = c->variant;
+        a.states = static_cast<RunState*>(bb.states.p);
+        a.vecs = bb.vecs.as<double>();
+        a.Zs = bb.Z.as<double>();
+        a.x0 = bb.x0.as<double>();
+        a.max_evals = max_evals;
+        a.o = o;
+        const int grid = std::min(a.n_runs, 4096);
+        HIPTRY(c, launch_solve(gr.N, a, grid, fork ? c->side[g] : c->stream));
+    }
+    if (fork) {
+        for (int g = 0; g < G; ++g) {
+            HIPTRY(c, hipEventRecord(c->side_ev[1 + g], c->side[g]));
+            HIPTRY(c, hipStreamWaitEvent(c->stream, c->side_ev[1 + g], 0));
+        }
+    }
+    HIPTRY(c, hipStreamSynchronize(c->stream));
+    return SQH_OK;
+}
+
 constexpr int kMaxJitter = 4096;  // restart-jitter vectors per run
 
 // Jitter vectors per run the optimiser reads (0 when it runs one loop).
 int jitter_count(const sqh_decompose_options* opts) {
     return (loops_of(opts) > 1 && opts->restart_jitter) ? opts->restart_jitter_count : 0;
+}
+
+// Reads the runs' results back.  rounds < 0: report the largest evaluation count of a
+// run (the persistent solver has no lockstep rounds).
+int collect(sqh_ctx* c, BatchBufs& bb, int B, int Ps, double* x_out, int x_stride, double* f_out, int* status_out,
+            long long* evals_out, long long* rounds_out, long long rounds) {
+    std::vector<char> st((size_t)B * bfgs_state_bytes());
+    HIPTRY(c, hipMemcpy(st.data(), bb.states.p, st.size(), hipMemcpyDeviceToHost));
+    std::vector<double> vecs((size_t)B * RV_COUNT * Ps);
+    HIPTRY(c, hipMemcpy(vecs.data(), bb.vecs.p, vecs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    long long most = 0;
+    for (int r = 0; r < B; ++r) {
+        RunState S;
+        std::memcpy(&S, st.data() + (size_t)r * bfgs_state_bytes(), sizeof(RunState));
+        std::memcpy(x_out + (size_t)r * x_stride, vecs.data() + ((size_t)r * RV_COUNT + RV_BEST) * Ps,
+                    (size_t)S.P * sizeof(double));
+        f_out[r] = S.cur_min;
+        const long long ev = S.total_calls + (S.finished ? 0 : S.calls);
+        if (status_out) status_out[r] = S.finished ? S.status : -1;
+        if (evals_out) evals_out[r] = ev;
+        most = std::max(most, ev);
+    }
+    if (rounds_out) *rounds_out = rounds >= 0 ? rounds : most;
+    return SQH_OK;
 }
 
 // Runs the optimiser over B runs in the given groups.  x0: B x Ps (row stride Ps);
@@ -235,6 +342,22 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
         opts->max_rounds > 0 ? opts->max_rounds
                              : (long long)opts->max_outer_iterations * loops_of(opts) *
                                    (opts->max_inner_iterations + 2) * (mixed ? 2 : 1);
+    const char* blocker = persistent_blocker(c, groups, Ps, max_rounds);
+    if (opts->engine == SQH_ENGINE_PERSISTENT && *blocker)
+        return fail(c, SQH_ERR_ARG, "persistent solver unavailable: %s", blocker);
+    // Automatic choice: the persistent solver where it measured faster (every run on at
+    // most kSolveAutoQubits qubits: one wavefront holds the whole matrix); above that
+    // one wavefront per run evaluates column slabs in turn and the lockstep rounds,
+    // which spread a run's evaluation over several wavefronts, are faster.
+    bool auto_persistent = true;
+    for (const Group& g : groups) auto_persistent = auto_persistent && g.N <= kSolveAutoQubits;
+    const bool persistent = opts->engine == SQH_ENGINE_PERSISTENT ||
+                            (opts->engine == SQH_ENGINE_AUTO && !*blocker && auto_persistent);
+    if (persistent) {
+        int rc = solve_persistent(c, groups, bb, o, B, run_group, grouped, max_rounds);
+        if (rc) return rc;
+        return collect(c, bb, B, Ps, x_out, x_stride, f_out, status_out, evals_out, rounds_out, -1);
+    }
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     if (!grouped && jit_wants(c, 2)) {
@@ -280,21 +403,7 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
     if (h_counts) (void)hipHostFree(h_counts);
     if (e != hipSuccess) return fail(c, SQH_ERR_DEVICE, "batched optimisation: %s", hipGetErrorString(e));
 
-    std::vector<char> st((size_t)B * bfgs_state_bytes());
-    HIPTRY(c, hipMemcpy(st.data(), bb.states.p, st.size(), hipMemcpyDeviceToHost));
-    std::vector<double> vecs((size_t)B * RV_COUNT * Ps);
-    HIPTRY(c, hipMemcpy(vecs.data(), bb.vecs.p, vecs.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int r = 0; r < B; ++r) {
-        RunState S;
-        std::memcpy(&S, st.data() + (size_t)r * bfgs_state_bytes(), sizeof(RunState));
-        std::memcpy(x_out + (size_t)r * x_stride, vecs.data() + ((size_t)r * RV_COUNT + RV_BEST) * Ps,
-                    (size_t)S.P * sizeof(double));
-        f_out[r] = S.cur_min;
-        if (status_out) status_out[r] = S.finished ? S.status : -1;
-        if (evals_out) evals_out[r] = S.total_calls + (S.finished ? 0 : S.calls);
-    }
-    if (rounds_out) *rounds_out = rounds;
-    return SQH_OK;
+    return collect(c, bb, B, Ps, x_out, x_stride, f_out, status_out, evals_out, rounds_out, rounds);
 }
 
 int check_opts(sqh_ctx* c, const sqh_decompose_options* opts) {

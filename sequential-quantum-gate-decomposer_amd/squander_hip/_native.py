@@ -55,7 +55,7 @@ class DecomposeOptions(ctypes.Structure):
                 ("mixed_precision", ctypes.c_int), ("switch_cost", ctypes.c_double),
                 ("rounds_per_graph", ctypes.c_int), ("max_rounds", ctypes.c_longlong),
                 ("iteration_loops", ctypes.c_int), ("restart_jitter_count", ctypes.c_int),
-                ("restart_jitter", ctypes.c_void_p)]
+                ("restart_jitter", ctypes.c_void_p), ("engine", ctypes.c_int)]
 
 
 class SqhError(Exception):

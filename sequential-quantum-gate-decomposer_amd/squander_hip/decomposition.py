@@ -204,7 +204,7 @@ class N_Qubit_Decomposition_custom:
             cfg.setdefault("max_inner_iterations", self.max_iterations)
         cfg.setdefault("optimization_tolerance", self.optimization_tolerance)
         keys = ("max_inner_iterations", "max_outer_iterations", "optimization_tolerance", "convergence_threshold",
-                "mixed_precision", "switch_cost", "rounds_per_graph", "max_rounds", "iteration_loops")
+                "mixed_precision", "switch_cost", "rounds_per_graph", "max_rounds", "iteration_loops", "engine")
         if "max_inner_iterations_bfgs" in cfg:
             cfg["max_inner_iterations"] = cfg["max_inner_iterations_bfgs"]
         cfg.setdefault("max_outer_iterations", self.max_outer_iterations)

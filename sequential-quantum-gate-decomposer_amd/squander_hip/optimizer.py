@@ -86,6 +86,7 @@ def default_options(qbit_num: int, **kw) -> nat.DecomposeOptions:
     o.rounds_per_graph = int(kw.get("rounds_per_graph", 16))
     o.max_rounds = int(kw.get("max_rounds", 0))
     o.iteration_loops = int(kw.get("iteration_loops", 1))
+    o.engine = {"auto": 0, "rounds": 1, "persistent": 2}[str(kw.get("engine", "auto"))]
     o.restart_jitter_count = 0
     o.restart_jitter = None
     return o

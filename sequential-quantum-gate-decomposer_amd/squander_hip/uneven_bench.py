@@ -45,10 +45,10 @@ def make_problems(count=256, seed=2028, qubits=(3, 6), cnots=(4, 20), matrix=Non
     return out
 
 
-def run(count=256, seed=2028, tol=1e-8, mixed=True, log=print):
+def run(count=256, seed=2028, tol=1e-8, mixed=True, log=print, engine="auto"):
     ctx = nat.Context(0)
     probs = make_problems(count, seed)
-    opts = default_options(4, optimization_tolerance=tol, mixed_precision=mixed, switch_cost=1e-4)
+    opts = default_options(4, optimization_tolerance=tol, mixed_precision=mixed, switch_cost=1e-4, engine=engine)
     t0 = time.perf_counter()
     res = decompose_problems(ctx, [(n, g, U, x0) for n, g, U, x0, _ in probs], opts)
     dt = time.perf_counter() - t0
@@ -56,7 +56,7 @@ def run(count=256, seed=2028, tol=1e-8, mixed=True, log=print):
     ok = res["f"] <= tol
     out = {"problems": count, "seconds": dt, "solves_per_s": float(ok.sum() / dt), "rounds": res["rounds"],
            "evals_p50": float(np.median(res["evals"])), "evals_p99": float(np.percentile(res["evals"], 99)),
-           "mixed_precision": bool(mixed)}
+           "mixed_precision": bool(mixed), "engine": engine}
     for k in ("reachable", "haar"):
         sel = kinds == k
         out[k] = {"count": int(sel.sum()), "success": float(ok[sel].mean()),
@@ -77,6 +77,8 @@ if __name__ == "__main__":
     ap.add_argument("--count", type=int, default=256)
     ap.add_argument("--seed", type=int, default=2028)
     ap.add_argument("--fp64", action="store_true", help="FP64 throughout (default: complex64 descent)")
+    ap.add_argument("--engine", default="auto", choices=["auto", "rounds", "persistent"])
     a = ap.parse_args()
-    res, _, _ = run(count=a.count, seed=a.seed, mixed=not a.fp64, log=lambda *m: print(*m, file=sys.stderr))
+    res, _, _ = run(count=a.count, seed=a.seed, mixed=not a.fp64, log=lambda *m: print(*m, file=sys.stderr),
+                    engine=a.engine)
     print(json.dumps(res))

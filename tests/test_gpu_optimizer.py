@@ -49,12 +49,18 @@ def reachable_targets(n, gates, count, seed):
     return np.array(Ts)
 
 
-@pytest.mark.parametrize("n,layers,loops", [(2, 3, 3), (2, 2, 1), (3, 4, 1), (3, 8, 1), (3, 4, 2)])
-def test_batch_matches_oracle_driver(oracle, n, layers, loops):
+ENGINES = ["persistent", "rounds"]
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+@pytest.mark.parametrize("n,layers,loops", [(2, 3, 3), (2, 2, 1), (3, 4, 1), (3, 8, 1), (3, 4, 2), (5, 3, 1)])
+def test_batch_matches_oracle_driver(oracle, n, layers, loops, engine):
     """Iteration loops: 3 at 2 qubits is the reference's default (BFGS.cpp:65-83 with
-    Optimization_Interface.cpp:149); 2 at 3 qubits stands for config max_iteration_loops."""
+    Optimization_Interface.cpp:149); 2 at 3 qubits stands for config max_iteration_loops.
+    Both engines: the persistent solver (one wavefront per run, one launch) and the
+    lockstep rounds."""
     from squander_hip.optimizer import restart_jitter
-    say(f"decompose_batch vs oracle n={n} layers={layers} loops={loops}")
+    say(f"decompose_batch vs oracle n={n} layers={layers} loops={loops} engine={engine}")
     gates = W.ansatz_layers(n, W.alternating_pairs(n, layers))
     P = W.param_count(gates)
     T = reachable_targets(n, gates, 6, 100 + n)
@@ -63,7 +69,8 @@ def test_batch_matches_oracle_driver(oracle, n, layers, loops):
     jitter = np.array([restart_jitter(P, s, "RANDOM", 4 * loops) for s in seeds]) if loops > 1 else None
     ctx = nat.Context(0)
     ctx.set_circuit(n, gates)
-    opts = default_options(n, optimization_tolerance=1e-8, max_outer_iterations=4, iteration_loops=loops)
+    opts = default_options(n, optimization_tolerance=1e-8, max_outer_iterations=4, iteration_loops=loops,
+                           engine=engine)
     res = decompose_batch(ctx, T, np.arange(6, dtype=np.int32), x0, opts, jitter)
     pairs = []
     for r in range(6):
@@ -149,8 +156,9 @@ def test_decompose_ibm_challenge_fixture(oracle):
     assert err < 1e-3
 
 
-def test_mixed_precision_batch_4q(oracle):
-    say("mixed-precision batch 4q")
+@pytest.mark.parametrize("engine", ENGINES)
+def test_mixed_precision_batch_4q(oracle, engine):
+    say(f"mixed-precision batch 4q engine={engine}")
     n = 4
     gates = W.ansatz_4q_12()
     P = W.param_count(gates)
@@ -160,9 +168,10 @@ def test_mixed_precision_batch_4q(oracle):
     run_target = np.repeat(np.arange(16, dtype=np.int32), 2)
     ctx = nat.Context(0)
     ctx.set_circuit(n, gates)
-    res64 = decompose_batch(ctx, T, run_target, x0, default_options(n, optimization_tolerance=1e-8))
+    res64 = decompose_batch(ctx, T, run_target, x0, default_options(n, optimization_tolerance=1e-8, engine=engine))
     resmx = decompose_batch(ctx, T, run_target, x0,
-                            default_options(n, optimization_tolerance=1e-8, mixed_precision=True, switch_cost=1e-4))
+                            default_options(n, optimization_tolerance=1e-8, mixed_precision=True, switch_cost=1e-4,
+                                            engine=engine))
     for res in (res64, resmx):
         for r in range(32):
             assert abs(oracle.optimization_problem(gates, T[run_target[r]], n, res["x"][r]) - res["f"][r]) < 1e-12
@@ -172,9 +181,10 @@ def test_mixed_precision_batch_4q(oracle):
     assert s64 > 0 and smx > 0
 
 
-def test_decompose_problems_matches_oracle_driver(oracle):
-    """Problems with their own circuits and qubit counts in one lockstep batch, each
-    compared with the oracle's driver from the same mt19937 start point."""
+@pytest.mark.parametrize("engine", ENGINES)
+def test_decompose_problems_matches_oracle_driver(oracle, engine):
+    """Problems with their own circuits and qubit counts in one batch, each compared
+    with the oracle's driver from the same mt19937 start point."""
     from squander_hip import decompose_problems
     specs = [(2, 2, 1), (3, 4, 2), (4, 3, 3), (5, 2, 4), (3, 6, 5), (4, 5, 6), (2, 3, 7), (5, 4, 8), (6, 2, 9)]
     probs, ref = [], []
@@ -186,7 +196,8 @@ def test_decompose_problems_matches_oracle_driver(oracle):
         probs.append((n, gates, T, initial_guess(P, seed)))
         ref.append(oracle.decompose_custom(gates, T, n, seed, max_outer_iterations=4, optimization_tolerance=1e-8))
     ctx = nat.Context(0)
-    res = decompose_problems(ctx, probs, default_options(4, optimization_tolerance=1e-8, max_outer_iterations=4))
+    res = decompose_problems(ctx, probs, default_options(4, optimization_tolerance=1e-8, max_outer_iterations=4,
+                                                         engine=engine))
     pairs = []
     for i, ((n, gates, T, _), (fo, xo, co)) in enumerate(zip(probs, ref)):
         fg = res["f"][i]
@@ -198,7 +209,8 @@ def test_decompose_problems_matches_oracle_driver(oracle):
     check_against_oracle(pairs, 1e-8)
 
 
-def test_decompose_problems_iteration_loops_match_oracle(oracle):
+@pytest.mark.parametrize("engine", ENGINES)
+def test_decompose_problems_iteration_loops_match_oracle(oracle, engine):
     """Uneven problems with three iteration loops and per-problem restart jitter."""
     from squander_hip import decompose_problems
     from squander_hip.optimizer import restart_jitter
@@ -214,7 +226,7 @@ def test_decompose_problems_iteration_loops_match_oracle(oracle):
                                            optimization_tolerance=1e-8))
     ctx = nat.Context(0)
     res = decompose_problems(ctx, probs, default_options(4, optimization_tolerance=1e-8, max_outer_iterations=4,
-                                                         iteration_loops=3), jit)
+                                                         iteration_loops=3, engine=engine), jit)
     pairs = []
     for i, ((n, gates, T, _), (fo, xo, co)) in enumerate(zip(probs, ref)):
         say(f"  problem {i} (n={n}): gpu f={res['f'][i]:.3e} evals={res['evals'][i]}  oracle f={fo:.3e} evals={co}")
@@ -368,3 +380,36 @@ def test_combined_targets_matches_oracle(oracle):
                 fo, go = oracle.optimization_problem_combined(gates, T[i], n, ps[b], variant=variant)
                 assert abs(f[b].item() - fo) < 1e-12
                 assert np.max(np.abs(g[b].cpu().numpy() - go)) < 1e-12
+
+
+def test_persistent_engine_with_trace_offset_and_hs_cost(oracle):
+    """The persistent solver with a rectangular target and trace offset (FROBENIUS) and
+    with the Hilbert-Schmidt cost, against the oracle's driver; the correction variants
+    are refused by the persistent engine and run in lockstep rounds under AUTO."""
+    n = 3
+    gates = W.ansatz_layers(n, W.alternating_pairs(n, 5))
+    P = W.param_count(gates)
+    T = reachable_targets(n, gates, 4, 810)
+    seeds = [21, 22, 23, 24]
+    x0 = np.array([initial_guess(P, s) for s in seeds])
+    for variant, off, cols in ((nat.COST_FROBENIUS, 2, 6), (nat.COST_HILBERT_SCHMIDT, 0, 8)):
+        say(f"  persistent variant={variant} offset={off} cols={cols}")
+        Tv = np.ascontiguousarray(T[:, :, :cols])
+        ctx = nat.Context(0)
+        ctx.set_circuit(n, gates)
+        ctx.set_cost_function(variant)
+        ctx.set_trace_offset(off)
+        res = decompose_batch(ctx, Tv, np.arange(4, dtype=np.int32), x0,
+                              default_options(n, optimization_tolerance=1e-8, engine="persistent"))
+        pairs = []
+        for r in range(4):
+            fo, xo, co = oracle.decompose_custom(gates, Tv[r], n, seeds[r], max_outer_iterations=4,
+                                                 optimization_tolerance=1e-8, variant=variant, trace_offset=off)
+            assert abs(oracle.optimization_problem(gates, Tv[r], n, res["x"][r], variant, off) - res["f"][r]) < 1e-12
+            pairs.append((res["f"][r], res["evals"][r], fo, co))
+        check_against_oracle(pairs, 1e-8)
+    ctx.set_cost_function(nat.COST_FROBENIUS_CORRECTION1)
+    ctx.set_trace_offset(0)
+    with pytest.raises(nat.SqhError):
+        decompose_batch(ctx, T, np.arange(4, dtype=np.int32), x0, default_options(n, engine="persistent"))
+    decompose_batch(ctx, T, np.arange(4, dtype=np.int32), x0, default_options(n, max_outer_iterations=1))
