@@ -105,8 +105,9 @@ def test_jit_decompose_batch_matches_oracle_driver(oracle):
     ctx = nat.Context(0)
     ctx.set_circuit(n, gates)
     ctx.set_jit(True)
+    # the circuit-specialised kernels serve the lockstep rounds
     res = decompose_batch(ctx, T, np.arange(4, dtype=np.int32), x0,
-                          default_options(n, optimization_tolerance=1e-8, mixed_precision=True))
+                          default_options(n, optimization_tolerance=1e-8, mixed_precision=True, engine="rounds"))
     for r in range(4):
         fo, xo, co = oracle.decompose_custom(gates, T[r], n, r + 1, optimization_tolerance=1e-8)
         assert abs(oracle.optimization_problem(gates, T[r], n, res["x"][r]) - res["f"][r]) < 1e-12
