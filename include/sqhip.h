@@ -166,6 +166,19 @@ int sqh_optimization_problem_combined_batch_dev(sqh_ctx* ctx, const double* d_pa
 int sqh_optimization_problem_combined_targets_dev(sqh_ctx* ctx, const double* d_targets, int n_targets,
                                                   const int* d_target_index, const double* d_params, int B,
                                                   int precision, double* d_f0, double* d_grad);
+/* The same costs and gradients as sqh_optimization_problem_combined_batch_dev
+ * (d_grad may be NULL: costs only) without allocating anything: no hipMalloc, no
+ * hipFree, no allocating library routine and no run-time kernel compilation (a
+ * circuit-specialised kernel is used only if an earlier call compiled it).  No
+ * counterpart in the reference: it is for callers that cannot allocate while they
+ * run (code under stream capture, serving loops, harnesses that forbid hipMalloc).
+ * Its scratch memory (kernel tables, trace channels, gradient correlations) is one
+ * 64 MiB device array per GPU, held until the process ends; a batch that needs more
+ * is rejected with SQH_ERR_ARG (at 8 qubits with a gradient that is B > ~170, costs
+ * only B > ~6800).  Up to 10 qubits.  All calls on one GPU share that array: calls
+ * that overlap on a GPU race and must be serialised by the caller. */
+int sqh_optimization_problem_combined_batch_noalloc_dev(sqh_ctx* ctx, const double* d_params, int B, int precision,
+                                                        double* d_f0, double* d_grad);
 /* optimization_problem_combined_unitary: C.Umtx (rows x cols dense) and the P
  * derivative matrices (P x rows x cols). */
 int sqh_optimization_problem_combined_unitary(sqh_ctx* ctx, const double* params, double* out_unitary,

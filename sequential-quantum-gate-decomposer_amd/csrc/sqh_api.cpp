@@ -146,22 +146,55 @@ int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lon
     return SQH_OK;
 }
 
-// costs (and, with grad, the gradient) for B parameter vectors on the device.
-int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad) {
+void eval_scratch_bytes(sqh_ctx* c, int B, bool f32, bool grad, size_t bytes[4]) {
+    const int N = c->prog.N, m = c->cols, ns = std::max((int)c->prog.slots.size(), 1);
+    const TraceSpec ts = trace_spec(c, B, m);
+    bytes[0] = (size_t)B * ns * 4 * (f32 ? 8 : 16);
+    bytes[1] = (size_t)(ts.nch + 1) * B * m * 2 * sizeof(double);
+    bytes[2] = grad ? (size_t)B * waves_per_batch(N, m_pad_for(N, m)) * ns * 8 * sizeof(double) : 0;
+    bytes[3] = grad ? (size_t)B * 6 * sizeof(double) : 0;
+}
+
+// costs (and, with grad, the gradient) for B parameter vectors on the device; with
+// `fixed` into the caller's scratch memory (tile path only, nothing allocated).
+int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad,
+               const EvalScratch* fixed) {
     int rc = require_target(c);
     if (rc) return rc;
     if (B <= 0) return fail(c, SQH_ERR_ARG, "batch size must be positive");
     if (precision != SQH_PREC_F64 && precision != SQH_PREC_F32) return fail(c, SQH_ERR_ARG, "bad precision");
     const bool f32 = precision == SQH_PREC_F32;
     const int N = c->prog.N, m = c->cols, ns = (int)c->prog.slots.size();
+    if (fixed && N > kMaxTileQubits)
+        return fail(c, SQH_ERR_ARG, "evaluation without allocation supports up to %d qubits", kMaxTileQubits);
     const int off = uses_offset(c->variant) ? c->trace_offset : 0;
     const TraceSpec ts = trace_spec(c, B, m);
     const bool hs_corr = ts.nch > 0 && !uses_offset(c->variant);
-    if ((rc = build_kernels(c, d_params, B, f32))) return rc;
-    HIPTRY(c, c->d_diag.ensure((size_t)(ts.nch + 1) * B * m * 2 * sizeof(double)));
+    void* kern = nullptr;
+    double* diag = nullptr;
+    if (fixed) {
+        kern = fixed->kern;
+        diag = fixed->diag;
+        HIPTRY(c, launch_build_kernels(f32, d_params, c->prog.P, B, c->d_slots.as<Slot>(), ns, kern, c->stream));
+    } else {
+        if ((rc = build_kernels(c, d_params, B, f32))) return rc;
+        HIPTRY(c, c->d_diag.ensure((size_t)(ts.nch + 1) * B * m * 2 * sizeof(double)));
+        kern = c->d_kern.p;
+        diag = c->d_diag.as<double>();
+    }
+    // the run-time kernel when the context uses one (with fixed scratch: only if compiled)
+    auto run = [&](const TileLaunch& t, int kind) -> int {
+        if (jit_wants(c, kind)) {
+            const int r = jit_launch(c, t, kind, c->stream, fixed == nullptr);
+            if (r != kJitNotReady) return r;
+        }
+        HIPTRY(c, kind == 2 ? launch_tile_grad(t, c->stream) : launch_tile_forward(t, c->stream));
+        return SQH_OK;
+    };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
     if (N <= kMaxTileQubits) {
         TileLaunch t = base_launch(c, f32);
+        t.kern = kern;
         t.in = U;
         t.in_bstride = 0;
         t.in_stride = m;
@@ -170,43 +203,48 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
         t.B = B;
         t.kern_bstride = (long long)ns * 4;
         t.trace_off = off;
-        t.diag = c->d_diag.as<double>();
+        t.diag = diag;
         t.ts = ts;
         const int wpb = waves_per_batch(N, t.m_pad);
         if (d_grad) {
-            HIPTRY(c, c->d_mpart.ensure((size_t)B * wpb * std::max(ns, 1) * 8 * sizeof(double)));
-            t.mpart = c->d_mpart.as<double>();
+            double* seed = nullptr;
+            if (fixed) {
+                t.mpart = fixed->mpart;
+                seed = fixed->seed;
+            } else {
+                HIPTRY(c, c->d_mpart.ensure((size_t)B * wpb * std::max(ns, 1) * 8 * sizeof(double)));
+                t.mpart = c->d_mpart.as<double>();
+            }
             if (hs_corr) {
                 // |t_k|^2 terms: a forward pass gives the channel traces, whose conjugates
                 // (weighted) seed the adjoint sweep.
-                HIPTRY(c, c->d_seed.ensure((size_t)B * 6 * sizeof(double)));
+                if (!fixed) {
+                    HIPTRY(c, c->d_seed.ensure((size_t)B * 6 * sizeof(double)));
+                    seed = c->d_seed.as<double>();
+                }
                 TileLaunch tf = t;
                 tf.mode = 1;
                 tf.mpart = nullptr;
                 HIPTRY(c, launch_tile_forward(tf, c->stream));
                 HIPTRY(c, launch_finish(t.diag, m, B, c->variant, nullptr, nullptr, 0, ns, t.slots, d_params,
-                                        c->prog.P, nullptr, c->stream, BatchIndex(), ts, c->d_seed.as<double>()));
-                t.ts.seed = c->d_seed.as<double>();
+                                        c->prog.P, nullptr, c->stream, BatchIndex(), ts, seed));
+                t.ts.seed = seed;
             }
-            {
+            if (fixed) {
+                if ((rc = run(t, 2))) return rc;
+            } else {
                 TimedLaunch tl(c);
-                if (jit_wants(c, 2)) {
-                    if ((rc = jit_launch(c, t, 2, c->stream))) return rc;
-                } else {
-                    HIPTRY(c, launch_tile_grad(t, c->stream));
-                }
+                if ((rc = run(t, 2))) return rc;
             }
             HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, t.mpart, wpb, ns, t.slots, d_params, c->prog.P,
                                     d_grad, c->stream, BatchIndex(), ts));
         } else {
             t.mode = 1;
-            {
+            if (fixed) {
+                if ((rc = run(t, 1))) return rc;
+            } else {
                 TimedLaunch tl(c);
-                if (jit_wants(c, 1)) {
-                    if ((rc = jit_launch(c, t, 1, c->stream))) return rc;
-                } else {
-                    HIPTRY(c, launch_tile_forward(t, c->stream));
-                }
+                if ((rc = run(t, 1))) return rc;
             }
             HIPTRY(c, launch_finish(t.diag, m, B, c->variant, d_costs, nullptr, 0, ns, t.slots, d_params, c->prog.P,
                                     nullptr, c->stream, BatchIndex(), ts));
@@ -363,6 +401,9 @@ int sqh_create(int device, sqh_ctx** out) {
         delete c;
         return SQH_ERR_DEVICE;
     }
+    // the allocation-free entry point's scratch address on this device (its failure
+    // is reported by that entry point, not here)
+    (void)noalloc_scratch_init(device);
     *out = c;
     return SQH_OK;
 }

@@ -137,7 +137,23 @@ int build_kernels(sqh_ctx* c, const double* d_params, int B, bool f32);
 TileLaunch base_launch(sqh_ctx* c, bool f32);
 int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long long bstride, int B, bool f32,
                  long long kern_bstride);
-int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad);
+// Device memory an evaluation writes besides its outputs.  Given to eval_batch by a
+// caller that cannot allocate: the evaluation then allocates nothing, compiles no
+// run-time kernel (it uses one only if already compiled) and records no timers.
+struct EvalScratch {
+    void* kern = nullptr;     // [B][ns][4] 2x2 kernels (complex128, or complex64 in f32)
+    double* diag = nullptr;   // [nch + 1][B][m] complex trace channels
+    double* mpart = nullptr;  // [B][waves][ns][8] per-wavefront gradient correlations
+    double* seed = nullptr;   // [B][3] complex adjoint seed weights (HS corrections)
+};
+// Bytes of each EvalScratch part for B elements of the context's circuit and target
+// (kern, diag, mpart, seed; mpart and seed are 0 without a gradient).
+void eval_scratch_bytes(sqh_ctx* c, int B, bool f32, bool grad, size_t bytes[4]);
+int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double* d_costs, double* d_grad,
+               const EvalScratch* fixed = nullptr);
+// Looks up, once per device (the device current), the address of the file-scope
+// scratch array of the allocation-free entry point (sqh_noalloc.hip).
+hipError_t noalloc_scratch_init(int device);
 // FP64 costs of B parameter vectors, element b against the d_index[b]-th of the
 // complex128 rows x cols targets at d_targets (contiguous, rows = 2^N); the context's
 // circuit, cost variant and trace offset.  N <= 10.
@@ -152,7 +168,10 @@ int fused_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lon
                 long long kern_bstride);
 // Run-time specialised kernel for the current program: kind 0 forward store, 1
 // forward diagonal, 2 gradient (sqh_jit.cpp).
-int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s);
+// With compile = false a kernel that is not compiled yet is not compiled: the call
+// returns kJitNotReady and launches nothing.
+constexpr int kJitNotReady = 1;
+int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s, bool compile = true);
 // Whether the context uses a run-time kernel of this kind for its program.
 bool jit_wants(sqh_ctx* c, int kind);
 // Compiles the kernel jit_launch would use for these shapes (before a graph capture).

@@ -426,7 +426,14 @@ bool jit_compile(const std::string& src, const std::string& arch, std::vector<ch
 
 // Compiles (once) and returns the kernel for the current program; nullptr with the
 // context's error set on failure.
-static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni, const JitShape& sh) {
+static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni, const JitShape& sh, bool compile = true) {
+    if (!compile) {  // cache lookup only (allocates nothing)
+        if (!c->jit_cache) return nullptr;
+        for (auto& e : c->jit_cache->entries)
+            if (e.prog_version == c->prog_version && e.kind == kind && e.f32 == f32 && e.uni == uni && e.shape == sh)
+                return e.fn;
+        return nullptr;
+    }
     if (!c->jit_cache) c->jit_cache = new JitCache();
     JitCache& jc = *c->jit_cache;
     for (auto& e : jc.entries)
@@ -479,12 +486,12 @@ static hipFunction_t jit_get(sqh_ctx* c, int kind, bool f32, bool uni, const Jit
     return e.fn;
 }
 
-int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s) {
+int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s, bool compile) {
     const int cw = tile_columns_per_wave(t.N);
     const bool uni = t.N > 4 && t.m_pad % cw == 0;
     if (t.bi.run_prog) return fail(c, SQH_ERR_ARG, "run-time kernels serve one circuit");
-    hipFunction_t fn = jit_get(c, kind, t.f32, uni, shape_of(t));
-    if (!fn) return SQH_ERR_DEVICE;
+    hipFunction_t fn = jit_get(c, kind, t.f32, uni, shape_of(t), compile);
+    if (!fn) return compile ? SQH_ERR_DEVICE : kJitNotReady;
     JitArgs a;
     std::memset(&a, 0, sizeof(a));  // pad0 = 0: the gradient kernel's table offset
     a.in = t.in;

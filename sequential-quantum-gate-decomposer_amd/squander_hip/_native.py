@@ -38,7 +38,7 @@ EXPORTS = [
     "sqh_apply_derivate_to", "sqh_apply_derivate_to_dev", "sqh_optimization_problem_batch",
     "sqh_optimization_problem_batch_dev", "sqh_optimization_problem_combined_batch",
     "sqh_optimization_problem_combined_batch_dev", "sqh_optimization_problem_combined_unitary",
-    "sqh_inspect_program", "sqh_inspect_slots", "sqh_inspect_fusion", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
+    "sqh_optimization_problem_combined_batch_noalloc_dev", "sqh_inspect_program", "sqh_inspect_slots", "sqh_inspect_fusion", "sqh_set_kernel_timing", "sqh_get_kernel_timing", "sqh_decompose_batch",
     "sqh_mlp_create", "sqh_mlp_destroy", "sqh_mlp_param_count", "sqh_mlp_get_params", "sqh_mlp_set_params",
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
@@ -135,6 +135,7 @@ def lib():
             "sqh_optimization_problem_combined_batch": ([vp, vp, i, i, vp, vp], i),
             "sqh_optimization_problem_combined_batch_dev": ([vp, vp, i, i, vp, vp], i),
             "sqh_optimization_problem_combined_unitary": ([vp, vp, vp, vp], i),
+            "sqh_optimization_problem_combined_batch_noalloc_dev": ([vp, vp, i, i, vp, vp], i),
             "sqh_set_kernel_timing": ([vp, i], i),
             "sqh_mlp_create": ([vp, i, i, i, i, ctypes.c_ulonglong, ctypes.POINTER(vp)], i),
             "sqh_mlp_destroy": ([vp], None),
