@@ -304,8 +304,9 @@ int sqh_agents_batch(sqh_ctx* ctx, const double* targets, int n_runs, int rows, 
 int sqh_set_jit(sqh_ctx* ctx, int enable);
 int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
 /* Introspection (no GPU needed): generate and compile, for gfx950, the run-time
- * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient;
- * uniform = one batch element per wavefront).  Returns SQH_OK when it compiles;
+ * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient, 3 the
+ * persistent solver of sqh_decompose_batch, up to 4 qubits; uniform = one batch element
+ * per wavefront).  Returns SQH_OK when it compiles;
  * log receives the compiler log. */
 int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32, int uniform,
                           char* log, int log_cap, long long* code_bytes, long long* source_bytes);

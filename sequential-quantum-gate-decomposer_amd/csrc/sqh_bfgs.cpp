@@ -220,7 +220,13 @@ int solve_persistent(sqh_ctx* c, std::vector<Group>& groups, BatchBufs& bb, cons
         a.max_evals = max_evals;
         a.o = o;
         const int grid = std::min(a.n_runs, 4096);
-        HIPTRY(c, launch_solve(gr.N, a, grid, fork ? c->side[g] : c->stream));
+        if (!grouped && jit_wants(c, kJitSolve)) {
+            const int rc = jit_launch_solve(c, a, grid, solve_lds_bytes(gr.N, o.P, a.ns_max),
+                                            fork ? c->side[g] : c->stream);
+            if (rc) return rc;
+        } else {
+            HIPTRY(c, launch_solve(gr.N, a, grid, fork ? c->side[g] : c->stream));
+        }
     }
     if (fork) {
         for (int g = 0; g < G; ++g) {

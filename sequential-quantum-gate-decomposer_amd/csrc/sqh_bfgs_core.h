@@ -3,14 +3,13 @@
 // per run and round) and the persistent solver (k_solve, one wavefront per run,
 // looping over its own evaluations).
 #pragma once
-#include <cfloat>
-
 #include "sqh_bfgs_device.h"
 #include "sqh_device.h"
 
 namespace sqh {
 
 constexpr double kNumPrecision = 1.42e-14;  // Grad_Descend::num_precision (grad_descend.cpp:40)
+constexpr double kTwoPi = 6.283185307179586476925286766559;
 
 // A double the whole wavefront holds the same value of, as a wave-uniform (scalar)
 // value: lane 0's bits.
@@ -258,17 +257,17 @@ __device__ bool bfgs_advance_run(RunState& S, int r, double* __restrict__ V, dou
                 __syncthreads();
             }
             dir_ready = false;
-            double dg = 0.0, gn = 0.0, ms = INFINITY;
+            double dg = 0.0, gn = 0.0, ms = __builtin_inf();
             for (int k = tid; k < P; k += nt) {
                 dg += d[k] * g[k];
                 gn += g[k] * g[k];
-                if (fabs(d[k]) >= 1e-5) ms = fmin(ms, fabs(2.0 * M_PI / d[k]));
+                if (fabs(d[k]) >= 1e-5) ms = fmin(ms, fabs(kTwoPi / d[k]));
             }
             dg = wg_sum<NT>(dg, sh);
             gn = wg_sum<NT>(gn, sh);
             ms = wg_min<NT>(ms, sh);
             double maxstep = 0.0;
-            if (dg < 0.0) maxstep = (ms == INFINITY) ? 0.0 : ms;
+            if (dg < 0.0) maxstep = (ms == __builtin_inf()) ? 0.0 : ms;
             if (gn <= 1e-19 * 1e-19) { S.status = BS_MINIMUM; S.phase = PH_BFGS_END; break; }
             if (dg >= 0.0) { S.status = BS_NO_DESCENT; S.phase = PH_BFGS_END; break; }
             if (S.f >= S.fprev) { S.status = BS_MINIMUM; S.phase = PH_BFGS_END; break; }

@@ -13,7 +13,9 @@
 // its precision.  All runs move in lockstep, one evaluation per round, so every
 // round is one batched cost+gradient launch.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #include "sqh_internal.h"
 

@@ -174,6 +174,11 @@ constexpr int kJitNotReady = 1;
 int jit_launch(sqh_ctx* c, const TileLaunch& t, int kind, hipStream_t s, bool compile = true);
 // Whether the context uses a run-time kernel of this kind for its program.
 bool jit_wants(sqh_ctx* c, int kind);
+// Run-time kernel kind of the persistent solver specialised to the circuit.
+constexpr int kJitSolve = 3;
+struct SolveArgs;
+// Launches the circuit-specialised persistent solver (compiled on first use).
+int jit_launch_solve(sqh_ctx* c, const SolveArgs& a, int grid, size_t shm, hipStream_t s);
 // Compiles the kernel jit_launch would use for these shapes (before a graph capture).
 int jit_prepare(sqh_ctx* c, const TileLaunch& t, int kind);
 double jit_compile_ms(sqh_ctx* c, int* compiles);

@@ -21,11 +21,16 @@
 #include <vector>
 
 #include "sqh_context.h"
+#include "sqh_solve.h"
 
 namespace sqh {
 
 extern const char* const kJitHeaderInternal;  // sqh_internal.h (generated at build time)
 extern const char* const kJitHeaderDevice;    // sqh_device.h
+extern const char* const kJitHeaderBfgsDevice;  // sqh_bfgs_device.h
+extern const char* const kJitHeaderBfgsCore;    // sqh_bfgs_core.h
+extern const char* const kJitHeaderSolve;       // sqh_solve.h
+extern const char* const kJitHeaderSolveLoop;   // sqh_solve_loop.h
 
 namespace {
 
@@ -356,6 +361,95 @@ struct Gen {
         o << "}\n";
         return o.str();
     }
+
+    // The persistent solver (sqh_solve_loop.h) with the evaluation specialised to the
+    // circuit: N <= 4, so one wavefront holds the whole matrix (one column per lane)
+    // and no remaps occur.  jit_eval<R> is the straight-line forward pass, trace,
+    // adjoint sweep and per-gate correlations of k_tile_grad for one run, reduced in
+    // the wavefront into LDS, followed by the gradient from the derivative kernels.
+    std::string solve_kernel() {
+        o << "#include \"sqh_solve_loop.h\"\nusing namespace sqh;\nconstexpr int N = " << N
+          << ";\ntypedef Geo<N> G;\nconstexpr int SM = " << sh.m << ", STR = " << sh.trace_off
+          << ", SNS = " << sh.ns << ";\n"
+          << "struct JitLane { bool valid; int col; };\n"
+          << "template <typename R>\n__device__ double jit_eval(const cpx<R>* in, const double* xr, double* gr, "
+             "cpx<double>* kt_raw, double* mc, const Slot* slots, int variant) {\n"
+          << "  const int lane = threadIdx.x;\n"
+          << "  cpx<R>* kw = reinterpret_cast<cpx<R>*>(kt_raw);\n"
+          << "  for (int sl = lane; sl < SNS; sl += 64) { const Slot q = slots[sl]; cpx<double> k[4]; "
+             "dev_gate_kernel(q.type, xr + q.param_start, k); for (int e = 0; e < 4; ++e) kw[4 * sl + e] = cvt<R>(k[e]); }\n"
+          << "  __syncthreads();\n"
+          << "  JitLane li; li.col = lane; li.valid = lane < SM;\n"
+          << "  const int s = 0, cc = lane; (void)s; (void)cc;\n"
+          << "  const cpx<R>* kb = kw;\n"
+          << "  cpx<R> x[G::NR], y[G::NR];\n";
+        load("x", "in", "SM");
+        const int cur = forward("x");
+        const Layout& lf = pr.layouts[cur];
+        o << "  double tre = 0.0, tim = 0.0;\n"
+          << "  { const int sf = " << sub_expr(N, lf) << ";\n    const int tr = li.col + STR;\n";
+        for (int i = 0; i < NR; ++i) {
+            o << "    { const bool hit = li.valid && sf + " << lf.reg_off[i] << " == tr;\n"
+              << "      if (hit) { tre += (double)x[" << i << "].re; tim += (double)x[" << i << "].im; }\n"
+              << "      y[" << i << "] = mk<R>(hit ? (R)1 : (R)0, (R)0); }\n";
+        }
+        o << "  }\n  const cpx<R>* kbb = kb;\n";
+        std::vector<int> par = order_fwd();
+        std::reverse(par.begin(), par.end());
+        size_t next = 0;
+        if (next < par.size()) {
+            o << "  {\n";
+            declare_b(par[next++]);
+        }
+        for (int p = (int)pr.ops.size() - 1; p >= 0; --p) {
+            const Op& op = pr.ops[p];
+            if (op.kind == OP_X || op.kind == OP_Z) {
+                gate("x", p);
+                gate("y", p);
+            } else if (op.kind == OP_U || op.kind == OP_CU) {
+                if (next < par.size()) declare_b(par[next++]);
+                const bool masked = op.kind == OP_CU;
+                const std::string mk = masked ? mask_expr(N, op) : std::string("0u");
+                const char* mb = masked ? "true" : "false";
+                o << "  { const cpx<R>* k = kb" << p << ";\n"
+                  << "    cpx<R> kd[4] = {mk(k[0].re, -k[0].im), mk(k[2].re, -k[2].im), mk(k[1].re, -k[1].im), "
+                     "mk(k[3].re, -k[3].im)};\n"
+                  << "    cpx<R> kt[4] = {k[0], k[2], k[1], k[3]};\n"
+                  << "    cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};\n"
+                  << "    const unsigned msk = " << mk << ";\n"
+                  << "    pairs_apply_f<" << form(op) << ", " << int(op.t) << ", " << mb << ">(x, kd, msk);\n"
+                  << "    pairs_corr<" << int(op.t) << ">(y, x, acc, msk, " << mb << ");\n"
+                  << "    pairs_apply_f<" << form(op) << ", " << int(op.t) << ", " << mb << ">(y, kt, msk);\n"
+                  << "    double r[8];\n"
+                  << "    for (int e = 0; e < 4; ++e) { r[2 * e] = wave_sum((double)acc[e].re); "
+                     "r[2 * e + 1] = wave_sum((double)acc[e].im); }\n"
+                  << "    if (lane == 0) { double* d = mc + " << 8 * op.idx << "; for (int e = 0; e < 8; ++e) d[e] = r[e]; }\n"
+                  << "  }\n";
+            }
+            barrier();
+        }
+        if (!par.empty()) o << "  }\n";
+        o << "  tre = wave_sum(tre);\n  tim = wave_sum(tim);\n  __syncthreads();\n"
+          << "  for (int sl = lane; sl < SNS; sl += 64) {\n"
+          << "    const Slot q = slots[sl];\n    const double* M = mc + 8 * sl;\n"
+          << "    for (int j = 0; j < q.nparam; ++j) {\n"
+          << "      cpx<double> dk[4]; dev_deriv_kernel(q.type, xr + q.param_start, j, dk);\n"
+          << "      double kre = 0.0, kim = 0.0;\n"
+          << "      for (int e = 0; e < 4; ++e) { kre += dk[e].re * M[2 * e] - dk[e].im * M[2 * e + 1]; "
+             "kim += dk[e].re * M[2 * e + 1] + dk[e].im * M[2 * e]; }\n"
+          << "      gr[q.param_start + j] = grad_of(variant, tre, tim, kre, kim, (double)SM);\n"
+          << "    }\n  }\n  __syncthreads();\n"
+          << "  return cost_of(variant, tre, tim, (double)SM);\n}\n"
+          << "extern \"C\" __global__ __launch_bounds__(64) void sqh_jit_kernel(SolveArgs a) {\n"
+          << "  const int var = a.variant;\n"
+          << "  solve_runs<1>(a,\n"
+          << "    [=](const cpx<double>* U, const Op*, int, const Layout*, const Slot* sl, int, const double* xr, "
+             "double* gr, cpx<double>* kt, double* mc, cpx<double>*) { return jit_eval<double>(U, xr, gr, kt, mc, sl, var); },\n"
+          << "    [=](const cpx<float>* U, const Op*, int, const Layout*, const Slot* sl, int, const double* xr, "
+             "double* gr, cpx<double>* kt, double* mc, cpx<float>*) { return jit_eval<float>(U, xr, gr, kt, mc, sl, var); });\n"
+          << "}\n";
+        return o.str();
+    }
 };
 
 }  // namespace
@@ -389,15 +483,18 @@ namespace {
 
 std::string jit_source(const Program& pr, int kind, bool f32, bool uni, const JitShape& sh) {
     Gen g(pr, sh);
+    if (kind == kJitSolve) return g.solve_kernel();
     return kind == 2 ? g.grad_kernel(uni, f32) : g.forward_kernel(uni, f32, kind);
 }
 
 // hiprtc compile of a generated source for arch; code object in code, log on failure.
 bool jit_compile(const std::string& src, const std::string& arch, std::vector<char>& code, std::string& log) {
     hiprtcProgram p = nullptr;
-    const char* hs[] = {kJitHeaderInternal, kJitHeaderDevice};
-    const char* hn[] = {"sqh_internal.h", "sqh_device.h"};
-    if (hiprtcCreateProgram(&p, src.c_str(), "sqh_jit.hip", 2, hs, hn) != HIPRTC_SUCCESS) {
+    const char* hs[] = {kJitHeaderInternal, kJitHeaderDevice, kJitHeaderBfgsDevice, kJitHeaderBfgsCore,
+                        kJitHeaderSolve, kJitHeaderSolveLoop};
+    const char* hn[] = {"sqh_internal.h", "sqh_device.h", "sqh_bfgs_device.h", "sqh_bfgs_core.h", "sqh_solve.h",
+                        "sqh_solve_loop.h"};
+    if (hiprtcCreateProgram(&p, src.c_str(), "sqh_jit.hip", 6, hs, hn) != HIPRTC_SUCCESS) {
         log = "hiprtcCreateProgram failed";
         return false;
     }
@@ -530,9 +627,23 @@ constexpr size_t kJitMaxGradOps = 256;
 
 bool jit_wants(sqh_ctx* c, int kind) {
     if (!c->jit || c->prog.N > kMaxTileQubits) return false;
+    if (kind == kJitSolve) return c->prog.N <= 4 && correction_channels(c->variant) == 0;
     // the generated epilogues read the diagonal only
     if ((kind == 1 || kind == 2) && correction_channels(c->variant) > 0) return false;
     return kind != 2 || c->prog.ops.size() <= kJitMaxGradOps;
+}
+
+int jit_launch_solve(sqh_ctx* c, const SolveArgs& a, int grid, size_t shm, hipStream_t s) {
+    JitShape sh;
+    sh.m = a.m;
+    sh.trace_off = a.trace_off;
+    sh.ns = a.ns;
+    hipFunction_t fn = jit_get(c, kJitSolve, false, false, sh);
+    if (!fn) return SQH_ERR_DEVICE;
+    SolveArgs arg = a;
+    void* params[] = {&arg};
+    HIPTRY(c, hipModuleLaunchKernel(fn, grid, 1, 1, 64, 1, 1, (unsigned)shm, s, params, nullptr));
+    return SQH_OK;
 }
 
 int jit_prepare(sqh_ctx* c, const TileLaunch& t, int kind) {
@@ -557,12 +668,12 @@ using namespace sqh;
 extern "C" int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32,
                                      int uniform, char* log, int log_cap, long long* code_bytes,
                                      long long* source_bytes) {
-    if (n_gates < 0 || (n_gates > 0 && !gates) || kind < 0 || kind > 2) return SQH_ERR_ARG;
+    if (n_gates < 0 || (n_gates > 0 && !gates) || kind < 0 || kind > kJitSolve) return SQH_ERR_ARG;
     std::vector<GateRec> g(n_gates);
     for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
     Program pr;
     std::string err;
-    if (!compile_program(qbit_num, g, pr, err) || pr.N > kMaxTileQubits) {
+    if (!compile_program(qbit_num, g, pr, err) || pr.N > kMaxTileQubits || (kind == kJitSolve && pr.N > 4)) {
         if (log && log_cap > 0) std::snprintf(log, (size_t)log_cap, "%s", err.c_str());
         return SQH_ERR_GATE;
     }

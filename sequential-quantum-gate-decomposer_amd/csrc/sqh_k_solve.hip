@@ -14,6 +14,7 @@
 #include "sqh_device.h"
 #include "sqh_kernels.h"
 #include "sqh_solve.h"
+#include "sqh_solve_loop.h"
 
 namespace sqh {
 
@@ -166,109 +167,24 @@ __device__ double wave_eval(const cpx<R>* __restrict__ U, int m, int trace_off, 
 template <int N>
 __global__ __launch_bounds__(64, (N <= 2) ? 2 : 1) void k_solve(SolveArgs a) {
     using G = Geo<N>;
-    extern __shared__ __align__(16) unsigned char solve_lds[];
-    const BfgsOpts& o = a.o;
-    const int Ps = o.P;
-    // LDS: remap buffer, kernel table, correlations, evaluation point and gradient,
-    // the optimiser's scratch (zs, rot_c, rot_s, one partial-direction row), sh
-    char* p = reinterpret_cast<char*>(solve_lds);
-    cpx<double>* lds_remap = reinterpret_cast<cpx<double>*>(p);
-    p += sizeof(cpx<double>) * (size_t)G::LDS;
-    cpx<double>* kt = reinterpret_cast<cpx<double>*>(p);
-    p += sizeof(cpx<double>) * 4 * (size_t)a.ns_max;
-    double* mc = reinterpret_cast<double*>(p);
-    p += sizeof(double) * 8 * (size_t)a.ns_max;
-    double* xr = reinterpret_cast<double*>(p);
-    double* gr = xr + Ps;
-    double* zs = gr + Ps;
-    double* rot_c = zs + Ps;
-    double* rot_s = rot_c + Ps;
-    double* pd = rot_s + Ps;
-    double* sh = pd + Ps;
-    RunState* s_state = reinterpret_cast<RunState*>(sh + 8);
-    const int lane = threadIdx.x;
-    for (;;) {
-        int q = 0;
-        if (lane == 0) q = atomicAdd(a.next, 1);
-        q = __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
-        if (q >= a.n_runs) return;  // every wavefront leaves once the queue is empty
-        const int r = a.queue ? a.queue[q] : q;
-        // the run's problem
-        const Op* ops = a.ops;
-        int nops = a.nops;
-        const Layout* lays = a.lays;
-        const Slot* slots = a.slots;
-        int ns = a.ns;
-        if (a.run_prog) {
-            const ProgRef pg = a.run_prog[r];
-            ops += pg.op_off;
-            nops = pg.nops;
-            lays += pg.lay_off;
-            slots += pg.slot_off;
-            ns = pg.ns;
-        }
-        const int P = a.run_P ? a.run_P[r] : Ps;
-        const long long t = a.run_target ? a.run_target[r] : 0;
-        const cpx<double>* U64 = static_cast<const cpx<double>*>(a.tgt64) + t * a.tgt_stride;
-        const cpx<float>* U32 = static_cast<const cpx<float>*>(a.tgt32) + t * a.tgt_stride;
-        double* V = a.vecs + (size_t)r * RV_COUNT * Ps;
-        double* Z = a.Zs + (size_t)r * Ps * Ps;
-        for (int k = lane; k < P; k += 64) V[RV_BEST * Ps + k] = a.x0[(size_t)r * Ps + k];
-        // the run's state lives in LDS (every lane reads it; all lanes write the same
-        // values), which leaves the registers to the evaluation and the vector loops
-        __syncthreads();
-        RunState& S = *s_state;
-        S.phase = PH_OUTER_START;
-        S.status = BS_INITIAL;
-        S.prec = o.mixed ? 1 : 0;
-        S.outer = 0;
-        S.ls_iter = 0;
-        S.finished = 0;
-        S.P = P;
-        S.loop = 0;
-        S.jit_used = S.pad1 = S.pad2 = S.pad3 = 0;
-        S.calls = S.total_calls = S.calls32 = 0;
-        S.f = S.fprev = 0.0;
-        S.norm_Z = -1.0;
-        S.step = S.f_lowest = S.f_highest = S.step_highest = S.step_lowest = S.step_best = S.f_best = 0.0;
-        S.dg_abs_best = S.dg_highest = S.dg0 = S.maxstep = 0.0;
-        S.cur_min = DBL_MAX;
-        for (int i = 0; i < kMinWindow; ++i) S.minvec[i] = 0.0;
-        __syncthreads();
-        double fe = 0.0;
-        long long evals = 0;
-        int idle = 0;
-        for (;;) {
-            const bool request = bfgs_advance_run<64>(S, r, V, Z, fe, gr, xr, o, sh, zs, rot_c, rot_s, pd);
-            if (!request) {
-                // a call either asks for an evaluation or finishes the run; the bound
-                // only keeps a wavefront from spinning should neither happen
-                if (S.phase == PH_FINISHED || ++idle >= 4) break;
-                continue;
-            }
-            idle = 0;
-            if (evals >= a.max_evals) break;  // cap: reported as unfinished
-            ++evals;
-            __syncthreads();
-            const int prec = S.prec;
-            if (prec == 1)
-                fe = wave_eval<N, float>(U32, a.m, a.trace_off, a.variant, ops, nops, lays, slots, ns, xr, gr, kt, mc,
-                                         reinterpret_cast<cpx<float>*>(lds_remap));
-            else
-                fe = wave_eval<N, double>(U64, a.m, a.trace_off, a.variant, ops, nops, lays, slots, ns, xr, gr, kt,
-                                          mc, lds_remap);
-        }
-        __syncthreads();
-        if (lane == 0) a.states[r] = S;
-        __syncthreads();
-    }
+    const int m = a.m, toff = a.trace_off, var = a.variant;
+    solve_runs<G::LDS>(
+        a,
+        [=](const cpx<double>* U, const Op* ops, int nops, const Layout* lays, const Slot* slots, int ns,
+            const double* xr, double* gr, cpx<double>* kt, double* mc, cpx<double>* lds) {
+            return wave_eval<N, double>(U, m, toff, var, ops, nops, lays, slots, ns, xr, gr, kt, mc, lds);
+        },
+        [=](const cpx<float>* U, const Op* ops, int nops, const Layout* lays, const Slot* slots, int ns,
+            const double* xr, double* gr, cpx<double>* kt, double* mc, cpx<float>* lds) {
+            return wave_eval<N, float>(U, m, toff, var, ops, nops, lays, slots, ns, xr, gr, kt, mc, lds);
+        });
 }
 
 size_t solve_lds_bytes(int N, int Ps, int ns_max) {
     const int cw = tile_columns_per_wave(N);
     const size_t remap = N > 4 ? (size_t)(1 << N) * cw : 1;
     return sizeof(cpx<double>) * remap + sizeof(cpx<double>) * 4 * (size_t)ns_max + sizeof(double) * 8 * ns_max +
-           sizeof(double) * 6 * (size_t)Ps + sizeof(double) * 8 + sizeof(RunState);
+           sizeof(double) * (6 + RV_COUNT) * (size_t)Ps + sizeof(double) * 8 + sizeof(RunState);
 }
 
 hipError_t launch_solve(int N, const SolveArgs& a, int grid, hipStream_t s) {

@@ -1,6 +1,8 @@
 // sqh_solve.h -- arguments and launcher of the persistent batched solver (sqh_k_solve.hip).
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
+#endif
 
 #include "sqh_bfgs_device.h"
 #include "sqh_internal.h"
@@ -40,7 +42,9 @@ struct SolveArgs {
     BfgsOpts o;
 };
 
+#ifndef __HIPCC_RTC__
 size_t solve_lds_bytes(int N, int Ps, int ns_max);
 hipError_t launch_solve(int N, const SolveArgs& a, int grid, hipStream_t s);
+#endif
 
 }  // namespace sqh

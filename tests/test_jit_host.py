@@ -20,3 +20,13 @@ def test_compile_check_rejects_bad_circuit():
     ok, log, _, _ = nat.jit_compile_check(2, [[nat.GATE_U3, 3, -1, 0]], kind=1)
     assert not ok and log
 
+
+def test_persistent_solver_kernel_compiles():
+    """The persistent solver with the evaluation specialised to the circuit (kind 3,
+    up to 4 qubits) compiles for the 4-qubit ansatz; above 4 qubits it is refused."""
+    ok, log, code, src = nat.jit_compile_check(4, W.ansatz_4q_12(), kind=3)
+    assert ok, log
+    assert code > 0 and src > 0
+    ok, _, _, _ = nat.jit_compile_check(5, W.ansatz_layers(5, [0, 1]), kind=3)
+    assert not ok
+
