@@ -57,6 +57,27 @@ __device__ inline double wg_min(double v, double* sh) {
     }
 }
 
+// d = -Z (Z^T g) from ztg = Z^T g (BFGS_Powell.cpp:396-439), threads over the rows
+// (coalesced column reads), two rows per thread for twice the loads in flight.
+template <int NT>
+__device__ inline void bfgs_direction(const double* __restrict__ Z, const double* __restrict__ ztg,
+                                      double* __restrict__ d, int P) {
+    for (int row = threadIdx.x; row < P; row += 2 * NT) {
+        const int r2 = row + NT;
+        const bool two = r2 < P;
+        double s = 0.0, t = 0.0;
+#pragma unroll 16
+        for (int c = 0; c < P; ++c) {
+            const double zc = ztg[c];
+            s += Z[(size_t)c * P + row] * zc;
+            t += (two ? Z[(size_t)c * P + r2] : 0.0) * zc;
+        }
+        d[row] = -s;
+        if (two) d[r2] = -t;
+    }
+    __syncthreads();
+}
+
 // BFGS_Powell::BFGS_iteration (BFGS_Powell.cpp:202-386) across the workgroup.
 // Z is P x P stored column-major (Z[row][c] at c * P + row), so a column is
 // contiguous: the Givens sweep reads whole columns with consecutive threads on
@@ -91,38 +112,41 @@ __device__ bool bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
     dtg = fmax(kNumPrecision, dtg);
     // Givens rotations folding Z^T g into its first entry (Powell 1987, Eq. 1.7):
     // angles first (sequential in c), then the column pairs (c-1, c) of every row.
+    // The angles' recurrence runs through the carried norm only (a square root per
+    // step); the divisions hang off it, so the loop is branch-free and unrolled to let
+    // them overlap.  A zero carried entry is an identity rotation.
     if (tid == 0) {
         double carry = ztg[P - 1];
+#pragma unroll 4
         for (int c = P - 1; c > 0; --c) {
             const double zc = carry, zm = ztg[c - 1];
-            if (zc == 0.0) {
-                rc[c] = 1.0;
-                rs[c] = 0.0;
-                carry = zm;
-                continue;
-            }
+            const bool skip = zc == 0.0;
             const double a = sqrt(zm * zm + zc * zc);
-            rc[c] = zm / a;
-            rs[c] = zc / a;
-            carry = a;
+            rc[c] = skip ? 1.0 : zm / a;
+            rs[c] = skip ? 0.0 : zc / a;
+            carry = skip ? zm : a;
         }
     }
     __syncthreads();
-    for (int row = tid; row < P; row += nt) {
-        double z1 = Z[(size_t)(P - 1) * P + row];
-#pragma unroll 8
+    // Each thread rotates two rows at once (two independent chains, twice the loads in
+    // flight); an identity rotation (1, 0) leaves the pair as the swap-free update does.
+    for (int row = tid; row < P; row += 2 * nt) {
+        const int r2 = row + nt;
+        const bool two = r2 < P;
+        double a1 = Z[(size_t)(P - 1) * P + row];
+        double b1 = two ? Z[(size_t)(P - 1) * P + r2] : 0.0;
+#pragma unroll 16
         for (int c = P - 1; c > 0; --c) {
             const double cv = rc[c], sv = rs[c];
-            const double z0 = Z[(size_t)(c - 1) * P + row];
-            if (sv == 0.0 && cv == 1.0) {
-                Z[(size_t)c * P + row] = z1;
-                z1 = z0;
-                continue;
-            }
-            Z[(size_t)c * P + row] = cv * z1 - sv * z0;
-            z1 = cv * z0 + sv * z1;
+            const double a0 = Z[(size_t)(c - 1) * P + row];
+            const double b0 = two ? Z[(size_t)(c - 1) * P + r2] : 0.0;
+            Z[(size_t)c * P + row] = cv * a1 - sv * a0;
+            a1 = cv * a0 + sv * a1;
+            if (two) Z[(size_t)c * P + r2] = cv * b1 - sv * b0;
+            b1 = cv * b0 + sv * b1;
         }
-        Z[row] = z1;
+        Z[row] = a1;
+        if (two) Z[r2] = b1;
     }
     __syncthreads();
     if (norm_Z < 0.0) {
@@ -144,11 +168,11 @@ __device__ bool bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
         double tg = 0.0;
         if (c > 0) {
             double s = 0.0;
-#pragma unroll 8
+#pragma unroll 16
             for (int row = 0; row < P; ++row) s += gamma[row] * zc[row];
             const double gz = s / dtg;
             double nz = 0.0;
-#pragma unroll 8
+#pragma unroll 16
             for (int row = 0; row < P; ++row) {
                 const double v = zc[row] - gz * delta[row];
                 zc[row] = v;
@@ -168,14 +192,8 @@ __device__ bool bfgs_update(double* __restrict__ Z, double* __restrict__ V, int 
         ztg[c] = tg;
     }
     __syncthreads();
-    // d = -Z (Z^T g), threads over the rows (coalesced column reads)
-    for (int row = tid; row < P; row += nt) {
-        double s = 0.0;
-#pragma unroll 8
-        for (int c = 0; c < P; ++c) s += Z[(size_t)c * P + row] * ztg[c];
-        d[row] = -s;
-    }
-    __syncthreads();
+    // d = -Z (Z^T g), threads over the rows (coalesced column reads), two rows each
+    bfgs_direction<NT>(Z, ztg, d, P);
     return true;
 }
 
@@ -248,13 +266,7 @@ __device__ bool bfgs_advance_run(RunState& S, int r, double* __restrict__ V, dou
                     ztg[c] = s;
                 }
                 __syncthreads();
-                for (int row = tid; row < P; row += nt) {
-                    double s = 0.0;
-#pragma unroll 8
-                    for (int c = 0; c < P; ++c) s += Z[(size_t)c * P + row] * ztg[c];
-                    d[row] = -s;
-                }
-                __syncthreads();
+                bfgs_direction<NT>(Z, ztg, d, P);
             }
             dir_ready = false;
             double dg = 0.0, gn = 0.0, ms = __builtin_inf();
