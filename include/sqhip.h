@@ -213,7 +213,7 @@ typedef struct sqh_decompose_options {
     int restart_jitter_count;
     const double* restart_jitter;
     /* Execution engine: SQH_ENGINE_AUTO picks the persistent solver when every run has
-     * at most 4 qubits and the cost variant has no correction channels, else lockstep
+     * at most 5 qubits and the cost variant has no correction channels, else lockstep
      * rounds (the persistent solver serves up to 6 qubits when asked for);
      * SQH_ENGINE_ROUNDS forces the lockstep rounds; SQH_ENGINE_PERSISTENT requires the
      * persistent solver (an error where it does not apply).  Both run the same

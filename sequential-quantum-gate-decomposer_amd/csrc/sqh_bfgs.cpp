@@ -153,7 +153,7 @@ int loops_of(const sqh_decompose_options* opts) { return opts->iteration_loops >
 // The persistent solver (sqh_k_solve.hip) serves up to this many qubits, the cost
 // variants without correction channels, and runs capped at this many evaluations.
 constexpr int kSolveMaxQubits = 6;
-constexpr int kSolveAutoQubits = 4;
+constexpr int kSolveAutoQubits = 5;
 constexpr long long kSolveMaxEvals = 1000000;
 constexpr size_t kSolveMaxLds = 64 * 1024;
 

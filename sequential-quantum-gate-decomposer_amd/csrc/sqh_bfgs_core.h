@@ -250,8 +250,15 @@ __device__ bool bfgs_advance_run(RunState& S, int r, double* __restrict__ V, dou
         }
         case PH_DIRECTION: {
             if (S.prec == 1 && S.f < o.switch_cost) {
-                S.status = BS_MINIMUM;
-                S.phase = PH_BFGS_END;
+                // The complex64 descent hands over to FP64 inside the same BFGS run: the
+                // point is evaluated again in FP64 and the run goes on with its Powell
+                // factor (the inverse-Hessian estimate is kept, not rebuilt from the
+                // identity), so the FP64 stage only finishes the descent.
+                S.calls32 = S.calls;
+                S.prec = 0;
+                for (int k = tid; k < P; k += nt) xr[k] = x[k];
+                S.phase = PH_INIT_WAIT;
+                request = true;
                 break;
             }
             // d = -Z Z^T g (BFGS_Powell.cpp:396-439); Z column-major: Z^T g with a
@@ -383,16 +390,18 @@ __device__ bool bfgs_advance_run(RunState& S, int r, double* __restrict__ V, dou
             break;
         }
         case PH_BFGS_END: {
-            S.total_calls += S.calls;
             if (S.prec == 1) {
-                // The complex64 stage hands its point to the FP64 stage.
-                for (int k = tid; k < P; k += nt) best[k] = x[k];
-                S.calls32 += S.calls;
+                // The complex64 stage stopped before reaching switch_cost: it hands over
+                // to FP64 the same way, the point evaluated again and the run continued
+                // with its Powell factor.
+                S.calls32 = S.calls;
                 S.prec = 0;
-                S.phase = PH_OUTER_START;
-                __syncthreads();
+                for (int k = tid; k < P; k += nt) xr[k] = x[k];
+                S.phase = PH_INIT_WAIT;
+                request = true;
                 break;
             }
+            S.total_calls += S.calls;
             // BFGS.cpp:121-133: keep an improvement and jitter the guess by the next
             // U[0,2pi)/100 draws (consumed even when no further loop follows)
             if (S.cur_min > S.f) {
