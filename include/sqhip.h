@@ -275,15 +275,18 @@ typedef struct sqh_agents_options {
 } sqh_agents_options;
 
 /* Optimization_Interface::solve_layer_optimization_problem_AGENTS (AGENTS.cpp:42-870)
- * on the context's circuit and target, FROBENIUS cost (its three-point sinusoid
- * line search), the optimiser's std::mt19937 seeded with seed.  x: in the solution
+ * on the context's circuit and target, with the line search of its cost: FROBENIUS
+ * (three-point sinusoid, AGENTS.cpp:395-443) or HILBERT_SCHMIDT_TEST (five-point fit
+ * minimised by BFGS_Powell, AGENTS.cpp:544-675); other costs are rejected (the
+ * reference has no line search for them).  The optimiser's std::mt19937 is seeded
+ * with seed.  x: in the solution
  * guess, out optimized_parameters (P doubles, host).  *f_out: current_minimum;
  * *iters_out: iterations run; *evals_out: cost evaluations (all batched on the GPU). */
 int sqh_agents(sqh_ctx* ctx, double* x, const sqh_agents_options* opts, unsigned int seed, double* f_out,
                long long* iters_out, long long* evals_out);
 
-/* n_runs independent AGENTS runs on the context's circuit (FROBENIUS cost, its trace
- * offset, up to 10 qubits), run k against the k-th complex128 rows x cols target of
+/* n_runs independent AGENTS runs on the context's circuit (FROBENIUS or
+ * HILBERT_SCHMIDT_TEST cost, its trace offset, up to 10 qubits), run k against the k-th complex128 rows x cols target of
  * targets (host, contiguous, rows = 2^qbit_num), from the guess x + k*P with its
  * generator seeded with seeds[k].  Each run's results (x + k*P, f_out[k], iters_out[k],
  * evals_out[k]) equal those of sqh_agents on that target and seed; the runs advance in

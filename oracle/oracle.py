@@ -184,6 +184,10 @@ class Oracle:
         L.or_bfgs_powell_fn.argtypes = [_FG, ctypes.c_void_p, ctypes.c_long, dp, ctypes.c_long,
                                         ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_int)]
         L.or_bfgs_powell_fn.restype = ctypes.c_double
+        L.or_agents_hs_fit.argtypes = [dp, ctypes.c_double, dp, dp]
+        L.or_agents_hs_fit.restype = None
+        L.or_agents_hs_search.argtypes = [dp, ctypes.c_double, dp, ctypes.POINTER(ctypes.c_long)]
+        L.or_agents_hs_search.restype = ctypes.c_double
 
     # -- helpers -----------------------------------------------------------
     @staticmethod
@@ -349,6 +353,23 @@ class Oracle:
         f = self.lib.or_agents(ctypes.byref(pr), seed, ctypes.byref(o), self._dp(x), ctypes.byref(it),
                                ctypes.byref(ev))
         return f, x, it.value, ev.value
+
+    def agents_hs_fit(self, fv, p):
+        """The five-point fit: (model[5], start shift) from fv = f at p, p + pi/4,
+        p + pi/2, p + pi, p + 3pi/2."""
+        fv = np.ascontiguousarray(fv, dtype=np.float64)
+        model = np.zeros(5)
+        s0 = np.zeros(1)
+        self.lib.or_agents_hs_fit(self._dp(fv), float(p), self._dp(model), self._dp(s0))
+        return model, float(s0[0])
+
+    def agents_hs_search(self, fv, p):
+        """The five-point line search: (model minimum, shift, BFGS calls)."""
+        fv = np.ascontiguousarray(fv, dtype=np.float64)
+        s = np.zeros(1)
+        calls = ctypes.c_long(0)
+        f = self.lib.or_agents_hs_search(self._dp(fv), float(p), self._dp(s), ctypes.byref(calls))
+        return f, float(s[0]), calls.value
 
     def mt_draws(self, seed, plan):
         """Interleaved libstdc++ draws from one mt19937(seed): plan is a list of

@@ -887,9 +887,49 @@ static void or_randomize_parameters(or_mt19937* g, const double* in, double* out
     }
 }
 
+/* AGENTS.cpp:611-664: f(p') = kappa sin(2p' + xi) + gamma sin(p' + phi) + offset
+ * from its values at p, p + pi/4, p + pi/2, p + pi and p + 3pi/2. */
+void or_agents_hs_fit(const double fv[5], double p, double model[5], double* shift0) {
+    const double f0 = fv[0], f_pi4 = fv[1], f_pi2 = fv[2], f_pi = fv[3], f_3pi2 = fv[4];
+    const double f1 = f0 - f_pi;      /* 2 gamma sin(p + phi) */
+    const double f2 = f_pi2 - f_3pi2; /* 2 gamma cos(p + phi) */
+    const double gamma = sqrt(f1 * f1 + f2 * f2) * 0.5;
+    const double varphi = atan2(f1, f2) - p;
+    const double offset = 0.25 * (f0 + f_pi + f_pi2 + f_3pi2);
+    const double f3 = 0.5 * (f0 + f_pi - 2 * offset);                           /* kappa sin(2p + xi) */
+    const double f4 = f_pi4 - offset - gamma * sin(p + M_PI / 4 + varphi);     /* kappa cos(2p + xi) */
+    const double kappa = sqrt(f3 * f3 + f4 * f4);
+    const double xi = atan2(f3, f4) - 2 * p;
+    model[0] = kappa;
+    model[1] = xi + 2 * p;
+    model[2] = gamma;
+    model[3] = varphi + p;
+    model[4] = offset;
+    /* the larger term's minimiser; the double-frequency start subtracts p / 2, not p,
+     * as the reference does (AGENTS.cpp:655-660) */
+    double s = fabs(gamma) > fabs(kappa) ? 3 * M_PI / 2 - varphi - p : 3 * M_PI / 4 - xi / 2 - p / 2;
+    *shift0 = fmod(s, M_PI * 2);
+}
+
+/* HS_partial_optimization_problem_combined (Bayes_Opt.cpp:34-40,58-66,83-88). */
+static void hs_model_fg(void* user, const double* s, double* f, double* g) {
+    const double* m = (const double*)user;
+    *f = m[0] * sin(2 * s[0] + m[1]) + m[2] * sin(s[0] + m[3]) + m[4];
+    g[0] = 2 * m[0] * cos(2 * s[0] + m[1]) + m[2] * cos(s[0] + m[3]);
+}
+
+/* AGENTS.cpp:664-667: BFGS_Powell on the model, Start_Optimization(shift, 10). */
+double or_agents_hs_search(const double fv[5], double p, double* shift, long* calls) {
+    double model[5];
+    or_agents_hs_fit(fv, p, model, shift);
+    return or_bfgs_powell_fn(hs_model_fg, model, 1, shift, 10, calls, 0);
+}
+
 double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* o, double* x, long* iters,
                  long* evals) {
     const int P = pr->P, A = o->agent_num;
+    const int five = pr->variant == OR_COST_HS; /* AGENTS.cpp:335-337 */
+    if (pr->variant != OR_COST_FROBENIUS && !five) return DBL_MAX;
     or_mt19937 gen;
     or_mt_seed(&gen, seed);
     long n_eval = 0;
@@ -898,6 +938,8 @@ double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* 
     double* fmin_ag = (double*)malloc(sizeof(double) * (size_t)A);
     double* f_pi2 = (double*)malloc(sizeof(double) * (size_t)A);
     double* f_pi = (double*)malloc(sizeof(double) * (size_t)A);
+    double* f_pi4 = (double*)malloc(sizeof(double) * (size_t)A);
+    double* f_3pi2 = (double*)malloc(sizeof(double) * (size_t)A);
     double* saved = (double*)malloc(sizeof(double) * (size_t)A);
     int* pidx = (int*)malloc(sizeof(int) * (size_t)A);
     double* rnd = (double*)malloc(sizeof(double) * 2 * (size_t)A);
@@ -925,23 +967,42 @@ double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* 
             pidx[a] = or_mt_uniform_int(&gen, P);
             saved[a] = ag[(size_t)a * P + pidx[a]];
         }
-        /* three-point line search on the sinusoid f(p) = offset + A cos(p - phi)
-         * (AGENTS.cpp:395-443) */
-        for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += M_PI / 2;
-        or_optimization_problem_batched(pr, ag, A, 1, f_pi2);
-        for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += M_PI / 2;
-        or_optimization_problem_batched(pr, ag, A, 1, f_pi);
-        n_eval += 2 * A;
-        for (int a = 0; a < A; a++) {
-            const double f0 = fmin_ag[a];
-            const double a_cos = (f0 - f_pi[a]) / 2;
-            const double offset = (f0 + f_pi[a]) / 2;
-            const double a_sin = offset - f_pi2[a];
-            const double phi0 = atan2(a_sin, a_cos);
-            const double shift = phi0 > 0 ? M_PI - phi0 : -phi0 - M_PI;
-            const double amp = sqrt(a_sin * a_sin + a_cos * a_cos);
-            ag[(size_t)a * P + pidx[a]] = saved[a] + shift;
-            fmin_ag[a] = offset - amp;
+        if (five) {
+            /* five-point line search (AGENTS.cpp:544-675): the parameter at p + pi/4,
+             * + pi/4, + pi/2, + pi/2, accumulated in place */
+            double* fs[4] = {f_pi4, f_pi2, f_pi, f_3pi2};
+            const double steps[4] = {M_PI / 4, M_PI / 4, M_PI / 2, M_PI / 2};
+            for (int k = 0; k < 4; k++) {
+                for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += steps[k];
+                or_optimization_problem_batched(pr, ag, A, 1, fs[k]);
+            }
+            n_eval += 4 * A;
+            for (int a = 0; a < A; a++) {
+                const double fv[5] = {fmin_ag[a], f_pi4[a], f_pi2[a], f_pi[a], f_3pi2[a]};
+                double shift;
+                const double f = or_agents_hs_search(fv, saved[a], &shift, 0);
+                ag[(size_t)a * P + pidx[a]] = saved[a] + shift;
+                fmin_ag[a] = f;
+            }
+        } else {
+            /* three-point line search on the sinusoid f(p) = offset + A cos(p - phi)
+             * (AGENTS.cpp:395-443) */
+            for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += M_PI / 2;
+            or_optimization_problem_batched(pr, ag, A, 1, f_pi2);
+            for (int a = 0; a < A; a++) ag[(size_t)a * P + pidx[a]] += M_PI / 2;
+            or_optimization_problem_batched(pr, ag, A, 1, f_pi);
+            n_eval += 2 * A;
+            for (int a = 0; a < A; a++) {
+                const double f0 = fmin_ag[a];
+                const double a_cos = (f0 - f_pi[a]) / 2;
+                const double offset = (f0 + f_pi[a]) / 2;
+                const double a_sin = offset - f_pi2[a];
+                const double phi0 = atan2(a_sin, a_cos);
+                const double shift = phi0 > 0 ? M_PI - phi0 : -phi0 - M_PI;
+                const double amp = sqrt(a_sin * a_sin + a_cos * a_cos);
+                ag[(size_t)a * P + pidx[a]] = saved[a] + shift;
+                fmin_ag[a] = offset - amp;
+            }
         }
         /* AGENTS.cpp:696-703 */
         for (int i = 0; i < 2 * A; i++) rnd[i] = or_mt_canonical(&gen);
@@ -991,6 +1052,6 @@ double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* 
     memcpy(x, best, sizeof(double) * P);
     if (iters) *iters = it;
     if (evals) *evals = n_eval;
-    free(best); free(ag); free(fmin_ag); free(f_pi2); free(f_pi); free(saved); free(pidx); free(rnd); free(win);
+    free(best); free(ag); free(fmin_ag); free(f_pi2); free(f_pi); free(f_pi4); free(f_3pi2); free(saved); free(pidx); free(rnd); free(win);
     return current_minimum;
 }

@@ -172,11 +172,24 @@ typedef struct {
     int convergence_length;          /* convergence_length [20] */
 } or_agents_opts;
 
+/* The five-point line search of the HILBERT_SCHMIDT_TEST cost (AGENTS.cpp:544-675).
+ * fv = {f(p), f(p + pi/4), f(p + pi/2), f(p + pi), f(p + 3pi/2)} along one
+ * parameter whose value is p.  model receives {kappa, xi + 2p, gamma, varphi + p,
+ * offset} of f(p + s) = kappa sin(2s + model[1]) + gamma sin(s + model[3]) + offset
+ * (Bayes_Opt.cpp:34-40), *shift0 the start shift (the larger term's minimum, fmod
+ * 2pi).  or_agents_hs_search then minimises the model over s from *shift0 with
+ * BFGS_Powell capped at 10 calls; *shift receives the shift, the model's value is
+ * returned and *calls the calls made. */
+void   or_agents_hs_fit(const double fv[5], double p, double model[5], double* shift0);
+double or_agents_hs_search(const double fv[5], double p, double* shift, long* calls);
+
 /* Optimization_Interface::solve_layer_optimization_problem_AGENTS
  * (AGENTS.cpp:42-870) for the FROBENIUS cost (the three-point sinusoid line search
- * of AGENTS.cpp:402-443) with the optimiser's mt19937 freshly seeded with seed and
- * the solution guess in x.  x receives optimized_parameters; returns
- * current_minimum; *iters the iterations run, *evals the cost evaluations. */
+ * of AGENTS.cpp:402-443) or the HILBERT_SCHMIDT_TEST cost (the five-point search
+ * above) with the optimiser's mt19937 freshly seeded with seed and the solution
+ * guess in x.  x receives optimized_parameters; returns current_minimum; *iters the
+ * iterations run, *evals the cost evaluations.  Other cost variants: returns
+ * DBL_MAX without running (the reference has no line search for them). */
 double or_agents(const or_problem* pr, unsigned int seed, const or_agents_opts* o, double* x, long* iters,
                  long* evals);
 

@@ -280,12 +280,14 @@ def test_decompose_problems_rejects_bad_problem():
         decompose_problems(ctx, [(3, gates, np.eye(4), np.zeros(P))], default_options(3))
 
 
-@pytest.mark.parametrize("n,layers,seed", [(2, 2, 1), (3, 4, 2)])
-def test_agents_matches_oracle(oracle, n, layers, seed):
+@pytest.mark.parametrize("n,layers,seed,hs", [(2, 2, 1, False), (3, 4, 2, False), (2, 2, 1, True), (3, 4, 2, True)])
+def test_agents_matches_oracle(oracle, n, layers, seed, hs):
     """AGENTS with the batched GPU cost follows the oracle's AGENTS from the same
-    generator seed and guess: same iteration count, same minimum, same parameters."""
+    generator seed and guess: same iteration count, same minimum, same parameters.
+    FROBENIUS cost (three-point line search) and HILBERT_SCHMIDT_TEST cost (five-point
+    fit minimised by the one-variable BFGS_Powell)."""
     from squander_hip import agents, agents_options
-    say(f"AGENTS n={n} layers={layers} seed={seed}")
+    say(f"AGENTS n={n} layers={layers} seed={seed} hs={hs}")
     gates = W.ansatz_layers(n, W.alternating_pairs(n, layers))
     P = W.param_count(gates)
     T = reachable_targets(n, gates, 1, 500 + seed)[0]
@@ -293,19 +295,23 @@ def test_agents_matches_oracle(oracle, n, layers, seed):
     ctx = nat.Context(0)
     ctx.set_circuit(n, gates)
     ctx.set_target(T)
+    variant = orc.COST_HS if hs else orc.COST_FROBENIUS
+    if hs:
+        ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT)
     fg, xg, itg, evg = agents(ctx, x0, seed, agents_options(max_iterations=3000, optimization_tolerance=1e-8))
-    fo, xo, ito, evo = oracle.agents(gates, T, n, seed, x0, agent_num=64, max_iterations=3000, lifetime=1000,
-                                     tolerance=1e-8)
+    fo, xo, ito, evo = oracle.agents(gates, T, n, seed, x0, variant=variant, agent_num=64, max_iterations=3000,
+                                     lifetime=1000, tolerance=1e-8)
     say(f"  gpu f={fg:.3e} it={itg} evals={evg}; oracle f={fo:.3e} it={ito} evals={evo}")
-    assert abs(oracle.optimization_problem(gates, T, n, xg) - fg) < 1e-10
+    assert abs(oracle.optimization_problem(gates, T, n, xg, variant=variant) - fg) < 1e-10
     assert itg == ito and evg == evo
     assert abs(fg - fo) < 1e-10
     d = np.mod(xg - xo + np.pi, 2 * np.pi) - np.pi
     assert np.max(np.abs(d)) < 1e-6
 
 
-@pytest.mark.parametrize("n,layers,jit", [(3, 4, False), (4, 4, True), (8, 2, False)])
-def test_agents_batch_equals_single_runs(n, layers, jit):
+@pytest.mark.parametrize("n,layers,jit,hs", [(3, 4, False, False), (4, 4, True, False), (8, 2, False, False),
+                                             (3, 4, False, True)])
+def test_agents_batch_equals_single_runs(n, layers, jit, hs):
     """Runs against several targets advanced in lockstep (one launch per stage) give
     exactly the results of one sqh_agents call per target: the same parameters,
     minimum, iteration and evaluation counts.  Also with a trace offset."""
@@ -316,7 +322,7 @@ def test_agents_batch_equals_single_runs(n, layers, jit):
     d = 1 << n
     off = 1 if n == 4 else 0
     m = d - off
-    say(f"AGENTS batch n={n} layers={layers} jit={jit} K={K} offset={off}")
+    say(f"AGENTS batch n={n} layers={layers} jit={jit} hs={hs} K={K} offset={off}")
     Ts = np.array([T[:, :m] for T in reachable_targets(n, gates, K, 700 + n)])
     x0 = np.random.default_rng(n + 900).uniform(0, 2 * np.pi, (K, P))
     seeds = np.array([11 + 7 * k for k in range(K)], dtype=np.uint32)
@@ -327,6 +333,8 @@ def test_agents_batch_equals_single_runs(n, layers, jit):
     ctx.set_circuit(n, gates)
     ctx.set_trace_offset(off)
     ctx.set_jit(jit)
+    if hs:
+        ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT)
     fb, xb, itb, evb = agents_batch(ctx, Ts, x0, seeds, opts)
     for k in range(K):
         ctx.set_target(Ts[k])
@@ -358,7 +366,7 @@ def test_agents_rejects_other_costs():
     ctx = nat.Context(0)
     ctx.set_circuit(2, gates)
     ctx.set_target(np.eye(4))
-    ctx.set_cost_function(nat.COST_HILBERT_SCHMIDT)
+    ctx.set_cost_function(nat.COST_INFIDELITY)
     with pytest.raises(nat.SqhError):
         agents(ctx, np.zeros(W.param_count(gates)), 1)
 
