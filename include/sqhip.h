@@ -306,6 +306,16 @@ int sqh_agents_batch(sqh_ctx* ctx, const double* targets, int n_runs, int rows, 
  * reports the compile time spent so far. */
 int sqh_set_jit(sqh_ctx* ctx, int enable);
 int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
+/* Cost-only batched evaluations (sqh_optimization_problem_batch[_dev], the AGENTS
+ * populations) through the circuit's fused dense 4-qubit blocks on the matrix cores
+ * with every column resident on the chip (one launch: blocks applied as MFMA tiles,
+ * LDS exchanges between blocks, trace entries at the end).  Applies for 4 to 10 qubits
+ * and costs without correction channels; elsewhere the tile kernels run.  Results
+ * agree with the tile kernels to rounding (the blocks reassociate the products).  No
+ * counterpart in the reference: it replaces the per-gate loop of
+ * Gates_block::apply_to (Gates_block.cpp:606-710) inside optimization_problem_batched
+ * (Optimization_Interface.cpp:940-1033) for callers that want cost-only throughput. */
+int sqh_set_fused_cost(sqh_ctx* ctx, int enable);
 /* Introspection (no GPU needed): generate and compile, for gfx950, the run-time
  * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient, 3 the
  * persistent solver of sqh_decompose_batch, up to 4 qubits; uniform = one batch element

@@ -117,6 +117,66 @@ int stream_apply(sqh_ctx* c, void* mats, int rows, int cols, int stride, long lo
     return SQH_OK;
 }
 
+// Row tables of the tile-resident fused cost: per block, the row offsets of the 16
+// local indices (bit k <-> the k-th smallest qubit of the block) and the row bases of
+// the 2^N / 16 outer indices (bit k <-> the k-th smallest other qubit).
+std::vector<int> fused_row_maps(int N, const FusionPlan& fp) {
+    const int nb = (int)fp.qubits.size();
+    std::vector<int> maps((size_t)nb * kFusedMapStride, 0);
+    for (int b = 0; b < nb; ++b) {
+        int* mp = maps.data() + (size_t)b * kFusedMapStride;
+        const auto& q = fp.qubits[b];
+        int outer[16], no = 0;
+        for (int k = 0; k < N; ++k)
+            if (k != q[0] && k != q[1] && k != q[2] && k != q[3]) outer[no++] = k;
+        for (int i = 0; i < 16; ++i) {
+            int r = 0;
+            for (int k = 0; k < 4; ++k)
+                if ((i >> k) & 1) r |= 1 << q[k];
+            mp[i] = r;
+        }
+        for (int o = 0; o < (1 << no); ++o) {
+            int r = 0;
+            for (int k = 0; k < no; ++k)
+                if ((o >> k) & 1) r |= 1 << outer[k];
+            mp[16 + o] = r;
+        }
+    }
+    return maps;
+}
+
+// Whether a cost-only evaluation runs through the tile-resident fused-block kernel: the
+// switch is on, 4 <= N <= 10, the circuit has blocks and the cost has no correction
+// channels.
+bool fused_cost_applies(const sqh_ctx* c, bool grad, const TraceSpec& ts) {
+    return c->fused_cost && !grad && ts.nch == 0 && c->prog.N >= 4 && c->prog.N <= kFusedCostMaxQubits &&
+           c->fplan.k == 4 && !c->fplan.qubits.empty();
+}
+
+// Costs of B parameter vectors by the tile-resident fused-block kernel: kernel tables
+// (built by the caller into kern) -> block matrices -> one launch over every column ->
+// trace entries into diag.
+int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off) {
+    const FusionPlan& fp = c->fplan;
+    const int nb = (int)fp.qubits.size(), ns = (int)c->prog.slots.size();
+    const size_t esz = f32 ? 8 : 16;
+    HIPTRY(c, c->d_fused.ensure((size_t)B * nb * 256 * esz));
+    HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, B,
+                                 c->d_fused.p, c->stream));
+    FusedCostArgs a;
+    a.U = f32 ? c->d_U32.p : c->d_U64.p;
+    a.m = c->cols;
+    a.trace_off = trace_off;
+    a.B = B;
+    a.nb = nb;
+    a.F = c->d_fused.p;
+    a.maps = c->d_fmaps.as<int>();
+    a.diag = diag;
+    TimedLaunch tl(c);
+    HIPTRY(c, launch_fused_cost(f32, c->prog.N, a, c->stream));
+    return SQH_OK;
+}
+
 // The circuit as fused dense blocks (<= 4 qubits each): one HBM pass per block,
 // matrix cores for the 4-qubit FP64 blocks.  kern holds the kernel tables built for
 // this call (per matrix when kern_bstride != 0).
@@ -192,6 +252,12 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
         return SQH_OK;
     };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
+    if (fused_cost_applies(c, d_grad != nullptr, ts) && !fixed) {
+        if ((rc = fused_cost_eval(c, f32, B, kern, diag, off))) return rc;
+        HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(), d_params,
+                                c->prog.P, nullptr, c->stream, BatchIndex(), ts));
+        return SQH_OK;
+    }
     if (N <= kMaxTileQubits) {
         TileLaunch t = base_launch(c, f32);
         t.kern = kern;
@@ -463,6 +529,11 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
     if (!fp.gates.empty())
         HIPTRY(c, hipMemcpy(c->d_fgates.p, fp.gates.data(), fp.gates.size() * sizeof(FusedRec), hipMemcpyHostToDevice));
     HIPTRY(c, hipMemcpy(c->d_ffirst.p, fp.first.data(), fp.first.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (qbit_num >= 4 && qbit_num <= kFusedCostMaxQubits && fp.k == 4 && !fp.qubits.empty()) {
+        const std::vector<int> maps = fused_row_maps(qbit_num, fp);
+        HIPTRY(c, c->d_fmaps.ensure(maps.size() * sizeof(int)));
+        HIPTRY(c, hipMemcpy(c->d_fmaps.p, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     c->fplan = std::move(fp);
     c->prog = std::move(p);
     c->gates = std::move(g);
@@ -775,6 +846,13 @@ int sqh_set_jit(sqh_ctx* c, int enable) {
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
     c->jit = enable != 0;
+    return SQH_OK;
+}
+
+int sqh_set_fused_cost(sqh_ctx* c, int enable) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->fused_cost = enable != 0;
     return SQH_OK;
 }
 

@@ -82,6 +82,10 @@ struct sqh_ctx {
     // Dense-block fusion of the circuit for the streaming path.
     sqh::FusionPlan fplan;
     sqh::DevBuf d_fgates, d_ffirst, d_fused;
+    // Row tables of the tile-resident fused-block cost kernel ([blocks][kFusedMapStride];
+    // present for 4 <= N <= 10) and its switch (sqh_set_fused_cost).
+    sqh::DevBuf d_fmaps;
+    bool fused_cost = false;
 
     // Run-time specialised tile kernels (sqh_set_jit).
     bool jit = false;
@@ -105,7 +109,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_idx, &d_seed, &d_dtab, &d_zslot})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_fmaps, &d_idx, &d_seed, &d_dtab, &d_zslot})
             b->release();
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);

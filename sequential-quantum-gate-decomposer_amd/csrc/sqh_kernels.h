@@ -72,4 +72,22 @@ hipError_t launch_broadcast_copy(bool f32, const void* src, int rows, int m, int
                                  long long dbstride, int B, hipStream_t st);
 hipError_t launch_convert(bool to_f32, const void* src, void* dst, long long nreal, hipStream_t st);
 
+// Tile-resident cost of the circuit as fused 4-qubit blocks on the matrix cores
+// (sqh_k_fused_cost.hip), 4 <= N <= kFusedCostMaxQubits.  maps: per block
+// kFusedMapStride ints, the row offsets of the 16 local indices, then the row bases of
+// the 2^N / 16 outer indices.  F: [B][nb][16][16] block matrices.  diag receives the B x m
+// trace entries (complex, as double pairs).
+constexpr int kFusedMapStride = 16 + 64;
+constexpr int kFusedCostMaxQubits = 10;
+struct FusedCostArgs {
+    const void* U = nullptr;  // target, rows = 2^N, m columns, row stride m
+    int m = 0, trace_off = 0, B = 0, nb = 0;
+    const void* F = nullptr;
+    const int* maps = nullptr;
+    double* diag = nullptr;
+    int groups = 0;  // column groups per element (set by the launcher)
+};
+hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, hipStream_t s);
+size_t fused_cost_lds_bytes(bool f32, int nb);
+
 }  // namespace sqh

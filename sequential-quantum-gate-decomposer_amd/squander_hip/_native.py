@@ -43,7 +43,7 @@ EXPORTS = [
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
     "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
-    "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check",
+    "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_set_fused_cost",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
 
@@ -157,6 +157,7 @@ def lib():
             "sqh_get_block_config": ([vp, i, i, i, i, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                       d], i),
             "sqh_set_jit": ([vp, i], i),
+            "sqh_set_fused_cost": ([vp, i], i),
             "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
             "sqh_mlp_backward_step_dev": ([vp, vp, vp, i, ctypes.c_float], i),
             "sqh_jit_compile_check": ([i, ctypes.POINTER(_Gate), i, i, i, i, ctypes.c_char_p, i,
@@ -296,6 +297,11 @@ class Context:
     def set_jit(self, enable=True):
         """Use run-time specialised kernels for this context's circuit."""
         self.check(self._L.sqh_set_jit(self.h, int(bool(enable))))
+
+    def set_fused_cost(self, enable=True):
+        """Cost-only batched evaluations through the fused 4-qubit blocks on the matrix
+        cores (4-10 qubits, costs without correction channels)."""
+        self.check(self._L.sqh_set_fused_cost(self.h, int(bool(enable))))
 
     def jit_stats(self):
         ms, n = ctypes.c_double(0.0), ctypes.c_int(0)

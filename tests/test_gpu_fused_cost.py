@@ -1,0 +1,112 @@
+"""The tile-resident fused-block cost kernel (sqh_set_fused_cost: the circuit as dense
+4-qubit blocks on the matrix cores, every column resident on the chip) against the
+FP64 oracle: complex128 costs to 1e-12, complex64 to 1e-5, at 4-10 qubits, square and
+rectangular targets, trace offsets, every cost variant it serves, the 8- and
+10-qubit benchmark circuits, and the fallback of the correction variants."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from squander_hip import _native as nat
+from squander_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(nat.COST_FROBENIUS, orc.COST_FROBENIUS), (nat.COST_HILBERT_SCHMIDT, orc.COST_HS),
+            (nat.COST_INFIDELITY, orc.COST_INFIDELITY), (nat.COST_ABS_TRACE, orc.COST_ABS_TRACE)]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _costs(ctx, ps, prec):
+    c = np.empty(ps.shape[0])
+    ctx.check(ctx._L.sqh_optimization_problem_batch(ctx.h, np.ascontiguousarray(ps).ctypes.data, ps.shape[0], prec,
+                                                     c.ctypes.data))
+    return c
+
+
+@pytest.mark.parametrize("n", [4, 5, 6, 7, 8, 9, 10])
+def test_fused_cost_matches_oracle(oracle, n):
+    gates = W.mixed_gate_circuit(n, 4 * n + 8, 3100 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    B = 5
+    ps = W.random_params(P, 3200 + n, batch=B)
+    for m, off in [(d, 0), (d // 2 + 3, 2), (3, 1)]:
+        say(f"fused cost n={n} m={m} offset={off}")
+        U = W.haar_unitary(d, 3300 + n)[:, :m].copy()
+        ctx = nat.Context(0)
+        ctx.set_circuit(n, gates)
+        ctx.set_target(U)
+        ctx.set_trace_offset(off)
+        ctx.set_fused_cost(True)
+        for nv, ov in VARIANTS:
+            ctx.set_cost_function(nv)
+            f64 = _costs(ctx, ps, nat.PREC_F64)
+            f32 = _costs(ctx, ps, nat.PREC_F32)
+            toff = off if ov in (orc.COST_FROBENIUS, orc.COST_ABS_TRACE) else 0
+            for b in range(B):
+                fo = oracle.optimization_problem(gates, U, n, ps[b], variant=ov, trace_offset=toff)
+                assert abs(f64[b] - fo) < 1e-12, (nv, b, f64[b], fo)
+                assert abs(f32[b] - fo) < 1e-5, (nv, b, f32[b], fo)
+
+
+@pytest.mark.parametrize("which", ["8q_40", "10q_60"])
+def test_fused_cost_benchmark_circuits(oracle, which):
+    """The 8-qubit / 40-layer and 10-qubit / 60-layer circuits: fused-block costs
+    against the oracle and against the tile kernels."""
+    n, gates = (8, W.ansatz_8q_40()) if which == "8q_40" else (10, W.ansatz_10q_60())
+    say(f"fused cost {which}")
+    P = W.param_count(gates)
+    U = W.haar_unitary(1 << n, 2026 + n).conj().T.copy()
+    B = 4
+    ps = W.random_params(P, 41, batch=B)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    tile64, tile32 = _costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32)
+    ctx.set_fused_cost(True)
+    f64, f32 = _costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32)
+    for b in range(B):
+        fo = oracle.optimization_problem(gates, U, n, ps[b])
+        assert abs(f64[b] - fo) < 1e-12 and abs(tile64[b] - fo) < 1e-12
+        assert abs(f32[b] - fo) < 1e-5 and abs(tile32[b] - fo) < 1e-5
+    say(f"  max |fused - oracle| c128 {np.max(np.abs(f64 - tile64)):.2e} (vs tile), c64 "
+        f"{max(abs(f32[b] - oracle.optimization_problem(gates, U, n, ps[b])) for b in range(B)):.2e}")
+
+
+def test_fused_cost_batch_equals_single_evaluations():
+    n = 8
+    gates = W.ansatz_8q_40()
+    P = W.param_count(gates)
+    U = W.haar_unitary(256, 5).conj().T.copy()
+    ps = W.random_params(P, 6, batch=37)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    ctx.set_fused_cost(True)
+    full = _costs(ctx, ps, nat.PREC_F64)
+    for b in (0, 17, 36):
+        assert _costs(ctx, ps[b:b + 1], nat.PREC_F64)[0] == full[b]
+
+
+def test_correction_variants_fall_back_to_the_tile_kernels():
+    n = 5
+    gates = W.mixed_gate_circuit(n, 20, 77)
+    P = W.param_count(gates)
+    U = W.haar_unitary(32, 78).conj().T.copy()
+    ps = W.random_params(P, 79, batch=3)
+    out = {}
+    for fused in (False, True):
+        ctx = nat.Context(0)
+        ctx.set_circuit(n, gates)
+        ctx.set_target(U)
+        ctx.set_cost_function(nat.COST_FROBENIUS_CORRECTION2)
+        ctx.set_fused_cost(fused)
+        out[fused] = _costs(ctx, ps, nat.PREC_F64)
+    assert np.array_equal(out[False], out[True])
