@@ -2,12 +2,15 @@
 
 A step is one call of optimization_problem_batched on 512 parameter vectors
 (8 qubits, U3 layer + 40 seeded-random adjacent CNOT layers: 128 gates, 264
-angles; 256 x 256 complex128 Haar target), inputs resident in HBM.  The JSON line
-also carries: the dominant kernel's roofline (live HIP-event timing inside the
-library), the unfused streaming per-gate path's HBM bandwidth on the same batch
-(512 MiB of matrices), the complex64 leg with its error against the FP64 oracle,
-the 8-qubit single-vector cost+gradient latency, and the CPU oracle timed on the
-host cores on a bounded sample.
+angles; 256 x 256 complex128 Haar target), inputs resident in HBM.  The headline
+path runs the circuit as its 6 fused dense 4-qubit blocks on the matrix cores with
+every column resident on the chip (k_fused_cost); the JSON line also carries: that
+kernel's roofline (live HIP-event timing inside the library), the gate-by-gate
+register-tile kernels (circuit-specialised and generic) on the same batch, the
+unfused streaming per-gate path's HBM bandwidth (512 MiB of matrices), the complex64
+legs with their error against the FP64 oracle, the 10-qubit / 60-layer batch, the
+8-qubit cost+gradient, and the CPU oracle timed on the host cores on a bounded
+sample.
 """
 import argparse
 import json
@@ -27,6 +30,22 @@ FP64_PEAK_TFLOPS = 78.6
 TILE_FORWARD_8Q_B512_TRAFFIC = (11922.6 * 2.0 + 2048.0) * 1024
 # the same for the circuit-specialised kernel: profiles/sqh_jit_kernel_8q_b512.md
 JIT_FORWARD_8Q_B512_TRAFFIC = (24042.5 * 2.0 + 2048.0) * 1024
+# the same for the fused-block kernel: profiles/k_fused_cost_8q_b512.md (None until profiled)
+FUSED_COST_8Q_B512_TRAFFIC = None
+FP32_MFMA_PEAK_TFLOPS = 157.3
+
+
+def _fusion_blocks(n, gates):
+    from squander_hip import _native as nat
+    return nat.inspect_fusion(n, gates)
+
+
+def block_flops(gates, n, m):
+    """Algorithmic flops of one evaluation as fused dense 4-qubit blocks: a complex
+    16 x 16 block times every 16-row group of the d x m matrix is 128 real flop per
+    element, per block of the fusion plan."""
+    from squander_hip import _native as nat
+    return len(nat.inspect_fusion(n, gates)) * 128.0 * (1 << n) * m
 
 
 def log(*a):
@@ -107,12 +126,15 @@ def cpu_baseline(gates, U, n, P, o, seconds=10.0):
     return res
 
 
-def generic_leg(gates, U, params, costs, steps, flops):
-    """The same batch with the generic tile kernel (one program walked op by op)."""
+def tile_leg(gates, U, params, costs, steps, flops, jit):
+    """The same batch with the gate-by-gate register-tile kernel: specialised to the
+    circuit (jit) or the generic one (one program walked op by op)."""
     from squander_hip import PREC_F64, Context
     ctx = Context(0)
     ctx.set_circuit(8, gates)
     ctx.set_target(U)
+    ctx.set_jit(jit)
+    ctx.set_fused_cost(False)
     B = params.shape[0]
     step = lambda: ctx.check(ctx._L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64,
                                                                       costs.data_ptr()))
@@ -127,7 +149,10 @@ def generic_leg(gates, U, params, costs, steps, flops):
     kms, nl = ctx.kernel_timing()
     k_s = kms / max(nl, 1) / 1e3
     return {"evals_per_s": B * steps / dt, "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": flops / k_s / 1e12,
-            "frac": flops / k_s / 1e12 / FP64_PEAK_TFLOPS, "kernel": "k_tile_forward<8,double,1>"}
+            "frac_of_fp64_valu_peak": flops / k_s / 1e12 / FP64_PEAK_TFLOPS,
+            "flops": "gate arithmetic (14 flop per element per 1-qubit gate)",
+            "kernel": ("sqh_jit_kernel (k_tile_forward specialised to the circuit, diagonal mode)" if jit
+                       else "k_tile_forward<8,double,1>")}
 
 
 def ten_qubit_leg(steps, o):
@@ -146,6 +171,7 @@ def ten_qubit_leg(steps, o):
     ctx.set_circuit(n, gates)
     ctx.set_target(U)
     ctx.set_jit(True)
+    ctx.set_fused_cost(False)  # the register-tile kernels here; the fused legs below
     params = torch.from_numpy(W.random_params(P, 8, batch=B)).cuda()
     costs = torch.empty(B, dtype=torch.float64, device="cuda")
     flops, _ = gate_work(gates, d, d, 16)
@@ -170,6 +196,32 @@ def ten_qubit_leg(steps, o):
         tf = flops * B / (kms / max(nl, 1) / 1e3) / 1e12
         res[name] = {"evals_per_s": B * k / dt, "kernel_avg_ms": kms / max(nl, 1), "achieved_TFLOPs": tf,
                      "peak_TFLOPs": peak, "frac": tf / peak,
+                     "err_vs_fp64_oracle": float(np.abs(costs[:2].cpu().numpy() - ref).max())}
+        log(f"10q {name}: {res[name]['evals_per_s']:.0f} evals/s, {tf:.1f} TFLOP/s")
+    # the same through the fused dense 4-qubit blocks on the matrix cores
+    fctx = Context(0)
+    fctx.set_circuit(n, gates)
+    fctx.set_target(U)
+    fctx.set_fused_cost(True)
+    bfl = block_flops(gates, n, d)
+    for name, prec, peak in (("fused_c128", PREC_F64, FP64_PEAK_TFLOPS), ("fused_c64", PREC_F32, FP32_MFMA_PEAK_TFLOPS)):
+        step = lambda: fctx.check(fctx._L.sqh_optimization_problem_batch_dev(fctx.h, params.data_ptr(), B, prec,
+                                                                            costs.data_ptr()))
+        step()
+        fctx.synchronize()
+        fctx.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        k = max(2, steps // 4)
+        for _ in range(k):
+            step()
+        fctx.synchronize()
+        dt = time.perf_counter() - t0
+        kms, nl = fctx.kernel_timing()
+        fctx.set_kernel_timing(False)
+        tf = bfl * B / (kms / max(nl, 1) / 1e3) / 1e12
+        res[name] = {"evals_per_s": B * k / dt, "kernel_avg_ms": kms / max(nl, 1), "achieved_TFLOPs": tf,
+                     "peak_TFLOPs": peak, "frac": tf / peak, "blocks": len(_fusion_blocks(n, gates)),
+                     "algorithmic_flops_per_eval": bfl,
                      "err_vs_fp64_oracle": float(np.abs(costs[:2].cpu().numpy() - ref).max())}
         log(f"10q {name}: {res[name]['evals_per_s']:.0f} evals/s, {tf:.1f} TFLOP/s")
     # complex128 cost + gradient over the batch (specialised adjoint kernel)
@@ -456,7 +508,10 @@ def main():
     ap.add_argument("--only-gradient", action="store_true",
                     help="time the batched 8-qubit cost + gradient only (for profiling the adjoint kernel)")
     ap.add_argument("--skip-solves", action="store_true", help="skip the decomposition legs (4q solves, uneven batch)")
-    ap.add_argument("--generic", action="store_true", help="headline with the generic (not circuit-specialised) kernel")
+    ap.add_argument("--headline", choices=["fused", "jit", "generic"], default="fused",
+                    help="headline kernel: fused 4-qubit blocks on the matrix cores (default), the "
+                         "circuit-specialised register-tile kernel, or the generic one")
+    ap.add_argument("--generic", action="store_true", help="same as --headline generic")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -481,11 +536,21 @@ def main():
     B = args.batch
     U = W.haar_unitary(d, 2026).conj().T.copy()
 
+    headline = "generic" if args.generic else args.headline
+    # the context of the other legs: circuit-specialised register-tile kernels (compiled
+    # on first use) for the cost-only legs that name them
     ctx = Context(0)
     ctx.set_circuit(n, gates)
     ctx.set_target(U)
-    ctx.set_jit(not args.generic)  # circuit-specialised kernels (compiled in the first warm-up step)
+    ctx.set_jit(True)
+    ctx.set_fused_cost(False)
     L = ctx._L
+    hctx = ctx
+    if headline != "jit":
+        hctx = Context(0)
+        hctx.set_circuit(n, gates)
+        hctx.set_target(U)
+        hctx.set_fused_cost(headline == "fused")
     params = torch.from_numpy(W.random_params(P, 7, batch=B)).to(dev)
     costs = torch.empty(B, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
@@ -497,32 +562,35 @@ def main():
         return
 
     def step(prec=PREC_F64):
-        ctx.check(L.sqh_optimization_problem_batch_dev(ctx.h, params.data_ptr(), B, prec, costs.data_ptr()))
+        hctx.check(L.sqh_optimization_problem_batch_dev(hctx.h, params.data_ptr(), B, prec, costs.data_ptr()))
 
     log(f"warmup {args.warmup} steps")
     t_w = time.perf_counter()
     for _ in range(max(args.warmup, 1)):
         step()
-    ctx.synchronize()
-    jit_ms, _ = ctx.jit_stats()
+    hctx.synchronize()
+    jit_ms, _ = hctx.jit_stats()
     log(f"warm-up done in {time.perf_counter() - t_w:.1f} s (run-time kernel compilation {jit_ms:.0f} ms)")
-    ctx.kernel_timing()
-    ctx.set_kernel_timing(True)
-    log(f"timing {args.steps} steps of B={B} cost evaluations (FP64)")
+    hctx.kernel_timing()
+    hctx.set_kernel_timing(True)
+    log(f"timing {args.steps} steps of B={B} cost evaluations (FP64, headline kernel: {headline})")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step()
-    ctx.synchronize()
+    hctx.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kms, nl = ctx.kernel_timing()
-    ctx.set_kernel_timing(False)
+    kms, nl = hctx.kernel_timing()
+    hctx.set_kernel_timing(False)
     wall = t1 - t0
     value = B * args.steps / wall
     k_avg_s = kms / max(nl, 1) / 1e3
     flops_eval, _ = gate_work(gates, d, d, 16)
-    achieved = flops_eval * B / k_avg_s / 1e12
+    # the fused kernel's algorithmic flops are its blocks' (0.62x the gate arithmetic)
+    flops_launch = (block_flops(gates, n, d) if headline == "fused" else flops_eval) * B
+    achieved = flops_launch / k_avg_s / 1e12
+    log(f"headline: {value:.0f} evals/s, kernel {k_avg_s * 1e3:.3f} ms, {achieved:.1f} TFLOP/s")
 
     # error of the GPU costs against the FP64 oracle on a sample
     o = orc.Oracle()
@@ -551,15 +619,25 @@ def main():
         "err_vs_fp64_oracle": err64,
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": (None if B != 512 else
-                                 TILE_FORWARD_8Q_B512_TRAFFIC if args.generic else JIT_FORWARD_8Q_B512_TRAFFIC),
-                     "kernel": ("k_tile_forward<8,double,1>" if args.generic else
-                                "sqh_jit_kernel (k_tile_forward specialised to the circuit, diagonal mode)"),
-                     "kernel_avg_ms": k_avg_s * 1e3, "algorithmic_flops_per_launch": flops_eval * B},
+                     "traffic": (None if B != 512 else {"fused": FUSED_COST_8Q_B512_TRAFFIC,
+                                                        "jit": JIT_FORWARD_8Q_B512_TRAFFIC,
+                                                        "generic": TILE_FORWARD_8Q_B512_TRAFFIC}[headline]),
+                     "kernel": {"fused": "k_fused_cost<double,8> (6 fused 4-qubit blocks, v_mfma_f64_16x16x4f64)",
+                                "jit": "sqh_jit_kernel (k_tile_forward specialised to the circuit, diagonal mode)",
+                                "generic": "k_tile_forward<8,double,1>"}[headline],
+                     "peak_note": ("FP64 dense matrix-core peak" if headline == "fused" else
+                                   "FP64 vector peak (equal to the FP64 matrix peak on MI355X): a VALU kernel"),
+                     "kernel_avg_ms": k_avg_s * 1e3, "algorithmic_flops_per_launch": flops_launch,
+                     "flops": ("fused blocks: 128 flop per element per block, 6 blocks" if headline == "fused"
+                               else "gate arithmetic: 14 flop per element per 1-qubit gate")},
         "kernel_compile_ms": jit_ms,
     }
-    if not args.generic and not args.only_main:
-        out["generic_kernel"] = generic_leg(gates, U, params, costs, args.steps, flops_eval * B)
+    if not args.only_main:
+        if headline != "jit":
+            out["tile_kernel_specialised"] = tile_leg(gates, U, params, costs, args.steps, flops_eval * B, True)
+            log(f"tile kernel (specialised): {out['tile_kernel_specialised']['evals_per_s']:.0f} evals/s")
+        if headline != "generic":
+            out["tile_kernel_generic"] = tile_leg(gates, U, params, costs, args.steps, flops_eval * B, False)
     if args.copy_calibration:
         nbytes = 256 << 20
         src = torch.ones(nbytes, dtype=torch.uint8, device=dev)
@@ -573,27 +651,41 @@ def main():
         print(json.dumps(out), flush=True)
         return
 
-    # complex64 leg
-    for _ in range(2):
-        step(PREC_F32)
-    ctx.synchronize()
-    ctx.set_kernel_timing(True)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(PREC_F32)
-    ctx.synchronize()
-    t1 = time.perf_counter()
-    kms32, nl32 = ctx.kernel_timing()
-    ctx.set_kernel_timing(False)
-    err32 = float(np.abs(costs[:8].cpu().numpy() - ref).max())
-    k32_s = kms32 / max(nl32, 1) / 1e3
-    # packed FP32 (two FMAs per lane per cycle, which the 157.3 TFLOP/s figure assumes)
-    # is not generated here, so the attainable FP32 rate equals the FP64 one
-    out["c64"] = {"value": B * args.steps / (t1 - t0), "unit": "cost evals/s", "err_vs_fp64_oracle": err32,
-                  "kernel_avg_ms": k32_s * 1e3, "achieved_TFLOPs": flops_eval * B / k32_s / 1e12,
-                  "peak_TFLOPs_packed": 157.3, "peak_TFLOPs_unpacked": FP64_PEAK_TFLOPS,
-                  "frac_of_unpacked_peak": flops_eval * B / k32_s / 1e12 / FP64_PEAK_TFLOPS}
-    log(f"c64: {out['c64']['value']:.0f} evals/s, err {err32:.2e}")
+    # complex64 legs: the headline path and the specialised register-tile kernel
+    def c64_leg(c, fl, fused):
+        run = lambda: c.check(L.sqh_optimization_problem_batch_dev(c.h, params.data_ptr(), B, PREC_F32,
+                                                                   costs.data_ptr()))
+        for _ in range(2):
+            run()
+        c.synchronize()
+        c.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        c.synchronize()
+        t1 = time.perf_counter()
+        kms32, nl32 = c.kernel_timing()
+        c.set_kernel_timing(False)
+        k32_s = kms32 / max(nl32, 1) / 1e3
+        tf = fl / k32_s / 1e12
+        r = {"value": B * args.steps / (t1 - t0), "unit": "cost evals/s",
+             "err_vs_fp64_oracle": float(np.abs(costs[:8].cpu().numpy() - ref).max()),
+             "kernel_avg_ms": k32_s * 1e3, "achieved_TFLOPs": tf}
+        if fused:
+            r.update({"kernel": "k_fused_cost<float,8> (v_mfma_f32_16x16x4f32, FP32 accumulation)",
+                      "peak_TFLOPs": FP32_MFMA_PEAK_TFLOPS, "frac": tf / FP32_MFMA_PEAK_TFLOPS})
+        else:
+            # packed FP32 (two FMAs per lane per cycle, which the 157.3 TFLOP/s figure
+            # assumes) is not generated here, so the attainable FP32 VALU rate is the FP64 one
+            r.update({"kernel": "sqh_jit_kernel (complex64)", "peak_TFLOPs_unpacked": FP64_PEAK_TFLOPS,
+                      "frac_of_unpacked_peak": tf / FP64_PEAK_TFLOPS})
+        return r
+
+    out["c64"] = c64_leg(hctx, flops_launch, headline == "fused")
+    log(f"c64: {out['c64']['value']:.0f} evals/s, err {out['c64']['err_vs_fp64_oracle']:.2e}")
+    if headline != "jit":
+        out["c64_tile_kernel_specialised"] = c64_leg(ctx, flops_eval * B, False)
+        log(f"c64 tile kernel: {out['c64_tile_kernel_specialised']['value']:.0f} evals/s")
 
     def apply_err(fn, f32=False):
         """Max relative Frobenius error of two matrices applied once against the oracle."""

@@ -307,15 +307,32 @@ int sqh_agents_batch(sqh_ctx* ctx, const double* targets, int n_runs, int rows, 
 int sqh_set_jit(sqh_ctx* ctx, int enable);
 int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
 /* Cost-only batched evaluations (sqh_optimization_problem_batch[_dev], the AGENTS
- * populations) through the circuit's fused dense 4-qubit blocks on the matrix cores
- * with every column resident on the chip (one launch: blocks applied as MFMA tiles,
- * LDS exchanges between blocks, trace entries at the end).  Applies for 4 to 10 qubits
- * and costs without correction channels; elsewhere the tile kernels run.  Results
- * agree with the tile kernels to rounding (the blocks reassociate the products).  No
- * counterpart in the reference: it replaces the per-gate loop of
- * Gates_block::apply_to (Gates_block.cpp:606-710) inside optimization_problem_batched
- * (Optimization_Interface.cpp:940-1033) for callers that want cost-only throughput. */
-int sqh_set_fused_cost(sqh_ctx* ctx, int enable);
+ * populations, single and batched) through the circuit's fused dense 4-qubit blocks on
+ * the matrix cores with every column resident on the chip (one launch: blocks applied
+ * as MFMA tiles, LDS exchanges between blocks, trace entries at the end).  Applies for
+ * 4 to 10 qubits and costs without correction channels; elsewhere the tile kernels
+ * run.  Results agree with the tile kernels to rounding (the blocks reassociate the
+ * products).  mode: SQH_FUSED_COST_OFF, _ON, or _AUTO (the default: on from 7 qubits,
+ * or from 5 when the run-time specialised kernels are off -- where it measured faster
+ * than the tile kernels).  No counterpart in the reference: it replaces the per-gate
+ * loop of Gates_block::apply_to (Gates_block.cpp:606-710) inside
+ * optimization_problem_batched (Optimization_Interface.cpp:940-1033). */
+#define SQH_FUSED_COST_OFF 0
+#define SQH_FUSED_COST_ON 1
+#define SQH_FUSED_COST_AUTO 2
+int sqh_set_fused_cost(sqh_ctx* ctx, int mode);
+/* Launch shape of the fused-block cost kernel: 16-column tiles per wavefront (2 or 4),
+ * LDS exchange groups per block (1, or 2 with 4 tiles and prefetch: one group's
+ * exchange overlaps the other's MFMAs), wavefronts per workgroup (1, 2 or 4), and
+ * prefetch (1: the next block's matrix is loaded while a block runs, at the cost of
+ * registers).  All zero (the default):
+ * the first evaluation of each (precision, qubits, columns, batch) shape times the
+ * valid shapes on its own inputs and keeps the fastest.  Every shape gives
+ * bit-identical results.  sqh_get_fused_cost_config reports the shape in use for a
+ * precision and batch size of the context's circuit and target. */
+int sqh_set_fused_cost_config(sqh_ctx* ctx, int tiles_per_wave, int exchange_groups, int waves, int prefetch);
+int sqh_get_fused_cost_config(sqh_ctx* ctx, int is_f32, int B, int* tiles_per_wave, int* exchange_groups,
+                              int* waves, int* prefetch);
 /* Introspection (no GPU needed): generate and compile, for gfx950, the run-time
  * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient, 3 the
  * persistent solver of sqh_decompose_batch, up to 4 qubits; uniform = one batch element

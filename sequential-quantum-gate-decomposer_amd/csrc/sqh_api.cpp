@@ -145,18 +145,25 @@ std::vector<int> fused_row_maps(int N, const FusionPlan& fp) {
     return maps;
 }
 
-// Whether a cost-only evaluation runs through the tile-resident fused-block kernel: the
-// switch is on, 4 <= N <= 10, the circuit has blocks and the cost has no correction
-// channels.
+// Whether a cost-only evaluation runs through the tile-resident fused-block kernel:
+// 4 <= N <= 10, the circuit has blocks, the cost has no correction channels, and the
+// switch is on -- or automatic (the default) where it measured faster on MI355X than
+// the register-tile kernels: from 7 qubits, or from 5 when the circuit-specialised
+// kernels are off.
 bool fused_cost_applies(const sqh_ctx* c, bool grad, const TraceSpec& ts) {
-    return c->fused_cost && !grad && ts.nch == 0 && c->prog.N >= 4 && c->prog.N <= kFusedCostMaxQubits &&
-           c->fplan.k == 4 && !c->fplan.qubits.empty();
+    const int N = c->prog.N;
+    if (grad || ts.nch != 0 || N < 4 || N > kFusedCostMaxQubits || c->fplan.k != 4 || c->fplan.qubits.empty())
+        return false;
+    if (c->fused_cost == SQH_FUSED_COST_AUTO) return N >= 7 || (N >= 5 && !c->jit);
+    return c->fused_cost == SQH_FUSED_COST_ON;
 }
 
 // Costs of B parameter vectors by the tile-resident fused-block kernel: kernel tables
 // (built by the caller into kern) -> block matrices -> one launch over every column ->
-// trace entries into diag.
-int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off) {
+// trace entries into diag.  U: the target (rows x m, row stride m), or with in_index
+// element b's target at U + in_index[b] * in_bstride.
+int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off, const void* U, int m,
+                    const int* in_index, long long in_bstride) {
     const FusionPlan& fp = c->fplan;
     const int nb = (int)fp.qubits.size(), ns = (int)c->prog.slots.size();
     const size_t esz = f32 ? 8 : 16;
@@ -164,8 +171,10 @@ int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, B,
                                  c->d_fused.p, c->stream));
     FusedCostArgs a;
-    a.U = f32 ? c->d_U32.p : c->d_U64.p;
-    a.m = c->cols;
+    a.U = U;
+    a.in_index = in_index;
+    a.in_bstride = in_bstride;
+    a.m = m;
     a.trace_off = trace_off;
     a.B = B;
     a.nb = nb;
@@ -173,8 +182,7 @@ int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     a.maps = c->d_fmaps.as<int>();
     a.diag = diag;
     TimedLaunch tl(c);
-    HIPTRY(c, launch_fused_cost(f32, c->prog.N, a, c->stream));
-    return SQH_OK;
+    return fused_cost_launch(c, f32, a);
 }
 
 // The circuit as fused dense blocks (<= 4 qubits each): one HBM pass per block,
@@ -253,7 +261,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
     };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
     if (fused_cost_applies(c, d_grad != nullptr, ts) && !fixed) {
-        if ((rc = fused_cost_eval(c, f32, B, kern, diag, off))) return rc;
+        if ((rc = fused_cost_eval(c, f32, B, kern, diag, off, U, m, nullptr, 0))) return rc;
         HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(), d_params,
                                 c->prog.P, nullptr, c->stream, BatchIndex(), ts));
         return SQH_OK;
@@ -358,7 +366,10 @@ int eval_targets(sqh_ctx* c, const double* d_params, int B, const void* d_target
     t.diag = c->d_diag.as<double>();
     t.ts = ts;
     t.mode = 1;
-    {
+    if (fused_cost_applies(c, false, ts)) {
+        if ((rc = fused_cost_eval(c, false, B, c->d_kern.p, t.diag, off, d_targets, m, d_index, t.in_bstride)))
+            return rc;
+    } else {
         TimedLaunch tl(c);
         if (jit_wants(c, 1)) {
             if ((rc = jit_launch(c, t, 1, c->stream))) return rc;
@@ -535,6 +546,7 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
         HIPTRY(c, hipMemcpy(c->d_fmaps.p, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     c->fplan = std::move(fp);
+    c->fused_tuned.clear();
     c->prog = std::move(p);
     c->gates = std::move(g);
     c->have_circuit = true;
@@ -849,10 +861,12 @@ int sqh_set_jit(sqh_ctx* c, int enable) {
     return SQH_OK;
 }
 
-int sqh_set_fused_cost(sqh_ctx* c, int enable) {
+int sqh_set_fused_cost(sqh_ctx* c, int mode) {
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
-    c->fused_cost = enable != 0;
+    if (mode != SQH_FUSED_COST_OFF && mode != SQH_FUSED_COST_ON && mode != SQH_FUSED_COST_AUTO)
+        return fail(c, SQH_ERR_ARG, "fused cost mode must be 0 (off), 1 (on) or 2 (automatic)");
+    c->fused_cost = mode;
     return SQH_OK;
 }
 

@@ -85,7 +85,11 @@ struct sqh_ctx {
     // Row tables of the tile-resident fused-block cost kernel ([blocks][kFusedMapStride];
     // present for 4 <= N <= 10) and its switch (sqh_set_fused_cost).
     sqh::DevBuf d_fmaps;
-    bool fused_cost = false;
+    int fused_cost = SQH_FUSED_COST_AUTO;
+    // its launch shape: forced (sqh_set_fused_cost_config) or picked per (precision,
+    // qubits, columns, batch) by the run-time tuner
+    sqh::FusedCostCfg fused_force;
+    std::map<std::array<long long, 4>, sqh::FusedCostCfg> fused_tuned;
 
     // Run-time specialised tile kernels (sqh_set_jit).
     bool jit = false;
@@ -197,6 +201,8 @@ int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out);
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
                               long long kern_bstride, int Bk, void* out, hipStream_t s);
 size_t fused_gate_bytes();
+// The fused-block cost launch with the context's launch shape (forced, tuned or default).
+int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a);
 
 // Brackets one kernel launch with timer events when timing is enabled.
 struct TimedLaunch {

@@ -171,36 +171,58 @@ struct FusedGate {
     int32_t slot;            // parametric slot (kernel table index) or -1
 };
 
+// Gates staged per pass and batch elements per workgroup (x 16 block columns = one
+// wavefront) of k_fuse_blocks.
+constexpr int kFuseChunk = 32;
+constexpr int kFuseElems = 4;
+
+// Workgroup -> (block, 16 batch elements); thread -> (element, column j of the block).
+// The block's gate records and the elements' 2x2 kernels of those gates are staged in
+// LDS a chunk at a time (all loads in flight together), then every thread applies the
+// gates to its unit column e_j.
 template <typename R>
-__global__ void k_fuse_blocks(const FusedGate* __restrict__ fg, const int* __restrict__ block_first, int nblocks,
-                              const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
-                              cpx<R>* __restrict__ out) {
-    const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (long long)Bk * nblocks * 16) return;
-    const int j = (int)(tid % 16);
-    const int blk = (int)((tid / 16) % nblocks);
-    const int b = (int)(tid / (16 * nblocks));
+__global__ __launch_bounds__(64) void k_fuse_blocks(const FusedGate* __restrict__ fg,
+                                                     const int* __restrict__ block_first, int nblocks,
+                                                     const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
+                                                     cpx<R>* __restrict__ out) {
+    __shared__ FusedGate srec[kFuseChunk];
+    __shared__ cpx<R> skern[kFuseElems][kFuseChunk][4];
+    const int blk = blockIdx.x % nblocks;
+    const int b0 = (blockIdx.x / nblocks) * kFuseElems;
+    const int le = threadIdx.x >> 4, j = threadIdx.x & 15;
     cpx<R> v[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[i] = mk<R>(i == j ? (R)1 : (R)0, (R)0);
-    const cpx<R>* kb = kern + (long long)b * kern_bstride;
-    for (int p = block_first[blk]; p < block_first[blk + 1]; ++p) {
-        const FusedGate gt = fg[p];
-        Op op;
-        op.kind = (uint8_t)gt.kind;
-        op.t = (uint8_t)gt.t;
-        op.cmode = gt.c >= 0 ? CM_REG : CM_NONE;
-        op.c = (uint8_t)(gt.c >= 0 ? gt.c : 0);
-        op.idx = gt.slot;
-        cpx<R> k[4];
-        if (gt.slot >= 0) {
-            k[0] = kb[4 * gt.slot];
-            k[1] = kb[4 * gt.slot + 1];
-            k[2] = kb[4 * gt.slot + 2];
-            k[3] = kb[4 * gt.slot + 3];
+    const int first = block_first[blk], last = block_first[blk + 1];
+    for (int p0 = first; p0 < last; p0 += kFuseChunk) {
+        const int cnt = min(kFuseChunk, last - p0);
+        __syncthreads();  // the previous chunk is consumed
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) srec[i] = fg[p0 + i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < kFuseElems * cnt * 4; i += blockDim.x) {
+            const int e = i / (cnt * 4), gi = (i / 4) % cnt, k = i % 4;
+            const int slot = srec[gi].slot;
+            if (slot >= 0 && b0 + e < Bk) skern[e][gi][k] = kern[(long long)(b0 + e) * kern_bstride + 4 * slot + k];
         }
-        exec_gate<4, false, R>(v, op, k, 0);
+        __syncthreads();
+        for (int gi = 0; gi < cnt; ++gi) {
+            const FusedGate gt = srec[gi];
+            Op op;
+            op.kind = (uint8_t)gt.kind;
+            op.t = (uint8_t)gt.t;
+            op.cmode = gt.c >= 0 ? CM_REG : CM_NONE;
+            op.c = (uint8_t)(gt.c >= 0 ? gt.c : 0);
+            op.idx = gt.slot;
+            cpx<R> k[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+            if (gt.slot >= 0) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) k[e] = skern[le][gi][e];
+            }
+            exec_gate<4, false, R>(v, op, k, 0);
+        }
     }
+    const int b = b0 + le;
+    if (b >= Bk) return;
     // out[b][blk][i][j]
     cpx<R>* o = out + ((long long)b * nblocks + blk) * 256;
 #pragma unroll
@@ -280,16 +302,15 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
                               long long kern_bstride, int Bk, void* out, hipStream_t s) {
-    const long long n = (long long)Bk * nblocks * 16;
-    if (n == 0) return hipSuccess;
+    if ((long long)Bk * nblocks == 0) return hipSuccess;
+    const long long grid = (long long)nblocks * ((Bk + kFuseElems - 1) / kFuseElems);
+    const int threads = 16 * kFuseElems;
     if (f32)
-        hipLaunchKernelGGL(k_fuse_blocks<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           (const FusedGate*)fg, block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk,
-                           (cpx<float>*)out);
+        hipLaunchKernelGGL(k_fuse_blocks<float>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
+                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out);
     else
-        hipLaunchKernelGGL(k_fuse_blocks<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
-                           (const FusedGate*)fg, block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk,
-                           (cpx<double>*)out);
+        hipLaunchKernelGGL(k_fuse_blocks<double>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
+                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out);
     return hipGetLastError();
 }
 

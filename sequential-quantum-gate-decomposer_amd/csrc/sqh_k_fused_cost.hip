@@ -69,16 +69,24 @@ __device__ __forceinline__ int fc_slot(int j) {
 // moving the loads up).
 __device__ __forceinline__ void fc_wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
-// Wavefronts per workgroup: two keep the complex128 LDS regions (16 KiB each) and the
-// row tables inside the 64 KiB a workgroup may allocate.
-constexpr int kFcWaves = 2;
-
-template <typename R, int N>
-__global__ __launch_bounds__(64 * kFcWaves) void k_fused_cost(FusedCostArgs a) {
+// One workgroup's launch shape (FusedCostCfg): T 16-column tiles per wavefront (2 or
+// 4: 32 or 64 vector columns), H exchange groups per block (the tiles go through LDS in
+// H independent groups, so the exchange of one overlaps the MFMAs of the next; a group
+// must hold whole columns), PF: block b + 1's matrix loaded while block b runs (more
+// registers, fewer waves), waves per workgroup at run time.  Every shape performs the
+// same operations on every column in the same order: results are bit-identical.
+//
+// Workgroups are ordered for the XCDs: hardware workgroup i runs on XCD i mod 8, and the
+// logical order gives each XCD a contiguous range of (element, column group) work, so
+// an element's block matrices and the target stay in one XCD's L2.
+template <typename R, int N, int T, int H, bool PF>
+__global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
     typedef FcMfma<R> MF;
     constexpr int D = 1 << N;
-    constexpr int O = D >> 4;  // outer groups per column
-    constexpr int W = 64 / O;  // columns per wavefront
+    constexpr int O = D >> 4;      // outer groups per column
+    constexpr int W = 16 * T / O;  // columns per wavefront
+    constexpr int TH = T / H;      // tiles per exchange group
+    static_assert(W >= 1 && (16 * TH) % O == 0, "an exchange group holds whole columns");
     extern __shared__ __align__(16) unsigned char fc_smem[];
     int* smap = reinterpret_cast<int*>(fc_smem);
     const int nmap = a.nb * kFusedMapStride;
@@ -86,28 +94,30 @@ __global__ __launch_bounds__(64 * kFcWaves) void k_fused_cost(FusedCostArgs a) {
     for (int i = threadIdx.x; i < nmap; i += blockDim.x) smap[i] = a.maps[i];
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const long long wave = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long per_xcd = gridDim.x >> 3;  // the grid is a multiple of 8
+    const long long lwg = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const long long wave = lwg * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (wave >= (long long)a.B * a.groups) return;
     const int e = (int)(wave / a.groups);
     const int c0 = (int)(wave % a.groups) * W;
-    cpx<R>* lds = region + (threadIdx.x >> 6) * 1024;
+    cpx<R>* lds = region + (threadIdx.x >> 6) * (16 * T * 16);
     const int nl = lane & 15, g = lane >> 4;
     // vector column 16 t + nl of tile t: column c0 + w[t], outer group o[t]
-    int w[4], o[4];
+    int w[T], o[T];
 #pragma unroll
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < T; ++t) {
         const int n = 16 * t + nl;
         w[t] = n / O;
         o[t] = n % O;
     }
-    const cpx<R>* U = static_cast<const cpx<R>*>(a.U);
+    const cpx<R>* U = static_cast<const cpx<R>*>(a.U) + (a.in_index ? (long long)a.in_index[e] * a.in_bstride : 0);
     const int m = a.m;
     // B operand of step s, tile t: X[4 s + g][16 t + nl] (block 0's grouping, from U)
-    R xr[4][4], xi[4][4];
+    R xr[T][4], xi[T][4];
     {
         const int* mp = smap;
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < T; ++t) {
             const int col = c0 + w[t];
             const int base = mp[16 + o[t]];
 #pragma unroll
@@ -120,67 +130,108 @@ __global__ __launch_bounds__(64 * kFcWaves) void k_fused_cost(FusedCostArgs a) {
         }
     }
     const cpx<R>* Fe = static_cast<const cpx<R>*>(a.F) + (long long)e * a.nb * 256;
-    for (int b = 0; b < a.nb; ++b) {
-        // A operand of step s: F_b[nl][4 s + g] (and its negated imaginary part)
-        R ar[4], ai[4], ain[4];
+    // A operand of step s: F_b[nl][4 s + g] (with PF, block b + 1's is loaded while
+    // block b runs)
+    R far[4], fai[4];
+    if constexpr (PF) {
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const cpx<R> f = Fe[(long long)b * 256 + nl * 16 + 4 * s + g];
-            ar[s] = f.re;
-            ai[s] = f.im;
-            ain[s] = -f.im;
+            const cpx<R> f = Fe[nl * 16 + 4 * s + g];
+            far[s] = f.re;
+            fai[s] = f.im;
         }
-        typename MF::V yr[4], yi[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            yr[t] = typename MF::V{0, 0, 0, 0};
-            yi[t] = typename MF::V{0, 0, 0, 0};
+    }
+    for (int b = 0; b < a.nb; ++b) {
+        R ar[4], ai[4], ain[4];
+        if constexpr (PF) {
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                yr[t] = MF::op(ar[s], xr[t][s], yr[t]);
-                yr[t] = MF::op(ain[s], xi[t][s], yr[t]);
-                yi[t] = MF::op(ar[s], xi[t][s], yi[t]);
-                yi[t] = MF::op(ai[s], xr[t][s], yi[t]);
+                ar[s] = far[s];
+                ai[s] = fai[s];
+                ain[s] = -fai[s];
+            }
+            if (b + 1 < a.nb) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const cpx<R> f = Fe[(long long)(b + 1) * 256 + nl * 16 + 4 * s + g];
+                    far[s] = f.re;
+                    fai[s] = f.im;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> f = Fe[(long long)b * 256 + nl * 16 + 4 * s + g];
+                ar[s] = f.re;
+                ai[s] = f.im;
+                ain[s] = -f.im;
             }
         }
         const int* mp = smap + b * kFusedMapStride;
-        if (b + 1 < a.nb) {
-            // through LDS in natural row order, back in block b + 1's grouping
+        const int* mn = mp + kFusedMapStride;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int base = mp[16 + o[t]];
+        for (int h = 0; h < H; ++h) {
+            typename MF::V yr[TH], yi[TH];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = base + mp[MF::row(r, g)];
-                    lds[w[t] * D + fc_slot<N>(j)] = mk<R>(yr[t][r], yi[t][r]);
+            for (int u = 0; u < TH; ++u) {
+                yr[u] = typename MF::V{0, 0, 0, 0};
+                yi[u] = typename MF::V{0, 0, 0, 0};
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+#pragma unroll
+                for (int u = 0; u < TH; ++u) {
+                    const int t = h * TH + u;
+                    yr[u] = MF::op(ar[s], xr[t][s], yr[u]);
+                    yi[u] = MF::op(ar[s], xi[t][s], yi[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < TH; ++u) {
+                    const int t = h * TH + u;
+                    yr[u] = MF::op(ain[s], xi[t][s], yr[u]);
+                    yi[u] = MF::op(ai[s], xr[t][s], yi[u]);
                 }
             }
-            fc_wave_fence();
-            const int* mn = mp + kFusedMapStride;
+            if (b + 1 < a.nb) {
+                // through LDS in natural row order, back in block b + 1's grouping
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int base = mn[16 + o[t]];
+                for (int u = 0; u < TH; ++u) {
+                    const int t = h * TH + u;
+                    const int base = mp[16 + o[t]];
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const cpx<R> v = lds[w[t] * D + fc_slot<N>(base + mn[4 * s + g])];
-                    xr[t][s] = v.re;
-                    xi[t][s] = v.im;
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = base + mp[MF::row(r, g)];
+                        lds[w[t] * D + fc_slot<N>(j)] = mk<R>(yr[u][r], yi[u][r]);
+                    }
                 }
-            }
-            fc_wave_fence();
-        } else {
-            // trace entries: row c + trace_off of column c
+                fc_wave_fence();
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int col = c0 + w[t];
-                const int base = mp[16 + o[t]];
+                for (int u = 0; u < TH; ++u) {
+                    const int t = h * TH + u;
+                    const int base = mn[16 + o[t]];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int j = base + mp[MF::row(r, g)];
-                    if (col < m && j == col + a.trace_off) {
-                        double* d = a.diag + 2 * ((long long)e * m + col);
-                        d[0] = (double)yr[t][r];
-                        d[1] = (double)yi[t][r];
+                    for (int s = 0; s < 4; ++s) {
+                        const cpx<R> v = lds[w[t] * D + fc_slot<N>(base + mn[4 * s + g])];
+                        xr[t][s] = v.re;
+                        xi[t][s] = v.im;
+                    }
+                }
+                fc_wave_fence();
+            } else {
+                // trace entries: row c + trace_off of column c
+#pragma unroll
+                for (int u = 0; u < TH; ++u) {
+                    const int t = h * TH + u;
+                    const int col = c0 + w[t];
+                    const int base = mp[16 + o[t]];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = base + mp[MF::row(r, g)];
+                        if (col < m && j == col + a.trace_off) {
+                            double* d = a.diag + 2 * ((long long)e * m + col);
+                            d[0] = (double)yr[u][r];
+                            d[1] = (double)yi[u][r];
+                        }
                     }
                 }
             }
@@ -188,38 +239,62 @@ __global__ __launch_bounds__(64 * kFcWaves) void k_fused_cost(FusedCostArgs a) {
     }
 }
 
-size_t fused_cost_lds_bytes(bool f32, int nb) {
-    return (((size_t)nb * kFusedMapStride * sizeof(int) + 15) & ~(size_t)15) + (size_t)kFcWaves * 1024 * (f32 ? 8 : 16);
+bool fused_cost_cfg_valid(int N, bool f32, int nb, FusedCostCfg cfg) {
+    if (N < 4 || N > kFusedCostMaxQubits) return false;
+    const int O = (1 << N) >> 4;
+    if (!((cfg.tiles == 4 && cfg.groups == 1) || (cfg.tiles == 4 && cfg.groups == 2 && cfg.prefetch) ||
+          (cfg.tiles == 2 && cfg.groups == 1)))
+        return false;
+    if (cfg.waves != 1 && cfg.waves != 2 && cfg.waves != 4) return false;
+    if (16 * cfg.tiles < O || (16 * cfg.tiles / cfg.groups) % O != 0) return false;
+    return fused_cost_lds_bytes(f32, nb, cfg) <= 64 * 1024;
+}
+
+size_t fused_cost_lds_bytes(bool f32, int nb, FusedCostCfg cfg) {
+    return (((size_t)nb * kFusedMapStride * sizeof(int) + 15) & ~(size_t)15) +
+           (size_t)cfg.waves * cfg.tiles * 256 * (f32 ? 8 : 16);
+}
+
+template <typename R, int N, int T, int H, bool PF>
+static void launch_fc_shape(const FusedCostArgs& a, int grid, int threads, size_t shm, hipStream_t s) {
+    if constexpr (16 * T >= ((1 << N) >> 4) && (16 * T / H) % ((1 << N) >> 4) == 0)
+        hipLaunchKernelGGL((k_fused_cost<R, N, T, H, PF>), dim3(grid), dim3(threads), shm, s, a);
+}
+
+template <typename R, int N>
+static void launch_fc_n(const FusedCostArgs& a, FusedCostCfg cfg, int grid, size_t shm, hipStream_t s) {
+    const int threads = 64 * cfg.waves;
+    if (cfg.tiles == 4 && cfg.groups == 2) launch_fc_shape<R, N, 4, 2, true>(a, grid, threads, shm, s);
+    else if (cfg.tiles == 4 && cfg.prefetch) launch_fc_shape<R, N, 4, 1, true>(a, grid, threads, shm, s);
+    else if (cfg.tiles == 4) launch_fc_shape<R, N, 4, 1, false>(a, grid, threads, shm, s);
+    else if (cfg.prefetch) launch_fc_shape<R, N, 2, 1, true>(a, grid, threads, shm, s);
+    else launch_fc_shape<R, N, 2, 1, false>(a, grid, threads, shm, s);
 }
 
 template <typename R>
-static hipError_t launch_fc(int N, const FusedCostArgs& a, int grid, size_t shm, hipStream_t s) {
+static hipError_t launch_fc(int N, const FusedCostArgs& a, FusedCostCfg cfg, int grid, size_t shm, hipStream_t s) {
     switch (N) {
-#define SQH_FC_CASE(n) \
-    case n: hipLaunchKernelGGL((k_fused_cost<R, n>), dim3(grid), dim3(64 * kFcWaves), shm, s, a); break;
-        SQH_FC_CASE(4)
-        SQH_FC_CASE(5)
-        SQH_FC_CASE(6)
-        SQH_FC_CASE(7)
-        SQH_FC_CASE(8)
-        SQH_FC_CASE(9)
-        SQH_FC_CASE(10)
-#undef SQH_FC_CASE
-    default:
-        return hipErrorInvalidValue;
+    case 4: launch_fc_n<R, 4>(a, cfg, grid, shm, s); break;
+    case 5: launch_fc_n<R, 5>(a, cfg, grid, shm, s); break;
+    case 6: launch_fc_n<R, 6>(a, cfg, grid, shm, s); break;
+    case 7: launch_fc_n<R, 7>(a, cfg, grid, shm, s); break;
+    case 8: launch_fc_n<R, 8>(a, cfg, grid, shm, s); break;
+    case 9: launch_fc_n<R, 9>(a, cfg, grid, shm, s); break;
+    case 10: launch_fc_n<R, 10>(a, cfg, grid, shm, s); break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, hipStream_t s) {
-    if (N < 4 || N > kFusedCostMaxQubits || a.nb <= 0 || a.m <= 0 || a.B <= 0) return hipErrorInvalidValue;
-    const int W = 64 / ((1 << N) >> 4);
+hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, FusedCostCfg cfg, hipStream_t s) {
+    if (a.nb <= 0 || a.m <= 0 || a.B <= 0 || !fused_cost_cfg_valid(N, f32, a.nb, cfg)) return hipErrorInvalidValue;
+    const int W = 16 * cfg.tiles / ((1 << N) >> 4);
     a.groups = (a.m + W - 1) / W;
     const long long waves = (long long)a.B * a.groups;
-    const int grid = (int)((waves + kFcWaves - 1) / kFcWaves);
-    const size_t shm = fused_cost_lds_bytes(f32, a.nb);
-    if (shm > 64 * 1024) return hipErrorInvalidValue;
-    return f32 ? launch_fc<float>(N, a, grid, shm, s) : launch_fc<double>(N, a, grid, shm, s);
+    const long long wgs = (waves + cfg.waves - 1) / cfg.waves;
+    const int grid = (int)((wgs + 7) / 8 * 8);
+    const size_t shm = fused_cost_lds_bytes(f32, a.nb, cfg);
+    return f32 ? launch_fc<float>(N, a, cfg, grid, shm, s) : launch_fc<double>(N, a, cfg, grid, shm, s);
 }
 
 }  // namespace sqh

@@ -81,13 +81,22 @@ constexpr int kFusedMapStride = 16 + 64;
 constexpr int kFusedCostMaxQubits = 10;
 struct FusedCostArgs {
     const void* U = nullptr;  // target, rows = 2^N, m columns, row stride m
+    const int* in_index = nullptr;  // element b's target at U + in_index[b] * in_bstride (null: U for all)
+    long long in_bstride = 0;
     int m = 0, trace_off = 0, B = 0, nb = 0;
     const void* F = nullptr;
     const int* maps = nullptr;
     double* diag = nullptr;
     int groups = 0;  // column groups per element (set by the launcher)
 };
-hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, hipStream_t s);
-size_t fused_cost_lds_bytes(bool f32, int nb);
+// Launch shape: tiles per wavefront (2 or 4), LDS exchange groups per block (1, or 2
+// with 4 tiles and prefetch), waves per workgroup (1, 2 or 4), next block's matrix
+// prefetched (0 or 1).  Every valid shape gives bit-identical results.
+struct FusedCostCfg {
+    int tiles = 0, groups = 0, waves = 0, prefetch = 0;
+};
+bool fused_cost_cfg_valid(int N, bool f32, int nb, FusedCostCfg cfg);
+size_t fused_cost_lds_bytes(bool f32, int nb, FusedCostCfg cfg);
+hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, FusedCostCfg cfg, hipStream_t s);
 
 }  // namespace sqh

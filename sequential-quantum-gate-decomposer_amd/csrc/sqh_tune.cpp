@@ -1,4 +1,6 @@
-// sqh_tune.cpp -- run-time tuner of the 4-qubit matrix-core block kernel's launch shape.
+// sqh_tune.cpp -- run-time tuners of the launch shapes of the matrix-core kernels: the
+// 4-qubit block kernel of the streaming path and the tile-resident fused-block cost
+// kernel.
 //
 // Candidates: 1, 2 or 4 sixteen-column tiles per wavefront x 64 or 256 threads per
 // workgroup.  The first fused-block application of a (precision, rows, columns, batch)
@@ -98,9 +100,118 @@ int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out) {
     return SQH_OK;
 }
 
+// Fused-block cost kernel: candidates of tiles per wavefront x exchange groups x waves
+// per workgroup.  The first evaluation of a (precision, qubits, columns, batch) shape
+// times every valid candidate on that call's own inputs (one warm-up and three timed
+// launches each, HIP events) and keeps the fastest; every candidate writes the same
+// trace entries (the shapes are bit-identical), and the chosen one then runs for the
+// call.  Under stream capture, or with tuning off, the default shape runs.
+namespace {
+const FusedCostCfg kFcCands[] = {{4, 1, 2, 0}, {4, 1, 2, 1}, {4, 2, 2, 1}, {4, 1, 1, 0}, {4, 2, 1, 1},
+                                 {2, 1, 2, 0}, {2, 1, 4, 0}, {2, 1, 4, 1}};
+
+FusedCostCfg fused_default(int N, bool f32, int nb) {
+    // complex128: all four tiles in one exchange (eight independent MFMA chains);
+    // complex64: two exchange groups where the columns allow
+    const FusedCostCfg pref = f32 ? FusedCostCfg{4, 2, 2, 1} : FusedCostCfg{4, 1, 2, 0};
+    if (fused_cost_cfg_valid(N, f32, nb, pref)) return pref;
+    for (const FusedCostCfg& cfg : kFcCands)
+        if (fused_cost_cfg_valid(N, f32, nb, cfg)) return cfg;
+    return pref;
+}
+}  // namespace
+
+int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
+    const int N = c->prog.N, nb = a.nb;
+    if (c->fused_force.tiles != 0) {
+        if (!fused_cost_cfg_valid(N, f32, nb, c->fused_force))
+            return fail(c, SQH_ERR_ARG,
+                        "fused-block launch shape %d/%d/%d/%d does not fit %d qubits (a column must lie in one "
+                        "exchange group) or the 64 KiB of LDS per workgroup",
+                        c->fused_force.tiles, c->fused_force.groups, c->fused_force.waves, c->fused_force.prefetch, N);
+        HIPTRY(c, launch_fused_cost(f32, N, a, c->fused_force, c->stream));
+        return SQH_OK;
+    }
+    const std::array<long long, 4> key{f32 ? 1 : 0, N, a.m, a.B};
+    auto it = c->fused_tuned.find(key);
+    if (it != c->fused_tuned.end() && fused_cost_cfg_valid(N, f32, nb, it->second)) {
+        HIPTRY(c, launch_fused_cost(f32, N, a, it->second, c->stream));
+        return SQH_OK;
+    }
+    FusedCostCfg best = fused_default(N, f32, nb);
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPTRY(c, hipStreamIsCapturing(c->stream, &cs));
+    if (c->autotune && cs == hipStreamCaptureStatusNone) {
+        const auto t_start = std::chrono::steady_clock::now();
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPTRY(c, hipEventCreate(&e0));
+        hipError_t err = hipEventCreate(&e1);
+        float best_ms = -1.0f;
+        for (const FusedCostCfg& cand : kFcCands) {
+            if (err != hipSuccess) break;
+            if (!fused_cost_cfg_valid(N, f32, nb, cand)) continue;
+            if ((err = launch_fused_cost(f32, N, a, cand, c->stream)) != hipSuccess) break;  // warm-up
+            if ((err = hipEventRecord(e0, c->stream)) != hipSuccess) break;
+            for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch_fused_cost(f32, N, a, cand, c->stream);
+            if (err != hipSuccess || (err = hipEventRecord(e1, c->stream)) != hipSuccess ||
+                (err = hipEventSynchronize(e1)) != hipSuccess)
+                break;
+            float ms = 0.0f;
+            if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            if (best_ms < 0.0f || ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        HIPTRY(c, err);
+        c->fused_tuned[key] = best;
+        c->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    HIPTRY(c, launch_fused_cost(f32, N, a, best, c->stream));
+    return SQH_OK;
+}
+
 }  // namespace sqh
 
 using namespace sqh;
+
+extern "C" int sqh_set_fused_cost_config(sqh_ctx* c, int tiles_per_wave, int exchange_groups, int waves,
+                                         int prefetch) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    const FusedCostCfg cfg{tiles_per_wave, exchange_groups, waves, prefetch};
+    if (!(tiles_per_wave == 0 && exchange_groups == 0 && waves == 0 && prefetch == 0) &&
+        !((tiles_per_wave == 2 || tiles_per_wave == 4) && (exchange_groups == 1 || exchange_groups == 2) &&
+          (waves == 1 || waves == 2 || waves == 4) && (prefetch == 0 || prefetch == 1) &&
+          (exchange_groups == 1 || (tiles_per_wave == 4 && prefetch == 1))))
+        return fail(c, SQH_ERR_ARG,
+                    "tiles per wavefront must be 2 or 4, waves 1, 2 or 4, prefetch 0 or 1, exchange groups 1 "
+                    "(or 2 with 4 tiles and prefetch); all 0 for the tuner");
+    c->fused_force = cfg;
+    return SQH_OK;
+}
+
+extern "C" int sqh_get_fused_cost_config(sqh_ctx* c, int is_f32, int B, int* tiles_per_wave, int* exchange_groups,
+                                         int* waves, int* prefetch) {
+    if (!c || !tiles_per_wave || !exchange_groups || !waves || !prefetch) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->have_circuit || !c->have_target) return fail(c, SQH_ERR_STATE, "set a circuit and a target first");
+    const int N = c->prog.N, nb = (int)c->fplan.qubits.size();
+    FusedCostCfg cfg = fused_default(N, is_f32 != 0, nb);
+    if (fused_cost_cfg_valid(N, is_f32 != 0, nb, c->fused_force)) {
+        cfg = c->fused_force;
+    } else {
+        auto it = c->fused_tuned.find({is_f32 ? 1 : 0, N, c->cols, B});
+        if (it != c->fused_tuned.end()) cfg = it->second;
+    }
+    *tiles_per_wave = cfg.tiles;
+    *exchange_groups = cfg.groups;
+    *waves = cfg.waves;
+    *prefetch = cfg.prefetch;
+    return SQH_OK;
+}
 
 extern "C" int sqh_set_autotune(sqh_ctx* c, int enable) {
     if (!c) return SQH_ERR_ARG;

@@ -29,6 +29,7 @@ COST_FROBENIUS, COST_HILBERT_SCHMIDT, COST_INFIDELITY, COST_ABS_TRACE = 0, 3, 9,
 COST_FROBENIUS_CORRECTION1, COST_FROBENIUS_CORRECTION2 = 1, 2
 COST_HILBERT_SCHMIDT_CORRECTION1, COST_HILBERT_SCHMIDT_CORRECTION2 = 4, 5
 PREC_F64, PREC_F32 = 0, 1
+FUSED_COST_OFF, FUSED_COST_ON, FUSED_COST_AUTO = 0, 1, 2
 
 # Exported symbols (checked by the CPU test suite against include/sqhip.h).
 EXPORTS = [
@@ -44,6 +45,7 @@ EXPORTS = [
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
     "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_set_fused_cost",
+    "sqh_set_fused_cost_config", "sqh_get_fused_cost_config",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
 
@@ -158,6 +160,9 @@ def lib():
                                       d], i),
             "sqh_set_jit": ([vp, i], i),
             "sqh_set_fused_cost": ([vp, i], i),
+            "sqh_set_fused_cost_config": ([vp, i, i, i, i], i),
+            "sqh_get_fused_cost_config": ([vp, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
+                                           ctypes.POINTER(i)], i),
             "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
             "sqh_mlp_backward_step_dev": ([vp, vp, vp, i, ctypes.c_float], i),
             "sqh_jit_compile_check": ([i, ctypes.POINTER(_Gate), i, i, i, i, ctypes.c_char_p, i,
@@ -298,10 +303,23 @@ class Context:
         """Use run-time specialised kernels for this context's circuit."""
         self.check(self._L.sqh_set_jit(self.h, int(bool(enable))))
 
-    def set_fused_cost(self, enable=True):
+    def set_fused_cost(self, mode=True):
         """Cost-only batched evaluations through the fused 4-qubit blocks on the matrix
-        cores (4-10 qubits, costs without correction channels)."""
-        self.check(self._L.sqh_set_fused_cost(self.h, int(bool(enable))))
+        cores (4-10 qubits, costs without correction channels): True / False, or
+        "auto" (the default: from 7 qubits, or from 5 without specialised kernels)."""
+        m = FUSED_COST_AUTO if mode == "auto" else (FUSED_COST_ON if mode else FUSED_COST_OFF)
+        self.check(self._L.sqh_set_fused_cost(self.h, m))
+
+    def set_fused_cost_config(self, tiles=0, groups=0, waves=0, prefetch=0):
+        """Force the fused-block cost kernel's launch shape (all 0: run-time tuner)."""
+        self.check(self._L.sqh_set_fused_cost_config(self.h, int(tiles), int(groups), int(waves), int(prefetch)))
+
+    def fused_cost_config(self, f32, B):
+        """(tiles per wavefront, exchange groups, waves per workgroup, prefetch) in use."""
+        t, g, w, p = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        self.check(self._L.sqh_get_fused_cost_config(self.h, int(bool(f32)), int(B), ctypes.byref(t), ctypes.byref(g),
+                                                      ctypes.byref(w), ctypes.byref(p)))
+        return t.value, g.value, w.value, p.value
 
     def jit_stats(self):
         ms, n = ctypes.c_double(0.0), ctypes.c_int(0)

@@ -110,3 +110,44 @@ def test_correction_variants_fall_back_to_the_tile_kernels():
         ctx.set_fused_cost(fused)
         out[fused] = _costs(ctx, ps, nat.PREC_F64)
     assert np.array_equal(out[False], out[True])
+
+
+SHAPES = [(t, g, w, p) for t in (2, 4) for g in (1, 2) for w in (1, 2, 4) for p in (0, 1)
+          if g == 1 or (t == 4 and p == 1)]
+
+
+@pytest.mark.parametrize("n", [4, 6, 8, 9, 10])
+def test_fused_cost_launch_shapes_are_bit_identical(n):
+    """Every launch shape the tuner may pick (tiles per wavefront, exchange groups,
+    waves per workgroup, prefetch) gives bit-identical costs, in both precisions; the
+    tuner's own pick equals them too.  Shapes that do not fit the qubit count (a column
+    must lie in one exchange group; 64 KiB of LDS per workgroup) are rejected."""
+    gates = W.mixed_gate_circuit(n, 5 * n, 3400 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    U = W.haar_unitary(d, 3500 + n)[:, :d - 3].copy()
+    ps = W.random_params(P, 3600 + n, batch=9)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    ctx.set_fused_cost(True)
+    for prec in (nat.PREC_F64, nat.PREC_F32):
+        ctx.set_fused_cost_config(0, 0, 0, 0)
+        tuned = _costs(ctx, ps, prec)
+        pick = ctx.fused_cost_config(prec == nat.PREC_F32, ps.shape[0])
+        say(f"fused shapes n={n} prec={prec}: tuner picked {pick}")
+        ran = 0
+        for shape in SHAPES:
+            ctx.set_fused_cost_config(*shape)
+            try:
+                got = _costs(ctx, ps, prec)
+            except nat.SqhError:
+                # too few vector columns per group for this qubit count, or too much LDS
+                O = d // 16
+                assert 16 * shape[0] < O or (16 * shape[0] // shape[1]) % O != 0 or (prec == nat.PREC_F64 and
+                                                                                     shape[0] * shape[2] == 16)
+                continue
+            ran += 1
+            assert np.array_equal(got, tuned), shape
+        assert ran >= 4
+    ctx.set_fused_cost_config(0, 0, 0, 0)
