@@ -340,6 +340,39 @@ def derivate_leg(gates, P, U):
     return res
 
 
+def mlp_train_leg(samples=65536, epochs=4, batch=1024):
+    """Training throughput of the warm-start network at its configured size (512 ->
+    3 x 1024 GELU -> 168, BF16 weights and activations, FP32 accumulation and master
+    weights, Adam) on synthetic inputs and targets: every layer's forward and both
+    backward products run on k_gemm_bf16 (v_mfma_f32_16x16x32_bf16).  Algorithmic
+    flops per sample and step: 2 * sum(in * out) forward, twice that backward."""
+    import numpy as np
+    import torch
+
+    from squander_hip import Context
+    from squander_hip.warmstart import WarmStart
+    ctx = Context(0)
+    in_dim, P, hidden = 512, 84, 1024
+    ws = WarmStart(ctx, in_dim, P, hidden=hidden, n_hidden=3, seed=1)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((samples, in_dim)).astype(np.float32)
+    Y = rng.uniform(-1, 1, (samples, 2 * P)).astype(np.float32)
+    ws.train(X[:batch * 4], Y[:batch * 4], epochs=1, batch=batch, lr=1e-3, seed=0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = ws.train(X, Y, epochs=epochs, batch=batch, lr=1e-3, seed=0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    dims = [in_dim, hidden, hidden, hidden, 2 * P]
+    fwd = 2.0 * sum(a * b for a, b in zip(dims[:-1], dims[1:]))
+    steps = epochs * (samples // batch)
+    flops = 3.0 * fwd * batch * steps
+    return {"workload": f"warm-start MLP 512 -> 3 x 1024 GELU -> 168, batch {batch}, {steps} Adam steps, synthetic data",
+            "seconds": dt, "samples_per_s": batch * steps / dt, "wall_TFLOPs": flops / dt / 1e12,
+            "peak_TFLOPs_bf16_dense": 2500.0, "algorithmic_flops_per_step": 3.0 * fwd * batch,
+            "loss_first_last": [losses[0], losses[-1]]}
+
+
 def agents_leg(o):
     """AGENTS (gradient-free population search, AGENTS.cpp:323-600) on the 8-qubit
     ansatz: 64 agents, each iteration's population costs in one batched launch.  The
@@ -512,6 +545,8 @@ def main():
                     help="headline kernel: fused 4-qubit blocks on the matrix cores (default), the "
                          "circuit-specialised register-tile kernel, or the generic one")
     ap.add_argument("--generic", action="store_true", help="same as --headline generic")
+    ap.add_argument("--only-mlp", action="store_true",
+                    help="time the warm-start network's training steps only (for profiling the BF16 GEMM kernel)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (the reference kernel for PMC byte counters)")
     args = ap.parse_args()
@@ -554,6 +589,11 @@ def main():
     params = torch.from_numpy(W.random_params(P, 7, batch=B)).to(dev)
     costs = torch.empty(B, dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
+    if args.only_mlp:
+        res = mlp_train_leg()
+        print(json.dumps({"metric": "warm-start MLP training samples/s (BF16 MFMA)", "value": res["samples_per_s"],
+                          "unit": "samples/s", "higher_is_better": True, "dtype": "bf16", "mlp": res}), flush=True)
+        return
     if args.only_gradient:
         res = gradient_leg(ctx, params, gates, U, orc.Oracle(), reps=args.steps)
         print(json.dumps({"metric": "cost+gradient evals/s @8q batch 512 (FP64)", "value": res["grads_per_s"],
