@@ -160,19 +160,25 @@ __global__ __launch_bounds__(256) void k_out_grad(const float* __restrict__ dy, 
     dyb[i] = (bf16)g;
 }
 
-// db[n] = sum over the first B rows of dz[:, n]: a 64 x 4 block per 64 columns, each
-// of the 4 row-slices summing every 4th row with coalesced reads, then combined in LDS.
-__global__ __launch_bounds__(256) void k_bias_grad(const float* __restrict__ dz, int ld, int B, int N,
-                                                   float* __restrict__ db) {
-    __shared__ float part[4][64];
-    const int cx = threadIdx.x & 63, ry = threadIdx.x >> 6;
-    const int n = blockIdx.x * 64 + cx;
+// db[n] = sum over the first B rows of dz[:, n]: a workgroup per 16 columns, its 64
+// row slices each summing every 64th row, then combined in LDS in a fixed order (the
+// sum is the same on every run).  64 row slices keep 64 workgroups of 16 wavefronts
+// busy for a 1024-wide layer, where 4 slices per 64 columns kept 16 workgroups.
+__global__ __launch_bounds__(1024) void k_bias_grad(const float* __restrict__ dz, int ld, int B, int N,
+                                                    float* __restrict__ db) {
+    __shared__ float part[64][17];
+    const int cx = threadIdx.x & 15, ry = threadIdx.x >> 4;
+    const int n = blockIdx.x * 16 + cx;
     float s = 0.0f;
     if (n < N)
-        for (int r = ry; r < B; r += 4) s += dz[(size_t)r * ld + n];
+        for (int r = ry; r < B; r += 64) s += dz[(size_t)r * ld + n];
     part[ry][cx] = s;
     __syncthreads();
-    if (ry == 0 && n < N) db[n] = part[0][cx] + part[1][cx] + part[2][cx] + part[3][cx];
+    if (ry == 0 && n < N) {
+        float t = 0.0f;
+        for (int k = 0; k < 64; ++k) t += part[k][cx];
+        db[n] = t;
+    }
 }
 
 // Adam on fp32 master weights; refreshes the bf16 copy when wb is given.
@@ -252,7 +258,7 @@ hipError_t mlp_out_grad(const float* dy, int B, int out, int ldy, int Bp, float*
 }
 
 hipError_t mlp_bias_grad(const float* dz, int ld, int B, int N, float* db, hipStream_t s) {
-    hipLaunchKernelGGL(k_bias_grad, dim3((N + 63) / 64), dim3(256), 0, s, dz, ld, B, N, db);
+    hipLaunchKernelGGL(k_bias_grad, dim3((N + 15) / 16), dim3(1024), 0, s, dz, ld, B, N, db);
     return hipGetLastError();
 }
 
