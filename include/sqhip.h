@@ -213,8 +213,8 @@ typedef struct sqh_decompose_options {
     int restart_jitter_count;
     const double* restart_jitter;
     /* Execution engine: SQH_ENGINE_AUTO picks the persistent solver when every run has
-     * at most 5 qubits and the cost variant has no correction channels, else lockstep
-     * rounds (the persistent solver serves up to 6 qubits when asked for);
+     * at most 6 qubits, the cost variant has no correction channels and the run's state
+     * fits a workgroup's LDS, else lockstep rounds;
      * SQH_ENGINE_ROUNDS forces the lockstep rounds; SQH_ENGINE_PERSISTENT requires the
      * persistent solver (an error where it does not apply).  Both run the same
      * optimiser; results agree to rounding. */

@@ -11,6 +11,7 @@
 // sqh_decompose_problems: every run brings its own circuit (any qubit count up to
 // the tile limit); runs with the same qubit count share a launch, each wavefront
 // reading its run's program through a per-run table.
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -153,9 +154,8 @@ int loops_of(const sqh_decompose_options* opts) { return opts->iteration_loops >
 // The persistent solver (sqh_k_solve.hip) serves up to this many qubits, the cost
 // variants without correction channels, and runs capped at this many evaluations.
 constexpr int kSolveMaxQubits = 6;
-constexpr int kSolveAutoQubits = 5;
+constexpr int kSolveAutoQubits = 6;
 constexpr long long kSolveMaxEvals = 1000000;
-constexpr size_t kSolveMaxLds = 64 * 1024;
 
 // Whether the persistent solver can run these groups ("" if so, else the reason).
 const char* persistent_blocker(sqh_ctx* c, const std::vector<Group>& groups, int Ps, long long max_evals) {
@@ -163,41 +163,27 @@ const char* persistent_blocker(sqh_ctx* c, const std::vector<Group>& groups, int
     if (max_evals > kSolveMaxEvals) return "the iteration limits allow more evaluations per run than it runs";
     for (const Group& g : groups) {
         if (g.N > kSolveMaxQubits) return "it serves up to 6 qubits";
-        if (solve_lds_bytes(g.N, Ps, std::max(g.ns, 1)) > kSolveMaxLds) return "the problem needs too much LDS";
+        if (solve_lds_bytes(g.N, Ps, std::max(g.ns, 1)) > device_max_lds()) return "the problem needs too much LDS";
     }
     return "";
 }
 
-// Every run to completion with one persistent launch per qubit-count group (the
-// groups' launches run side by side on forked streams).
+// Every run to completion with persistent launches: groups of more than five qubits
+// one launch each, the groups of at most five qubits together in one combined launch
+// (when there are several).  The launches run side by side, the largest qubit count on
+// the context's stream and every other launch on a forked stream; merging the small
+// groups keeps the streams within the hardware queues a process gets by default (four,
+// one taken by the default stream), where launches on a shared queue would run one
+// after the other.
 int solve_persistent(sqh_ctx* c, std::vector<Group>& groups, BatchBufs& bb, const BfgsOpts& o, int B,
                      const int* run_group, bool grouped, long long max_evals) {
     const int G = (int)groups.size();
     std::vector<std::vector<int>> queue(G);
     for (int r = 0; r < B; ++r) queue[run_group ? run_group[r] : 0].push_back(r);
-    std::vector<int> flat;
-    std::vector<size_t> first(G);
-    for (int g = 0; g < G; ++g) {
-        first[g] = flat.size();
-        flat.insert(flat.end(), queue[g].begin(), queue[g].end());
-    }
-    DevBuf d_queue, d_next;
-    HIPTRY(c, d_queue.ensure(flat.size() * sizeof(int)));
-    HIPTRY(c, d_next.ensure((size_t)G * sizeof(int)));
-    HIPTRY(c, hipMemcpyAsync(d_queue.p, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPTRY(c, hipMemsetAsync(d_next.p, 0, (size_t)G * sizeof(int), c->stream));
-    const bool fork = G > 1;
-    if (fork) {
-        HIPTRY(c, hipEventRecord(c->side_ev[0], c->stream));
-        for (int g = 0; g < G; ++g) HIPTRY(c, hipStreamWaitEvent(c->side[g], c->side_ev[0], 0));
-    }
-    for (int g = 0; g < G; ++g) {
-        Group& gr = groups[g];
-        if (queue[g].empty()) continue;
+    // the group's arguments (queue fields filled per launch below)
+    auto group_args = [&](int g) {
+        const Group& gr = groups[g];
         SolveArgs a;
-        a.next = d_next.as<int>() + g;
-        a.n_runs = (int)queue[g].size();
-        a.queue = d_queue.as<int>() + first[g];
         a.ops = gr.ops;
         a.nops = gr.nops;
         a.lays = gr.lays;
@@ -219,19 +205,89 @@ int solve_persistent(sqh_ctx* c, std::vector<Group>& groups, BatchBufs& bb, cons
         a.x0 = bb.x0.as<double>();
         a.max_evals = max_evals;
         a.o = o;
-        const int grid = std::min(a.n_runs, 4096);
+        return a;
+    };
+    // launch units: a group on its own, or every small group together (unit `small`)
+    std::vector<std::vector<int>> units;
+    std::vector<int> small;
+    for (int g = 0; g < G; ++g) {
+        if (queue[g].empty()) continue;
+        if (groups[g].N <= kSolveSmallMaxQubits) small.push_back(g);
+        else units.push_back({g});
+    }
+    if (small.size() == 1) units.push_back(small);
+    const bool merged = small.size() > 1;
+    if (merged) units.push_back(small);
+    auto unit_n = [&](const std::vector<int>& u) {
+        int n = 0;
+        for (int g : u) n = std::max(n, groups[g].N);
+        return n;
+    };
+    std::stable_sort(units.begin(), units.end(),
+                     [&](const std::vector<int>& x, const std::vector<int>& y) { return unit_n(x) > unit_n(y); });
+    const int U = (int)units.size();
+    // device queues: per unit, its runs (and for the merged unit each entry's qubit count)
+    std::vector<int> flat, flat_n;
+    std::vector<size_t> first(U);
+    for (int u = 0; u < U; ++u) {
+        first[u] = flat.size();
+        for (int g : units[u])
+            for (int r : queue[g]) {
+                flat.push_back(r);
+                flat_n.push_back(groups[g].N);
+            }
+    }
+    DevBuf d_queue, d_queue_n, d_next;
+    HIPTRY(c, d_queue.ensure(std::max<size_t>(flat.size(), 1) * sizeof(int)));
+    HIPTRY(c, d_queue_n.ensure(std::max<size_t>(flat.size(), 1) * sizeof(int)));
+    HIPTRY(c, d_next.ensure((size_t)std::max(U, 1) * sizeof(int)));
+    if (!flat.empty()) {
+        HIPTRY(c, hipMemcpyAsync(d_queue.p, flat.data(), flat.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPTRY(c, hipMemcpyAsync(d_queue_n.p, flat_n.data(), flat_n.size() * sizeof(int), hipMemcpyHostToDevice,
+                                 c->stream));
+    }
+    HIPTRY(c, hipMemsetAsync(d_next.p, 0, (size_t)std::max(U, 1) * sizeof(int), c->stream));
+    const bool fork = U > 1;
+    if (fork) {
+        HIPTRY(c, hipEventRecord(c->side_ev[0], c->stream));
+        for (int u = 1; u < U; ++u) HIPTRY(c, hipStreamWaitEvent(c->side[u], c->side_ev[0], 0));
+    }
+    for (int u = 0; u < U; ++u) {
+        hipStream_t st = u == 0 ? c->stream : c->side[u];
+        const std::vector<int>& un = units[u];
+        int n_runs = 0;
+        for (int g : un) n_runs += (int)queue[g].size();
+        const int grid = std::min(n_runs, 4096);
+        if (merged && un == small) {
+            SolveSmallArgs sa;
+            size_t shm = 0;
+            for (int g : un) {
+                sa.g[groups[g].N - 1] = group_args(g);
+                shm = std::max(shm, solve_lds_bytes(groups[g].N, o.P, std::max(groups[g].ns, 1)));
+            }
+            sa.next = d_next.as<int>() + u;
+            sa.n_runs = n_runs;
+            sa.queue = d_queue.as<int>() + first[u];
+            sa.queue_n = d_queue_n.as<int>() + first[u];
+            HIPTRY(c, launch_solve_small(sa, grid, shm, st));
+            continue;
+        }
+        const int g = un[0];
+        SolveArgs a = group_args(g);
+        a.next = d_next.as<int>() + u;
+        a.n_runs = n_runs;
+        a.queue = d_queue.as<int>() + first[u];
         if (!grouped && jit_wants(c, kJitSolve)) {
-            const int rc = jit_launch_solve(c, a, grid, solve_lds_bytes(gr.N, o.P, a.ns_max),
-                                            fork ? c->side[g] : c->stream);
+            const int rc = jit_launch_solve(c, a, grid, solve_lds_bytes(groups[g].N, o.P, a.ns_max), st);
             if (rc) return rc;
         } else {
-            HIPTRY(c, launch_solve(gr.N, a, grid, fork ? c->side[g] : c->stream));
+            HIPTRY(c, launch_solve(groups[g].N, a, grid, st));
         }
     }
     if (fork) {
-        for (int g = 0; g < G; ++g) {
-            HIPTRY(c, hipEventRecord(c->side_ev[1 + g], c->side[g]));
-            HIPTRY(c, hipStreamWaitEvent(c->stream, c->side_ev[1 + g], 0));
+        for (int u = 1; u < U; ++u) {
+            HIPTRY(c, hipEventRecord(c->side_ev[1 + u], c->side[u]));
+            HIPTRY(c, hipStreamWaitEvent(c->stream, c->side_ev[1 + u], 0));
         }
     }
     HIPTRY(c, hipStreamSynchronize(c->stream));
@@ -351,10 +407,10 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
     const char* blocker = persistent_blocker(c, groups, Ps, max_rounds);
     if (opts->engine == SQH_ENGINE_PERSISTENT && *blocker)
         return fail(c, SQH_ERR_ARG, "persistent solver unavailable: %s", blocker);
-    // Automatic choice: the persistent solver where it measured faster (every run on at
-    // most kSolveAutoQubits qubits: one wavefront holds the whole matrix); above that
-    // one wavefront per run evaluates column slabs in turn and the lockstep rounds,
-    // which spread a run's evaluation over several wavefronts, are faster.
+    // Automatic choice: the persistent solver where it measured faster than the lockstep
+    // rounds (every run on at most kSolveAutoQubits qubits; at 6 qubits with four
+    // wavefronts per run, one per column slab), and where it applies (no correction
+    // channels, the LDS a workgroup may allocate).
     bool auto_persistent = true;
     for (const Group& g : groups) auto_persistent = auto_persistent && g.N <= kSolveAutoQubits;
     const bool persistent = opts->engine == SQH_ENGINE_PERSISTENT ||

@@ -247,7 +247,7 @@ bool fused_cost_cfg_valid(int N, bool f32, int nb, FusedCostCfg cfg) {
         return false;
     if (cfg.waves != 1 && cfg.waves != 2 && cfg.waves != 4) return false;
     if (16 * cfg.tiles < O || (16 * cfg.tiles / cfg.groups) % O != 0) return false;
-    return fused_cost_lds_bytes(f32, nb, cfg) <= 64 * 1024;
+    return fused_cost_lds_bytes(f32, nb, cfg) <= device_max_lds();
 }
 
 size_t fused_cost_lds_bytes(bool f32, int nb, FusedCostCfg cfg) {

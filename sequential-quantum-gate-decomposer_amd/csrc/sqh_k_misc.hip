@@ -16,6 +16,8 @@ hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st) {
 hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st) {
     return t.f32 ? launch_tile_grad_float(t, st) : launch_tile_grad_double(t, st);
 }
+size_t device_max_lds() { return max_lds_per_group(); }
+
 int tile_columns_per_wave(int N) {
     if (N < 1 || N > kMaxTileQubits) return 0;
     const int K = N < 4 ? N : 4;

@@ -47,6 +47,9 @@ struct TileLaunch {
     int stage_lays = 0;        //      ... and up to this many layouts
 };
 
+// LDS a workgroup may allocate on the current device (160 KiB on MI355X).
+size_t device_max_lds();
+
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st);
 int tile_columns_per_wave(int N);

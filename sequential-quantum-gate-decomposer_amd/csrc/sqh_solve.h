@@ -42,9 +42,23 @@ struct SolveArgs {
     BfgsOpts o;
 };
 
+// Groups of 1..5 qubits in one launch: g[N - 1] describes the group of N qubits (its
+// next / n_runs / queue unused); the combined queue lists the runs, queue_n[q] the
+// qubit count of entry q.
+struct SolveSmallArgs {
+    SolveArgs g[5];
+    int* next = nullptr;
+    int n_runs = 0;
+    const int* queue = nullptr;
+    const int* queue_n = nullptr;
+};
+
+constexpr int kSolveSmallMaxQubits = 5;
+
 #ifndef __HIPCC_RTC__
 size_t solve_lds_bytes(int N, int Ps, int ns_max);
 hipError_t launch_solve(int N, const SolveArgs& a, int grid, hipStream_t s);
+hipError_t launch_solve_small(const SolveSmallArgs& s, int grid, size_t shm, hipStream_t st);
 #endif
 
 }  // namespace sqh

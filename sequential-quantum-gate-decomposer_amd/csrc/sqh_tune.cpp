@@ -127,7 +127,7 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
         if (!fused_cost_cfg_valid(N, f32, nb, c->fused_force))
             return fail(c, SQH_ERR_ARG,
                         "fused-block launch shape %d/%d/%d/%d does not fit %d qubits (a column must lie in one "
-                        "exchange group) or the 64 KiB of LDS per workgroup",
+                        "exchange group) or the LDS a workgroup may allocate",
                         c->fused_force.tiles, c->fused_force.groups, c->fused_force.waves, c->fused_force.prefetch, N);
         HIPTRY(c, launch_fused_cost(f32, N, a, c->fused_force, c->stream));
         return SQH_OK;
