@@ -442,6 +442,12 @@ def uneven_leg(o):
         err = max(err, abs(float(raw["f"][i]) - o.optimization_problem(g, U, n, raw["x"][i])))
     res["max_err_reported_cost_vs_fp64_oracle"] = err
     log(f"uneven: max |reported cost - FP64 oracle cost| = {err:.1e}")
+    # the same batch in FP64 throughout, and through the lockstep rounds
+    for key, kw in (("fp64", {"mixed": False}), ("rounds_mixed", {"mixed": True, "engine": "rounds"})):
+        r2, _, _ = uneven_bench.run(count=256, seed=2028, tol=1e-8, log=log, **kw)
+        res[key] = {k: r2[k] for k in ("seconds", "solves_per_s", "evals_p50", "evals_p99", "engine")}
+        res[key]["success_reachable"] = r2["reachable"]["success"]
+        res[key]["success_haar"] = r2["haar"]["success"]
     return res
 
 

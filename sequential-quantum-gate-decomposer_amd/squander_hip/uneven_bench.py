@@ -1,5 +1,7 @@
 """Uneven persistent batch: independent decompositions of 3..6-qubit targets with
-their own circuits, run as one lockstep batch (sqh_decompose_problems).
+their own circuits, run as one batch (sqh_decompose_problems): by default the
+persistent solver (whole optimisations in one launch per qubit-count group, the groups
+side by side), or the lockstep rounds with engine="rounds".
 
 Half of the problems are Haar targets, half are ansatz-reachable targets built from
 seeded random angles; every problem's circuit is the U3 layer + L random adjacent
