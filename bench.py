@@ -794,6 +794,37 @@ def main():
                            "err_rel_fro_vs_fp64_oracle": apply_err(L.sqh_apply_to_fused_dev),
                            "kernel": "k_block_mfma4<double> (v_mfma_f64_16x16x4f64)"}
     log(f"fused stream: {out['fused_stream']['evals_per_s']:.0f} evals/s")
+
+    # apply_to of the same batch in place, every column resident on the chip: the fused
+    # blocks' store pass on the matrix cores, and the specialised register-tile kernel
+    def apply_leg(c, name, kernel, fl):
+        run = lambda: c.check(L.sqh_apply_to_dev(c.h, params.data_ptr(), P, mats.data_ptr(), d, d, d, d * d, B, 0))
+        run()
+        c.synchronize()
+        c.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            run()
+        c.synchronize()
+        t1 = time.perf_counter()
+        akms, anl = c.kernel_timing()
+        c.set_kernel_timing(False)
+        k_s = akms / max(anl, 1) / 1e3
+        r = {"evals_per_s": B * reps / (t1 - t0), "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": fl * B / k_s / 1e12,
+             "hbm_GBs": 2.0 * 16 * d * d * B / k_s / 1e9, "kernel": kernel,
+             "err_rel_fro_vs_fp64_oracle": apply_err(lambda h, pp, Pp, mm, dd, cc, ss, bs, bb, f: L.sqh_apply_to_dev(
+                 c.h, pp, Pp, mm, dd, cc, ss, bs, bb, f))}
+        log(f"apply_to {name}: {r['evals_per_s']:.0f} matrices/s, kernel {k_s * 1e3:.3f} ms")
+        return r
+
+    fctx = Context(0)
+    fctx.set_circuit(n, gates)
+    fctx.set_fused_cost(True)
+    out["apply_to_fused_resident"] = apply_leg(fctx, "fused resident",
+                                               "k_fused_cost<double,8,...> store pass (v_mfma_f64_16x16x4f64)",
+                                               block_flops(gates, n, d))
+    out["apply_to_tile_specialised"] = apply_leg(ctx, "tile specialised",
+                                                 "sqh_jit_kernel (k_tile_forward specialised, store mode)", flops_eval)
     del mats
 
     # the same in complex64: 16x16 blocks on v_mfma_f32_16x16x4f32, FP32 accumulation

@@ -109,7 +109,11 @@ int sqh_set_correction_weights(sqh_ctx* ctx, double prev_cost, double scale1, do
 /* In place on a host matrix. is_f32: complex64 matrix + float32 parameters. */
 int sqh_apply_to(sqh_ctx* ctx, const void* params, void* inout, int rows, int cols, int stride, int is_f32);
 /* Device variant: B matrices at inout + b*bstride elements, parameters shared
- * (pbstride = 0) or per matrix (pbstride = P). */
+ * (pbstride = 0) or per matrix (pbstride = P).  Up to 10 qubits it runs the register-tile
+ * kernels, or -- where sqh_set_fused_cost selects it (by default from 7 qubits) -- the
+ * tile-resident pass of the circuit's fused 4-qubit blocks on the matrix cores, which
+ * writes each column back to where it read it; above 10 qubits the fused streaming
+ * path. */
 int sqh_apply_to_dev(sqh_ctx* ctx, const double* d_params, int pbstride, void* d_inout, int rows, int cols,
                      int stride, long long bstride, int B, int is_f32);
 /* The same result computed by one streaming HBM pass per gate (the unfused path). */

@@ -150,12 +150,15 @@ std::vector<int> fused_row_maps(int N, const FusionPlan& fp) {
 // switch is on -- or automatic (the default) where it measured faster on MI355X than
 // the register-tile kernels: from 7 qubits, or from 5 when the circuit-specialised
 // kernels are off.
-bool fused_cost_applies(const sqh_ctx* c, bool grad, const TraceSpec& ts) {
+bool fused_pass_applies(const sqh_ctx* c) {
     const int N = c->prog.N;
-    if (grad || ts.nch != 0 || N < 4 || N > kFusedCostMaxQubits || c->fplan.k != 4 || c->fplan.qubits.empty())
-        return false;
+    if (N < 4 || N > kFusedCostMaxQubits || c->fplan.k != 4 || c->fplan.qubits.empty()) return false;
     if (c->fused_cost == SQH_FUSED_COST_AUTO) return N >= 7 || (N >= 5 && !c->jit);
     return c->fused_cost == SQH_FUSED_COST_ON;
+}
+
+bool fused_cost_applies(const sqh_ctx* c, bool grad, const TraceSpec& ts) {
+    return !grad && ts.nch == 0 && fused_pass_applies(c);
 }
 
 // Costs of B parameter vectors by the tile-resident fused-block kernel: kernel tables
@@ -173,7 +176,9 @@ int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     FusedCostArgs a;
     a.U = U;
     a.in_index = in_index;
-    a.in_bstride = in_bstride;
+    a.in_bstride = in_index ? in_bstride : 0;
+    a.in_stride = m;
+    a.F_bstride = (long long)nb * 256;
     a.m = m;
     a.trace_off = trace_off;
     a.B = B;
@@ -395,6 +400,30 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
     if ((rc = build_kernels(c, d_params, Bk, f32))) return rc;
     const long long kb = pbstride ? (long long)ns * 4 : 0;
     if (path == 0 && N > kMaxTileQubits) path = N >= 4 ? 2 : 1;
+    if (path == 0 && fused_pass_applies(c)) {
+        // the circuit's fused 4-qubit blocks on the matrix cores, every column resident
+        const FusionPlan& fp = c->fplan;
+        const int nb = (int)fp.qubits.size();
+        const size_t esz = f32 ? 8 : 16;
+        HIPTRY(c, c->d_fused.ensure((size_t)Bk * nb * 256 * esz));
+        HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, c->d_kern.p, kb, Bk, c->d_fused.p,
+                                     c->stream));
+        FusedCostArgs a;
+        a.U = d_inout;
+        a.in_bstride = bstride;
+        a.in_stride = stride;
+        a.m = cols;
+        a.B = B;
+        a.nb = nb;
+        a.F = c->d_fused.p;
+        a.F_bstride = Bk > 1 ? (long long)nb * 256 : 0;
+        a.maps = c->d_fmaps.as<int>();
+        a.out = d_inout;
+        a.out_bstride = bstride;
+        a.out_stride = stride;
+        TimedLaunch tl(c);
+        return fused_cost_launch(c, f32, a);
+    }
     if (path == 0) {
         TileLaunch t = base_launch(c, f32);
         t.mode = 0;

@@ -110,8 +110,9 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
         w[t] = n / O;
         o[t] = n % O;
     }
-    const cpx<R>* U = static_cast<const cpx<R>*>(a.U) + (a.in_index ? (long long)a.in_index[e] * a.in_bstride : 0);
-    const int m = a.m;
+    const cpx<R>* U =
+        static_cast<const cpx<R>*>(a.U) + (a.in_index ? (long long)a.in_index[e] : (long long)e) * a.in_bstride;
+    const int m = a.m, ist = a.in_stride;
     // B operand of step s, tile t: X[4 s + g][16 t + nl] (block 0's grouping, from U)
     R xr[T][4], xi[T][4];
     {
@@ -123,13 +124,13 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const int j = base + mp[4 * s + g];
-                const cpx<R> v = col < m ? U[(long long)j * m + col] : mk<R>(0, 0);
+                const cpx<R> v = col < m ? U[(long long)j * ist + col] : mk<R>(0, 0);
                 xr[t][s] = v.re;
                 xi[t][s] = v.im;
             }
         }
     }
-    const cpx<R>* Fe = static_cast<const cpx<R>*>(a.F) + (long long)e * a.nb * 256;
+    const cpx<R>* Fe = static_cast<const cpx<R>*>(a.F) + (long long)e * a.F_bstride;
     // A operand of step s: F_b[nl][4 s + g] (with PF, block b + 1's is loaded while
     // block b runs)
     R far[4], fai[4];
@@ -217,6 +218,21 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
                     }
                 }
                 fc_wave_fence();
+            } else if (a.out) {
+                // apply_to: the transformed columns back to HBM (each wavefront writes
+                // only the columns it read, so the pass may run in place)
+                cpx<R>* O = static_cast<cpx<R>*>(a.out) + (long long)e * a.out_bstride;
+#pragma unroll
+                for (int u = 0; u < TH; ++u) {
+                    const int t = h * TH + u;
+                    const int col = c0 + w[t];
+                    const int base = mp[16 + o[t]];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int j = base + mp[MF::row(r, g)];
+                        if (col < m) O[(long long)j * a.out_stride + col] = mk<R>(yr[u][r], yi[u][r]);
+                    }
+                }
             } else {
                 // trace entries: row c + trace_off of column c
 #pragma unroll
@@ -288,6 +304,7 @@ static hipError_t launch_fc(int N, const FusedCostArgs& a, FusedCostCfg cfg, int
 
 hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, FusedCostCfg cfg, hipStream_t s) {
     if (a.nb <= 0 || a.m <= 0 || a.B <= 0 || !fused_cost_cfg_valid(N, f32, a.nb, cfg)) return hipErrorInvalidValue;
+    if (a.in_stride < a.m || (a.out && a.out_stride < a.m)) return hipErrorInvalidValue;
     const int W = 16 * cfg.tiles / ((1 << N) >> 4);
     a.groups = (a.m + W - 1) / W;
     const long long waves = (long long)a.B * a.groups;

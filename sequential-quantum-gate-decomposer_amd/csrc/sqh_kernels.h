@@ -75,21 +75,28 @@ hipError_t launch_broadcast_copy(bool f32, const void* src, int rows, int m, int
                                  long long dbstride, int B, hipStream_t st);
 hipError_t launch_convert(bool to_f32, const void* src, void* dst, long long nreal, hipStream_t st);
 
-// Tile-resident cost of the circuit as fused 4-qubit blocks on the matrix cores
-// (sqh_k_fused_cost.hip), 4 <= N <= kFusedCostMaxQubits.  maps: per block
+// Tile-resident pass of the circuit as fused 4-qubit blocks on the matrix cores
+// (sqh_k_fused_cost.hip), 4 <= N <= kFusedCostMaxQubits: the trace entries of every
+// column (cost) or, with out, the transformed matrices (apply_to).  maps: per block
 // kFusedMapStride ints, the row offsets of the 16 local indices, then the row bases of
-// the 2^N / 16 outer indices.  F: [B][nb][16][16] block matrices.  diag receives the B x m
-// trace entries (complex, as double pairs).
+// the 2^N / 16 outer indices.  F: block matrices, element b's [nb][16][16] at
+// F + b * F_bstride (0: shared).  diag receives the B x m trace entries (complex, as
+// double pairs).
 constexpr int kFusedMapStride = 16 + 64;
 constexpr int kFusedCostMaxQubits = 10;
 struct FusedCostArgs {
-    const void* U = nullptr;  // target, rows = 2^N, m columns, row stride m
-    const int* in_index = nullptr;  // element b's target at U + in_index[b] * in_bstride (null: U for all)
+    const void* U = nullptr;  // input: element b's at U + (in_index ? in_index[b] : b) * in_bstride
+    const int* in_index = nullptr;
     long long in_bstride = 0;
+    int in_stride = 0;  // row stride of the input (elements)
     int m = 0, trace_off = 0, B = 0, nb = 0;
     const void* F = nullptr;
+    long long F_bstride = 0;
     const int* maps = nullptr;
     double* diag = nullptr;
+    void* out = nullptr;  // non-null: store element b's result at out + b * out_bstride (row stride out_stride)
+    long long out_bstride = 0;
+    int out_stride = 0;
     int groups = 0;  // column groups per element (set by the launcher)
 };
 // Launch shape: tiles per wavefront (2 or 4), LDS exchange groups per block (1, or 2

@@ -132,7 +132,8 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
         HIPTRY(c, launch_fused_cost(f32, N, a, c->fused_force, c->stream));
         return SQH_OK;
     }
-    const std::array<long long, 4> key{f32 ? 1 : 0, N, a.m, a.B};
+    // the store pass (apply_to) moves the matrices through HBM: its own key
+    const std::array<long long, 4> key{(f32 ? 1 : 0) + (a.out ? 2 : 0), N, a.m, a.B};
     auto it = c->fused_tuned.find(key);
     if (it != c->fused_tuned.end() && fused_cost_cfg_valid(N, f32, nb, it->second)) {
         HIPTRY(c, launch_fused_cost(f32, N, a, it->second, c->stream));

@@ -150,3 +150,41 @@ def test_fused_cost_launch_shapes_are_bit_identical(n):
             assert np.array_equal(got, tuned), shape
         assert ran >= 4
     ctx.set_fused_cost_config(0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("n", [4, 6, 7, 8, 10])
+def test_fused_apply_to_matches_oracle(oracle, n):
+    """Gates_block::apply_to through the fused blocks on the matrix cores (the store pass
+    of the tile-resident kernel): a batch of matrices in place, per-element and shared
+    parameters, a rectangular strided view inside a wider buffer; complex128 to 1e-12
+    relative Frobenius norm, complex64 to 1e-5."""
+    import torch
+    gates = W.mixed_gate_circuit(n, 4 * n + 6, 3700 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    m, stride = d // 2 + 3, d + 5
+    B = 3
+    rng = np.random.default_rng(3800 + n)
+    X = rng.standard_normal((B, d, stride)) + 1j * rng.standard_normal((B, d, stride))
+    ps = W.random_params(P, 3900 + n, batch=B)
+    for f32 in (False, True):
+        for shared in (False, True):
+            say(f"fused apply_to n={n} f32={f32} shared={shared}")
+            ctx = nat.Context(0)
+            ctx.set_circuit(n, gates)
+            ctx.set_fused_cost(True)
+            dt = np.complex64 if f32 else np.complex128
+            Xd = torch.from_numpy(X.astype(dt)).cuda()
+            pd = torch.from_numpy(ps).cuda()
+            torch.cuda.synchronize()
+            ctx.check(ctx._L.sqh_apply_to_dev(ctx.h, pd.data_ptr(), 0 if shared else P, Xd.data_ptr(), d, m, stride,
+                                              d * stride, B, int(f32)))
+            ctx.synchronize()
+            R = Xd.cpu().numpy()
+            for b in range(B):
+                p = ps[0] if shared else ps[b]
+                ref = oracle.apply_to(gates, p, X[b][:, :m].astype(dt).astype(np.complex128))
+                err = np.linalg.norm(R[b][:, :m] - ref) / np.linalg.norm(ref)
+                assert err < (1e-5 if f32 else 1e-12), (b, err)
+                # the columns beyond the view are untouched
+                assert np.array_equal(R[b][:, m:], X[b][:, m:].astype(dt))
