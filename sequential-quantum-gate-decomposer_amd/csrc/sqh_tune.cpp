@@ -100,13 +100,17 @@ int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out) {
     return SQH_OK;
 }
 
-// Fused-block cost kernel: candidates of tiles per wavefront x exchange groups x waves
-// per workgroup.  The first evaluation of a (precision, qubits, columns, batch) shape
-// times every valid candidate on that call's own inputs (one warm-up and three timed
-// launches each, HIP events) and keeps the fastest; every candidate writes the same
-// trace entries (the shapes are bit-identical), and the chosen one then runs for the
-// call.  Under stream capture, or with tuning off, the default shape runs.
+// Fused-block kernel: candidates of tiles per wavefront x exchange groups x waves per
+// workgroup x prefetch.  The first call of a (pass, precision, qubits, columns, batch)
+// shape times every valid candidate on that call's own inputs (one warm-up and three
+// timed launches each, HIP events) and keeps the fastest; the chosen one then runs for
+// the call.  The cost pass's candidates all write the same trace entries (the shapes
+// are bit-identical).  The store pass (apply_to) runs in place, so its candidates read
+// the call's input and write a scratch copy of at most kFcTuneBytes instead (the first
+// elements of the batch).  Under stream capture, or with tuning off, the default shape
+// runs.
 namespace {
+constexpr size_t kFcTuneBytes = size_t(256) << 20;
 const FusedCostCfg kFcCands[] = {{4, 1, 2, 0}, {4, 1, 2, 1}, {4, 2, 2, 1}, {4, 1, 1, 0}, {4, 2, 1, 1},
                                  {2, 1, 2, 0}, {2, 1, 4, 0}, {2, 1, 4, 1}};
 
@@ -142,7 +146,34 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
     FusedCostCfg best = fused_default(N, f32, nb);
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIPTRY(c, hipStreamIsCapturing(c->stream, &cs));
-    if (c->autotune && cs == hipStreamCaptureStatusNone) {
+    // the arguments the candidates run with: the call's own, or for the store pass a
+    // scratch output (rows x m per element, row stride m)
+    FusedCostArgs ta = a;
+    struct Scratch {
+        DevBuf b;
+        hipStream_t s = nullptr;
+        ~Scratch() {
+            if (b.p) {
+                (void)hipStreamSynchronize(s);  // released only after the candidates have run
+                b.release();
+            }
+        }
+    } scratch_guard;
+    scratch_guard.s = c->stream;
+    DevBuf& scratch = scratch_guard.b;
+    bool can_tune = c->autotune && cs == hipStreamCaptureStatusNone;
+    if (can_tune && a.out) {
+        const size_t per = ((size_t)1 << N) * a.m * (f32 ? 8 : 16);
+        ta.B = (int)std::max<size_t>(1, std::min<size_t>((size_t)a.B, kFcTuneBytes / per));
+        if (scratch.ensure((size_t)ta.B * per) != hipSuccess) {
+            (void)hipGetLastError();  // no room for the scratch copy: keep the default
+            can_tune = false;
+        }
+        ta.out = scratch.p;
+        ta.out_stride = a.m;
+        ta.out_bstride = ((long long)1 << N) * a.m;
+    }
+    if (can_tune) {
         const auto t_start = std::chrono::steady_clock::now();
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIPTRY(c, hipEventCreate(&e0));
@@ -151,9 +182,9 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
         for (const FusedCostCfg& cand : kFcCands) {
             if (err != hipSuccess) break;
             if (!fused_cost_cfg_valid(N, f32, nb, cand)) continue;
-            if ((err = launch_fused_cost(f32, N, a, cand, c->stream)) != hipSuccess) break;  // warm-up
+            if ((err = launch_fused_cost(f32, N, ta, cand, c->stream)) != hipSuccess) break;  // warm-up
             if ((err = hipEventRecord(e0, c->stream)) != hipSuccess) break;
-            for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch_fused_cost(f32, N, a, cand, c->stream);
+            for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch_fused_cost(f32, N, ta, cand, c->stream);
             if (err != hipSuccess || (err = hipEventRecord(e1, c->stream)) != hipSuccess ||
                 (err = hipEventSynchronize(e1)) != hipSuccess)
                 break;
