@@ -190,6 +190,53 @@ int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     return fused_cost_launch(c, f32, a);
 }
 
+// 16-column tiles per wavefront of the fused-block gradient kernel: two (three resident
+// wavefronts per SIMD in complex128), four at 10 qubits (a column of 1024 rows is 64
+// vector columns).
+int fused_grad_tiles(int N) { return N >= 10 ? 4 : 2; }
+
+// Whether a cost + gradient evaluation runs through the fused blocks: the switch on
+// (SQH_FUSED_COST_ON), 4 <= N <= 10, no correction channels, and the kernel's LDS fits.
+bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32) {
+    if (ts.nch != 0 || c->fused_cost != SQH_FUSED_COST_ON || !fused_pass_applies(c)) return false;
+    const int N = c->prog.N;
+    return fused_grad_wgs(f32, N, c->cols, (int)c->fplan.qubits.size(), fused_grad_tiles(N)) > 0;
+}
+
+// Costs and gradients of B parameter vectors through the fused blocks: block matrices,
+// the forward + backward kernel (trace entries into diag, per-workgroup correlation
+// partials), then each block's gate correlations into d_fcorr for k_finish.
+int fused_grad_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off) {
+    const FusionPlan& fp = c->fplan;
+    const int N = c->prog.N, nb = (int)fp.qubits.size(), ns = (int)c->prog.slots.size(), m = c->cols;
+    const int T = fused_grad_tiles(N);
+    const int wgs = fused_grad_wgs(f32, N, m, nb, T);
+    const size_t esz = f32 ? 8 : 16;
+    HIPTRY(c, c->d_fused.ensure((size_t)B * nb * 256 * esz));
+    HIPTRY(c, c->d_mpart.ensure((size_t)B * wgs * nb * 512 * sizeof(double)));
+    HIPTRY(c, c->d_fcorr.ensure((size_t)B * std::max(ns, 1) * 8 * sizeof(double)));
+    HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, B,
+                                 c->d_fused.p, c->stream));
+    FusedGradArgs a;
+    a.U = f32 ? c->d_U32.p : c->d_U64.p;
+    a.m = m;
+    a.trace_off = trace_off;
+    a.B = B;
+    a.nb = nb;
+    a.F = c->d_fused.p;
+    a.maps = c->d_fmaps.as<int>();
+    a.diag = diag;
+    a.mpart = c->d_mpart.as<double>();
+    a.wgs = wgs;
+    {
+        TimedLaunch tl(c);
+        HIPTRY(c, launch_fused_grad(f32, N, a, T, c->stream));
+    }
+    HIPTRY(c, launch_fused_block_corr(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4,
+                                      c->d_mpart.as<double>(), wgs, B, c->d_fcorr.as<double>(), ns, c->stream));
+    return SQH_OK;
+}
+
 // The circuit as fused dense blocks (<= 4 qubits each): one HBM pass per block,
 // matrix cores for the 4-qubit FP64 blocks.  kern holds the kernel tables built for
 // this call (per matrix when kern_bstride != 0).
@@ -265,6 +312,12 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
         return SQH_OK;
     };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
+    if (d_grad && !fixed && fused_grad_applies(c, ts, f32)) {
+        if ((rc = fused_grad_eval(c, f32, B, kern, diag, off))) return rc;
+        HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, c->d_fcorr.as<double>(), 1, ns, c->d_slots.as<Slot>(),
+                                d_params, c->prog.P, d_grad, c->stream, BatchIndex(), ts));
+        return SQH_OK;
+    }
     if (fused_cost_applies(c, d_grad != nullptr, ts) && !fixed) {
         if ((rc = fused_cost_eval(c, f32, B, kern, diag, off, U, m, nullptr, 0))) return rc;
         HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(), d_params,

@@ -85,6 +85,7 @@ struct sqh_ctx {
     // Row tables of the tile-resident fused-block cost kernel ([blocks][kFusedMapStride];
     // present for 4 <= N <= 10) and its switch (sqh_set_fused_cost).
     sqh::DevBuf d_fmaps;
+    sqh::DevBuf d_fcorr;  // [B][ns][8] gate correlations of the fused-block gradient
     int fused_cost = SQH_FUSED_COST_AUTO;
     // its launch shape: forced (sqh_set_fused_cost_config) or picked per (precision,
     // qubits, columns, batch) by the run-time tuner
@@ -113,7 +114,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_fmaps, &d_idx, &d_seed, &d_dtab, &d_zslot})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_fmaps, &d_fcorr, &d_idx, &d_seed, &d_dtab, &d_zslot})
             b->release();
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);

@@ -164,12 +164,8 @@ __global__ __launch_bounds__(256) void k_block_mfma4(cpx<R>* __restrict__ mats, 
     }
 }
 
-// Fused block matrices: thread -> (batch element b, block, column j of the block);
-// the block's gates (local qubits 0..3, the block's slots) are applied to e_j.
-struct FusedGate {
-    int8_t kind, t, c, pad;  // OP_U / OP_CU / OP_X / OP_Z; local target; local control (-1 none)
-    int32_t slot;            // parametric slot (kernel table index) or -1
-};
+// Fused block matrices: the block's gates (FusedGate: local qubits 0..3, the block's
+// slots) applied to the unit columns e_j.
 
 // Gates staged per pass and batch elements per workgroup (x 16 block columns = one
 // wavefront) of k_fuse_blocks.

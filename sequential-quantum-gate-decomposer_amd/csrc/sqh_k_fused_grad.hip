@@ -1,0 +1,440 @@
+// sqh_k_fused_grad.hip -- batched cost and gradient of the circuit as fused dense
+// 4-qubit blocks on the matrix cores, every column resident on the chip.
+//
+// Forward: as the fused cost kernel (sqh_k_fused_cost.hip), X_b = F_b X_{b-1} from the
+// target, and the trace entries.  Backward, with Y the trace selector propagated
+// through the blocks (Y_{b-1} = F_b^T Y_b, so Tr = sum Y_b * X_b for every b), per block
+// from the last:
+//   X_{b-1} = F_b^dagger X_b                                   (MFMA, F unitary)
+//   M_b[i][k] += sum over the wavefront's (outer group, column) of Y_b[(i,o)] X_{b-1}[(k,o)]
+//                                                              (MFMA on transposed tiles)
+//   Y_{b-1} = F_b^T Y_b                                        (MFMA)
+// so that dTr/dtheta = sum_ik (dF_b/dtheta)_ik M_b[i][k] for a parameter of block b.
+// The wavefronts of a workgroup (one batch element) sum their M_b in LDS in a fixed
+// order and write one partial per workgroup and block; k_fused_block_corr then sums the
+// partials and runs the block's gates on 16-dimensional vectors (x from e_k, y from
+// column k of M_b) backwards, producing each gate's 2x2 correlations in the layout of
+// the register-tile adjoint kernel, from which k_finish forms the gradient.
+//
+// Layouts follow sqh_k_fused_cost.hip: row j of a column under block b is
+// base_b[o] + off_b[i]; 16x16x4 MFMA operands A[l % 16][l / 16], B[l / 16][l % 16]; the
+// B operand of step s holds rows 4 s + lane / 16, the D layout rows row(r, lane / 16).
+#include <hip/hip_runtime.h>
+
+#include "sqh_device.h"
+#include "sqh_kernels.h"
+
+namespace sqh {
+
+typedef double fg_f64x4 __attribute__((ext_vector_type(4)));
+typedef float fg_f32x4 __attribute__((ext_vector_type(4)));
+
+template <typename R>
+struct FgMfma;
+template <>
+struct FgMfma<double> {
+    typedef fg_f64x4 V;
+    static __device__ __forceinline__ V op(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int g) { return 4 * r + g; }
+};
+template <>
+struct FgMfma<float> {
+    typedef fg_f32x4 V;
+    static __device__ __forceinline__ V op(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int r, int g) { return 4 * g + r; }
+};
+
+template <int N>
+__device__ __forceinline__ int fg_slot(int j) {
+    if constexpr (N <= 4) {
+        return j;
+    } else {
+        int h = j >> 4;
+        if constexpr (N > 8) h ^= h >> 4;
+        return j ^ (h & 15);
+    }
+}
+
+__device__ __forceinline__ void fg_wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
+
+// Complex 16x16 tile product Y = A X for T tiles: A given by (ar, ai, ain = -ai) per
+// step, X the B operand (xr, xi) per tile and step; result in the D layout.
+template <typename R, int T>
+__device__ __forceinline__ void fg_product(const R (&ar)[4], const R (&ai)[4], const R (&ain)[4], const R (&xr)[T][4],
+                                           const R (&xi)[T][4], typename FgMfma<R>::V (&yr)[T],
+                                           typename FgMfma<R>::V (&yi)[T]) {
+    typedef FgMfma<R> MF;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        yr[t] = typename MF::V{0, 0, 0, 0};
+        yi[t] = typename MF::V{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            yr[t] = MF::op(ar[s], xr[t][s], yr[t]);
+            yi[t] = MF::op(ar[s], xi[t][s], yi[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            yr[t] = MF::op(ain[s], xi[t][s], yr[t]);
+            yi[t] = MF::op(ai[s], xr[t][s], yi[t]);
+        }
+    }
+}
+
+// D-layout tiles (block b's grouping, tables mp) through the wavefront's LDS region into
+// B-layout operands of the grouping of tables mn.
+template <typename R, int N, int T>
+__device__ __forceinline__ void fg_remap(const typename FgMfma<R>::V (&dr)[T], const typename FgMfma<R>::V (&di)[T],
+                                         R (&xr)[T][4], R (&xi)[T][4], const int* mp, const int* mn, const int (&w)[T],
+                                         const int (&o)[T], int g, cpx<R>* lds) {
+    typedef FgMfma<R> MF;
+    constexpr int D = 1 << N;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int base = mp[16 + o[t]];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) lds[w[t] * D + fg_slot<N>(base + mp[MF::row(r, g)])] = mk<R>(dr[t][r], di[t][r]);
+    }
+    fg_wave_fence();
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int base = mn[16 + o[t]];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> v = lds[w[t] * D + fg_slot<N>(base + mn[4 * s + g])];
+            xr[t][s] = v.re;
+            xi[t][s] = v.im;
+        }
+    }
+    fg_wave_fence();
+}
+
+template <typename R, int N, int T>
+__global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
+    typedef FgMfma<R> MF;
+    constexpr int D = 1 << N;
+    constexpr int O = D >> 4;
+    constexpr int W = 16 * T / O;
+    static_assert(W >= 1, "a wavefront holds whole columns");
+    constexpr int RG = 16 * T * 16;  // remap region (complex entries) per wavefront
+    constexpr int TS = 16 * 17;      // one transposed tile, rows padded to 17
+    extern __shared__ __align__(16) unsigned char fg_smem[];
+    const int nwv = blockDim.x >> 6;
+    int* smap = reinterpret_cast<int*>(fg_smem);
+    const int nmap = a.nb * kFusedMapStride;
+    unsigned char* p = fg_smem + (((size_t)nmap * sizeof(int) + 15) & ~(size_t)15);
+    cpx<R>* region = reinterpret_cast<cpx<R>*>(p);
+    p += sizeof(cpx<R>) * (size_t)(RG + 2 * TS) * nwv;
+    double* red = reinterpret_cast<double*>(p);  // [waves][256][2] partial M of the block
+    for (int i = threadIdx.x; i < nmap; i += blockDim.x) smap[i] = a.maps[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long long per_xcd = gridDim.x >> 3;  // the grid is a multiple of 8
+    const long long lwg = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    const long long wave = lwg * nwv + wv;
+    // whole workgroups past the batch leave together (the workgroups of one element
+    // hold whole column groups: groups % waves == 0)
+    if (lwg * nwv >= (long long)a.B * a.groups) return;
+    const int e = (int)(wave / a.groups);
+    const int cg = (int)(wave % a.groups);
+    const int c0 = cg * W;
+    const int wgi = cg / nwv;  // the workgroup's index within its element
+    cpx<R>* lds = region + (size_t)wv * (RG + 2 * TS);
+    cpx<R>* ys = lds + RG;
+    cpx<R>* xs = ys + TS;
+    const int nl = lane & 15, g = lane >> 4;
+    int w[T], o[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int n = 16 * t + nl;
+        w[t] = n / O;
+        o[t] = n % O;
+    }
+    const cpx<R>* U = static_cast<const cpx<R>*>(a.U);
+    const int m = a.m;
+    R xr[T][4], xi[T][4];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int col = c0 + w[t];
+        const int base = smap[16 + o[t]];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> v = col < m ? U[(long long)(base + smap[4 * s + g]) * m + col] : mk<R>(0, 0);
+            xr[t][s] = v.re;
+            xi[t][s] = v.im;
+        }
+    }
+    const cpx<R>* Fe = static_cast<const cpx<R>*>(a.F) + (long long)e * a.nb * 256;
+    typename MF::V dr[T], di[T];
+    // forward
+    for (int b = 0; b < a.nb; ++b) {
+        R ar[4], ai[4], ain[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> f = Fe[(long long)b * 256 + nl * 16 + 4 * s + g];
+            ar[s] = f.re;
+            ai[s] = f.im;
+            ain[s] = -f.im;
+        }
+        fg_product<R, T>(ar, ai, ain, xr, xi, dr, di);
+        const int* mp = smap + b * kFusedMapStride;
+        // next block's grouping, or for the last block its own B layout
+        fg_remap<R, N, T>(dr, di, xr, xi, mp, b + 1 < a.nb ? mp + kFusedMapStride : mp, w, o, g, lds);
+    }
+    // trace entries from X (B layout of the last block) and the selector Y
+    R yr_[T][4], yi_[T][4];
+    {
+        const int* mp = smap + (a.nb - 1) * kFusedMapStride;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int col = c0 + w[t];
+            const int base = mp[16 + o[t]];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const bool hit = col < m && base + mp[4 * s + g] == col + a.trace_off;
+                if (hit) {
+                    double* d = a.diag + 2 * ((long long)e * m + col);
+                    d[0] = (double)xr[t][s];
+                    d[1] = (double)xi[t][s];
+                }
+                yr_[t][s] = hit ? (R)1 : (R)0;
+                yi_[t][s] = (R)0;
+            }
+        }
+    }
+    // backward
+    for (int b = a.nb - 1; b >= 0; --b) {
+        // F_b^T as the A operand: A[i][k] = F[k][i]; F^dagger: its conjugate
+        R tr_[4], ti_[4], tin_[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> f = Fe[(long long)b * 256 + (4 * s + g) * 16 + nl];
+            tr_[s] = f.re;
+            ti_[s] = f.im;
+            tin_[s] = -f.im;
+        }
+        // X_{b-1} = F^dagger X_b (A = conj(F^T): imaginary part negated)
+        fg_product<R, T>(tr_, tin_, ti_, xr, xi, dr, di);
+        // M_b += Y_b X_{b-1}^T over the wavefront's vector columns, tile by tile through
+        // the transposed buffers: ys[i][vc] from Y (B layout), xs[k][vc] from X_{b-1}
+        typename MF::V mr = typename MF::V{0, 0, 0, 0}, mi = typename MF::V{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) ys[(4 * s + g) * 17 + nl] = mk<R>(yr_[t][s], yi_[t][s]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xs[MF::row(r, g) * 17 + nl] = mk<R>(dr[t][r], di[t][r]);
+            fg_wave_fence();
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> yv = ys[nl * 17 + 4 * s + g];
+                const cpx<R> xv = xs[nl * 17 + 4 * s + g];
+                mr = MF::op(yv.re, xv.re, mr);
+                mi = MF::op(yv.re, xv.im, mi);
+                mr = MF::op(-yv.im, xv.im, mr);
+                mi = MF::op(yv.im, xv.re, mi);
+            }
+            fg_wave_fence();
+        }
+        // the workgroup's wavefronts sum their M_b in a fixed order
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int idx = MF::row(r, g) * 16 + nl;
+            red[((size_t)wv * 256 + idx) * 2] = (double)mr[r];
+            red[((size_t)wv * 256 + idx) * 2 + 1] = (double)mi[r];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+            double sum = 0.0;
+            for (int v = 0; v < nwv; ++v) sum += red[(size_t)v * 512 + i];
+            a.mpart[(((long long)e * a.wgs + wgi) * a.nb + b) * 512 + i] = sum;
+        }
+        __syncthreads();
+        if (b == 0) break;
+        // Y_{b-1} = F^T Y_b, then both to the grouping of block b - 1
+        typename MF::V er[T], ei[T];
+        fg_product<R, T>(tr_, ti_, tin_, yr_, yi_, er, ei);
+        const int* mp = smap + b * kFusedMapStride;
+        const int* mn = mp - kFusedMapStride;
+        fg_remap<R, N, T>(dr, di, xr, xi, mp, mn, w, o, g, lds);
+        fg_remap<R, N, T>(er, ei, yr_, yi_, mp, mn, w, o, g, lds);
+    }
+}
+
+// Per (element, block): the workgroup partials of M_b summed in a fixed order, then the
+// block's gates run on 16-dimensional vectors -- thread k takes x = F_b e_k and
+// y = column k of M_b and walks the gates backwards (x <- G^dagger x, the pair
+// correlations of y and x, y <- G^T y) -- giving each parametric gate's 2x2
+// correlations, summed over k, into corr[e][slot][8] (the register-tile adjoint's
+// layout).  Workgroup = 4 (element, block) pairs x 16 threads.
+template <typename R>
+__global__ __launch_bounds__(64) void k_fused_block_corr(const FusedGate* __restrict__ fg,
+                                                         const int* __restrict__ block_first, int nb,
+                                                         const cpx<R>* __restrict__ kern, long long kern_bstride,
+                                                         const double* __restrict__ mpart, int wgs, int B,
+                                                         double* __restrict__ corr, int ns) {
+    const int pair = blockIdx.x * 4 + (threadIdx.x >> 4);
+    const int k = threadIdx.x & 15;
+    const bool live = pair < B * nb;
+    const int e = live ? pair / nb : 0, blk = live ? pair % nb : 0;
+    cpx<R> x[16], y[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        double re = 0.0, im = 0.0;
+        if (live) {
+            for (int v = 0; v < wgs; ++v) {
+                const double* src = mpart + (((long long)e * wgs + v) * nb + blk) * 512 + (i * 16 + k) * 2;
+                re += src[0];
+                im += src[1];
+            }
+        }
+        y[i] = mk<R>((R)re, (R)im);
+        x[i] = mk<R>(i == k ? (R)1 : (R)0, (R)0);
+    }
+    const cpx<R>* kb = kern + (long long)e * kern_bstride;
+    const int first = block_first[blk], last = block_first[blk + 1];
+    auto op_of = [&](const FusedGate& gt) {
+        Op op;
+        op.kind = (uint8_t)gt.kind;
+        op.t = (uint8_t)gt.t;
+        op.cmode = gt.c >= 0 ? CM_REG : CM_NONE;
+        op.c = (uint8_t)(gt.c >= 0 ? gt.c : 0);
+        op.idx = gt.slot;
+        return op;
+    };
+    for (int q = first; q < last; ++q) {
+        const FusedGate gt = fg[q];
+        cpx<R> kk[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+        if (gt.slot >= 0)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) kk[t] = kb[4 * gt.slot + t];
+        exec_gate<4, false, R>(x, op_of(gt), kk, 0);
+    }
+    for (int q = last - 1; q >= first; --q) {
+        const FusedGate gt = fg[q];
+        const Op op = op_of(gt);
+        cpx<R> kk[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+        if (gt.slot >= 0)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) kk[t] = kb[4 * gt.slot + t];
+        if (op.kind == OP_X || op.kind == OP_Z) {
+            exec_gate<4, false, R>(x, op, kk, 0);
+            exec_gate<4, false, R>(y, op, kk, 0);
+            continue;
+        }
+        cpx<R> kd[4] = {mk(kk[0].re, -kk[0].im), mk(kk[2].re, -kk[2].im), mk(kk[1].re, -kk[1].im),
+                        mk(kk[3].re, -kk[3].im)};
+        cpx<R> kt[4] = {kk[0], kk[2], kk[1], kk[3]};
+        cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+        switch (op.kind * 4 + op.t) {
+            SQH_ARMS(OP_U, 4, (pairs_apply<T, false, false>(x, kd, 0u), pairs_corr<T>(y, x, acc, 0u, false),
+                               pairs_apply<T, false, false>(y, kt, 0u)))
+            SQH_ARMS(OP_CU, 4,
+                     (pairs_apply<T, true, false>(x, kd, ctrl_mask<4>(op.cmode, op.c, 0)),
+                      pairs_corr<T>(y, x, acc, ctrl_mask<4>(op.cmode, op.c, 0), true),
+                      pairs_apply<T, true, false>(y, kt, ctrl_mask<4>(op.cmode, op.c, 0))))
+        default:
+            break;
+        }
+        // the 16 threads of the pair sum their correlations (fixed butterfly order)
+        double r[8];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            r[2 * t] = (double)acc[t].re;
+            r[2 * t + 1] = (double)acc[t].im;
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+#pragma unroll
+            for (int off = 8; off >= 1; off >>= 1) r[t] += __shfl_xor(r[t], off, 16);
+        }
+        if (live && k == 0 && gt.slot >= 0) {
+            double* d = corr + ((long long)e * ns + gt.slot) * 8;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) d[t] = r[t];
+        }
+    }
+}
+
+size_t fused_grad_lds_bytes(bool f32, int nb, int T, int waves) {
+    const size_t esz = f32 ? 8 : 16;
+    return (((size_t)nb * kFusedMapStride * sizeof(int) + 15) & ~(size_t)15) +
+           esz * (size_t)(16 * T * 16 + 2 * 16 * 17) * waves + sizeof(double) * 512 * (size_t)waves;
+}
+
+template <typename R>
+static hipError_t launch_fg(int N, const FusedGradArgs& a, int T, int grid, int threads, size_t shm, hipStream_t s) {
+#define SQH_FG_CASE(n, t) \
+    if (N == n && T == t) { hipLaunchKernelGGL((k_fused_grad<R, n, t>), dim3(grid), dim3(threads), shm, s, a); return hipGetLastError(); }
+    SQH_FG_CASE(4, 4)
+    SQH_FG_CASE(5, 4)
+    SQH_FG_CASE(6, 4)
+    SQH_FG_CASE(7, 4)
+    SQH_FG_CASE(8, 4)
+    SQH_FG_CASE(9, 4)
+    SQH_FG_CASE(10, 4)
+    SQH_FG_CASE(4, 2)
+    SQH_FG_CASE(5, 2)
+    SQH_FG_CASE(6, 2)
+    SQH_FG_CASE(7, 2)
+    SQH_FG_CASE(8, 2)
+    SQH_FG_CASE(9, 2)
+#undef SQH_FG_CASE
+    return hipErrorInvalidValue;
+}
+
+// Wavefronts per workgroup: the largest of 8, 4, 2 dividing the column groups whose LDS
+// fits, else 1.
+int fused_grad_waves(bool f32, int N, int m, int nb, int T) {
+    const int W = 16 * T / ((1 << N) >> 4);
+    if (W < 1) return 0;
+    const int groups = (m + W - 1) / W;
+    for (int wv : {8, 4, 2})
+        if (groups % wv == 0 && fused_grad_lds_bytes(f32, nb, T, wv) <= device_max_lds()) return wv;
+    return fused_grad_lds_bytes(f32, nb, T, 1) <= device_max_lds() ? 1 : 0;
+}
+
+int fused_grad_wgs(bool f32, int N, int m, int nb, int T) {
+    const int wv = fused_grad_waves(f32, N, m, nb, T);
+    if (wv <= 0) return 0;
+    const int W = 16 * T / ((1 << N) >> 4);
+    return ((m + W - 1) / W) / wv;
+}
+
+hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, hipStream_t s) {
+    if (N < 4 || N > kFusedCostMaxQubits || a.nb <= 0 || a.m <= 0 || a.B <= 0) return hipErrorInvalidValue;
+    const int O = (1 << N) >> 4;
+    if ((T != 2 && T != 4) || 16 * T < O || (N == 10 && T != 4)) return hipErrorInvalidValue;
+    const int W = 16 * T / O;
+    a.groups = (a.m + W - 1) / W;
+    const int wv = fused_grad_waves(f32, N, a.m, a.nb, T);
+    if (wv <= 0 || a.wgs != a.groups / wv) return hipErrorInvalidValue;  // the partials' layout the caller sized
+    const long long wgs_total = (long long)a.B * a.groups / wv;
+    const int grid = (int)((wgs_total + 7) / 8 * 8);
+    const size_t shm = fused_grad_lds_bytes(f32, a.nb, T, wv);
+    return f32 ? launch_fg<float>(N, a, T, grid, 64 * wv, shm, s) : launch_fg<double>(N, a, T, grid, 64 * wv, shm, s);
+}
+
+hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_first, int nb, const void* kern,
+                                   long long kern_bstride, const double* mpart, int wgs, int B, double* corr, int ns,
+                                   hipStream_t s) {
+    const long long pairs = (long long)B * nb;
+    if (pairs == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((pairs + 3) / 4);
+    if (f32)
+        hipLaunchKernelGGL(k_fused_block_corr<float>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
+                           nb, (const cpx<float>*)kern, kern_bstride, mpart, wgs, B, corr, ns);
+    else
+        hipLaunchKernelGGL(k_fused_block_corr<double>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
+                           nb, (const cpx<double>*)kern, kern_bstride, mpart, wgs, B, corr, ns);
+    return hipGetLastError();
+}
+
+}  // namespace sqh

@@ -99,6 +99,38 @@ struct FusedCostArgs {
     int out_stride = 0;
     int groups = 0;  // column groups per element (set by the launcher)
 };
+// A gate of a fused block: OP_U / OP_CU / OP_X / OP_Z, local target, local control (-1
+// none), parametric slot (kernel table index) or -1 (the host's FusedRec layout).
+struct FusedGate {
+    int8_t kind, t, c, pad;
+    int32_t slot;
+};
+
+// Cost and gradient through the fused blocks (sqh_k_fused_grad.hip): the forward pass
+// and trace entries as the cost kernel, then the blocks backwards with the trace
+// selector, writing per workgroup and block the 16x16 correlation partials
+// mpart[B][wgs][nb][256] (complex, as double pairs).  T: 16-column tiles per wavefront.
+struct FusedGradArgs {
+    const void* U = nullptr;  // target, rows = 2^N, m columns, row stride m
+    int m = 0, trace_off = 0, B = 0, nb = 0;
+    const void* F = nullptr;  // [B][nb][16][16] block matrices
+    const int* maps = nullptr;
+    double* diag = nullptr;
+    double* mpart = nullptr;
+    int wgs = 0;     // workgroups per element (the partials' layout)
+    int groups = 0;  // column groups per element (set by the launcher)
+};
+int fused_grad_waves(bool f32, int N, int m, int nb, int T);  // wavefronts per workgroup (0: does not fit)
+int fused_grad_wgs(bool f32, int N, int m, int nb, int T);    // workgroups per element (0: does not fit)
+size_t fused_grad_lds_bytes(bool f32, int nb, int T, int waves);
+hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, hipStream_t s);
+// Per (element, block): the partials summed, then the block's gates run backwards on
+// 16-dimensional vectors, giving each parametric slot's 2x2 correlations in
+// corr[B][ns][8] (the register-tile adjoint's layout, one partial per element).
+hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_first, int nb, const void* kern,
+                                   long long kern_bstride, const double* mpart, int wgs, int B, double* corr, int ns,
+                                   hipStream_t s);
+
 // Launch shape: tiles per wavefront (2 or 4), LDS exchange groups per block (1, or 2
 // with 4 tiles and prefetch), waves per workgroup (1, 2 or 4), next block's matrix
 // prefetched (0 or 1).  Every valid shape gives bit-identical results.

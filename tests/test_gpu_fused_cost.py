@@ -188,3 +188,85 @@ def test_fused_apply_to_matches_oracle(oracle, n):
                 assert err < (1e-5 if f32 else 1e-12), (b, err)
                 # the columns beyond the view are untouched
                 assert np.array_equal(R[b][:, m:], X[b][:, m:].astype(dt))
+
+
+def _combined(ctx, ps, prec):
+    B, P = ps.shape
+    f = np.empty(B)
+    g = np.empty((B, P))
+    ctx.check(ctx._L.sqh_optimization_problem_combined_batch(ctx.h, np.ascontiguousarray(ps).ctypes.data, B, prec,
+                                                              f.ctypes.data, g.ctypes.data))
+    return f, g
+
+
+@pytest.mark.parametrize("n", [4, 5, 6, 7, 8, 9, 10])
+def test_fused_gradient_matches_oracle(oracle, n):
+    """Costs and gradients through the fused blocks (forward on the matrix cores, the
+    adjoint sweep block by block, each block's correlations resolved to its gates)
+    against the FP64 oracle: complex128 to 1e-12, complex64 to 1e-5, every cost variant,
+    square and rectangular targets, trace offsets."""
+    gates = W.mixed_gate_circuit(n, 4 * n + 8, 3700 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    B = 3
+    ps = W.random_params(P, 3800 + n, batch=B)
+    for m, off in [(d, 0), (d // 2 + 3, 2), (3, 1)]:
+        say(f"fused gradient n={n} m={m} offset={off}")
+        U = W.haar_unitary(d, 3900 + n)[:, :m].copy()
+        ctx = nat.Context(0)
+        ctx.set_circuit(n, gates)
+        ctx.set_target(U)
+        ctx.set_trace_offset(off)
+        ctx.set_fused_cost(True)
+        for nv, ov in VARIANTS:
+            ctx.set_cost_function(nv)
+            f64, g64 = _combined(ctx, ps, nat.PREC_F64)
+            f32, g32 = _combined(ctx, ps, nat.PREC_F32)
+            toff = off if ov in (orc.COST_FROBENIUS, orc.COST_ABS_TRACE) else 0
+            for b in range(B):
+                fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b], ov, toff)
+                assert abs(f64[b] - fo) < 1e-12, (nv, b, f64[b], fo)
+                assert np.abs(g64[b] - go).max() < 1e-12, (nv, b, np.abs(g64[b] - go).max())
+                assert abs(f32[b] - fo) < 1e-5, (nv, b, f32[b], fo)
+                assert np.abs(g32[b] - go).max() < 1e-5, (nv, b, np.abs(g32[b] - go).max())
+
+
+@pytest.mark.parametrize("which", ["8q_40", "10q_60"])
+def test_fused_gradient_benchmark_circuits(oracle, which):
+    """The 8-qubit / 40-layer and 10-qubit / 60-layer circuits: fused-block costs and
+    gradients against the oracle and against the tile adjoint kernels."""
+    n, gates = (8, W.ansatz_8q_40()) if which == "8q_40" else (10, W.ansatz_10q_60())
+    say(f"fused gradient {which}")
+    P = W.param_count(gates)
+    U = W.haar_unitary(1 << n, 2126 + n).conj().T.copy()
+    B = 3
+    ps = W.random_params(P, 43, batch=B)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    ctx.set_fused_cost(False)
+    tf, tg = _combined(ctx, ps, nat.PREC_F64)
+    ctx.set_fused_cost(True)
+    f64, g64 = _combined(ctx, ps, nat.PREC_F64)
+    f32, g32 = _combined(ctx, ps, nat.PREC_F32)
+    for b in range(B):
+        fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b])
+        assert abs(f64[b] - fo) < 1e-12 and np.abs(g64[b] - go).max() < 1e-12
+        assert abs(f32[b] - fo) < 1e-5 and np.abs(g32[b] - go).max() < 1e-5
+    say(f"  max |fused - tile| gradient c128 {np.abs(g64 - tg).max():.2e}, c64 vs c128 {np.abs(g32 - g64).max():.2e}")
+
+
+def test_fused_gradient_batch_equals_single_evaluations():
+    n = 8
+    gates = W.ansatz_8q_40()
+    P = W.param_count(gates)
+    U = W.haar_unitary(256, 15).conj().T.copy()
+    ps = W.random_params(P, 16, batch=21)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    ctx.set_fused_cost(True)
+    f, g = _combined(ctx, ps, nat.PREC_F64)
+    for b in (0, 9, 20):
+        fb, gb = _combined(ctx, ps[b:b + 1], nat.PREC_F64)
+        assert fb[0] == f[b] and np.array_equal(gb[0], g[b])
