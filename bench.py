@@ -224,8 +224,31 @@ def ten_qubit_leg(steps, o):
                      "algorithmic_flops_per_eval": bfl,
                      "err_vs_fp64_oracle": float(np.abs(costs[:2].cpu().numpy() - ref).max())}
         log(f"10q {name}: {res[name]['evals_per_s']:.0f} evals/s, {tf:.1f} TFLOP/s")
-    # complex128 cost + gradient over the batch (specialised adjoint kernel)
+    # cost + gradient over the batch through the fused blocks (four block products per
+    # block: 4 x the cost evaluation's flops), complex128 and complex64
     grad = torch.empty((B, P), dtype=torch.float64, device="cuda")
+    _, gr = o.optimization_problem_combined(gates, U, n, params[0].cpu().numpy())
+    for name, prec, peak in (("c128_gradient_fused", PREC_F64, FP64_PEAK_TFLOPS),
+                             ("c64_gradient_fused", PREC_F32, FP32_MFMA_PEAK_TFLOPS)):
+        fstep = lambda: fctx.check(fctx._L.sqh_optimization_problem_combined_batch_dev(
+            fctx.h, params.data_ptr(), B, prec, costs.data_ptr(), grad.data_ptr()))
+        fstep()
+        fctx.synchronize()
+        fctx.set_kernel_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            fstep()
+        fctx.synchronize()
+        dt = (time.perf_counter() - t0) / 3
+        kms, nl = fctx.kernel_timing()
+        fctx.set_kernel_timing(False)
+        k_s = kms / max(nl, 1) / 1e3
+        gfl = 4.0 * bfl * B
+        res[name] = {"kernel_avg_ms": k_s * 1e3, "ms_per_call": dt * 1e3, "grads_per_s": B / dt,
+                     "achieved_TFLOPs": gfl / k_s / 1e12, "peak_TFLOPs": peak, "frac": gfl / k_s / 1e12 / peak,
+                     "err_grad_vs_oracle": float(np.abs(grad[0].cpu().numpy() - gr).max())}
+        log(f"10q {name}: {dt * 1e3:.2f} ms per call ({k_s * 1e3:.2f} ms kernel)")
+    # complex128 cost + gradient over the batch (specialised register-tile adjoint kernel)
     gstep = lambda: ctx.check(ctx._L.sqh_optimization_problem_combined_batch_dev(ctx.h, params.data_ptr(), B, PREC_F64,
                                                                                 costs.data_ptr(), grad.data_ptr()))
     gstep()
@@ -246,9 +269,10 @@ def ten_qubit_leg(steps, o):
     return res
 
 
-def gradient_leg(ctx, params, gates, U, o, reps=5):
+def gradient_leg(ctx, params, gates, U, o, reps=5, fused=False):
     """Batched cost + gradient (optimization_problem_combined over the batch) of the
-    8-qubit ansatz with the circuit-specialised adjoint kernel."""
+    8-qubit ansatz: through the fused 4-qubit blocks on the matrix cores (fused), or the
+    circuit-specialised register-tile adjoint kernel."""
     import numpy as np
     import torch
 
@@ -256,7 +280,6 @@ def gradient_leg(ctx, params, gates, U, o, reps=5):
     n, d = 8, 256
     B, P = params.shape
     L = ctx._L
-    flops_eval, _ = gate_work(gates, d, d, 16)
     G = torch.empty((B, P), dtype=torch.float64, device="cuda")
     fb = torch.empty(B, dtype=torch.float64, device="cuda")
 
@@ -276,15 +299,24 @@ def gradient_leg(ctx, params, gates, U, o, reps=5):
     ctx.set_kernel_timing(False)
     gk_s = gkms / max(gnl, 1) / 1e3
     _, gr1 = o.optimization_problem_combined(gates, U, n, params[1].cpu().numpy())
-    # adjoint sweep: forward pass, G^dagger on the state, G^T on the adjoint and the
-    # pair correlations (32 flop per pair against 28 for a gate): (3 + 8/7) x forward
-    gflops = flops_eval * B * (3.0 + 8.0 / 7.0)
-    log(f"8q B={B} cost+gradient: {gk_s * 1e3:.3f} ms, {gflops / gk_s / 1e12:.1f} TFLOP/s")
+    if fused:
+        # four block products per block and vector column: X forward, F^dagger X and
+        # F^T Y backward, and the correlation Y X^T -- 4 x the cost evaluation's flops
+        gflops = 4.0 * block_flops(gates, n, d) * B
+        peak = FP64_PEAK_TFLOPS
+        kernel = "k_fused_grad<double,8,T> (v_mfma_f64_16x16x4f64)"
+    else:
+        # adjoint sweep: forward pass, G^dagger on the state, G^T on the adjoint and the
+        # pair correlations (32 flop per pair against 28 for a gate): (3 + 8/7) x forward
+        gflops = gate_work(gates, d, d, 16)[0] * B * (3.0 + 8.0 / 7.0)
+        peak = FP64_PEAK_TFLOPS
+        kernel = "sqh_jit_kernel (adjoint gradient specialised to the circuit)"
+    log(f"8q B={B} cost+gradient ({'fused' if fused else 'tile'}): {gk_s * 1e3:.3f} ms kernel, "
+        f"{(t1 - t0) / reps * 1e3:.3f} ms per call, {gflops / gk_s / 1e12:.1f} TFLOP/s")
     return {"ms": (t1 - t0) / reps * 1e3, "kernel_avg_ms": gk_s * 1e3, "grads_per_s": B / ((t1 - t0) / reps),
-            "algorithmic_flops": gflops, "achieved_TFLOPs": gflops / gk_s / 1e12,
-            "frac_of_fp64_peak": gflops / gk_s / 1e12 / FP64_PEAK_TFLOPS,
-            "err_grad_vs_oracle": float(np.abs(G[1].cpu().numpy() - gr1).max()),
-            "kernel": "sqh_jit_kernel (adjoint gradient specialised to the circuit)"}
+            "algorithmic_flops": gflops, "achieved_TFLOPs": gflops / gk_s / 1e12, "peak_TFLOPs": peak,
+            "frac_of_peak": gflops / gk_s / 1e12 / peak,
+            "err_grad_vs_oracle": float(np.abs(G[1].cpu().numpy() - gr1).max()), "kernel": kernel}
 
 
 def derivate_leg(gates, P, U):
@@ -601,7 +633,11 @@ def main():
                           "unit": "samples/s", "higher_is_better": True, "dtype": "bf16", "mlp": res}), flush=True)
         return
     if args.only_gradient:
-        res = gradient_leg(ctx, params, gates, U, orc.Oracle(), reps=args.steps)
+        gctx = Context(0)
+        gctx.set_circuit(n, gates)
+        gctx.set_target(U)
+        gctx.set_fused_cost(True)
+        res = gradient_leg(gctx, params, gates, U, orc.Oracle(), reps=args.steps, fused=True)
         print(json.dumps({"metric": "cost+gradient evals/s @8q batch 512 (FP64)", "value": res["grads_per_s"],
                           "unit": "gradients/s", "higher_is_better": True, "dtype": "f64", "gradient": res}),
               flush=True)
@@ -852,28 +888,37 @@ def main():
         f"{out['fused_stream_c64']['achieved_GBs']:.0f} GB/s, err {out['fused_stream_c64']['err_rel_fro_vs_fp64_oracle']:.1e}")
     del mats32
 
-    # single-vector cost + gradient latency (8 qubits, 264 angles)
+    # single-vector cost + gradient latency (8 qubits, 264 angles): the default path
+    # (fused blocks) and the register-tile adjoint
     f0 = torch.empty(1, dtype=torch.float64, device=dev)
     g0 = torch.empty(P, dtype=torch.float64, device=dev)
     p1 = params[:1].contiguous()
-
-    def grad_step():
-        ctx.check(L.sqh_optimization_problem_combined_batch_dev(ctx.h, p1.data_ptr(), 1, PREC_F64, f0.data_ptr(),
-                                                                 g0.data_ptr()))
-
-    for _ in range(3):
-        grad_step()
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(20):
-        grad_step()
-    ctx.synchronize()
-    t1 = time.perf_counter()
+    fctx = Context(0)
+    fctx.set_circuit(n, gates)
+    fctx.set_target(U)
     fr, gr = o.optimization_problem_combined(gates, U, n, p1.cpu().numpy()[0])
-    out["grad_8q_single"] = {"ms": (t1 - t0) / 20 * 1e3, "err_grad_vs_oracle": float(np.abs(g0.cpu().numpy() - gr).max()),
-                             "err_cost_vs_oracle": float(abs(f0.item() - fr))}
+    for name, c in (("grad_8q_single", fctx), ("grad_8q_single_tile", ctx)):
+        def grad_step():
+            c.check(L.sqh_optimization_problem_combined_batch_dev(c.h, p1.data_ptr(), 1, PREC_F64, f0.data_ptr(),
+                                                                   g0.data_ptr()))
 
-    out["grad_8q_b512"] = gradient_leg(ctx, params, gates, U, o)
+        for _ in range(3):
+            grad_step()
+        c.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(20):
+            grad_step()
+        c.synchronize()
+        t1 = time.perf_counter()
+        out[name] = {"ms": (t1 - t0) / 20 * 1e3, "err_grad_vs_oracle": float(np.abs(g0.cpu().numpy() - gr).max()),
+                     "err_cost_vs_oracle": float(abs(f0.item() - fr)),
+                     "path": "fused blocks (default)" if c is fctx else "register-tile adjoint (specialised)"}
+    log(f"8q single cost+gradient: fused {out['grad_8q_single']['ms']:.3f} ms, tile "
+        f"{out['grad_8q_single_tile']['ms']:.3f} ms")
+
+    fctx.set_fused_cost(True)
+    out["grad_8q_b512"] = gradient_leg(fctx, params, gates, U, o, fused=True)
+    out["grad_8q_b512_tile"] = gradient_leg(ctx, params, gates, U, o)
     out["ten_qubit_b64"] = ten_qubit_leg(args.steps, o)
     out["apply_derivate_8q"] = derivate_leg(gates, P, U)
     if not args.skip_solves:

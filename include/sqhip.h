@@ -337,6 +337,13 @@ int sqh_set_fused_cost(sqh_ctx* ctx, int mode);
 int sqh_set_fused_cost_config(sqh_ctx* ctx, int tiles_per_wave, int exchange_groups, int waves, int prefetch);
 int sqh_get_fused_cost_config(sqh_ctx* ctx, int is_f32, int B, int* tiles_per_wave, int* exchange_groups,
                               int* waves, int* prefetch);
+/* Launch shape of the fused-block cost + gradient kernel: 16-column tiles per
+ * wavefront (2 or 4; 10 qubits need 4) and wavefronts per workgroup (1, 2, 4 or 8,
+ * dividing the element's column groups).  Both zero (the default): a fixed rule per
+ * qubit count.  A shape that does not fit the circuit, target or the device's LDS falls
+ * back to the rule.  The shape sets how the correlation sums are grouped, so different
+ * shapes agree to rounding (each shape is deterministic). */
+int sqh_set_fused_grad_config(sqh_ctx* ctx, int tiles_per_wave, int waves);
 /* Introspection (no GPU needed): generate and compile, for gfx950, the run-time
  * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient, 3 the
  * persistent solver of sqh_decompose_batch, up to 4 qubits; uniform = one batch element

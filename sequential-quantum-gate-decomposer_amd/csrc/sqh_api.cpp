@@ -150,9 +150,15 @@ std::vector<int> fused_row_maps(int N, const FusionPlan& fp) {
 // switch is on -- or automatic (the default) where it measured faster on MI355X than
 // the register-tile kernels: from 7 qubits, or from 5 when the circuit-specialised
 // kernels are off.
+// The circuit fuses into 4-qubit blocks the fused kernels take (4 to 10 qubits).
+bool fused_plan_fits(const sqh_ctx* c) {
+    const int N = c->prog.N;
+    return N >= 4 && N <= kFusedCostMaxQubits && c->fplan.k == 4 && !c->fplan.qubits.empty();
+}
+
 bool fused_pass_applies(const sqh_ctx* c) {
     const int N = c->prog.N;
-    if (N < 4 || N > kFusedCostMaxQubits || c->fplan.k != 4 || c->fplan.qubits.empty()) return false;
+    if (!fused_plan_fits(c)) return false;
     if (c->fused_cost == SQH_FUSED_COST_AUTO) return N >= 7 || (N >= 5 && !c->jit);
     return c->fused_cost == SQH_FUSED_COST_ON;
 }
@@ -190,17 +196,38 @@ int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     return fused_cost_launch(c, f32, a);
 }
 
-// 16-column tiles per wavefront of the fused-block gradient kernel: two (three resident
-// wavefronts per SIMD in complex128), four at 10 qubits (a column of 1024 rows is 64
-// vector columns).
-int fused_grad_tiles(int N) { return N >= 10 ? 4 : 2; }
+// Launch shape of the fused-block gradient for the context's circuit and target and a
+// batch of B: the one set by sqh_set_fused_grad_config when it fits, else four 16-column
+// tiles per wavefront from 9 qubits or from 64 elements (fewer, larger partials), two
+// below (more wavefronts for small batches), and the most wavefronts per workgroup, up
+// to four, that divide the column groups -- the fastest shapes measured on MI355X.
+// Returns false when no shape fits the device's LDS.
+bool fused_grad_shape(const sqh_ctx* c, bool f32, int B, int& T, int& waves) {
+    const int N = c->prog.N, m = c->cols, nb = (int)c->fplan.qubits.size();
+    if (c->fg_force_tiles && fused_grad_shape_valid(f32, N, m, nb, c->fg_force_tiles, c->fg_force_waves)) {
+        T = c->fg_force_tiles;
+        waves = c->fg_force_waves;
+        return true;
+    }
+    for (int t : {N >= 9 || B >= 64 ? 4 : 2, 4})
+        for (int w : {4, 2, 1})
+            if (fused_grad_shape_valid(f32, N, m, nb, t, w)) {
+                T = t;
+                waves = w;
+                return true;
+            }
+    return false;
+}
 
 // Whether a cost + gradient evaluation runs through the fused blocks: the switch on
-// (SQH_FUSED_COST_ON), 4 <= N <= 10, no correction channels, and the kernel's LDS fits.
-bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32) {
-    if (ts.nch != 0 || c->fused_cost != SQH_FUSED_COST_ON || !fused_pass_applies(c)) return false;
-    const int N = c->prog.N;
-    return fused_grad_wgs(f32, N, c->cols, (int)c->fplan.qubits.size(), fused_grad_tiles(N)) > 0;
+// (SQH_FUSED_COST_ON, or _AUTO from 6 qubits, where it measured faster than the
+// register-tile adjoint at every batch size), 4 <= N <= 10, no correction channels, and
+// a launch shape fits.
+bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32, int B) {
+    if (ts.nch != 0 || !fused_plan_fits(c)) return false;
+    if (c->fused_cost == SQH_FUSED_COST_OFF || (c->fused_cost == SQH_FUSED_COST_AUTO && c->prog.N < 6)) return false;
+    int T, waves;
+    return fused_grad_shape(c, f32, B, T, waves);
 }
 
 // Costs and gradients of B parameter vectors through the fused blocks: block matrices,
@@ -209,8 +236,9 @@ bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32) {
 int fused_grad_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off) {
     const FusionPlan& fp = c->fplan;
     const int N = c->prog.N, nb = (int)fp.qubits.size(), ns = (int)c->prog.slots.size(), m = c->cols;
-    const int T = fused_grad_tiles(N);
-    const int wgs = fused_grad_wgs(f32, N, m, nb, T);
+    int T = 0, waves = 0;
+    if (!fused_grad_shape(c, f32, B, T, waves)) return fail(c, SQH_ERR_ARG, "no fused gradient launch shape fits");
+    const int wgs = fused_grad_wgs(N, m, T, waves);
     const size_t esz = f32 ? 8 : 16;
     HIPTRY(c, c->d_fused.ensure((size_t)B * nb * 256 * esz));
     HIPTRY(c, c->d_mpart.ensure((size_t)B * wgs * nb * 512 * sizeof(double)));
@@ -230,10 +258,11 @@ int fused_grad_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     a.wgs = wgs;
     {
         TimedLaunch tl(c);
-        HIPTRY(c, launch_fused_grad(f32, N, a, T, c->stream));
+        HIPTRY(c, launch_fused_grad(f32, N, a, T, waves, c->stream));
     }
     HIPTRY(c, launch_fused_block_corr(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4,
-                                      c->d_mpart.as<double>(), wgs, B, c->d_fcorr.as<double>(), ns, c->stream));
+                                      c->d_fused.p, c->d_mpart.as<double>(), wgs, B, c->d_fcorr.as<double>(), ns,
+                                      c->stream));
     return SQH_OK;
 }
 
@@ -312,7 +341,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
         return SQH_OK;
     };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
-    if (d_grad && !fixed && fused_grad_applies(c, ts, f32)) {
+    if (d_grad && !fixed && fused_grad_applies(c, ts, f32, B)) {
         if ((rc = fused_grad_eval(c, f32, B, kern, diag, off))) return rc;
         HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, c->d_fcorr.as<double>(), 1, ns, c->d_slots.as<Slot>(),
                                 d_params, c->prog.P, d_grad, c->stream, BatchIndex(), ts));

@@ -87,6 +87,7 @@ struct sqh_ctx {
     sqh::DevBuf d_fmaps;
     sqh::DevBuf d_fcorr;  // [B][ns][8] gate correlations of the fused-block gradient
     int fused_cost = SQH_FUSED_COST_AUTO;
+    int fg_force_tiles = 0, fg_force_waves = 0;  // fused gradient launch shape (0: automatic)
     // its launch shape: forced (sqh_set_fused_cost_config) or picked per (precision,
     // qubits, columns, batch) by the run-time tuner
     sqh::FusedCostCfg fused_force;

@@ -61,6 +61,10 @@ __device__ __forceinline__ int fg_slot(int j) {
 
 __device__ __forceinline__ void fg_wave_fence() { __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront"); }
 
+// Entry (r, c) of a 16 x 16 tile in LDS, columns rotated by the row (XOR) so that both a
+// row-wise and a column-wise sweep of 16 lanes touch 16 different bank groups.
+__device__ __forceinline__ int fg_tile(int r, int c) { return r * 16 + (c ^ r); }
+
 // Complex 16x16 tile product Y = A X for T tiles: A given by (ar, ai, ain = -ai) per
 // step, X the B operand (xr, xi) per tile and step; result in the D layout.
 template <typename R, int T>
@@ -123,16 +127,16 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
     constexpr int O = D >> 4;
     constexpr int W = 16 * T / O;
     static_assert(W >= 1, "a wavefront holds whole columns");
-    constexpr int RG = 16 * T * 16;  // remap region (complex entries) per wavefront
-    constexpr int TS = 16 * 17;      // one transposed tile, rows padded to 17
+    // Per wavefront one LDS region of RG complex entries, used in turn by the remaps, the
+    // two transposed tiles of the correlation product (2 x 256 entries) and the
+    // workgroup's reduction of M_b (512 doubles).
+    constexpr int RG = 16 * T * 16;
+    static_assert(RG >= 512 && RG * sizeof(cpx<R>) >= 512 * sizeof(double), "the region holds the tiles");
     extern __shared__ __align__(16) unsigned char fg_smem[];
     const int nwv = blockDim.x >> 6;
     int* smap = reinterpret_cast<int*>(fg_smem);
     const int nmap = a.nb * kFusedMapStride;
-    unsigned char* p = fg_smem + (((size_t)nmap * sizeof(int) + 15) & ~(size_t)15);
-    cpx<R>* region = reinterpret_cast<cpx<R>*>(p);
-    p += sizeof(cpx<R>) * (size_t)(RG + 2 * TS) * nwv;
-    double* red = reinterpret_cast<double*>(p);  // [waves][256][2] partial M of the block
+    cpx<R>* region = reinterpret_cast<cpx<R>*>(fg_smem + (((size_t)nmap * sizeof(int) + 15) & ~(size_t)15));
     for (int i = threadIdx.x; i < nmap; i += blockDim.x) smap[i] = a.maps[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -146,9 +150,9 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
     const int cg = (int)(wave % a.groups);
     const int c0 = cg * W;
     const int wgi = cg / nwv;  // the workgroup's index within its element
-    cpx<R>* lds = region + (size_t)wv * (RG + 2 * TS);
-    cpx<R>* ys = lds + RG;
-    cpx<R>* xs = ys + TS;
+    cpx<R>* lds = region + (size_t)wv * RG;
+    cpx<R>* ys = lds;
+    cpx<R>* xs = lds + 256;
     const int nl = lane & 15, g = lane >> 4;
     int w[T], o[T];
 #pragma unroll
@@ -228,14 +232,14 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
 #pragma unroll
         for (int t = 0; t < T; ++t) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) ys[(4 * s + g) * 17 + nl] = mk<R>(yr_[t][s], yi_[t][s]);
+            for (int s = 0; s < 4; ++s) ys[fg_tile(4 * s + g, nl)] = mk<R>(yr_[t][s], yi_[t][s]);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) xs[MF::row(r, g) * 17 + nl] = mk<R>(dr[t][r], di[t][r]);
+            for (int r = 0; r < 4; ++r) xs[fg_tile(MF::row(r, g), nl)] = mk<R>(dr[t][r], di[t][r]);
             fg_wave_fence();
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const cpx<R> yv = ys[nl * 17 + 4 * s + g];
-                const cpx<R> xv = xs[nl * 17 + 4 * s + g];
+                const cpx<R> yv = ys[fg_tile(nl, 4 * s + g)];
+                const cpx<R> xv = xs[fg_tile(nl, 4 * s + g)];
                 mr = MF::op(yv.re, xv.re, mr);
                 mi = MF::op(yv.re, xv.im, mi);
                 mr = MF::op(-yv.im, xv.im, mr);
@@ -243,17 +247,19 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
             }
             fg_wave_fence();
         }
-        // the workgroup's wavefronts sum their M_b in a fixed order
+        // the workgroup's wavefronts sum their M_b in a fixed order (each wavefront's
+        // region as 512 doubles: [256][re, im])
+        double* red = reinterpret_cast<double*>(lds);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int idx = MF::row(r, g) * 16 + nl;
-            red[((size_t)wv * 256 + idx) * 2] = (double)mr[r];
-            red[((size_t)wv * 256 + idx) * 2 + 1] = (double)mi[r];
+            red[idx * 2] = (double)mr[r];
+            red[idx * 2 + 1] = (double)mi[r];
         }
         __syncthreads();
         for (int i = threadIdx.x; i < 512; i += blockDim.x) {
             double sum = 0.0;
-            for (int v = 0; v < nwv; ++v) sum += red[(size_t)v * 512 + i];
+            for (int v = 0; v < nwv; ++v) sum += reinterpret_cast<const double*>(region + (size_t)v * RG)[i];
             a.mpart[(((long long)e * a.wgs + wgi) * a.nb + b) * 512 + i] = sum;
         }
         __syncthreads();
@@ -268,105 +274,125 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
     }
 }
 
-// Per (element, block): the workgroup partials of M_b summed in a fixed order, then the
-// block's gates run on 16-dimensional vectors -- thread k takes x = F_b e_k and
-// y = column k of M_b and walks the gates backwards (x <- G^dagger x, the pair
-// correlations of y and x, y <- G^T y) -- giving each parametric gate's 2x2
-// correlations, summed over k, into corr[e][slot][8] (the register-tile adjoint's
-// layout).  Workgroup = 4 (element, block) pairs x 16 threads.
+// The workgroup partials of every (element, block) M summed in a fixed order, in place
+// into the first partial: thread per double, consecutive threads on consecutive
+// entries.
+__global__ __launch_bounds__(256) void k_fused_sum_partials(double* __restrict__ mpart, int wgs, int nb,
+                                                            long long total) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;  // total = B * nb * 512
+    const long long per_e = (long long)nb * 512;
+    const long long e = i / per_e, r = i % per_e;
+    double* base = mpart + e * wgs * per_e + r;
+    double sum = base[0];
+    for (int v = 1; v < wgs; ++v) sum += base[(long long)v * per_e];
+    base[0] = sum;
+}
+
+// Gates staged per pass and batch elements per workgroup (x 16 block columns = one
+// wavefront) of k_fused_block_corr.
+constexpr int kCorrChunk = 32;
+constexpr int kCorrElems = 4;
+
+// Workgroup -> (block, 4 batch elements); thread -> (element, column k of the block).
+// Thread k starts from x = F_b e_k (column k of the block matrix) and y = column k of
+// M_b (the workgroup partials, summed into the first) and walks the block's gates
+// backwards: x <- G^dagger x, the pair correlations of y and x, y <- G^T y.  The 16
+// threads of an element sum their correlations in a fixed butterfly order, giving each
+// parametric gate's 2x2 correlations in corr[e][slot][8] (the register-tile adjoint's
+// layout).  Gate records and the elements' 2x2 kernels are staged in LDS a chunk at a
+// time (from the block's end), so the walk issues no dependent global loads.
 template <typename R>
 __global__ __launch_bounds__(64) void k_fused_block_corr(const FusedGate* __restrict__ fg,
                                                          const int* __restrict__ block_first, int nb,
                                                          const cpx<R>* __restrict__ kern, long long kern_bstride,
-                                                         const double* __restrict__ mpart, int wgs, int B,
-                                                         double* __restrict__ corr, int ns) {
-    const int pair = blockIdx.x * 4 + (threadIdx.x >> 4);
-    const int k = threadIdx.x & 15;
-    const bool live = pair < B * nb;
-    const int e = live ? pair / nb : 0, blk = live ? pair % nb : 0;
+                                                         const cpx<R>* __restrict__ F, const double* __restrict__ mpart,
+                                                         int wgs, int B, double* __restrict__ corr, int ns) {
+    __shared__ FusedGate srec[kCorrChunk];
+    __shared__ cpx<R> skern[kCorrElems][kCorrChunk][4];
+    const int blk = blockIdx.x % nb;
+    const int e0 = (blockIdx.x / nb) * kCorrElems;
+    const int le = threadIdx.x >> 4, k = threadIdx.x & 15;
+    const int e = e0 + le;
+    const bool live = e < B;
     cpx<R> x[16], y[16];
+    {
+        const cpx<R>* Fb = F + ((long long)(live ? e : 0) * nb + blk) * 256;
+        const double* Mb = mpart + ((long long)(live ? e : 0) * wgs * nb + blk) * 512;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) {
-        double re = 0.0, im = 0.0;
-        if (live) {
-            for (int v = 0; v < wgs; ++v) {
-                const double* src = mpart + (((long long)e * wgs + v) * nb + blk) * 512 + (i * 16 + k) * 2;
-                re += src[0];
-                im += src[1];
-            }
+        for (int i = 0; i < 16; ++i) {
+            x[i] = live ? Fb[i * 16 + k] : mk<R>(0, 0);
+            y[i] = live ? mk<R>((R)Mb[(i * 16 + k) * 2], (R)Mb[(i * 16 + k) * 2 + 1]) : mk<R>(0, 0);
         }
-        y[i] = mk<R>((R)re, (R)im);
-        x[i] = mk<R>(i == k ? (R)1 : (R)0, (R)0);
     }
-    const cpx<R>* kb = kern + (long long)e * kern_bstride;
     const int first = block_first[blk], last = block_first[blk + 1];
-    auto op_of = [&](const FusedGate& gt) {
-        Op op;
-        op.kind = (uint8_t)gt.kind;
-        op.t = (uint8_t)gt.t;
-        op.cmode = gt.c >= 0 ? CM_REG : CM_NONE;
-        op.c = (uint8_t)(gt.c >= 0 ? gt.c : 0);
-        op.idx = gt.slot;
-        return op;
-    };
-    for (int q = first; q < last; ++q) {
-        const FusedGate gt = fg[q];
-        cpx<R> kk[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
-        if (gt.slot >= 0)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) kk[t] = kb[4 * gt.slot + t];
-        exec_gate<4, false, R>(x, op_of(gt), kk, 0);
-    }
-    for (int q = last - 1; q >= first; --q) {
-        const FusedGate gt = fg[q];
-        const Op op = op_of(gt);
-        cpx<R> kk[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
-        if (gt.slot >= 0)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) kk[t] = kb[4 * gt.slot + t];
-        if (op.kind == OP_X || op.kind == OP_Z) {
-            exec_gate<4, false, R>(x, op, kk, 0);
-            exec_gate<4, false, R>(y, op, kk, 0);
-            continue;
+    for (int p1 = last; p1 > first; p1 -= kCorrChunk) {
+        const int p0 = max(first, p1 - kCorrChunk), cnt = p1 - p0;
+        __syncthreads();  // the previous chunk is consumed
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) srec[i] = fg[p0 + i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < kCorrElems * cnt * 4; i += blockDim.x) {
+            const int el = i / (cnt * 4), gi = (i / 4) % cnt, t = i % 4;
+            const int slot = srec[gi].slot;
+            if (slot >= 0 && e0 + el < B) skern[el][gi][t] = kern[(long long)(e0 + el) * kern_bstride + 4 * slot + t];
         }
-        cpx<R> kd[4] = {mk(kk[0].re, -kk[0].im), mk(kk[2].re, -kk[2].im), mk(kk[1].re, -kk[1].im),
-                        mk(kk[3].re, -kk[3].im)};
-        cpx<R> kt[4] = {kk[0], kk[2], kk[1], kk[3]};
-        cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
-        switch (op.kind * 4 + op.t) {
-            SQH_ARMS(OP_U, 4, (pairs_apply<T, false, false>(x, kd, 0u), pairs_corr<T>(y, x, acc, 0u, false),
-                               pairs_apply<T, false, false>(y, kt, 0u)))
-            SQH_ARMS(OP_CU, 4,
-                     (pairs_apply<T, true, false>(x, kd, ctrl_mask<4>(op.cmode, op.c, 0)),
-                      pairs_corr<T>(y, x, acc, ctrl_mask<4>(op.cmode, op.c, 0), true),
-                      pairs_apply<T, true, false>(y, kt, ctrl_mask<4>(op.cmode, op.c, 0))))
-        default:
-            break;
-        }
-        // the 16 threads of the pair sum their correlations (fixed butterfly order)
-        double r[8];
+        __syncthreads();
+        for (int gi = cnt - 1; gi >= 0; --gi) {
+            const FusedGate gt = srec[gi];
+            Op op;
+            op.kind = (uint8_t)gt.kind;
+            op.t = (uint8_t)gt.t;
+            op.cmode = gt.c >= 0 ? CM_REG : CM_NONE;
+            op.c = (uint8_t)(gt.c >= 0 ? gt.c : 0);
+            op.idx = gt.slot;
+            cpx<R> kk[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+            if (gt.slot >= 0) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            r[2 * t] = (double)acc[t].re;
-            r[2 * t + 1] = (double)acc[t].im;
-        }
+                for (int t = 0; t < 4; ++t) kk[t] = skern[le][gi][t];
+            }
+            if (op.kind == OP_X || op.kind == OP_Z) {
+                // self-inverse real symmetric: G^dagger = G^T = G
+                exec_gate<4, false, R>(x, op, kk, 0);
+                exec_gate<4, false, R>(y, op, kk, 0);
+                continue;
+            }
+            cpx<R> kd[4] = {mk(kk[0].re, -kk[0].im), mk(kk[2].re, -kk[2].im), mk(kk[1].re, -kk[1].im),
+                            mk(kk[3].re, -kk[3].im)};
+            cpx<R> kt[4] = {kk[0], kk[2], kk[1], kk[3]};
+            cpx<R> acc[4] = {mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0), mk<R>(0, 0)};
+            switch (op.kind * 4 + op.t) {
+                SQH_ARMS(OP_U, 4, (pairs_apply<T, false, false>(x, kd, 0u), pairs_corr<T>(y, x, acc, 0u, false),
+                                   pairs_apply<T, false, false>(y, kt, 0u)))
+                SQH_ARMS(OP_CU, 4,
+                         (pairs_apply<T, true, false>(x, kd, ctrl_mask<4>(op.cmode, op.c, 0)),
+                          pairs_corr<T>(y, x, acc, ctrl_mask<4>(op.cmode, op.c, 0), true),
+                          pairs_apply<T, true, false>(y, kt, ctrl_mask<4>(op.cmode, op.c, 0))))
+            default:
+                break;
+            }
+            double r[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
+            for (int t = 0; t < 4; ++t) {
+                r[2 * t] = (double)acc[t].re;
+                r[2 * t + 1] = (double)acc[t].im;
+            }
 #pragma unroll
-            for (int off = 8; off >= 1; off >>= 1) r[t] += __shfl_xor(r[t], off, 16);
-        }
-        if (live && k == 0 && gt.slot >= 0) {
-            double* d = corr + ((long long)e * ns + gt.slot) * 8;
+            for (int t = 0; t < 8; ++t) {
 #pragma unroll
-            for (int t = 0; t < 8; ++t) d[t] = r[t];
+                for (int off = 8; off >= 1; off >>= 1) r[t] += __shfl_xor(r[t], off, 16);
+            }
+            if (live && k == 0 && gt.slot >= 0) {
+                double* d = corr + ((long long)e * ns + gt.slot) * 8;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) d[t] = r[t];
+            }
         }
     }
 }
 
 size_t fused_grad_lds_bytes(bool f32, int nb, int T, int waves) {
     const size_t esz = f32 ? 8 : 16;
-    return (((size_t)nb * kFusedMapStride * sizeof(int) + 15) & ~(size_t)15) +
-           esz * (size_t)(16 * T * 16 + 2 * 16 * 17) * waves + sizeof(double) * 512 * (size_t)waves;
+    return (((size_t)nb * kFusedMapStride * sizeof(int) + 15) & ~(size_t)15) + esz * (size_t)(16 * T * 16) * waves;
 }
 
 template <typename R>
@@ -390,50 +416,51 @@ static hipError_t launch_fg(int N, const FusedGradArgs& a, int T, int grid, int 
     return hipErrorInvalidValue;
 }
 
-// Wavefronts per workgroup: the largest of 8, 4, 2 dividing the column groups whose LDS
-// fits, else 1.
-int fused_grad_waves(bool f32, int N, int m, int nb, int T) {
+// Column groups (one per wavefront) of an element: 16 T vector columns per wavefront,
+// 2^N / 16 of them per matrix column.
+static int fg_groups(int N, int m, int T) {
     const int W = 16 * T / ((1 << N) >> 4);
-    if (W < 1) return 0;
-    const int groups = (m + W - 1) / W;
-    for (int wv : {8, 4, 2})
-        if (groups % wv == 0 && fused_grad_lds_bytes(f32, nb, T, wv) <= device_max_lds()) return wv;
-    return fused_grad_lds_bytes(f32, nb, T, 1) <= device_max_lds() ? 1 : 0;
+    return W < 1 ? 0 : (m + W - 1) / W;
 }
 
-int fused_grad_wgs(bool f32, int N, int m, int nb, int T) {
-    const int wv = fused_grad_waves(f32, N, m, nb, T);
-    if (wv <= 0) return 0;
-    const int W = 16 * T / ((1 << N) >> 4);
-    return ((m + W - 1) / W) / wv;
+bool fused_grad_shape_valid(bool f32, int N, int m, int nb, int T, int waves) {
+    if (N < 4 || N > kFusedCostMaxQubits || (T != 2 && T != 4) || m <= 0 || nb <= 0) return false;
+    if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
+    const int groups = fg_groups(N, m, T);
+    return groups > 0 && groups % waves == 0 && fused_grad_lds_bytes(f32, nb, T, waves) <= device_max_lds();
 }
 
-hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, hipStream_t s) {
-    if (N < 4 || N > kFusedCostMaxQubits || a.nb <= 0 || a.m <= 0 || a.B <= 0) return hipErrorInvalidValue;
-    const int O = (1 << N) >> 4;
-    if ((T != 2 && T != 4) || 16 * T < O || (N == 10 && T != 4)) return hipErrorInvalidValue;
-    const int W = 16 * T / O;
-    a.groups = (a.m + W - 1) / W;
-    const int wv = fused_grad_waves(f32, N, a.m, a.nb, T);
-    if (wv <= 0 || a.wgs != a.groups / wv) return hipErrorInvalidValue;  // the partials' layout the caller sized
-    const long long wgs_total = (long long)a.B * a.groups / wv;
+int fused_grad_wgs(int N, int m, int T, int waves) { return fg_groups(N, m, T) / waves; }
+
+hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, int waves, hipStream_t s) {
+    if (a.B <= 0 || !fused_grad_shape_valid(f32, N, a.m, a.nb, T, waves)) return hipErrorInvalidValue;
+    a.groups = fg_groups(N, a.m, T);
+    if (a.wgs != a.groups / waves) return hipErrorInvalidValue;  // the partials' layout the caller sized
+    const long long wgs_total = (long long)a.B * a.wgs;
     const int grid = (int)((wgs_total + 7) / 8 * 8);
-    const size_t shm = fused_grad_lds_bytes(f32, a.nb, T, wv);
-    return f32 ? launch_fg<float>(N, a, T, grid, 64 * wv, shm, s) : launch_fg<double>(N, a, T, grid, 64 * wv, shm, s);
+    const size_t shm = fused_grad_lds_bytes(f32, a.nb, T, waves);
+    return f32 ? launch_fg<float>(N, a, T, grid, 64 * waves, shm, s)
+               : launch_fg<double>(N, a, T, grid, 64 * waves, shm, s);
 }
 
 hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_first, int nb, const void* kern,
-                                   long long kern_bstride, const double* mpart, int wgs, int B, double* corr, int ns,
-                                   hipStream_t s) {
+                                   long long kern_bstride, const void* F, double* mpart, int wgs, int B, double* corr,
+                                   int ns, hipStream_t s) {
     const long long pairs = (long long)B * nb;
     if (pairs == 0) return hipSuccess;
-    const unsigned grid = (unsigned)((pairs + 3) / 4);
+    if (wgs > 1) {
+        const long long total = pairs * 512;
+        hipLaunchKernelGGL(k_fused_sum_partials, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
+                           mpart, wgs, nb, total);
+    }
+    const long long groups = (B + kCorrElems - 1) / kCorrElems;
+    const unsigned grid = (unsigned)(groups * nb);
     if (f32)
         hipLaunchKernelGGL(k_fused_block_corr<float>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
-                           nb, (const cpx<float>*)kern, kern_bstride, mpart, wgs, B, corr, ns);
+                           nb, (const cpx<float>*)kern, kern_bstride, (const cpx<float>*)F, mpart, wgs, B, corr, ns);
     else
         hipLaunchKernelGGL(k_fused_block_corr<double>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
-                           nb, (const cpx<double>*)kern, kern_bstride, mpart, wgs, B, corr, ns);
+                           nb, (const cpx<double>*)kern, kern_bstride, (const cpx<double>*)F, mpart, wgs, B, corr, ns);
     return hipGetLastError();
 }
 

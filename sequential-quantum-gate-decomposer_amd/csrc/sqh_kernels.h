@@ -120,16 +120,20 @@ struct FusedGradArgs {
     int wgs = 0;     // workgroups per element (the partials' layout)
     int groups = 0;  // column groups per element (set by the launcher)
 };
-int fused_grad_waves(bool f32, int N, int m, int nb, int T);  // wavefronts per workgroup (0: does not fit)
-int fused_grad_wgs(bool f32, int N, int m, int nb, int T);    // workgroups per element (0: does not fit)
+// Launch shape: T 16-column tiles per wavefront (2 or 4; 4 at 10 qubits), waves 1, 2, 4
+// or 8 per workgroup dividing the element's column groups; wgs = workgroups per element
+// (the partials' count).
+bool fused_grad_shape_valid(bool f32, int N, int m, int nb, int T, int waves);
+int fused_grad_wgs(int N, int m, int T, int waves);
 size_t fused_grad_lds_bytes(bool f32, int nb, int T, int waves);
-hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, hipStream_t s);
+hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, int waves, hipStream_t s);
 // Per (element, block): the partials summed, then the block's gates run backwards on
-// 16-dimensional vectors, giving each parametric slot's 2x2 correlations in
-// corr[B][ns][8] (the register-tile adjoint's layout, one partial per element).
+// 16-dimensional vectors (from the columns of F, the block matrices [B][nb][16][16]),
+// giving each parametric slot's 2x2 correlations in corr[B][ns][8] (the register-tile
+// adjoint's layout, one partial per element).
 hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_first, int nb, const void* kern,
-                                   long long kern_bstride, const double* mpart, int wgs, int B, double* corr, int ns,
-                                   hipStream_t s);
+                                   long long kern_bstride, const void* F, double* mpart, int wgs, int B, double* corr,
+                                   int ns, hipStream_t s);
 
 // Launch shape: tiles per wavefront (2 or 4), LDS exchange groups per block (1, or 2
 // with 4 tiles and prefetch), waves per workgroup (1, 2 or 4), next block's matrix

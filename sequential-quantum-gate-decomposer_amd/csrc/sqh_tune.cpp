@@ -225,6 +225,17 @@ extern "C" int sqh_set_fused_cost_config(sqh_ctx* c, int tiles_per_wave, int exc
     return SQH_OK;
 }
 
+extern "C" int sqh_set_fused_grad_config(sqh_ctx* c, int tiles_per_wave, int waves) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!(tiles_per_wave == 0 && waves == 0) &&
+        !((tiles_per_wave == 2 || tiles_per_wave == 4) && (waves == 1 || waves == 2 || waves == 4 || waves == 8)))
+        return fail(c, SQH_ERR_ARG, "tiles per wavefront must be 2 or 4 and waves 1, 2, 4 or 8; both 0 for automatic");
+    c->fg_force_tiles = tiles_per_wave;
+    c->fg_force_waves = waves;
+    return SQH_OK;
+}
+
 extern "C" int sqh_get_fused_cost_config(sqh_ctx* c, int is_f32, int B, int* tiles_per_wave, int* exchange_groups,
                                          int* waves, int* prefetch) {
     if (!c || !tiles_per_wave || !exchange_groups || !waves || !prefetch) return SQH_ERR_ARG;

@@ -45,7 +45,7 @@ EXPORTS = [
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
     "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_set_fused_cost",
-    "sqh_set_fused_cost_config", "sqh_get_fused_cost_config",
+    "sqh_set_fused_cost_config", "sqh_get_fused_cost_config", "sqh_set_fused_grad_config",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
 
@@ -161,6 +161,7 @@ def lib():
             "sqh_set_jit": ([vp, i], i),
             "sqh_set_fused_cost": ([vp, i], i),
             "sqh_set_fused_cost_config": ([vp, i, i, i, i], i),
+            "sqh_set_fused_grad_config": ([vp, i, i], i),
             "sqh_get_fused_cost_config": ([vp, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                            ctypes.POINTER(i)], i),
             "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
@@ -313,6 +314,10 @@ class Context:
     def set_fused_cost_config(self, tiles=0, groups=0, waves=0, prefetch=0):
         """Force the fused-block cost kernel's launch shape (all 0: run-time tuner)."""
         self.check(self._L.sqh_set_fused_cost_config(self.h, int(tiles), int(groups), int(waves), int(prefetch)))
+
+    def set_fused_grad_config(self, tiles=0, waves=0):
+        """Force the fused-block cost + gradient kernel's launch shape (both 0: the rule)."""
+        self.check(self._L.sqh_set_fused_grad_config(self.h, int(tiles), int(waves)))
 
     def fused_cost_config(self, f32, B):
         """(tiles per wavefront, exchange groups, waves per workgroup, prefetch) in use."""

@@ -270,3 +270,28 @@ def test_fused_gradient_batch_equals_single_evaluations():
     for b in (0, 9, 20):
         fb, gb = _combined(ctx, ps[b:b + 1], nat.PREC_F64)
         assert fb[0] == f[b] and np.array_equal(gb[0], g[b])
+
+
+@pytest.mark.parametrize("n", [4, 7, 9, 10])
+def test_fused_gradient_launch_shapes(oracle, n):
+    """Every launch shape of the cost + gradient kernel (tiles per wavefront, waves per
+    workgroup) agrees with the oracle; shapes that do not fit fall back to the rule."""
+    gates = W.mixed_gate_circuit(n, 4 * n, 4000 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    U = W.haar_unitary(d, 4100 + n)[:, :d - 8].copy()
+    ps = W.random_params(P, 4200 + n, batch=4)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_target(U)
+    ctx.set_fused_cost(True)
+    ref = [oracle.optimization_problem_combined(gates, U, n, ps[b]) for b in range(4)]
+    for t in (2, 4):
+        for w in (1, 2, 4, 8):
+            say(f"fused gradient shape n={n} tiles={t} waves={w}")
+            ctx.set_fused_grad_config(t, w)
+            f, g = _combined(ctx, ps, nat.PREC_F64)
+            for b in range(4):
+                assert abs(f[b] - ref[b][0]) < 1e-12 and np.abs(g[b] - ref[b][1]).max() < 1e-12
+    with pytest.raises(nat.SqhError):
+        ctx.set_fused_grad_config(3, 2)

@@ -20,10 +20,13 @@ def say(*a):
 
 
 def _ctx(n, gates, U, jit):
+    """A context on the register-tile kernels (the fused-block path off), specialised
+    to the circuit when jit."""
     ctx = nat.Context(0)
     ctx.set_circuit(n, gates)
     ctx.set_target(U)
     ctx.set_jit(jit)
+    ctx.set_fused_cost(False)
     return ctx
 
 
