@@ -318,9 +318,13 @@ int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
  * run.  Results agree with the tile kernels to rounding (the blocks reassociate the
  * products).  mode: SQH_FUSED_COST_OFF, _ON, or _AUTO (the default: on from 7 qubits,
  * or from 5 when the run-time specialised kernels are off -- where it measured faster
- * than the tile kernels).  No counterpart in the reference: it replaces the per-gate
- * loop of Gates_block::apply_to (Gates_block.cpp:606-710) inside
- * optimization_problem_batched (Optimization_Interface.cpp:940-1033). */
+ * than the tile kernels).  The same mode selects the fused blocks' adjoint for cost +
+ * gradient batches (sqh_optimization_problem_combined_batch[_dev]; _AUTO: from 6
+ * qubits): forward and backward block products on the matrix cores, each block's
+ * correlations then resolved to its gates.  No counterpart in the reference: it
+ * replaces the per-gate loop of Gates_block::apply_to (Gates_block.cpp:606-710) inside
+ * optimization_problem_batched (Optimization_Interface.cpp:940-1033) and the derivative
+ * products of optimization_problem_combined (Optimization_Interface.cpp:1401-1450). */
 #define SQH_FUSED_COST_OFF 0
 #define SQH_FUSED_COST_ON 1
 #define SQH_FUSED_COST_AUTO 2

@@ -305,9 +305,10 @@ class Context:
         self.check(self._L.sqh_set_jit(self.h, int(bool(enable))))
 
     def set_fused_cost(self, mode=True):
-        """Cost-only batched evaluations through the fused 4-qubit blocks on the matrix
-        cores (4-10 qubits, costs without correction channels): True / False, or
-        "auto" (the default: from 7 qubits, or from 5 without specialised kernels)."""
+        """Batched evaluations through the fused 4-qubit blocks on the matrix cores
+        (4-10 qubits, costs without correction channels): True / False, or "auto" (the
+        default: costs from 7 qubits, or from 5 without specialised kernels; costs with
+        gradients from 6 qubits)."""
         m = FUSED_COST_AUTO if mode == "auto" else (FUSED_COST_ON if mode else FUSED_COST_OFF)
         self.check(self._L.sqh_set_fused_cost(self.h, m))
 
