@@ -348,6 +348,20 @@ int sqh_get_fused_cost_config(sqh_ctx* ctx, int is_f32, int B, int* tiles_per_wa
  * back to the rule.  The shape sets how the correlation sums are grouped, so different
  * shapes agree to rounding (each shape is deterministic). */
 int sqh_set_fused_grad_config(sqh_ctx* ctx, int tiles_per_wave, int waves);
+/* Launch options of the register-tile kernels (the batched costs and gradients when
+ * the fused-block path does not run).  waves_per_group: for the forward pass with one
+ * batch element per wavefront (above 4 qubits), 4, 2 or 1 wavefronts of an element per
+ * workgroup sharing its kernel table in LDS (fewer when a group does not fit), or -1
+ * for the packed layout (several elements per wavefront).  stage_program: 1 stages the
+ * gradient pass's program (ops, layouts, kernel table) in LDS where it fits, 0 reads
+ * it through the scalar cache.  waves_per_group = 0 and stage_program = -1 (the
+ * default): the first call of each (pass, precision, qubits, columns, batch) shape times
+ * the distinct options on its own inputs and keeps the fastest.  Every option gives
+ * bit-identical results.  sqh_get_tile_config reports, for kind 1 (forward) or 2
+ * (gradient), the options in effect for the context's circuit, target and a batch of
+ * B: wavefronts per workgroup (-1 packed) and whether the program is staged. */
+int sqh_set_tile_config(sqh_ctx* ctx, int waves_per_group, int stage_program);
+int sqh_get_tile_config(sqh_ctx* ctx, int kind, int is_f32, int B, int* waves_per_group, int* stage_program);
 /* Introspection (no GPU needed): generate and compile, for gfx950, the run-time
  * kernel of a gate list (kind 0 forward store, 1 forward diagonal, 2 gradient, 3 the
  * persistent solver of sqh_decompose_batch, up to 4 qubits; uniform = one batch element

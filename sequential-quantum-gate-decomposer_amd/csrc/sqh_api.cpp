@@ -337,8 +337,7 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
             const int r = jit_launch(c, t, kind, c->stream, fixed == nullptr);
             if (r != kJitNotReady) return r;
         }
-        HIPTRY(c, kind == 2 ? launch_tile_grad(t, c->stream) : launch_tile_forward(t, c->stream));
-        return SQH_OK;
+        return tile_launch(c, t, kind, fixed == nullptr);
     };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
     if (d_grad && !fixed && fused_grad_applies(c, ts, f32, B)) {
@@ -658,6 +657,7 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
     }
     c->fplan = std::move(fp);
     c->fused_tuned.clear();
+    c->tile_tuned.clear();
     c->prog = std::move(p);
     c->gates = std::move(g);
     c->have_circuit = true;

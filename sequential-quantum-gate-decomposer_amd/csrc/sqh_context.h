@@ -102,6 +102,9 @@ struct sqh_ctx {
     bool autotune = true;
     sqh::BlockCfg block_force;
     std::map<std::array<long long, 4>, sqh::BlockCfg> block_tuned;
+    std::map<std::array<long long, 4>, sqh::TileCfg> tile_tuned;  // (pass, precision, N, m_pad, B)
+    sqh::TileCfg tile_force;                                       // forced options (tile_forced)
+    bool tile_forced = false;
     double tune_ms = 0.0;
 
     // Kernel timers: HIP events around the dominant kernel of each call.
@@ -205,6 +208,13 @@ hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, 
 size_t fused_gate_bytes();
 // The fused-block cost launch with the context's launch shape (forced, tuned or default).
 int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a);
+// A register-tile pass (kind 1: forward trace entries, 2: gradient) with the context's
+// launch options: forced, tuned for this (pass, precision, N, m_pad, B), or -- with
+// tune, tuning on, not under graph capture -- tuned now on the call's own inputs.
+int tile_launch(sqh_ctx* c, const TileLaunch& t, int kind, bool tune);
+// The launch t with options cfg applied (the gradient's staged program sized from the
+// context's program).
+TileLaunch tile_with(const sqh_ctx* c, const TileLaunch& t, int kind, const TileCfg& cfg);
 
 // Brackets one kernel launch with timer events when timing is enabled.
 struct TimedLaunch {

@@ -10,6 +10,25 @@ hipError_t launch_tile_forward_float(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_grad_double(const TileLaunch& t, hipStream_t st);
 hipError_t launch_tile_grad_float(const TileLaunch& t, hipStream_t st);
 
+// Wavefronts per workgroup the forward pass of t runs with under option wpg (-1: the
+// packed layout, several elements per wavefront).
+int tile_forward_group(const TileLaunch& t, int wpg) {
+#define SQH_FG_N(n) \
+    case n: return t.f32 ? forward_group<n, float>(t, wpg) : forward_group<n, double>(t, wpg);
+    switch (t.N) {
+        SQH_FG_N(1) SQH_FG_N(2) SQH_FG_N(3) SQH_FG_N(4) SQH_FG_N(5) SQH_FG_N(6) SQH_FG_N(7) SQH_FG_N(8) SQH_FG_N(9)
+        SQH_FG_N(10)
+    default: return -1;
+    }
+#undef SQH_FG_N
+}
+
+bool tile_grad_can_stage(const TileLaunch& t) {
+    const int cw = tile_columns_per_wave(t.N);
+    if (t.N <= 4 || t.m_pad % cw != 0 || t.stage_ops <= 0) return false;
+    return (t.f32 ? stage_bytes<float>(t) : stage_bytes<double>(t)) <= kStageLdsBytes;
+}
+
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st) {
     return t.f32 ? launch_tile_forward_float(t, st) : launch_tile_forward_double(t, st);
 }

@@ -45,12 +45,27 @@ struct TileLaunch {
                                  // derivative mode (control-0 block zeroed), or -1
     int stage_ops = 0;         // > 0: gradient kernel stages the program in LDS (largest op count)
     int stage_lays = 0;        //      ... and up to this many layouts
+    // Launch options that leave every column's arithmetic unchanged:
+    int wpg = 0;     // forward, one element per wavefront: 4, 2 or 1 wavefronts per workgroup
+                     // (falling back to fewer when the group does not fit), -1 the packed
+                     // layout, 0 the largest group that fits
+    int stage = -1;  // gradient: 0 never stage the program, otherwise stage when stage_ops > 0
 };
 
 // LDS a workgroup may allocate on the current device (160 KiB on MI355X).
 size_t device_max_lds();
 
 hipError_t launch_tile_forward(const TileLaunch& t, hipStream_t st);
+int tile_forward_group(const TileLaunch& t, int wpg);
+// Whether the gradient pass of t can stage its program in LDS (one element per
+// wavefront; kernel table, layouts and ops within the staging budget).
+bool tile_grad_can_stage(const TileLaunch& t);
+// Launch options of the register-tile kernels the tuner picks among (TileLaunch::wpg,
+// TileLaunch::stage).
+struct TileCfg {
+    int wpg = 0;
+    int stage = -1;
+};
 hipError_t launch_tile_grad(const TileLaunch& t, hipStream_t st);
 int tile_columns_per_wave(int N);
 

@@ -70,9 +70,11 @@ inline bool group_fits(const TileLaunch& t) {
 template <int N, typename R, int MODE>
 hipError_t launch_forward_nm(const TileLaunch& t, hipStream_t st, unsigned waves) {
     if constexpr (N > 4) {
-        if (uniform_waves<N>(t)) {
-            if (group_fits<N, R, 4>(t)) return launch_forward_nmu<N, R, MODE, true, 4>(t, st, waves);
-            if (group_fits<N, R, 2>(t)) return launch_forward_nmu<N, R, MODE, true, 2>(t, st, waves);
+        if (uniform_waves<N>(t) && t.wpg >= 0) {
+            if ((t.wpg == 0 || t.wpg == 4) && group_fits<N, R, 4>(t))
+                return launch_forward_nmu<N, R, MODE, true, 4>(t, st, waves);
+            if ((t.wpg == 0 || t.wpg >= 2) && group_fits<N, R, 2>(t))
+                return launch_forward_nmu<N, R, MODE, true, 2>(t, st, waves);
             return launch_forward_nmu<N, R, MODE, true, 1>(t, st, waves);
         }
     }
@@ -85,6 +87,19 @@ hipError_t launch_forward_n(const TileLaunch& t, hipStream_t st) {
     const long long items = (long long)t.B * t.m_pad;
     const unsigned waves = blocks_for(items, G::CW);
     return t.mode == 0 ? launch_forward_nm<N, R, 0>(t, st, waves) : launch_forward_nm<N, R, 1>(t, st, waves);
+}
+
+// Whether the forward pass of t would group its wavefronts WPG to a workgroup (one
+// element per wavefront, the group's kernel table and LDS fit).
+template <int N, typename R>
+inline int forward_group(const TileLaunch& t, int wpg) {
+    if constexpr (N > 4) {
+        if (!uniform_waves<N>(t) || wpg < 0) return -1;
+        if ((wpg == 0 || wpg == 4) && group_fits<N, R, 4>(t)) return 4;
+        if ((wpg == 0 || wpg >= 2) && group_fits<N, R, 2>(t)) return 2;
+        return 1;
+    }
+    return -1;
 }
 
 // LDS bytes of a staged program (kernel table, layouts, padded ops).
@@ -107,7 +122,8 @@ hipError_t launch_grad_nus(const TileLaunch& t, hipStream_t st, unsigned grid) {
 template <int N, typename R, bool UNI>
 hipError_t launch_grad_nu(const TileLaunch& t, hipStream_t st, unsigned grid) {
     if constexpr (UNI) {
-        if (t.stage_ops > 0 && stage_bytes<R>(t) <= kStageLdsBytes) return launch_grad_nus<N, R, true, true>(t, st, grid);
+        if (t.stage != 0 && t.stage_ops > 0 && stage_bytes<R>(t) <= kStageLdsBytes)
+            return launch_grad_nus<N, R, true, true>(t, st, grid);
     }
     return launch_grad_nus<N, R, UNI, false>(t, st, grid);
 }

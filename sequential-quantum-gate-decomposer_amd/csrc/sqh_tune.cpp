@@ -1,6 +1,6 @@
-// sqh_tune.cpp -- run-time tuners of the launch shapes of the matrix-core kernels: the
-// 4-qubit block kernel of the streaming path and the tile-resident fused-block cost
-// kernel.
+// sqh_tune.cpp -- run-time tuners of launch shapes: the 4-qubit block kernel of the
+// streaming path, the tile-resident fused-block cost kernel, and the launch options of
+// the register-tile forward and gradient kernels.
 //
 // Candidates: 1, 2 or 4 sixteen-column tiles per wavefront x 64 or 256 threads per
 // workgroup.  The first fused-block application of a (precision, rows, columns, batch)
@@ -10,8 +10,10 @@
 // keeps the fastest for the context.  Every candidate computes the same arithmetic per
 // element, so the choice changes timing only, never results.  Under stream capture, or
 // with tuning off, the fixed per-precision default is used.
+#include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <vector>
 
 #include "sqh_context.h"
 
@@ -205,9 +207,110 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
     return SQH_OK;
 }
 
+// Register-tile kernels: launch options that leave every column's arithmetic unchanged,
+// so every option gives bit-identical results.  Forward pass (trace entries) with one
+// batch element per wavefront: 4, 2 or 1 wavefronts of an element per workgroup
+// (sharing the element's kernel table in LDS), or the packed layout (several elements
+// per wavefront, the table read through the caches).  Gradient pass with one element
+// per wavefront: the program (ops, layouts, kernel table) staged in LDS, or read
+// through the scalar cache.  The first call of a (pass, precision, qubits, padded
+// columns, batch) shape times each distinct option on its own inputs (both passes
+// write only trace entries and correlation partials, so repeating them changes
+// nothing) and keeps the fastest.  The store pass (apply_to) keeps the rule.
+TileLaunch tile_with(const sqh_ctx* c, const TileLaunch& t, int kind, const TileCfg& cfg) {
+    TileLaunch u = t;
+    if (kind == 2) {
+        u.stage = cfg.stage;
+        if (cfg.stage == 1 && u.stage_ops == 0) {
+            u.stage_ops = u.nops;
+            u.stage_lays = (int)c->prog.layouts.size();
+        }
+    } else {
+        u.wpg = cfg.wpg;
+    }
+    return u;
+}
+
+int tile_launch(sqh_ctx* c, const TileLaunch& t, int kind, bool tune) {
+    auto launch = [&](const TileLaunch& u) -> hipError_t {
+        return kind == 2 ? launch_tile_grad(u, c->stream) : launch_tile_forward(u, c->stream);
+    };
+    if (c->tile_forced) {
+        HIPTRY(c, launch(tile_with(c, t, kind, c->tile_force)));
+        return SQH_OK;
+    }
+    const std::array<long long, 4> key{kind * 2 + (t.f32 ? 1 : 0), t.N, t.m_pad, t.B};
+    auto it = c->tile_tuned.find(key);
+    if (it != c->tile_tuned.end()) {
+        HIPTRY(c, launch(tile_with(c, t, kind, it->second)));
+        return SQH_OK;
+    }
+    // the distinct options of this launch, the rule's first
+    std::vector<TileCfg> cands{TileCfg{}};
+    if (kind == 2) {
+        if (!t.bi.run_prog && tile_grad_can_stage(tile_with(c, t, kind, TileCfg{0, 1}))) cands.push_back(TileCfg{0, 1});
+        cands[0].stage = t.stage_ops > 0 ? 1 : 0;
+    } else if (t.mode == 1) {
+        std::vector<int> groups{tile_forward_group(t, 0)};
+        for (int w : {4, 2, 1, -1}) {
+            const int g = tile_forward_group(t, w);
+            if (std::find(groups.begin(), groups.end(), g) == groups.end()) {
+                groups.push_back(g);
+                cands.push_back(TileCfg{w, -1});
+            }
+        }
+    }
+    TileCfg best = cands[0];
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPTRY(c, hipStreamIsCapturing(c->stream, &cs));
+    if (tune && c->autotune && cs == hipStreamCaptureStatusNone && cands.size() > 1) {
+        const auto t_start = std::chrono::steady_clock::now();
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPTRY(c, hipEventCreate(&e0));
+        hipError_t err = hipEventCreate(&e1);
+        float best_ms = -1.0f;
+        for (const TileCfg& cand : cands) {
+            if (err != hipSuccess) break;
+            const TileLaunch u = tile_with(c, t, kind, cand);
+            if ((err = launch(u)) != hipSuccess) break;  // warm-up
+            if ((err = hipEventRecord(e0, c->stream)) != hipSuccess) break;
+            for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch(u);
+            if (err != hipSuccess || (err = hipEventRecord(e1, c->stream)) != hipSuccess ||
+                (err = hipEventSynchronize(e1)) != hipSuccess)
+                break;
+            float ms = 0.0f;
+            if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+            if (best_ms < 0.0f || ms < best_ms) {
+                best_ms = ms;
+                best = cand;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        HIPTRY(c, err);
+        c->tile_tuned[key] = best;
+        c->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    }
+    HIPTRY(c, launch(tile_with(c, t, kind, best)));
+    return SQH_OK;
+}
+
 }  // namespace sqh
 
 using namespace sqh;
+
+extern "C" int sqh_set_tile_config(sqh_ctx* c, int waves_per_group, int stage_program) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (waves_per_group != 0 && waves_per_group != 1 && waves_per_group != 2 && waves_per_group != 4 &&
+        waves_per_group != -1)
+        return fail(c, SQH_ERR_ARG, "waves per group must be 1, 2, 4, -1 (packed) or 0 (tuner)");
+    if (stage_program != -1 && stage_program != 0 && stage_program != 1)
+        return fail(c, SQH_ERR_ARG, "stage_program must be 0, 1 or -1 (tuner)");
+    c->tile_forced = !(waves_per_group == 0 && stage_program == -1);
+    c->tile_force = TileCfg{waves_per_group, stage_program};
+    return SQH_OK;
+}
 
 extern "C" int sqh_set_fused_cost_config(sqh_ctx* c, int tiles_per_wave, int exchange_groups, int waves,
                                          int prefetch) {
@@ -253,6 +356,29 @@ extern "C" int sqh_get_fused_cost_config(sqh_ctx* c, int is_f32, int B, int* til
     *exchange_groups = cfg.groups;
     *waves = cfg.waves;
     *prefetch = cfg.prefetch;
+    return SQH_OK;
+}
+
+extern "C" int sqh_get_tile_config(sqh_ctx* c, int kind, int is_f32, int B, int* waves_per_group,
+                                   int* stage_program) {
+    if (!c || !waves_per_group || !stage_program || (kind != 1 && kind != 2) || B <= 0) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->have_circuit || !c->have_target) return fail(c, SQH_ERR_STATE, "set a circuit and a target first");
+    TileLaunch t = base_launch(c, is_f32 != 0);
+    t.mode = 1;
+    t.m = c->cols;
+    t.m_pad = m_pad_for(t.N, t.m);
+    t.B = B;
+    TileCfg cfg;
+    if (c->tile_forced) {
+        cfg = c->tile_force;
+    } else {
+        auto it = c->tile_tuned.find({kind * 2 + (is_f32 ? 1 : 0), t.N, t.m_pad, B});
+        if (it != c->tile_tuned.end()) cfg = it->second;
+    }
+    const TileLaunch u = tile_with(c, t, kind, cfg);
+    *waves_per_group = kind == 1 ? tile_forward_group(u, u.wpg) : 1;
+    *stage_program = kind == 2 ? (u.stage != 0 && tile_grad_can_stage(u) ? 1 : 0) : 0;
     return SQH_OK;
 }
 

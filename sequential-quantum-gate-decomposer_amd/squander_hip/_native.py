@@ -46,6 +46,7 @@ EXPORTS = [
     "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_set_fused_cost",
     "sqh_set_fused_cost_config", "sqh_get_fused_cost_config", "sqh_set_fused_grad_config",
+    "sqh_set_tile_config", "sqh_get_tile_config",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
 
@@ -162,6 +163,8 @@ def lib():
             "sqh_set_fused_cost": ([vp, i], i),
             "sqh_set_fused_cost_config": ([vp, i, i, i, i], i),
             "sqh_set_fused_grad_config": ([vp, i, i], i),
+            "sqh_set_tile_config": ([vp, i, i], i),
+            "sqh_get_tile_config": ([vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
             "sqh_get_fused_cost_config": ([vp, i, i, ctypes.POINTER(i), ctypes.POINTER(i), ctypes.POINTER(i),
                                            ctypes.POINTER(i)], i),
             "sqh_optimization_problem_combined_targets_dev": ([vp, vp, i, vp, vp, i, i, vp, vp], i),
@@ -319,6 +322,18 @@ class Context:
     def set_fused_grad_config(self, tiles=0, waves=0):
         """Force the fused-block cost + gradient kernel's launch shape (both 0: the rule)."""
         self.check(self._L.sqh_set_fused_grad_config(self.h, int(tiles), int(waves)))
+
+    def set_tile_config(self, waves_per_group=0, stage_program=-1):
+        """Force the register-tile kernels' launch options (0, -1: run-time tuner)."""
+        self.check(self._L.sqh_set_tile_config(self.h, int(waves_per_group), int(stage_program)))
+
+    def tile_config(self, kind, f32, B):
+        """(wavefronts per workgroup or -1 packed, program staged) in effect for kind 1
+        (forward trace entries) or 2 (gradient)."""
+        w, st = ctypes.c_int(0), ctypes.c_int(0)
+        self.check(self._L.sqh_get_tile_config(self.h, int(kind), int(bool(f32)), int(B), ctypes.byref(w),
+                                                ctypes.byref(st)))
+        return w.value, st.value
 
     def fused_cost_config(self, f32, B):
         """(tiles per wavefront, exchange groups, waves per workgroup, prefetch) in use."""
