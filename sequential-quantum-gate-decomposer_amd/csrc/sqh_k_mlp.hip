@@ -40,11 +40,55 @@ __device__ __forceinline__ float gelu_grad(float z) {
     return cdf + z * pdf;
 }
 
-template <bool AT, bool BT, int EPI>
+// One operand tile (64 rows of M or N x KS values of K) between global memory, the
+// thread's registers and LDS ([row][k], k contiguous, rows padded by 8 values).  KMAJ:
+// the operand is stored K-major (row k holds the 64 rows' values contiguously), so its
+// 8-value vectors run along the rows and are transposed on the way into LDS.
+template <bool KMAJ, int KS>
+struct GemmTile {
+    static constexpr int V = 64 * KS / 8 / 256;  // 8-value vectors per thread
+    bf16x8 r[V];
+    __device__ __forceinline__ void load(const bf16* __restrict__ X, int ld, int row0, int k0, int tid) {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int v = tid + 256 * i;
+            if constexpr (!KMAJ) {
+                const int row = v / (KS / 8), kk = (v % (KS / 8)) * 8;
+                r[i] = *reinterpret_cast<const bf16x8*>(&X[(size_t)(row0 + row) * ld + k0 + kk]);
+            } else {
+                const int k = v / 8, rr = (v % 8) * 8;
+                r[i] = *reinterpret_cast<const bf16x8*>(&X[(size_t)(k0 + k) * ld + row0 + rr]);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(bf16 (*S)[KS + 8], int tid) const {
+#pragma unroll
+        for (int i = 0; i < V; ++i) {
+            const int v = tid + 256 * i;
+            if constexpr (!KMAJ) {
+                const int row = v / (KS / 8), kk = (v % (KS / 8)) * 8;
+                *reinterpret_cast<bf16x8*>(&S[row][kk]) = r[i];
+            } else {
+                const int k = v / 8, rr = (v % 8) * 8;
+#pragma unroll
+                for (int j = 0; j < 8; ++j) S[rr + j][k] = r[i][j];
+            }
+        }
+    }
+};
+
+// C[64 x 64 tile] = A B on the matrix cores: 4 wavefronts of 32 x 32 (2 x 2
+// v_mfma_f32_16x16x32_bf16 tiles), K in stages of KS through double-buffered LDS with
+// one barrier per stage; the next stage's global loads are in flight while the current
+// stage's MFMAs run.  AT: A stored K-major (A^T given); BT: B stored N-major (B^T
+// given).  Fused epilogues (EPI) write FP32 and BF16 results.
+template <bool AT, bool BT, int EPI, int KS>
 __global__ __launch_bounds__(256) void k_gemm_bf16(const bf16* __restrict__ A, int lda, const bf16* __restrict__ B,
                                                    int ldb, int K, EpiArgs e) {
-    __shared__ __align__(16) bf16 As[64][40];
-    __shared__ __align__(16) bf16 Bs[64][40];
+    // [2 buffers][A, B][64][KS + 8] in dynamic LDS (up to 135 KiB at KS = 256)
+    extern __shared__ __align__(16) unsigned char gemm_smem[];
+    typedef bf16 Row[KS + 8];
+    Row* const rows = reinterpret_cast<Row*>(gemm_smem);  // A0, B0, A1, B1: 64 rows each
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -53,39 +97,41 @@ __global__ __launch_bounds__(256) void k_gemm_bf16(const bf16* __restrict__ A, i
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        if constexpr (!AT) {
-            const int m = tid >> 2, kk = (tid & 3) * 8;
-            *reinterpret_cast<bf16x8*>(&As[m][kk]) =
-                *reinterpret_cast<const bf16x8*>(&A[(size_t)(m0 + m) * lda + k0 + kk]);
-        } else {
-            const int k = tid >> 3, mm = (tid & 7) * 8;
-            const bf16x8 v = *reinterpret_cast<const bf16x8*>(&A[(size_t)(k0 + k) * lda + m0 + mm]);
-#pragma unroll
-            for (int i = 0; i < 8; ++i) As[mm + i][k] = v[i];
+    GemmTile<AT, KS> ta;
+    GemmTile<!BT, KS> tb;
+    ta.load(A, lda, m0, 0, tid);
+    tb.load(B, ldb, n0, 0, tid);
+    ta.store(rows, tid);
+    tb.store(rows + 64, tid);
+    __syncthreads();
+    const int S = K / KS;
+    for (int st = 0; st < S; ++st) {
+        const int cur = st & 1;
+        Row* const As = rows + 128 * cur;
+        Row* const Bs = rows + 128 * cur + 64;
+        if (st + 1 < S) {
+            ta.load(A, lda, m0, (st + 1) * KS, tid);
+            tb.load(B, ldb, n0, (st + 1) * KS, tid);
         }
-        if constexpr (BT) {
-            const int n = tid >> 2, kk = (tid & 3) * 8;
-            *reinterpret_cast<bf16x8*>(&Bs[n][kk]) =
-                *reinterpret_cast<const bf16x8*>(&B[(size_t)(n0 + n) * ldb + k0 + kk]);
-        } else {
-            const int k = tid >> 3, nn = (tid & 7) * 8;
-            const bf16x8 v = *reinterpret_cast<const bf16x8*>(&B[(size_t)(k0 + k) * ldb + n0 + nn]);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) Bs[nn + i][k] = v[i];
+        for (int ks = 0; ks < KS; ks += 32) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + (lane & 15)][ks + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + (lane & 15)][ks + 8 * (lane >> 4)]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
-        __syncthreads();
-        bf16x8 a[2], b[2];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + i * 16 + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-            b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + j * 16 + (lane & 15)][8 * (lane >> 4)]);
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (st + 1 < S) {
+            ta.store(rows + 128 * (cur ^ 1), tid);
+            tb.store(rows + 128 * (cur ^ 1) + 64, tid);
+        }
         __syncthreads();
     }
 #pragma unroll
@@ -207,10 +253,21 @@ __global__ void k_f32_to_bf16(const float* __restrict__ a, bf16* __restrict__ b,
 template <bool AT, bool BT, int EPI>
 static hipError_t gemm(const void* A, int lda, const void* B, int ldb, int M, int N, int K, const EpiArgs& e,
                        hipStream_t s) {
-    if (M % 64 || N % 64 || K % 32) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_gemm_bf16<AT, BT, EPI>), dim3(N / 64, M / 64), dim3(256), 0, s, (const bf16*)A, lda,
-                       (const bf16*)B, ldb, K, e);
-    return hipGetLastError();
+    if (M % 64 || N % 64 || K % 32 || K <= 0) return hipErrorInvalidValue;
+    // the widest K stage dividing K: fewer stages, each with more MFMAs per load wait
+#define SQH_GEMM_KS(ks)                                                                                     \
+    if (K % ks == 0) {                                                                                      \
+        const size_t shm = 4 * 64 * (ks + 8) * sizeof(bf16);                                                \
+        hipLaunchKernelGGL((k_gemm_bf16<AT, BT, EPI, ks>), dim3(N / 64, M / 64), dim3(256), shm, s,         \
+                           (const bf16*)A, lda, (const bf16*)B, ldb, K, e);                                 \
+        return hipGetLastError();                                                                           \
+    }
+    SQH_GEMM_KS(256)
+    SQH_GEMM_KS(128)
+    SQH_GEMM_KS(64)
+    SQH_GEMM_KS(32)
+#undef SQH_GEMM_KS
+    return hipErrorInvalidValue;
 }
 
 // Z[M x N] = H[M x K] W[N x K]^T + b  (hidden: also H' = gelu(Z); output: Y)
