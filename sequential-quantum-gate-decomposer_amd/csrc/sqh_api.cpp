@@ -156,10 +156,27 @@ bool fused_plan_fits(const sqh_ctx* c) {
     return N >= 4 && N <= kFusedCostMaxQubits && c->fplan.k == 4 && !c->fplan.qubits.empty();
 }
 
+// Gate arithmetic of the register-tile kernels per matrix element: 14 flop per
+// parametric one-qubit gate, 7 per CRY, none for CNOT / CZ.
+double tile_gate_flops(const sqh_ctx* c) {
+    double w = 0.0;
+    for (const GateRec& g : c->gates) w += g.type == GT_CRY ? 7.0 : (g.type == GT_CNOT || g.type == GT_CZ) ? 0.0 : 14.0;
+    return w;
+}
+
+// Whether the fused blocks do less arithmetic than the gate-by-gate kernels (128 flop
+// per element per dense block; the cost + gradient: 4 nb - 1 block products against
+// (3 + 8/7) x the forward gate arithmetic).  Shallow circuits, with few gates per
+// block, stay on the tile kernels in the automatic mode.
+bool fused_pays(const sqh_ctx* c, bool grad) {
+    const double nb = (double)c->fplan.qubits.size(), w = tile_gate_flops(c);
+    return grad ? (4.0 * nb - 1.0) * 128.0 <= (3.0 + 8.0 / 7.0) * w : nb * 128.0 <= w;
+}
+
 bool fused_pass_applies(const sqh_ctx* c) {
     const int N = c->prog.N;
     if (!fused_plan_fits(c)) return false;
-    if (c->fused_cost == SQH_FUSED_COST_AUTO) return N >= 7 || (N >= 5 && !c->jit);
+    if (c->fused_cost == SQH_FUSED_COST_AUTO) return (N >= 7 || (N >= 5 && !c->jit)) && fused_pays(c, false);
     return c->fused_cost == SQH_FUSED_COST_ON;
 }
 
@@ -196,73 +213,89 @@ int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag,
     return fused_cost_launch(c, f32, a);
 }
 
-// Launch shape of the fused-block gradient for the context's circuit and target and a
-// batch of B: the one set by sqh_set_fused_grad_config when it fits, else four 16-column
+// Launch plan of the fused-block gradient for the context's circuit, B elements of m
+// columns: the shape set by sqh_set_fused_grad_config when it fits, else four 16-column
 // tiles per wavefront from 9 qubits or from 64 elements (fewer, larger partials), two
 // below (more wavefronts for small batches), and the most wavefronts per workgroup, up
-// to four, that divide the column groups -- the fastest shapes measured on MI355X.
-// Returns false when no shape fits the device's LDS.
-bool fused_grad_shape(const sqh_ctx* c, bool f32, int B, int& T, int& waves) {
-    const int N = c->prog.N, m = c->cols, nb = (int)c->fplan.qubits.size();
+// to four, that divide the column groups -- the fastest shapes measured on MI355X --
+// with the scratch it needs.  Returns false when no shape fits the device's LDS.
+bool fused_grad_plan(const sqh_ctx* c, bool f32, int B, int m, FusedGradPlan& p) {
+    const int N = c->prog.N, nb = (int)c->fplan.qubits.size(), ns = (int)c->prog.slots.size();
+    p.T = 0;
     if (c->fg_force_tiles && fused_grad_shape_valid(f32, N, m, nb, c->fg_force_tiles, c->fg_force_waves)) {
-        T = c->fg_force_tiles;
-        waves = c->fg_force_waves;
-        return true;
+        p.T = c->fg_force_tiles;
+        p.waves = c->fg_force_waves;
+    } else {
+        for (int t : {N >= 9 || B >= 64 ? 4 : 2, 4}) {
+            for (int w : {4, 2, 1})
+                if (fused_grad_shape_valid(f32, N, m, nb, t, w)) {
+                    p.T = t;
+                    p.waves = w;
+                    break;
+                }
+            if (p.T) break;
+        }
     }
-    for (int t : {N >= 9 || B >= 64 ? 4 : 2, 4})
-        for (int w : {4, 2, 1})
-            if (fused_grad_shape_valid(f32, N, m, nb, t, w)) {
-                T = t;
-                waves = w;
-                return true;
-            }
-    return false;
+    if (!p.T) return false;
+    p.wgs = fused_grad_wgs(N, m, p.T, p.waves);
+    p.fused_bytes = (size_t)B * nb * 256 * (f32 ? 8 : 16);
+    p.mpart_bytes = (size_t)B * p.wgs * nb * 512 * sizeof(double);
+    p.corr_bytes = (size_t)B * std::max(ns, 1) * 8 * sizeof(double);
+    return true;
 }
 
-// Whether a cost + gradient evaluation runs through the fused blocks: the switch on
-// (SQH_FUSED_COST_ON, or _AUTO from 6 qubits, where it measured faster than the
-// register-tile adjoint at every batch size), 4 <= N <= 10, no correction channels, and
-// a launch shape fits.
-bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32, int B) {
+// Whether a cost + gradient evaluation of B elements of m columns runs through the
+// fused blocks: the switch on (SQH_FUSED_COST_ON, or _AUTO from 6 qubits when the blocks
+// do less arithmetic than the gates -- where it measured faster than the register-tile
+// adjoint at every batch size), 4 <= N <= 10, no correction channels, and a launch
+// shape fits.
+bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32, int B, int m, FusedGradPlan* plan) {
     if (ts.nch != 0 || !fused_plan_fits(c)) return false;
-    if (c->fused_cost == SQH_FUSED_COST_OFF || (c->fused_cost == SQH_FUSED_COST_AUTO && c->prog.N < 6)) return false;
-    int T, waves;
-    return fused_grad_shape(c, f32, B, T, waves);
+    if (c->fused_cost == SQH_FUSED_COST_OFF ||
+        (c->fused_cost == SQH_FUSED_COST_AUTO && (c->prog.N < 6 || !fused_pays(c, true))))
+        return false;
+    FusedGradPlan p;
+    if (!fused_grad_plan(c, f32, B, m, p)) return false;
+    if (plan) *plan = p;
+    return true;
 }
 
-// Costs and gradients of B parameter vectors through the fused blocks: block matrices,
-// the forward + backward kernel (trace entries into diag, per-workgroup correlation
-// partials), then each block's gate correlations into d_fcorr for k_finish.
-int fused_grad_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off) {
-    const FusionPlan& fp = c->fplan;
-    const int N = c->prog.N, nb = (int)fp.qubits.size(), ns = (int)c->prog.slots.size(), m = c->cols;
-    int T = 0, waves = 0;
-    if (!fused_grad_shape(c, f32, B, T, waves)) return fail(c, SQH_ERR_ARG, "no fused gradient launch shape fits");
-    const int wgs = fused_grad_wgs(N, m, T, waves);
-    const size_t esz = f32 ? 8 : 16;
-    HIPTRY(c, c->d_fused.ensure((size_t)B * nb * 256 * esz));
-    HIPTRY(c, c->d_mpart.ensure((size_t)B * wgs * nb * 512 * sizeof(double)));
-    HIPTRY(c, c->d_fcorr.ensure((size_t)B * std::max(ns, 1) * 8 * sizeof(double)));
-    HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, B,
-                                 c->d_fused.p, c->stream));
+// Costs and gradients of B parameter vectors through the fused blocks on stream st:
+// block matrices (into fused), the forward + backward kernel (trace entries into diag,
+// per-workgroup correlation partials into mpart), then each block's gate correlations
+// into corr ([B][ns][8]) for k_finish.  kern: the [B][ns][4] kernel tables; U: the
+// target (m columns, row stride m), or with bi.count the optimiser's per-run targets
+// (element e's at U + run_target[elem_run[e]] * tgt_stride, *count live elements).
+// The buffers hold the plan's sizes; nothing is allocated.
+int fused_grad_run(sqh_ctx* c, bool f32, const FusedGradPlan& p, int B, int m, const void* kern, const void* U,
+                   double* diag, int trace_off, void* fused, double* mpart, double* corr, const BatchIndex& bi,
+                   hipStream_t st, bool timed) {
+    const int N = c->prog.N, nb = (int)c->fplan.qubits.size(), ns = (int)c->prog.slots.size();
+    HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, B, fused, st,
+                                 bi.count));
     FusedGradArgs a;
-    a.U = f32 ? c->d_U32.p : c->d_U64.p;
+    a.U = U;
     a.m = m;
     a.trace_off = trace_off;
     a.B = B;
     a.nb = nb;
-    a.F = c->d_fused.p;
+    a.F = fused;
     a.maps = c->d_fmaps.as<int>();
     a.diag = diag;
-    a.mpart = c->d_mpart.as<double>();
-    a.wgs = wgs;
-    {
+    a.mpart = mpart;
+    a.wgs = p.wgs;
+    a.count = bi.count;
+    a.elem_run = bi.elem_run;
+    a.run_target = bi.run_target;
+    a.tgt_stride = bi.tgt_stride;
+    if (timed) {
         TimedLaunch tl(c);
-        HIPTRY(c, launch_fused_grad(f32, N, a, T, waves, c->stream));
+        HIPTRY(c, launch_fused_grad(f32, N, a, p.T, p.waves, st));
+    } else {
+        HIPTRY(c, launch_fused_grad(f32, N, a, p.T, p.waves, st));
     }
-    HIPTRY(c, launch_fused_block_corr(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4,
-                                      c->d_fused.p, c->d_mpart.as<double>(), wgs, B, c->d_fcorr.as<double>(), ns,
-                                      c->stream));
+    HIPTRY(c, launch_fused_block_corr(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, fused,
+                                      mpart, p.wgs, B, corr, ns, st, bi.count));
     return SQH_OK;
 }
 
@@ -340,8 +373,14 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
         return tile_launch(c, t, kind, fixed == nullptr);
     };
     const void* U = f32 ? c->d_U32.p : c->d_U64.p;
-    if (d_grad && !fixed && fused_grad_applies(c, ts, f32, B)) {
-        if ((rc = fused_grad_eval(c, f32, B, kern, diag, off))) return rc;
+    FusedGradPlan fgp;
+    if (d_grad && !fixed && fused_grad_applies(c, ts, f32, B, m, &fgp)) {
+        HIPTRY(c, c->d_fused.ensure(fgp.fused_bytes));
+        HIPTRY(c, c->d_mpart.ensure(fgp.mpart_bytes));
+        HIPTRY(c, c->d_fcorr.ensure(fgp.corr_bytes));
+        if ((rc = fused_grad_run(c, f32, fgp, B, m, kern, U, diag, off, c->d_fused.p, c->d_mpart.as<double>(),
+                                 c->d_fcorr.as<double>(), BatchIndex(), c->stream, true)))
+            return rc;
         HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, c->d_fcorr.as<double>(), 1, ns, c->d_slots.as<Slot>(),
                                 d_params, c->prog.P, d_grad, c->stream, BatchIndex(), ts));
         return SQH_OK;

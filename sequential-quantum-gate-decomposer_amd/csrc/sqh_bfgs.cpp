@@ -49,8 +49,14 @@ struct Group {
     DevBuf tgt64, tgt32, d_ops, d_lays, d_slots;
     DevBuf kern64, kern32, diag, mpart;  // the group's own evaluation scratch
     DevBuf seed;                         // Hilbert-Schmidt corrections: per-element seed weights
+    // the fused-block gradient (single circuit, where it applies): plan per precision
+    // and its scratch (block matrices, correlation partials, gate correlations)
+    bool fused[2] = {false, false};
+    FusedGradPlan fplan[2];
+    DevBuf ffused, fmpart, fcorr;
     ~Group() {
-        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots, &kern64, &kern32, &diag, &mpart, &seed})
+        for (DevBuf* b : {&tgt64, &tgt32, &d_ops, &d_lays, &d_slots, &kern64, &kern32, &diag, &mpart, &seed, &ffused,
+                          &fmpart, &fcorr})
             b->release();
     }
 };
@@ -119,6 +125,17 @@ int enqueue_round(sqh_ctx* c, BatchBufs& bb, std::vector<Group>& groups, const B
             const TileLaunch t = group_launch(c, bb, gr, g, p, B, grouped);
             HIPTRY(c, launch_build_kernels(t.f32, bb.xt.as<double>(), o.P, gr.B, gr.slots, gr.ns,
                                            const_cast<void*>(t.kern), st, t.bi));
+            if (gr.fused[p]) {
+                // the fused blocks' adjoint: block matrices on the matrix cores forward and
+                // backward, each block's correlations resolved to its gates
+                const int rc = fused_grad_run(c, t.f32, gr.fplan[p], gr.B, gr.cols, t.kern, t.in, t.diag, t.trace_off,
+                                              gr.ffused.p, gr.fmpart.as<double>(), gr.fcorr.as<double>(), t.bi, st,
+                                              false);
+                if (rc) return rc;
+                HIPTRY(c, launch_finish(t.diag, gr.cols, gr.B, c->variant, bb.fe.as<double>(), gr.fcorr.as<double>(), 1,
+                                        gr.ns, gr.slots, bb.xt.as<double>(), o.P, bb.ge.as<double>(), st, t.bi, t.ts));
+                continue;
+            }
             if (!grouped && jit_wants(c, 2)) {
                 const int rc = jit_launch(c, t, 2, st);
                 if (rc) return rc;
@@ -344,6 +361,26 @@ int run_batch(sqh_ctx* c, std::vector<Group>& groups, int B, int Ps, const int* 
         HIPTRY(c, g.diag.ensure((size_t)(correction_channels(c->variant) + 1) * g.B * g.cols * 2 * sizeof(double)));
         HIPTRY(c, g.seed.ensure((size_t)g.B * 6 * sizeof(double)));
         HIPTRY(c, g.mpart.ensure((size_t)g.B * g.wpb * std::max(g.ns, 1) * 8 * sizeof(double)));
+        // single circuit, fused path switched on (SQH_FUSED_COST_ON): the fused blocks'
+        // adjoint where it applies (per precision used).  The automatic mode keeps the
+        // tile kernels here: a round evaluates at most one point per run, and the fused
+        // path's four extra small launches per round measured no faster on decompositions
+        // of 7-8 qubits.
+        size_t fb = 0, mb = 0, cb = 0;
+        for (int p = 0; p < (mixed ? 2 : 1); ++p) {
+            g.fused[p] = !grouped && c->fused_cost == SQH_FUSED_COST_ON &&
+                         fused_grad_applies(c, trace_spec(c, g.B, g.cols), p == 1, g.B, g.cols, &g.fplan[p]);
+            if (g.fused[p]) {
+                fb = std::max(fb, g.fplan[p].fused_bytes);
+                mb = std::max(mb, g.fplan[p].mpart_bytes);
+                cb = std::max(cb, g.fplan[p].corr_bytes);
+            }
+        }
+        if (fb) {
+            HIPTRY(c, g.ffused.ensure(fb));
+            HIPTRY(c, g.fmpart.ensure(mb));
+            HIPTRY(c, g.fcorr.ensure(cb));
+        }
     }
     if (G > 1) {  // side streams and fork/join events, kept by the context
         while ((int)c->side.size() < G) {

@@ -204,8 +204,19 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 // timing every candidate on a scratch batch of the same shape (sqh_tune.cpp).
 int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out);
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
-                              long long kern_bstride, int Bk, void* out, hipStream_t s);
+                              long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count = nullptr);
 size_t fused_gate_bytes();
+// Fused-block cost + gradient (sqh_api.cpp): launch shape and scratch sizes, the
+// applicability rule, and the enqueue (no allocation) on a stream.
+struct FusedGradPlan {
+    int T = 0, waves = 0, wgs = 0;
+    size_t fused_bytes = 0, mpart_bytes = 0, corr_bytes = 0;
+};
+bool fused_grad_plan(const sqh_ctx* c, bool f32, int B, int m, FusedGradPlan& p);
+bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32, int B, int m, FusedGradPlan* plan);
+int fused_grad_run(sqh_ctx* c, bool f32, const FusedGradPlan& p, int B, int m, const void* kern, const void* U,
+                   double* diag, int trace_off, void* fused, double* mpart, double* corr, const BatchIndex& bi,
+                   hipStream_t st, bool timed);
 // The fused-block cost launch with the context's launch shape (forced, tuned or default).
 int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a);
 // A register-tile pass (kind 1: forward trace entries, 2: gradient) with the context's

@@ -180,11 +180,13 @@ template <typename R>
 __global__ __launch_bounds__(64) void k_fuse_blocks(const FusedGate* __restrict__ fg,
                                                      const int* __restrict__ block_first, int nblocks,
                                                      const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
-                                                     cpx<R>* __restrict__ out) {
+                                                     cpx<R>* __restrict__ out, const int* __restrict__ count) {
     __shared__ FusedGate srec[kFuseChunk];
     __shared__ cpx<R> skern[kFuseElems][kFuseChunk][4];
     const int blk = blockIdx.x % nblocks;
     const int b0 = (blockIdx.x / nblocks) * kFuseElems;
+    if (count) Bk = min(*count, Bk);  // live elements (the optimiser's rounds)
+    if (b0 >= Bk) return;             // the whole workgroup
     const int le = threadIdx.x >> 4, j = threadIdx.x & 15;
     cpx<R> v[16];
 #pragma unroll
@@ -297,16 +299,16 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 }
 
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
-                              long long kern_bstride, int Bk, void* out, hipStream_t s) {
+                              long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count) {
     if ((long long)Bk * nblocks == 0) return hipSuccess;
     const long long grid = (long long)nblocks * ((Bk + kFuseElems - 1) / kFuseElems);
     const int threads = 16 * kFuseElems;
     if (f32)
         hipLaunchKernelGGL(k_fuse_blocks<float>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
-                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out);
+                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out, count);
     else
         hipLaunchKernelGGL(k_fuse_blocks<double>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
-                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out);
+                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out, count);
     return hipGetLastError();
 }
 

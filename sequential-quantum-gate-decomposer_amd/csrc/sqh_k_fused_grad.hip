@@ -140,12 +140,20 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
     for (int i = threadIdx.x; i < nmap; i += blockDim.x) smap[i] = a.maps[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const long long per_xcd = gridDim.x >> 3;  // the grid is a multiple of 8
-    const long long lwg = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    // XCD-ordered workgroups (the grid is a multiple of 8); with a live count (the
+    // optimiser's rounds) the plain order, so the live elements spread over the XCDs
+    int B = a.B;
+    long long lwg = blockIdx.x;
+    if (a.count) {
+        B = min(*a.count, a.B);
+    } else {
+        const long long per_xcd = gridDim.x >> 3;
+        lwg = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    }
     const long long wave = lwg * nwv + wv;
     // whole workgroups past the batch leave together (the workgroups of one element
     // hold whole column groups: groups % waves == 0)
-    if (lwg * nwv >= (long long)a.B * a.groups) return;
+    if (lwg * nwv >= (long long)B * a.groups) return;
     const int e = (int)(wave / a.groups);
     const int cg = (int)(wave % a.groups);
     const int c0 = cg * W;
@@ -162,6 +170,7 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
         o[t] = n % O;
     }
     const cpx<R>* U = static_cast<const cpx<R>*>(a.U);
+    if (a.count) U += (long long)a.run_target[a.elem_run[e]] * a.tgt_stride;
     const int m = a.m;
     R xr[T][4], xi[T][4];
 #pragma unroll
@@ -278,11 +287,12 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
 // into the first partial: thread per double, consecutive threads on consecutive
 // entries.
 __global__ __launch_bounds__(256) void k_fused_sum_partials(double* __restrict__ mpart, int wgs, int nb,
-                                                            long long total) {
+                                                            long long total, const int* __restrict__ count) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= total) return;  // total = B * nb * 512
     const long long per_e = (long long)nb * 512;
     const long long e = i / per_e, r = i % per_e;
+    if (count && e >= *count) return;
     double* base = mpart + e * wgs * per_e + r;
     double sum = base[0];
     for (int v = 1; v < wgs; ++v) sum += base[(long long)v * per_e];
@@ -307,11 +317,14 @@ __global__ __launch_bounds__(64) void k_fused_block_corr(const FusedGate* __rest
                                                          const int* __restrict__ block_first, int nb,
                                                          const cpx<R>* __restrict__ kern, long long kern_bstride,
                                                          const cpx<R>* __restrict__ F, const double* __restrict__ mpart,
-                                                         int wgs, int B, double* __restrict__ corr, int ns) {
+                                                         int wgs, int B, double* __restrict__ corr, int ns,
+                                                         const int* __restrict__ count) {
     __shared__ FusedGate srec[kCorrChunk];
     __shared__ cpx<R> skern[kCorrElems][kCorrChunk][4];
     const int blk = blockIdx.x % nb;
     const int e0 = (blockIdx.x / nb) * kCorrElems;
+    if (count) B = min(*count, B);
+    if (e0 >= B) return;  // the whole workgroup
     const int le = threadIdx.x >> 4, k = threadIdx.x & 15;
     const int e = e0 + le;
     const bool live = e < B;
@@ -445,22 +458,22 @@ hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, int waves,
 
 hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_first, int nb, const void* kern,
                                    long long kern_bstride, const void* F, double* mpart, int wgs, int B, double* corr,
-                                   int ns, hipStream_t s) {
+                                   int ns, hipStream_t s, const int* count) {
     const long long pairs = (long long)B * nb;
     if (pairs == 0) return hipSuccess;
     if (wgs > 1) {
         const long long total = pairs * 512;
         hipLaunchKernelGGL(k_fused_sum_partials, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                           mpart, wgs, nb, total);
+                           mpart, wgs, nb, total, count);
     }
     const long long groups = (B + kCorrElems - 1) / kCorrElems;
     const unsigned grid = (unsigned)(groups * nb);
     if (f32)
         hipLaunchKernelGGL(k_fused_block_corr<float>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
-                           nb, (const cpx<float>*)kern, kern_bstride, (const cpx<float>*)F, mpart, wgs, B, corr, ns);
+                           nb, (const cpx<float>*)kern, kern_bstride, (const cpx<float>*)F, mpart, wgs, B, corr, ns, count);
     else
         hipLaunchKernelGGL(k_fused_block_corr<double>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
-                           nb, (const cpx<double>*)kern, kern_bstride, (const cpx<double>*)F, mpart, wgs, B, corr, ns);
+                           nb, (const cpx<double>*)kern, kern_bstride, (const cpx<double>*)F, mpart, wgs, B, corr, ns, count);
     return hipGetLastError();
 }
 

@@ -134,6 +134,12 @@ struct FusedGradArgs {
     double* mpart = nullptr;
     int wgs = 0;     // workgroups per element (the partials' layout)
     int groups = 0;  // column groups per element (set by the launcher)
+    // optional optimiser indirection: live elements *count (of B), element e's target at
+    // U + run_target[elem_run[e]] * tgt_stride
+    const int* count = nullptr;
+    const int* elem_run = nullptr;
+    const int* run_target = nullptr;
+    long long tgt_stride = 0;
 };
 // Launch shape: T 16-column tiles per wavefront (2 or 4; 4 at 10 qubits), waves 1, 2, 4
 // or 8 per workgroup dividing the element's column groups; wgs = workgroups per element
@@ -148,7 +154,7 @@ hipError_t launch_fused_grad(bool f32, int N, FusedGradArgs a, int T, int waves,
 // adjoint's layout, one partial per element).
 hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_first, int nb, const void* kern,
                                    long long kern_bstride, const void* F, double* mpart, int wgs, int B, double* corr,
-                                   int ns, hipStream_t s);
+                                   int ns, hipStream_t s, const int* count = nullptr);
 
 // Launch shape: tiles per wavefront (2 or 4), LDS exchange groups per block (1, or 2
 // with 4 tiles and prefetch), waves per workgroup (1, 2 or 4), next block's matrix

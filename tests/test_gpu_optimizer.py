@@ -93,6 +93,40 @@ def test_batch_matches_oracle_driver(oracle, n, layers, loops, engine):
     check_against_oracle(pairs, 1e-8)
 
 
+@pytest.mark.parametrize("n,layers,fused", [(4, 3, True), (5, 3, True), (7, 3, True)])
+def test_rounds_with_fused_gradient_match_oracle_driver(oracle, n, layers, fused):
+    """The lockstep rounds evaluating through the fused blocks' adjoint (switched on):
+    reported costs equal the oracle's cost of the returned parameters, runs match the
+    oracle driver, and the mixed-precision rounds (complex64 fused blocks, FP64 finish)
+    report FP64 costs."""
+    say(f"rounds with fused gradient n={n} layers={layers} fused={fused}")
+    gates = W.ansatz_layers(n, W.alternating_pairs(n, layers))
+    P = W.param_count(gates)
+    T = reachable_targets(n, gates, 6, 300 + n)
+    seeds = list(range(11, 17))
+    x0 = np.array([initial_guess(P, s, "RANDOM") for s in seeds])
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_fused_cost(fused)
+    res = decompose_batch(ctx, T, np.arange(6, dtype=np.int32), x0,
+                          default_options(n, optimization_tolerance=1e-8, max_outer_iterations=2, engine="rounds"))
+    pairs = []
+    for r in range(6):
+        fo, xo, co = oracle.decompose_custom(gates, T[r], n, seeds[r], max_outer_iterations=2,
+                                             optimization_tolerance=1e-8)
+        fg = res["f"][r]
+        say(f"  run {r}: gpu f={fg:.3e} evals={res['evals'][r]}  oracle f={fo:.3e} evals={co}")
+        assert abs(oracle.optimization_problem(gates, T[r], n, res["x"][r]) - fg) < 1e-12
+        pairs.append((fg, res["evals"][r], fo, co))
+    check_against_oracle(pairs, 1e-8)
+    resmx = decompose_batch(ctx, T, np.arange(6, dtype=np.int32), x0,
+                            default_options(n, optimization_tolerance=1e-8, max_outer_iterations=2, engine="rounds",
+                                            mixed_precision=True, switch_cost=1e-4))
+    for r in range(6):
+        assert abs(oracle.optimization_problem(gates, T[r], n, resmx["x"][r]) - resmx["f"][r]) < 1e-12
+    say(f"  mixed: success {np.mean(resmx['f'] < 1e-8):.2f} (FP64 {np.mean(res['f'] < 1e-8):.2f})")
+
+
 @pytest.mark.parametrize("n,seed", [(3, 1), (3, 2), (3, 3), (2, 4), (2, 5)])
 def test_start_decomposition_matches_oracle_driver(oracle, n, seed):
     """N_Qubit_Decomposition_custom.Start_Decomposition with config random_seed gives
