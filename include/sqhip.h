@@ -318,10 +318,12 @@ int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
  * run.  Results agree with the tile kernels to rounding (the blocks reassociate the
  * products).  mode: SQH_FUSED_COST_OFF, _ON, or _AUTO (the default: on from 7 qubits,
  * or from 5 when the run-time specialised kernels are off -- where it measured faster
- * than the tile kernels).  The same mode selects the fused blocks' adjoint for cost +
- * gradient batches (sqh_optimization_problem_combined_batch[_dev]; _AUTO: from 6
- * qubits): forward and backward block products on the matrix cores, each block's
- * correlations then resolved to its gates.  No counterpart in the reference: it
+ * than the tile kernels -- when the blocks do no more arithmetic than the gates).  The
+ * same mode selects the fused blocks' adjoint for cost + gradient batches
+ * (sqh_optimization_problem_combined_batch[_dev]; _AUTO: from 6 qubits, with the same
+ * arithmetic condition): forward and backward block products on the matrix cores, each
+ * block's correlations then resolved to its gates.  sqh_decompose_batch's lockstep
+ * rounds use the fused adjoint with _ON only.  No counterpart in the reference: it
  * replaces the per-gate loop of Gates_block::apply_to (Gates_block.cpp:606-710) inside
  * optimization_problem_batched (Optimization_Interface.cpp:940-1033) and the derivative
  * products of optimization_problem_combined (Optimization_Interface.cpp:1401-1450). */
