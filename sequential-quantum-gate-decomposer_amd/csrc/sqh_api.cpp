@@ -145,11 +145,6 @@ std::vector<int> fused_row_maps(int N, const FusionPlan& fp) {
     return maps;
 }
 
-// Whether a cost-only evaluation runs through the tile-resident fused-block kernel:
-// 4 <= N <= 10, the circuit has blocks, the cost has no correction channels, and the
-// switch is on -- or automatic (the default) where it measured faster on MI355X than
-// the register-tile kernels: from 7 qubits, or from 5 when the circuit-specialised
-// kernels are off.
 // The circuit fuses into 4-qubit blocks the fused kernels take (4 to 10 qubits).
 bool fused_plan_fits(const sqh_ctx* c) {
     const int N = c->prog.N;
@@ -173,6 +168,12 @@ bool fused_pays(const sqh_ctx* c, bool grad) {
     return grad ? (4.0 * nb - 1.0) * 128.0 <= (3.0 + 8.0 / 7.0) * w : nb * 128.0 <= w;
 }
 
+// Whether a pass (costs, or the apply_to store pass) runs through the tile-resident
+// fused-block kernel: 4 <= N <= 10, the circuit has blocks, and the switch is on -- or
+// automatic (the default) where it measured faster on MI355X than the register-tile
+// kernels: from 7 qubits, or from 5 when the circuit-specialised kernels are off, and
+// the blocks do no more arithmetic than the gates.  Costs also need no correction
+// channels (fused_cost_applies).
 bool fused_pass_applies(const sqh_ctx* c) {
     const int N = c->prog.N;
     if (!fused_plan_fits(c)) return false;

@@ -10,8 +10,9 @@
 // 16 MFMAs each (four real products, K = 16 in four steps):
 // v_mfma_f64_16x16x4f64 for complex128, v_mfma_f32_16x16x4f32 (FP32 accumulation) for
 // complex64.  Between blocks the tiles go through the wavefront's own LDS region in
-// natural row order and come back in the next block's grouping; after the last block
-// the lane holding row c + trace_off of column c writes that trace entry.  The target
+// natural row order and come back in the next block's grouping.  Of the last block the
+// cost needs only row c + trace_off of each column c: a 16-term dot product with that
+// row of F_nb per column replaces the block product.  The target
 // is read once per wavefront (it is shared by the batch and stays in L2); nothing
 // else touches HBM but the block matrices and the per-column trace entries.
 //
@@ -142,7 +143,10 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
             fai[s] = f.im;
         }
     }
-    for (int b = 0; b < a.nb; ++b) {
+    // the cost pass stops before the last block: of it only row c + trace_off of each
+    // column c is needed (below)
+    const int nbl = a.out ? a.nb : a.nb - 1;
+    for (int b = 0; b < nbl; ++b) {
         R ar[4], ai[4], ain[4];
         if constexpr (PF) {
 #pragma unroll
@@ -151,7 +155,7 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
                 ai[s] = fai[s];
                 ain[s] = -fai[s];
             }
-            if (b + 1 < a.nb) {
+            if (b + 1 < nbl) {
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
                     const cpx<R> f = Fe[(long long)(b + 1) * 256 + nl * 16 + 4 * s + g];
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
                     }
                 }
                 fc_wave_fence();
-            } else if (a.out) {
+            } else {
                 // apply_to: the transformed columns back to HBM (each wavefront writes
                 // only the columns it read, so the pass may run in place)
                 cpx<R>* O = static_cast<cpx<R>*>(a.out) + (long long)e * a.out_bstride;
@@ -233,23 +237,43 @@ __global__ __launch_bounds__(256) void k_fused_cost(FusedCostArgs a) {
                         if (col < m) O[(long long)j * a.out_stride + col] = mk<R>(yr[u][r], yi[u][r]);
                     }
                 }
-            } else {
-                // trace entries: row c + trace_off of column c
+            }
+        }
+    }
+    if (a.out) return;
+    // cost pass, last block: one 16-term dot product with row c + trace_off of F per
+    // column c.  X is in the block's grouping; the vector column whose outer group holds
+    // the row contributes, its 4 lane groups summing their 4 entries each.
+    {
+        const int* mp = smap + (a.nb - 1) * kFusedMapStride;
+        const int mask = mp[15];  // the block's four row bits (off of local index 15)
+        const cpx<R>* Fl = Fe + (long long)(a.nb - 1) * 256;
 #pragma unroll
-                for (int u = 0; u < TH; ++u) {
-                    const int t = h * TH + u;
-                    const int col = c0 + w[t];
-                    const int base = mp[16 + o[t]];
+        for (int t = 0; t < T; ++t) {
+            const int col = c0 + w[t];
+            const int jt = col + a.trace_off;
+            const bool hit = col < m && (jt & ~mask) == mp[16 + o[t]];
+            // local index of row jt: its bits at the block's qubits
+            const int is = ((jt & mp[1]) ? 1 : 0) | ((jt & mp[2]) ? 2 : 0) | ((jt & mp[4]) ? 4 : 0) |
+                           ((jt & mp[8]) ? 8 : 0);
+            R sr = 0, si = 0;
+            if (hit) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        const int j = base + mp[MF::row(r, g)];
-                        if (col < m && j == col + a.trace_off) {
-                            double* d = a.diag + 2 * ((long long)e * m + col);
-                            d[0] = (double)yr[u][r];
-                            d[1] = (double)yi[u][r];
-                        }
-                    }
+                for (int s = 0; s < 4; ++s) {
+                    const cpx<R> f = Fl[is * 16 + 4 * s + g];
+                    sr += f.re * xr[t][s] - f.im * xi[t][s];
+                    si += f.re * xi[t][s] + f.im * xr[t][s];
                 }
+            }
+            // the four lane groups of the vector column (lanes nl + 16 g)
+            sr += __shfl_xor(sr, 16, 64);
+            si += __shfl_xor(si, 16, 64);
+            sr += __shfl_xor(sr, 32, 64);
+            si += __shfl_xor(si, 32, 64);
+            if (hit && g == 0) {
+                double* d = a.diag + 2 * ((long long)e * m + col);
+                d[0] = (double)sr;
+                d[1] = (double)si;
             }
         }
     }
