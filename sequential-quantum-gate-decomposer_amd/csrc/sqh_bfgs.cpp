@@ -525,9 +525,15 @@ int upload_targets(sqh_ctx* c, Group& g, const double* const* targets) {
     const size_t msz = (size_t)g.rows * g.cols;
     HIPTRY(c, g.tgt64.ensure((size_t)g.n_targets * msz * 16));
     HIPTRY(c, g.tgt32.ensure((size_t)g.n_targets * msz * 8));
-    for (int i = 0; i < g.n_targets; ++i)
-        HIPTRY(c, hipMemcpyAsync(g.tgt64.as<char>() + (size_t)i * msz * 16, targets[i], msz * 16,
+    // targets adjacent in host memory go up in one copy (a batch of single-circuit
+    // targets is one copy instead of one per target)
+    for (int i = 0; i < g.n_targets;) {
+        int j = i + 1;
+        while (j < g.n_targets && targets[j] == targets[j - 1] + msz * 2) ++j;
+        HIPTRY(c, hipMemcpyAsync(g.tgt64.as<char>() + (size_t)i * msz * 16, targets[i], (size_t)(j - i) * msz * 16,
                                  hipMemcpyHostToDevice, c->stream));
+        i = j;
+    }
     HIPTRY(c, launch_convert(true, g.tgt64.p, g.tgt32.p, (long long)g.n_targets * msz * 2, c->stream));
     return SQH_OK;
 }
