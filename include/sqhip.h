@@ -371,6 +371,11 @@ int sqh_get_tile_config(sqh_ctx* ctx, int kind, int is_f32, int B, int* waves_pe
  * log receives the compiler log. */
 int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32, int uniform,
                           char* log, int log_cap, long long* code_bytes, long long* source_bytes);
+/* Introspection (no GPU needed): the generated source of the run-time kernel that
+ * sqh_jit_compile_check compiles, into buf (at most cap - 1 bytes, NUL-terminated);
+ * source_bytes receives its full length. */
+int sqh_jit_source(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32, int uniform, char* buf,
+                   long long cap, long long* source_bytes);
 
 /* ---- warm-start network ----------------------------------------------------- */
 /* A multilayer perceptron in -> hidden (GELU) x n_hidden -> out with BF16 weights

@@ -371,8 +371,8 @@ struct Gen {
         o << "#include \"sqh_solve_loop.h\"\nusing namespace sqh;\nconstexpr int N = " << N
           << ";\ntypedef Geo<N> G;\nconstexpr int SM = " << sh.m << ", STR = " << sh.trace_off
           << ", SNS = " << sh.ns << ";\n"
-          << "struct JitLane { bool valid; int col; };\n"
-          << "template <typename R>\n__device__ double jit_eval(const cpx<R>* in, const double* xr, double* gr, "
+          << "struct JitLane { bool valid; int col; };\n";
+        o << "template <typename R>\n__device__ double jit_eval(const cpx<R>* in, const double* xr, double* gr, "
              "cpx<double>* kt_raw, double* mc, const Slot* slots, int variant) {\n"
           << "  const int lane = threadIdx.x;\n"
           << "  cpx<R>* kw = reinterpret_cast<cpx<R>*>(kt_raw);\n"
@@ -439,8 +439,12 @@ struct Gen {
              "kim += dk[e].re * M[2 * e + 1] + dk[e].im * M[2 * e]; }\n"
           << "      gr[q.param_start + j] = grad_of(variant, tre, tim, kre, kim, (double)SM);\n"
           << "    }\n  }\n  __syncthreads();\n"
-          << "  return cost_of(variant, tre, tim, (double)SM);\n}\n"
-          << "extern \"C\" __global__ __launch_bounds__(64) void sqh_jit_kernel(SolveArgs a) {\n"
+          << "  return cost_of(variant, tre, tim, (double)SM);\n}\n";
+        solve_entry();
+        return o.str();
+    }
+    void solve_entry() {
+        o << "extern \"C\" __global__ __launch_bounds__(64) void sqh_jit_kernel(SolveArgs a) {\n"
           << "  const int var = a.variant;\n"
           << "  solve_runs<1>(a,\n"
           << "    [=](const cpx<double>* U, const Op*, int, const Layout*, const Slot* sl, int, const double* xr, "
@@ -448,7 +452,6 @@ struct Gen {
           << "    [=](const cpx<float>* U, const Op*, int, const Layout*, const Slot* sl, int, const double* xr, "
              "double* gr, cpx<double>* kt, double* mc, cpx<float>*) { return jit_eval<float>(U, xr, gr, kt, mc, sl, var); });\n"
           << "}\n";
-        return o.str();
     }
 };
 
@@ -664,6 +667,33 @@ double jit_compile_ms(sqh_ctx* c, int* compiles) {
 }  // namespace sqh
 
 using namespace sqh;
+
+extern "C" int sqh_jit_source(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32, int uniform,
+                              char* buf, long long cap, long long* source_bytes) {
+    if (n_gates < 0 || (n_gates > 0 && !gates) || kind < 0 || kind > kJitSolve || cap < 0) return SQH_ERR_ARG;
+    std::vector<GateRec> g(n_gates);
+    for (int i = 0; i < n_gates; ++i) g[i] = {gates[i].type, gates[i].target, gates[i].control, gates[i].param_start};
+    Program pr;
+    std::string err;
+    if (!compile_program(qbit_num, g, pr, err) || pr.N > kMaxTileQubits || (kind == kJitSolve && pr.N > 4))
+        return SQH_ERR_GATE;
+    JitShape sh;  // the representative launch of sqh_jit_compile_check
+    const int d = 1 << pr.N;
+    sh.m = d;
+    sh.m_pad = d;
+    sh.in_stride = sh.out_stride = d;
+    sh.ns = (int)pr.slots.size();
+    sh.kern_bstride = 4LL * sh.ns;
+    sh.out_bstride = (long long)d * d;
+    const std::string src = jit_source(pr, kind, is_f32 != 0, uniform != 0, sh);
+    if (source_bytes) *source_bytes = (long long)src.size();
+    if (buf && cap > 0) {
+        const size_t n = std::min<size_t>(src.size(), (size_t)cap - 1);
+        std::memcpy(buf, src.data(), n);
+        buf[n] = '\0';
+    }
+    return SQH_OK;
+}
 
 extern "C" int sqh_jit_compile_check(int qbit_num, const sqh_gate* gates, int n_gates, int kind, int is_f32,
                                      int uniform, char* log, int log_cap, long long* code_bytes,

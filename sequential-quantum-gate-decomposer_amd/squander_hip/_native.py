@@ -44,7 +44,7 @@ EXPORTS = [
     "sqh_mlp_forward_dev", "sqh_mlp_train_step_dev", "sqh_mlp_gradients", "sqh_apply_to_fused_dev",
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
     "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
-    "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_set_fused_cost",
+    "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_jit_source", "sqh_set_fused_cost",
     "sqh_set_fused_cost_config", "sqh_get_fused_cost_config", "sqh_set_fused_grad_config",
     "sqh_set_tile_config", "sqh_get_tile_config",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
@@ -171,6 +171,7 @@ def lib():
             "sqh_mlp_backward_step_dev": ([vp, vp, vp, i, ctypes.c_float], i),
             "sqh_jit_compile_check": ([i, ctypes.POINTER(_Gate), i, i, i, i, ctypes.c_char_p, i,
                                        ctypes.POINTER(ll), ctypes.POINTER(ll)], i),
+            "sqh_jit_source": ([i, ctypes.POINTER(_Gate), i, i, i, i, ctypes.c_char_p, ll, ctypes.POINTER(ll)], i),
             "sqh_get_jit_stats": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_get_kernel_timing": ([vp, d, ctypes.POINTER(ctypes.c_int)], i),
             "sqh_inspect_slots": ([i, ctypes.POINTER(_Gate), i, vp, i, ctypes.POINTER(ctypes.c_int)], i),
@@ -370,6 +371,20 @@ class Context:
         n = ctypes.c_int(0)
         self.check(self._L.sqh_get_kernel_timing(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+
+def jit_source(qbit_num, gates, kind=1, is_f32=False, uniform=True):
+    """The generated source of the run-time specialised kernel of a gate list (no GPU
+    needed)."""
+    L = lib()
+    garr, ng = gate_array(gates)
+    sb = ctypes.c_longlong(0)
+    rc = L.sqh_jit_source(qbit_num, garr, ng, kind, int(is_f32), int(uniform), None, 0, ctypes.byref(sb))
+    if rc != 0:
+        raise SqhError(rc, "sqh_jit_source failed")
+    buf = ctypes.create_string_buffer(sb.value + 1)
+    L.sqh_jit_source(qbit_num, garr, ng, kind, int(is_f32), int(uniform), buf, sb.value + 1, ctypes.byref(sb))
+    return buf.value.decode()
 
 
 def jit_compile_check(qbit_num, gates, kind=1, is_f32=False, uniform=True):

@@ -62,7 +62,7 @@ def engine_opts(ctx, engine):
 
 
 @pytest.mark.parametrize("engine", ENGINES)
-@pytest.mark.parametrize("n,layers,loops", [(2, 3, 3), (2, 2, 1), (3, 4, 1), (3, 8, 1), (3, 4, 2), (5, 3, 1)])
+@pytest.mark.parametrize("n,layers,loops", [(2, 3, 3), (2, 2, 1), (3, 4, 1), (3, 8, 1), (3, 4, 2), (4, 3, 1), (5, 3, 1)])
 def test_batch_matches_oracle_driver(oracle, n, layers, loops, engine):
     """Iteration loops: 3 at 2 qubits is the reference's default (BFGS.cpp:65-83 with
     Optimization_Interface.cpp:149); 2 at 3 qubits stands for config max_iteration_loops.
@@ -89,6 +89,33 @@ def test_batch_matches_oracle_driver(oracle, n, layers, loops, engine):
         f_check = oracle.optimization_problem(gates, T[r], n, res["x"][r])
         say(f"  run {r}: gpu f={fg:.3e} evals={res['evals'][r]}  oracle f={fo:.3e} evals={co}")
         assert abs(f_check - fg) < 1e-12
+        pairs.append((fg, res["evals"][r], fo, co))
+    check_against_oracle(pairs, 1e-8)
+
+
+@pytest.mark.parametrize("engine", ENGINES)
+def test_mixed_gate_circuit_4q_matches_oracle_driver(oracle, engine):
+    """Every gate kind (U3, RX, RY, RZ, CRY, CNOT, CZ; controls below and above the
+    target) at 4 qubits, square target: the engines (the circuit-specialised persistent
+    solver splits each column over four lanes here) against the oracle driver."""
+    n = 4
+    say(f"mixed-gate 4q circuit vs oracle engine={engine}")
+    gates = W.mixed_gate_circuit(n, 26, 4711)
+    P = W.param_count(gates)
+    T = reachable_targets(n, gates, 6, 4712)
+    seeds = list(range(21, 27))
+    x0 = np.array([initial_guess(P, s, "RANDOM") for s in seeds])
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    opts = default_options(n, optimization_tolerance=1e-8, max_outer_iterations=2, engine=engine_opts(ctx, engine))
+    res = decompose_batch(ctx, T, np.arange(6, dtype=np.int32), x0, opts)
+    pairs = []
+    for r in range(6):
+        fo, xo, co = oracle.decompose_custom(gates, T[r], n, seeds[r], max_outer_iterations=2,
+                                             optimization_tolerance=1e-8)
+        fg = res["f"][r]
+        say(f"  run {r}: gpu f={fg:.3e} evals={res['evals'][r]}  oracle f={fo:.3e} evals={co}")
+        assert abs(oracle.optimization_problem(gates, T[r], n, res["x"][r]) - fg) < 1e-12
         pairs.append((fg, res["evals"][r], fo, co))
     check_against_oracle(pairs, 1e-8)
 

@@ -30,3 +30,19 @@ def test_persistent_solver_kernel_compiles():
     ok, _, _, _ = nat.jit_compile_check(5, W.ansatz_layers(5, [0, 1]), kind=3)
     assert not ok
 
+
+
+def test_jit_source_is_the_compiled_source():
+    """sqh_jit_source returns the generated source whose length sqh_jit_compile_check
+    reports: straight-line code naming every parametric slot's kernel, the kernel entry
+    point, and the circuit's constants."""
+    gates = W.mixed_gate_circuit(5, 20, 77)
+    ok, log, code, nbytes = nat.jit_compile_check(5, gates, kind=1, uniform=True)
+    assert ok, log
+    src = nat.jit_source(5, gates, kind=1, uniform=True)
+    assert len(src) == nbytes
+    assert "sqh_jit" in src and "constexpr int N = 5;" in src
+    solver = nat.jit_source(4, W.ansatz_4q_12(), kind=3)
+    assert "sqh_jit_kernel" in solver and "solve_runs" in solver
+    with pytest.raises(nat.SqhError):
+        nat.jit_source(5, W.ansatz_layers(5, [0, 1]), kind=3)
