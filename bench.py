@@ -339,10 +339,11 @@ def gradient_leg(ctx, params, gates, U, o, reps=5, fused=False):
 
 def derivate_leg(gates, P, U):
     """apply_derivate_to of the 8-qubit ansatz on a 256 x 256 matrix: 264 derivative
-    matrices of 1 MiB written to HBM per call.  The forward store kernel runs over 264
+    matrices of 1 MiB written to HBM per call.  A forward store pass runs over 264
     elements sharing the input, element k with gate k's kernel replaced by its
-    derivative: one circuit pass per parameter (the reference's replay count), with
-    the generic and the circuit-specialised kernel."""
+    derivative: one circuit pass per parameter (the reference's replay count), through
+    the fused blocks (the default path; 128 flop per element per block) or the generic
+    and the circuit-specialised register-tile kernels (the gate arithmetic)."""
     import torch
 
     from squander_hip import Context
@@ -356,10 +357,12 @@ def derivate_leg(gates, P, U):
     # one pass of the circuit per parameter
     flops = P * flops_eval
     res = {"bound": "mfma (FP64)", "algorithmic_flops": flops, "bytes_written": wbytes}
-    for name, jit in (("generic", False), ("specialised", True)):
+    bflops = P * block_flops(gates, 8, d)
+    for name, jit in (("fused", False), ("generic", False), ("specialised", True)):
         ctx = Context(0)
         ctx.set_circuit(8, gates)
         ctx.set_jit(jit)
+        ctx.set_fused_cost(name == "fused")
         call = lambda: ctx.check(ctx._L.sqh_apply_derivate_to_dev(ctx.h, p.data_ptr(), M.data_ptr(), d, d, d,
                                                                   out.data_ptr()))
         call()
@@ -373,10 +376,13 @@ def derivate_leg(gates, P, U):
         kms, nl = ctx.kernel_timing()
         ctx.set_kernel_timing(False)
         k_s = kms / max(nl, 1) / 1e3
-        res[name] = {"ms": dt * 1e3, "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": flops / k_s / 1e12,
-                     "frac_of_fp64_peak": flops / k_s / 1e12 / FP64_PEAK_TFLOPS, "write_GBs": wbytes / k_s / 1e9,
-                     "kernel": ("sqh_jit_kernel (forward store, per-parameter kernel tables)" if jit
-                                else "k_tile_forward<8,double,0> (per-parameter kernel tables)")}
+        fl = bflops if name == "fused" else flops
+        res[name] = {"ms": dt * 1e3, "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": fl / k_s / 1e12,
+                     "frac_of_fp64_peak": fl / k_s / 1e12 / FP64_PEAK_TFLOPS, "write_GBs": wbytes / k_s / 1e9,
+                     "kernel": {"fused": "k_fused_cost<double,8,...> store pass over 264 elements (per-parameter "
+                                         "block matrices)",
+                                "specialised": "sqh_jit_kernel (forward store, per-parameter kernel tables)",
+                                "generic": "k_tile_forward<8,double,0> (per-parameter kernel tables)"}[name]}
         log(f"apply_derivate_to 8q {name}: {k_s * 1e3:.3f} ms")
     # error of the 264 derivative matrices (the last call's) against the FP64 oracle
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

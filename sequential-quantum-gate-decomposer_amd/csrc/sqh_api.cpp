@@ -567,10 +567,12 @@ int apply_dev(sqh_ctx* c, const double* d_params, int pbstride, void* d_inout, i
     return stream_apply(c, d_inout, rows, cols, stride, bstride, B, f32, kb);
 }
 
-// Gates_block::apply_derivate_to: the forward store kernel over P elements that share
-// the input, element k running the circuit with parameter k's gate replaced by its
+// Gates_block::apply_derivate_to: a forward store pass over P elements that share the
+// input, element k running the circuit with parameter k's gate replaced by its
 // derivative kernel (per-element kernel tables; a controlled gate's control-0 block
-// zeroed).  Output k is at d_out + k * rows * cols, row stride cols.
+// zeroed) -- through the fused blocks where that path applies (fused_pass_applies),
+// else the register-tile store kernel.  Output k is at d_out + k * rows * cols, row
+// stride cols.
 int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, int cols, int stride, void* d_out) {
     int rc = require_circuit(c);
     if (rc) return rc;
@@ -588,6 +590,28 @@ int deriv_dev(sqh_ctx* c, const double* d_params, const void* d_in, int rows, in
     HIPTRY(c, c->d_zslot.ensure((size_t)P * sizeof(int)));
     HIPTRY(c, launch_deriv_tables(c->d_kern.p, c->d_slots.as<Slot>(), ns, d_params, P, c->d_dtab.p,
                                   c->d_zslot.as<int>(), c->stream));
+    if (fused_pass_applies(c)) {
+        // the fused blocks' store pass over the P elements (block matrices from the
+        // per-parameter kernel tables), every column resident on the chip
+        const int nb = (int)c->fplan.qubits.size();
+        HIPTRY(c, c->d_fused.ensure((size_t)P * nb * 256 * 16));
+        HIPTRY(c, launch_fuse_blocks(false, c->d_fgates.p, c->d_ffirst.as<int>(), nb, c->d_dtab.p, (long long)ns * 4, P,
+                                     c->d_fused.p, c->stream, nullptr, controlled ? c->d_zslot.as<int>() : nullptr));
+        FusedCostArgs a;
+        a.U = d_in;
+        a.in_stride = stride;
+        a.m = cols;
+        a.B = P;
+        a.nb = nb;
+        a.F = c->d_fused.p;
+        a.F_bstride = (long long)nb * 256;
+        a.maps = c->d_fmaps.as<int>();
+        a.out = d_out;
+        a.out_bstride = (long long)rows * cols;
+        a.out_stride = cols;
+        TimedLaunch tl(c);
+        return fused_cost_launch(c, false, a);
+    }
     TileLaunch t = base_launch(c, false);
     t.kern = c->d_dtab.p;
     t.kern_bstride = (long long)ns * 4;

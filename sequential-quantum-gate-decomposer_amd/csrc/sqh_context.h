@@ -204,7 +204,8 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 // timing every candidate on a scratch batch of the same shape (sqh_tune.cpp).
 int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out);
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
-                              long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count = nullptr);
+                              long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count = nullptr,
+                              const int* zslot = nullptr);
 size_t fused_gate_bytes();
 // Fused-block cost + gradient (sqh_api.cpp): launch shape and scratch sizes, the
 // applicability rule, and the enqueue (no allocation) on a stream.

@@ -180,7 +180,8 @@ template <typename R>
 __global__ __launch_bounds__(64) void k_fuse_blocks(const FusedGate* __restrict__ fg,
                                                      const int* __restrict__ block_first, int nblocks,
                                                      const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
-                                                     cpx<R>* __restrict__ out, const int* __restrict__ count) {
+                                                     cpx<R>* __restrict__ out, const int* __restrict__ count,
+                                                     const int* __restrict__ zslot) {
     __shared__ FusedGate srec[kFuseChunk];
     __shared__ cpx<R> skern[kFuseElems][kFuseChunk][4];
     const int blk = blockIdx.x % nblocks;
@@ -188,6 +189,9 @@ __global__ __launch_bounds__(64) void k_fuse_blocks(const FusedGate* __restrict_
     if (count) Bk = min(*count, Bk);  // live elements (the optimiser's rounds)
     if (b0 >= Bk) return;             // the whole workgroup
     const int le = threadIdx.x >> 4, j = threadIdx.x & 15;
+    // apply_derivate_to: the element's differentiated controlled gate (or -1) runs with
+    // its control-0 block zeroed (the derivative of a controlled kernel)
+    const int zs = (zslot && b0 + le < Bk) ? zslot[b0 + le] : -1;
     cpx<R> v[16];
 #pragma unroll
     for (int i = 0; i < 16; ++i) v[i] = mk<R>(i == j ? (R)1 : (R)0, (R)0);
@@ -216,7 +220,10 @@ __global__ __launch_bounds__(64) void k_fuse_blocks(const FusedGate* __restrict_
 #pragma unroll
                 for (int e = 0; e < 4; ++e) k[e] = skern[le][gi][e];
             }
-            exec_gate<4, false, R>(v, op, k, 0);
+            if (gt.slot == zs && op.kind == OP_CU)
+                exec_gate<4, true, R>(v, op, k, 0);
+            else
+                exec_gate<4, false, R>(v, op, k, 0);
         }
     }
     const int b = b0 + le;
@@ -299,16 +306,17 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 }
 
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
-                              long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count) {
+                              long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count,
+                              const int* zslot) {
     if ((long long)Bk * nblocks == 0) return hipSuccess;
     const long long grid = (long long)nblocks * ((Bk + kFuseElems - 1) / kFuseElems);
     const int threads = 16 * kFuseElems;
     if (f32)
         hipLaunchKernelGGL(k_fuse_blocks<float>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
-                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out, count);
+                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out, count, zslot);
     else
         hipLaunchKernelGGL(k_fuse_blocks<double>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
-                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out, count);
+                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out, count, zslot);
     return hipGetLastError();
 }
 

@@ -295,3 +295,43 @@ def test_fused_gradient_launch_shapes(oracle, n):
                 assert abs(f[b] - ref[b][0]) < 1e-12 and np.abs(g[b] - ref[b][1]).max() < 1e-12
     with pytest.raises(nat.SqhError):
         ctx.set_fused_grad_config(3, 2)
+
+
+@pytest.mark.parametrize("n", [4, 5, 7, 8, 10])
+def test_fused_apply_derivate_to_matches_oracle(oracle, n):
+    """apply_derivate_to through the fused blocks (one store pass over the P elements,
+    each element's block matrices built from its kernel table with that parameter's gate
+    differentiated; a controlled gate's control-0 block zeroed): every derivative matrix
+    within 1e-12 (relative Frobenius) of the oracle's, square and rectangular inputs, a
+    strided input view, every gate kind."""
+    import torch
+    gates = W.mixed_gate_circuit(n, 4 * n + 6, 4800 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    p = W.random_params(P, 4900 + n)
+    rng = np.random.default_rng(5000 + n)
+    ctx = nat.Context(0)
+    ctx.set_circuit(n, gates)
+    ctx.set_fused_cost(True)
+    for m in sorted({d, max(1, d // 2 + 3)}):
+        say(f"fused apply_derivate_to n={n} m={m} P={P}")
+        X = rng.standard_normal((d, m)) + 1j * rng.standard_normal((d, m))
+        ref = oracle.apply_derivate_to(gates, p, X)
+        out = np.empty((P, d, m), dtype=np.complex128)
+        ctx.check(ctx._L.sqh_apply_derivate_to(ctx.h, p.ctypes.data, X.ctypes.data, d, m, m, out.ctypes.data))
+        worst = max(np.linalg.norm(out[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300) for k in range(P))
+        assert worst < 1e-12, worst
+    # strided device input: the m columns from column 3 of a d x (m + 5) buffer
+    m = max(1, d // 2)
+    buf = rng.standard_normal((d, m + 5)) + 1j * rng.standard_normal((d, m + 5))
+    ref = oracle.apply_derivate_to(gates, p, buf[:, 3:3 + m].copy())
+    dbuf = torch.from_numpy(buf).cuda()
+    dps = torch.from_numpy(p).cuda()
+    dout = torch.zeros((P, d, m), dtype=torch.complex128, device="cuda")
+    torch.cuda.synchronize()
+    ctx.check(ctx._L.sqh_apply_derivate_to_dev(ctx.h, dps.data_ptr(), dbuf.data_ptr() + 3 * 16, d, m, m + 5,
+                                                dout.data_ptr()))
+    torch.cuda.synchronize()
+    got = dout.cpu().numpy()
+    worst = max(np.linalg.norm(got[k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300) for k in range(P))
+    assert worst < 1e-12, worst

@@ -182,6 +182,7 @@ def test_jit_apply_derivate_to_matches_oracle(oracle, n):
             ctx = nat.Context(0)
             ctx.set_circuit(n, gates)
             ctx.set_jit(jit)
+            ctx.set_fused_cost(False)  # the register-tile store kernels (the fused path: test_gpu_fused_cost)
             out = np.empty((P, d, m), dtype=np.complex128)
             ctx.check(ctx._L.sqh_apply_derivate_to(ctx.h, p.ctypes.data, X.ctypes.data, d, m, m, out.ctypes.data))
             for k in range(P):
