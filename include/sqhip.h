@@ -147,7 +147,10 @@ int sqh_get_block_config(sqh_ctx* ctx, int is_f32, int rows, int cols, int B, in
                          double* tune_ms);
 
 /* ---- Gates_block::apply_derivate_to / apply_to_combined ---------------------- */
-/* out receives P dense rows x cols complex128 matrices: out[k] = (dC/dp_k) . in. */
+/* out receives P dense rows x cols complex128 matrices: out[k] = (dC/dp_k) . in.  One
+ * store pass over P elements sharing the input, element k with parameter k's gate
+ * replaced by its derivative: through the fused 4-qubit blocks where sqh_set_fused_cost
+ * selects them (as for the cost pass), else the register-tile store kernel. */
 int sqh_apply_derivate_to(sqh_ctx* ctx, const double* params, const double* in, int rows, int cols, int stride,
                           double* out);
 int sqh_apply_derivate_to_dev(sqh_ctx* ctx, const double* d_params, const double* d_in, int rows, int cols,
@@ -323,7 +326,8 @@ int sqh_get_jit_stats(sqh_ctx* ctx, double* compile_ms, int* compiles);
  * (sqh_optimization_problem_combined_batch[_dev]; _AUTO: from 6 qubits, with the same
  * arithmetic condition): forward and backward block products on the matrix cores, each
  * block's correlations then resolved to its gates.  sqh_decompose_batch's lockstep
- * rounds use the fused adjoint with _ON only.  No counterpart in the reference: it
+ * rounds use the fused adjoint with _ON only.  apply_derivate_to follows the cost pass's
+ * rule (one fused store pass over the P derivative elements).  No counterpart in the reference: it
  * replaces the per-gate loop of Gates_block::apply_to (Gates_block.cpp:606-710) inside
  * optimization_problem_batched (Optimization_Interface.cpp:940-1033) and the derivative
  * products of optimization_problem_combined (Optimization_Interface.cpp:1401-1450). */
