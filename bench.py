@@ -751,6 +751,8 @@ def main():
                      "peak_note": ("FP64 dense matrix-core peak" if headline == "fused" else
                                    "FP64 vector peak (equal to the FP64 matrix peak on MI355X): a VALU kernel"),
                      "kernel_avg_ms": k_avg_s * 1e3, "algorithmic_flops_per_launch": flops_launch,
+                     "launch_shape": (dict(zip(("tiles_per_wave", "exchange_groups", "waves", "prefetch"),
+                                               hctx.fused_cost_config(False, B))) if headline == "fused" else None),
                      "flops": ("fused blocks: 128 flop per element per block for 5 of the 6 blocks, "
                                "128 flop per column for the last (the trace row only)" if headline == "fused"
                                else "gate arithmetic: 14 flop per element per 1-qubit gate")},
