@@ -177,7 +177,7 @@ constexpr int kFuseElems = 4;
 // LDS a chunk at a time (all loads in flight together), then every thread applies the
 // gates to its unit column e_j.
 template <typename R>
-__global__ __launch_bounds__(64) void k_fuse_blocks(const FusedGate* __restrict__ fg,
+__global__ __launch_bounds__(16 * kFuseElems) void k_fuse_blocks(const FusedGate* __restrict__ fg,
                                                      const int* __restrict__ block_first, int nblocks,
                                                      const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
                                                      cpx<R>* __restrict__ out, const int* __restrict__ count,

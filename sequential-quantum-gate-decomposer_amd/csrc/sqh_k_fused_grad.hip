@@ -313,7 +313,7 @@ constexpr int kCorrElems = 4;
 // layout).  Gate records and the elements' 2x2 kernels are staged in LDS a chunk at a
 // time (from the block's end), so the walk issues no dependent global loads.
 template <typename R>
-__global__ __launch_bounds__(64) void k_fused_block_corr(const FusedGate* __restrict__ fg,
+__global__ __launch_bounds__(16 * kCorrElems) void k_fused_block_corr(const FusedGate* __restrict__ fg,
                                                          const int* __restrict__ block_first, int nb,
                                                          const cpx<R>* __restrict__ kern, long long kern_bstride,
                                                          const cpx<R>* __restrict__ F, const double* __restrict__ mpart,
@@ -469,10 +469,10 @@ hipError_t launch_fused_block_corr(bool f32, const void* fg, const int* block_fi
     const long long groups = (B + kCorrElems - 1) / kCorrElems;
     const unsigned grid = (unsigned)(groups * nb);
     if (f32)
-        hipLaunchKernelGGL(k_fused_block_corr<float>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
+        hipLaunchKernelGGL(k_fused_block_corr<float>, dim3(grid), dim3(16 * kCorrElems), 0, s, (const FusedGate*)fg, block_first,
                            nb, (const cpx<float>*)kern, kern_bstride, (const cpx<float>*)F, mpart, wgs, B, corr, ns, count);
     else
-        hipLaunchKernelGGL(k_fused_block_corr<double>, dim3(grid), dim3(64), 0, s, (const FusedGate*)fg, block_first,
+        hipLaunchKernelGGL(k_fused_block_corr<double>, dim3(grid), dim3(16 * kCorrElems), 0, s, (const FusedGate*)fg, block_first,
                            nb, (const cpx<double>*)kern, kern_bstride, (const cpx<double>*)F, mpart, wgs, B, corr, ns, count);
     return hipGetLastError();
 }
