@@ -48,14 +48,62 @@ def block_flops(gates, n, m):
     return len(nat.inspect_fusion(n, gates)) * 128.0 * (1 << n) * m
 
 
-def fused_cost_flops(gates, n, m):
-    """Algorithmic flops of one cost evaluation through the fused blocks: every block
-    but the last over the whole matrix (128 flop per element per block), then of the
-    last block only row c + offset of each column c (a 16-term complex dot product,
-    128 flop per column)."""
+def fused_cost_flops(gates, n, m, chain=True):
+    """Algorithmic flops of one cost evaluation through the fused blocks.
+
+    chain (the library's default cost pass): per column, the needed row of the circuit
+    matrix built from a unit vector, the blocks transposed from the last; block b then
+    acts on the 2^|S_b| rows inside the support S_b (the union of the qubits of blocks
+    b .. nb-1): 128 flop per row for every block but the last (whose row is loaded),
+    then 8 flop per row for the product with the target's column.
+
+    forward: every block but the last over the whole matrix (128 flop per element per
+    block), then of the last block only row c + offset of each column c (a 16-term
+    complex dot product, 128 flop per column)."""
     from squander_hip import _native as nat
-    nb = len(nat.inspect_fusion(n, gates))
-    return (nb - 1) * 128.0 * (1 << n) * m + 128.0 * m
+    plan = nat.inspect_fusion(n, gates)
+    nb = len(plan)
+    if not chain or nb < 2:
+        return (nb - 1) * 128.0 * (1 << n) * m + 128.0 * m
+    support, rows = set(plan[-1][0]), []
+    for qs, _ in reversed(plan[:-1]):
+        support |= set(qs)
+        rows.append(1 << len(support))
+    return (128.0 * sum(rows) + 8.0 * rows[-1]) * m
+
+
+def fused_forward_leg(gates, U, params, costs, steps, n, d):
+    """The fused cost pass with the target's columns run forward through the blocks
+    (the transposed chain switched off): the comparison for the headline kernel."""
+    import torch
+    from squander_hip import _native as nat
+    B = params.shape[0]
+    fctx = nat.Context(0)
+    fctx.set_circuit(n, gates)
+    fctx.set_target(U)
+    fctx.set_fused_cost(True)
+    fctx.set_fused_cost_chain(False)
+    step = lambda: fctx.check(fctx._L.sqh_optimization_problem_batch_dev(fctx.h, params.data_ptr(), B,
+                                                                        nat.PREC_F64, costs.data_ptr()))
+    step()
+    fctx.synchronize()
+    fctx.set_kernel_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    fctx.synchronize()
+    dt = time.perf_counter() - t0
+    kms, nl = fctx.kernel_timing()
+    fctx.set_kernel_timing(False)
+    k_s = kms / max(nl, 1) / 1e3
+    fl = fused_cost_flops(gates, n, d, chain=False) * B
+    return {"evals_per_s": B * steps / dt, "kernel_avg_ms": k_s * 1e3, "achieved_TFLOPs": fl / k_s / 1e12,
+            "frac": fl / k_s / 1e12 / FP64_PEAK_TFLOPS, "algorithmic_flops_per_launch": fl,
+            "launch_shape": dict(zip(("tiles_per_wave", "exchange_groups", "waves", "prefetch"),
+                                     fctx.fused_cost_config(False, B))),
+            "kernel": "k_fused_cost<double,8> (5 blocks over the target's columns, the 6th as one trace-row "
+                      "dot product per column)"}
 
 
 def fused_grad_flops(gates, n, m):
@@ -708,8 +756,9 @@ def main():
     value = B * args.steps / wall
     k_avg_s = kms / max(nl, 1) / 1e3
     flops_eval, _ = gate_work(gates, d, d, 16)
-    # the fused kernel's algorithmic flops: 5 of its 6 blocks over the whole matrix and
-    # one row of the last per column (0.52x the gate arithmetic)
+    # the fused chain's algorithmic flops: per column 2 blocks on the 128 rows of a
+    # 7-qubit support and 3 on all 256, then the product with the target's column (0.42x
+    # the gate arithmetic)
     flops_launch = (fused_cost_flops(gates, n, d) if headline == "fused" else flops_eval) * B
     achieved = flops_launch / k_avg_s / 1e12
     log(f"headline: {value:.0f} evals/s, kernel {k_avg_s * 1e3:.3f} ms, {achieved:.1f} TFLOP/s")
@@ -744,8 +793,10 @@ def main():
                      "traffic": (None if B != 512 else {"fused": FUSED_COST_8Q_B512_TRAFFIC,
                                                         "jit": JIT_FORWARD_8Q_B512_TRAFFIC,
                                                         "generic": TILE_FORWARD_8Q_B512_TRAFFIC}[headline]),
-                     "kernel": {"fused": "k_fused_cost<double,8> (5 fused 4-qubit blocks on v_mfma_f64_16x16x4f64, "
-                                         "the 6th as one trace-row dot product per column)",
+                     "kernel": {"fused": "k_fused_trace<double,8> (the transposed chain of the 6 fused 4-qubit "
+                                         "blocks from unit vectors on v_mfma_f64_16x16x4f64: the last block's row "
+                                         "loaded, 2 blocks on the 7-qubit support, 3 on all rows, then the product "
+                                         "with the target's column)",
                                 "jit": "sqh_jit_kernel (k_tile_forward specialised to the circuit, diagonal mode)",
                                 "generic": "k_tile_forward<8,double,1>"}[headline],
                      "peak_note": ("FP64 dense matrix-core peak" if headline == "fused" else
@@ -753,11 +804,15 @@ def main():
                      "kernel_avg_ms": k_avg_s * 1e3, "algorithmic_flops_per_launch": flops_launch,
                      "launch_shape": (dict(zip(("tiles_per_wave", "exchange_groups", "waves", "prefetch"),
                                                hctx.fused_cost_config(False, B))) if headline == "fused" else None),
-                     "flops": ("fused blocks: 128 flop per element per block for 5 of the 6 blocks, "
-                               "128 flop per column for the last (the trace row only)" if headline == "fused"
+                     "flops": ("fused chain: per column 128 flop per support row for blocks 5..1 (128, "
+                               "128, 256, 256, 256 rows), 8 flop per row for the product with the target's "
+                               "column" if headline == "fused"
                                else "gate arithmetic: 14 flop per element per 1-qubit gate")},
         "kernel_compile_ms": jit_ms,
     }
+    if headline == "fused":
+        out["fused_forward_pass"] = fused_forward_leg(gates, U, params, costs, args.steps, n, d)
+        log(f"fused forward pass (no chain): {out['fused_forward_pass']['evals_per_s']:.0f} evals/s")
     if not args.only_main:
         if headline != "jit":
             out["tile_kernel_specialised"] = tile_leg(gates, U, params, costs, args.steps, flops_eval * B, True)

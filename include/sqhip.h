@@ -347,6 +347,17 @@ int sqh_set_fused_cost(sqh_ctx* ctx, int mode);
 int sqh_set_fused_cost_config(sqh_ctx* ctx, int tiles_per_wave, int exchange_groups, int waves, int prefetch);
 int sqh_get_fused_cost_config(sqh_ctx* ctx, int is_f32, int B, int* tiles_per_wave, int* exchange_groups,
                               int* waves, int* prefetch);
+/* How the fused cost pass forms the trace entries.  on = 1 (the default): the
+ * transposed chain -- entry c is row c + trace_offset of the circuit matrix V times
+ * column c of the target, and that row is built from a unit vector through the blocks
+ * transposed, last block first, touching only the rows inside the union of the blocks'
+ * qubits seen so far (the 8-qubit 40-layer ansatz: 4 block products of work instead of
+ * 5).  on = 0: the target's columns run forward through the blocks (the store pass
+ * always does).  The two agree to rounding; the chain takes one exchange group
+ * (sqh_set_fused_cost_config's groups = 1), and circuits of one block always run
+ * forward.  No counterpart in the reference (the same trace as
+ * Optimization_Interface.cpp:940-1033 computes). */
+int sqh_set_fused_cost_chain(sqh_ctx* ctx, int on);
 /* Launch shape of the fused-block cost + gradient kernel: 16-column tiles per
  * wavefront (2 or 4; 10 qubits need 4) and wavefronts per workgroup (1, 2, 4 or 8,
  * dividing the element's column groups).  Both zero (the default): a fixed rule per

@@ -145,6 +145,42 @@ std::vector<int> fused_row_maps(int N, const FusionPlan& fp) {
     return maps;
 }
 
+// Row tables of the transposed-chain cost kernel: per block b, with the support
+// S_b = Q_b u Q_{b+1} u ... u Q_{nb-1} (the qubits of the blocks the chain has run,
+// from the last), the row offsets of the 16 local indices, the row bases of the
+// support's outer indices (bit k <-> the k-th smallest qubit of S_b outside the block),
+// log2 of their count and the row bits outside S_b.
+std::vector<int> fused_trace_maps(int N, const FusionPlan& fp) {
+    const int nb = (int)fp.qubits.size();
+    std::vector<int> maps((size_t)nb * kTraceMapStride, 0);
+    unsigned S = 0;
+    for (int b = nb - 1; b >= 0; --b) {
+        int* mp = maps.data() + (size_t)b * kTraceMapStride;
+        const auto& q = fp.qubits[b];
+        unsigned Q = 0;
+        for (int k = 0; k < 4; ++k) Q |= 1u << q[k];
+        S |= Q;
+        for (int i = 0; i < 16; ++i) {
+            int r = 0;
+            for (int k = 0; k < 4; ++k)
+                if ((i >> k) & 1) r |= 1 << q[k];
+            mp[i] = r;
+        }
+        int outer[16], no = 0;
+        for (int k = 0; k < N; ++k)
+            if (((S >> k) & 1u) && !((Q >> k) & 1u)) outer[no++] = k;
+        for (int o = 0; o < (1 << no); ++o) {
+            int r = 0;
+            for (int k = 0; k < no; ++k)
+                if ((o >> k) & 1) r |= 1 << outer[k];
+            mp[16 + o] = r;
+        }
+        mp[80] = no;
+        mp[81] = (int)(~S & ((1u << N) - 1u));
+    }
+    return maps;
+}
+
 // The circuit fuses into 4-qubit blocks the fused kernels take (4 to 10 qubits).
 bool fused_plan_fits(const sqh_ctx* c) {
     const int N = c->prog.N;
@@ -718,6 +754,9 @@ int sqh_set_circuit(sqh_ctx* c, int qbit_num, const sqh_gate* gates, int n_gates
         const std::vector<int> maps = fused_row_maps(qbit_num, fp);
         HIPTRY(c, c->d_fmaps.ensure(maps.size() * sizeof(int)));
         HIPTRY(c, hipMemcpy(c->d_fmaps.p, maps.data(), maps.size() * sizeof(int), hipMemcpyHostToDevice));
+        const std::vector<int> tmaps = fused_trace_maps(qbit_num, fp);
+        HIPTRY(c, c->d_tmaps.ensure(tmaps.size() * sizeof(int)));
+        HIPTRY(c, hipMemcpy(c->d_tmaps.p, tmaps.data(), tmaps.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     c->fplan = std::move(fp);
     c->fused_tuned.clear();

@@ -85,6 +85,11 @@ struct sqh_ctx {
     // Row tables of the tile-resident fused-block cost kernel ([blocks][kFusedMapStride];
     // present for 4 <= N <= 10) and its switch (sqh_set_fused_cost).
     sqh::DevBuf d_fmaps;
+    // Row tables of the transposed-chain cost kernel ([blocks][kTraceMapStride]) and its
+    // switch (sqh_set_fused_cost_chain; on by default): the cost pass builds the rows of
+    // the circuit matrix the trace needs instead of running the target's columns forward.
+    sqh::DevBuf d_tmaps;
+    bool fused_chain = true;
     sqh::DevBuf d_fcorr;  // [B][ns][8] gate correlations of the fused-block gradient
     int fused_cost = SQH_FUSED_COST_AUTO;
     int fg_force_tiles = 0, fg_force_waves = 0;  // fused gradient launch shape (0: automatic)
@@ -118,7 +123,7 @@ struct sqh_ctx {
             (void)hipEventDestroy(e.second);
         }
         for (sqh::DevBuf* b : {&d_ops, &d_lays, &d_slots, &d_U64, &d_U32, &d_params, &d_kern, &d_diag, &d_mpart,
-                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_fmaps, &d_fcorr, &d_idx, &d_seed, &d_dtab, &d_zslot})
+                               &d_res, &d_mat, &d_out, &d_fgates, &d_ffirst, &d_fused, &d_fmaps, &d_tmaps, &d_fcorr, &d_idx, &d_seed, &d_dtab, &d_zslot})
             b->release();
         for (hipStream_t s : side) (void)hipStreamDestroy(s);
         for (hipEvent_t e : side_ev) (void)hipEventDestroy(e);

@@ -165,5 +165,15 @@ struct FusedCostCfg {
 bool fused_cost_cfg_valid(int N, bool f32, int nb, FusedCostCfg cfg);
 size_t fused_cost_lds_bytes(bool f32, int nb, FusedCostCfg cfg);
 hipError_t launch_fused_cost(bool f32, int N, FusedCostArgs a, FusedCostCfg cfg, hipStream_t s);
+// The cost pass as a transposed chain from unit vectors (k_fused_trace, nb >= 2): the
+// rows of the circuit matrix the trace needs, built block by block on the rows inside
+// the support of the blocks seen so far, then dotted with the target's columns.  maps:
+// per block kTraceMapStride ints (row offsets of the 16 local indices, row bases of the
+// support's outer groups, log2 of their count, the row bits outside the support).  Shapes
+// with one exchange group; every valid shape gives bit-identical results.
+constexpr int kTraceMapStride = 16 + 64 + 2;
+bool fused_trace_cfg_valid(int N, bool f32, int nb, FusedCostCfg cfg);
+size_t fused_trace_lds_bytes(bool f32, int nb, FusedCostCfg cfg);
+hipError_t launch_fused_trace(bool f32, int N, FusedCostArgs a, FusedCostCfg cfg, hipStream_t s);
 
 }  // namespace sqh

@@ -102,50 +102,72 @@ int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out) {
     return SQH_OK;
 }
 
-// Fused-block kernel: candidates of tiles per wavefront x exchange groups x waves per
+// Fused-block kernels: candidates of tiles per wavefront x exchange groups x waves per
 // workgroup x prefetch.  The first call of a (pass, precision, qubits, columns, batch)
 // shape times every valid candidate on that call's own inputs (one warm-up and three
 // timed launches each, HIP events) and keeps the fastest; the chosen one then runs for
-// the call.  The cost pass's candidates all write the same trace entries (the shapes
-// are bit-identical).  The store pass (apply_to) runs in place, so its candidates read
-// the call's input and write a scratch copy of at most kFcTuneBytes instead (the first
-// elements of the batch).  Under stream capture, or with tuning off, the default shape
-// runs.
+// the call.  The cost pass runs the transposed chain (k_fused_trace, one exchange group)
+// unless it is switched off or the circuit has a single block; its candidates all write
+// the same trace entries (the shapes are bit-identical).  The store pass (apply_to) runs
+// in place, so its candidates read the call's input and write a scratch copy of at most
+// kFcTuneBytes instead (the first elements of the batch).  Under stream capture, or with
+// tuning off, the default shape runs.
 namespace {
 constexpr size_t kFcTuneBytes = size_t(256) << 20;
 const FusedCostCfg kFcCands[] = {{4, 1, 2, 0}, {4, 1, 2, 1}, {4, 2, 2, 1}, {4, 1, 1, 0}, {4, 2, 1, 1},
                                  {2, 1, 2, 0}, {2, 1, 4, 0}, {2, 1, 4, 1}};
+const FusedCostCfg kFtCands[] = {{4, 1, 2, 0}, {4, 1, 2, 1}, {4, 1, 1, 0}, {4, 1, 4, 0},
+                                 {2, 1, 2, 0}, {2, 1, 4, 0}, {2, 1, 4, 1}, {2, 1, 2, 1}};
 
-FusedCostCfg fused_default(int N, bool f32, int nb) {
+FusedCostCfg fused_default(int N, bool f32, int nb, bool chain) {
     // complex128: all four tiles in one exchange (eight independent MFMA chains);
-    // complex64: two exchange groups where the columns allow
-    const FusedCostCfg pref = f32 ? FusedCostCfg{4, 2, 2, 1} : FusedCostCfg{4, 1, 2, 0};
-    if (fused_cost_cfg_valid(N, f32, nb, pref)) return pref;
-    for (const FusedCostCfg& cfg : kFcCands)
-        if (fused_cost_cfg_valid(N, f32, nb, cfg)) return cfg;
+    // complex64: two exchange groups where the columns allow (the chain: one)
+    const FusedCostCfg pref = f32 && !chain ? FusedCostCfg{4, 2, 2, 1} : FusedCostCfg{4, 1, 2, 0};
+    auto valid = [&](FusedCostCfg cfg) {
+        return chain ? fused_trace_cfg_valid(N, f32, nb, cfg) : fused_cost_cfg_valid(N, f32, nb, cfg);
+    };
+    if (valid(pref)) return pref;
+    for (const FusedCostCfg& cfg : chain ? kFtCands : kFcCands)
+        if (valid(cfg)) return cfg;
     return pref;
+}
+
+// whether a fused pass runs the transposed chain: the cost pass (no output), with the
+// chain on and at least two blocks
+bool fused_uses_chain(const sqh_ctx* c, bool store) {
+    return !store && c->fused_chain && c->fplan.qubits.size() >= 2;
 }
 }  // namespace
 
-int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
-    const int N = c->prog.N, nb = a.nb;
+int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a_in) {
+    const int N = c->prog.N, nb = a_in.nb;
+    const bool chain = fused_uses_chain(c, a_in.out != nullptr);
+    FusedCostArgs a = a_in;
+    if (chain) a.maps = c->d_tmaps.as<int>();
+    auto valid = [&](FusedCostCfg cfg) {
+        return chain ? fused_trace_cfg_valid(N, f32, nb, cfg) : fused_cost_cfg_valid(N, f32, nb, cfg);
+    };
+    auto launch = [&](const FusedCostArgs& x, FusedCostCfg cfg) {
+        return chain ? launch_fused_trace(f32, N, x, cfg, c->stream) : launch_fused_cost(f32, N, x, cfg, c->stream);
+    };
     if (c->fused_force.tiles != 0) {
-        if (!fused_cost_cfg_valid(N, f32, nb, c->fused_force))
+        if (!valid(c->fused_force))
             return fail(c, SQH_ERR_ARG,
                         "fused-block launch shape %d/%d/%d/%d does not fit %d qubits (a column must lie in one "
-                        "exchange group) or the LDS a workgroup may allocate",
+                        "exchange group; the transposed chain takes one group) or the LDS a workgroup may allocate",
                         c->fused_force.tiles, c->fused_force.groups, c->fused_force.waves, c->fused_force.prefetch, N);
-        HIPTRY(c, launch_fused_cost(f32, N, a, c->fused_force, c->stream));
+        HIPTRY(c, launch(a, c->fused_force));
         return SQH_OK;
     }
-    // the store pass (apply_to) moves the matrices through HBM: its own key
-    const std::array<long long, 4> key{(f32 ? 1 : 0) + (a.out ? 2 : 0), N, a.m, a.B};
+    // the store pass (apply_to) moves the matrices through HBM, the chain is another
+    // kernel: their own keys
+    const std::array<long long, 4> key{(f32 ? 1 : 0) + (a.out ? 2 : 0) + (chain ? 4 : 0), N, a.m, a.B};
     auto it = c->fused_tuned.find(key);
-    if (it != c->fused_tuned.end() && fused_cost_cfg_valid(N, f32, nb, it->second)) {
-        HIPTRY(c, launch_fused_cost(f32, N, a, it->second, c->stream));
+    if (it != c->fused_tuned.end() && valid(it->second)) {
+        HIPTRY(c, launch(a, it->second));
         return SQH_OK;
     }
-    FusedCostCfg best = fused_default(N, f32, nb);
+    FusedCostCfg best = fused_default(N, f32, nb, chain);
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     HIPTRY(c, hipStreamIsCapturing(c->stream, &cs));
     // the arguments the candidates run with: the call's own, or for the store pass a
@@ -181,12 +203,12 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
         HIPTRY(c, hipEventCreate(&e0));
         hipError_t err = hipEventCreate(&e1);
         float best_ms = -1.0f;
-        for (const FusedCostCfg& cand : kFcCands) {
+        for (const FusedCostCfg& cand : chain ? kFtCands : kFcCands) {
             if (err != hipSuccess) break;
-            if (!fused_cost_cfg_valid(N, f32, nb, cand)) continue;
-            if ((err = launch_fused_cost(f32, N, ta, cand, c->stream)) != hipSuccess) break;  // warm-up
+            if (!valid(cand)) continue;
+            if ((err = launch(ta, cand)) != hipSuccess) break;  // warm-up
             if ((err = hipEventRecord(e0, c->stream)) != hipSuccess) break;
-            for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch_fused_cost(f32, N, ta, cand, c->stream);
+            for (int r = 0; r < 3 && err == hipSuccess; ++r) err = launch(ta, cand);
             if (err != hipSuccess || (err = hipEventRecord(e1, c->stream)) != hipSuccess ||
                 (err = hipEventSynchronize(e1)) != hipSuccess)
                 break;
@@ -203,7 +225,7 @@ int fused_cost_launch(sqh_ctx* c, bool f32, const FusedCostArgs& a) {
         c->fused_tuned[key] = best;
         c->tune_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     }
-    HIPTRY(c, launch_fused_cost(f32, N, a, best, c->stream));
+    HIPTRY(c, launch(a, best));
     return SQH_OK;
 }
 
@@ -328,6 +350,14 @@ extern "C" int sqh_set_fused_cost_config(sqh_ctx* c, int tiles_per_wave, int exc
     return SQH_OK;
 }
 
+extern "C" int sqh_set_fused_cost_chain(sqh_ctx* c, int on) {
+    if (!c) return SQH_ERR_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (on != 0 && on != 1) return fail(c, SQH_ERR_ARG, "the chain switch is 0 or 1");
+    c->fused_chain = on != 0;
+    return SQH_OK;
+}
+
 extern "C" int sqh_set_fused_grad_config(sqh_ctx* c, int tiles_per_wave, int waves) {
     if (!c) return SQH_ERR_ARG;
     std::lock_guard<std::mutex> lk(c->mu);
@@ -345,11 +375,13 @@ extern "C" int sqh_get_fused_cost_config(sqh_ctx* c, int is_f32, int B, int* til
     std::lock_guard<std::mutex> lk(c->mu);
     if (!c->have_circuit || !c->have_target) return fail(c, SQH_ERR_STATE, "set a circuit and a target first");
     const int N = c->prog.N, nb = (int)c->fplan.qubits.size();
-    FusedCostCfg cfg = fused_default(N, is_f32 != 0, nb);
-    if (fused_cost_cfg_valid(N, is_f32 != 0, nb, c->fused_force)) {
+    const bool chain = fused_uses_chain(c, false);
+    FusedCostCfg cfg = fused_default(N, is_f32 != 0, nb, chain);
+    if (chain ? fused_trace_cfg_valid(N, is_f32 != 0, nb, c->fused_force)
+              : fused_cost_cfg_valid(N, is_f32 != 0, nb, c->fused_force)) {
         cfg = c->fused_force;
     } else {
-        auto it = c->fused_tuned.find({is_f32 ? 1 : 0, N, c->cols, B});
+        auto it = c->fused_tuned.find({(is_f32 ? 1 : 0) + (chain ? 4 : 0), N, c->cols, B});
         if (it != c->fused_tuned.end()) cfg = it->second;
     }
     *tiles_per_wave = cfg.tiles;

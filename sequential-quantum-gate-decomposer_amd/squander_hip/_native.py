@@ -45,7 +45,7 @@ EXPORTS = [
     "sqh_apply_large_kernel_dev", "sqh_decompose_problems", "sqh_agents", "sqh_agents_batch",
     "sqh_set_autotune", "sqh_set_block_config", "sqh_get_block_config",
     "sqh_set_jit", "sqh_get_jit_stats", "sqh_jit_compile_check", "sqh_jit_source", "sqh_set_fused_cost",
-    "sqh_set_fused_cost_config", "sqh_get_fused_cost_config", "sqh_set_fused_grad_config",
+    "sqh_set_fused_cost_config", "sqh_get_fused_cost_config", "sqh_set_fused_cost_chain", "sqh_set_fused_grad_config",
     "sqh_set_tile_config", "sqh_get_tile_config",
     "sqh_optimization_problem_combined_targets_dev", "sqh_mlp_backward_step_dev",
 ]
@@ -162,6 +162,7 @@ def lib():
             "sqh_set_jit": ([vp, i], i),
             "sqh_set_fused_cost": ([vp, i], i),
             "sqh_set_fused_cost_config": ([vp, i, i, i, i], i),
+            "sqh_set_fused_cost_chain": ([vp, i], i),
             "sqh_set_fused_grad_config": ([vp, i, i], i),
             "sqh_set_tile_config": ([vp, i, i], i),
             "sqh_get_tile_config": ([vp, i, i, i, ctypes.POINTER(i), ctypes.POINTER(i)], i),
@@ -319,6 +320,11 @@ class Context:
     def set_fused_cost_config(self, tiles=0, groups=0, waves=0, prefetch=0):
         """Force the fused-block cost kernel's launch shape (all 0: run-time tuner)."""
         self.check(self._L.sqh_set_fused_cost_config(self.h, int(tiles), int(groups), int(waves), int(prefetch)))
+
+    def set_fused_cost_chain(self, on=True):
+        """Fused cost pass as the transposed chain from unit vectors (True, the default)
+        or as the target's columns run forward through the blocks (False)."""
+        self.check(self._L.sqh_set_fused_cost_chain(self.h, int(bool(on))))
 
     def set_fused_grad_config(self, tiles=0, waves=0):
         """Force the fused-block cost + gradient kernel's launch shape (both 0: the rule)."""

@@ -47,13 +47,18 @@ def test_fused_cost_matches_oracle(oracle, n):
         ctx.set_fused_cost(True)
         for nv, ov in VARIANTS:
             ctx.set_cost_function(nv)
-            f64 = _costs(ctx, ps, nat.PREC_F64)
-            f32 = _costs(ctx, ps, nat.PREC_F32)
+            # the transposed chain (default) and the forward pass over the target's columns
+            got = {}
+            for chain in (True, False):
+                ctx.set_fused_cost_chain(chain)
+                got[chain] = (_costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32))
+            ctx.set_fused_cost_chain(True)
             toff = off if ov in (orc.COST_FROBENIUS, orc.COST_ABS_TRACE) else 0
             for b in range(B):
                 fo = oracle.optimization_problem(gates, U, n, ps[b], variant=ov, trace_offset=toff)
-                assert abs(f64[b] - fo) < 1e-12, (nv, b, f64[b], fo)
-                assert abs(f32[b] - fo) < 1e-5, (nv, b, f32[b], fo)
+                for chain, (f64, f32) in got.items():
+                    assert abs(f64[b] - fo) < 1e-12, (nv, chain, b, f64[b], fo)
+                    assert abs(f32[b] - fo) < 1e-5, (nv, chain, b, f32[b], fo)
 
 
 @pytest.mark.parametrize("which", ["8q_40", "10q_60"])
@@ -72,10 +77,12 @@ def test_fused_cost_benchmark_circuits(oracle, which):
     tile64, tile32 = _costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32)
     ctx.set_fused_cost(True)
     f64, f32 = _costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32)
+    ctx.set_fused_cost_chain(False)
+    fw64, fw32 = _costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32)
     for b in range(B):
         fo = oracle.optimization_problem(gates, U, n, ps[b])
-        assert abs(f64[b] - fo) < 1e-12 and abs(tile64[b] - fo) < 1e-12
-        assert abs(f32[b] - fo) < 1e-5 and abs(tile32[b] - fo) < 1e-5
+        assert abs(f64[b] - fo) < 1e-12 and abs(tile64[b] - fo) < 1e-12 and abs(fw64[b] - fo) < 1e-12
+        assert abs(f32[b] - fo) < 1e-5 and abs(tile32[b] - fo) < 1e-5 and abs(fw32[b] - fo) < 1e-5
     say(f"  max |fused - oracle| c128 {np.max(np.abs(f64 - tile64)):.2e} (vs tile), c64 "
         f"{max(abs(f32[b] - oracle.optimization_problem(gates, U, n, ps[b])) for b in range(B)):.2e}")
 
@@ -116,12 +123,14 @@ SHAPES = [(t, g, w, p) for t in (2, 4) for g in (1, 2) for w in (1, 2, 4) for p 
           if g == 1 or (t == 4 and p == 1)]
 
 
+@pytest.mark.parametrize("chain", [True, False])
 @pytest.mark.parametrize("n", [4, 6, 8, 9, 10])
-def test_fused_cost_launch_shapes_are_bit_identical(n):
+def test_fused_cost_launch_shapes_are_bit_identical(n, chain):
     """Every launch shape the tuner may pick (tiles per wavefront, exchange groups,
     waves per workgroup, prefetch) gives bit-identical costs, in both precisions; the
     tuner's own pick equals them too.  Shapes that do not fit the qubit count (a column
-    must lie in one exchange group) are rejected."""
+    must lie in one exchange group) are rejected, and so are two exchange groups for the
+    transposed chain."""
     gates = W.mixed_gate_circuit(n, 5 * n, 3400 + n)
     P = W.param_count(gates)
     d = 1 << n
@@ -131,6 +140,7 @@ def test_fused_cost_launch_shapes_are_bit_identical(n):
     ctx.set_circuit(n, gates)
     ctx.set_target(U)
     ctx.set_fused_cost(True)
+    ctx.set_fused_cost_chain(chain)
     for prec in (nat.PREC_F64, nat.PREC_F32):
         ctx.set_fused_cost_config(0, 0, 0, 0)
         tuned = _costs(ctx, ps, prec)
@@ -144,7 +154,7 @@ def test_fused_cost_launch_shapes_are_bit_identical(n):
             except nat.SqhError:
                 # too few vector columns per group for this qubit count
                 O = d // 16
-                assert 16 * shape[0] < O or (16 * shape[0] // shape[1]) % O != 0
+                assert 16 * shape[0] < O or (16 * shape[0] // shape[1]) % O != 0 or (chain and shape[1] != 1)
                 continue
             ran += 1
             assert np.array_equal(got, tuned), shape
