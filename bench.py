@@ -30,8 +30,8 @@ FP64_PEAK_TFLOPS = 78.6
 TILE_FORWARD_8Q_B512_TRAFFIC = (11922.6 * 2.0 + 2048.0) * 1024
 # the same for the circuit-specialised kernel: profiles/sqh_jit_kernel_8q_b512.md
 JIT_FORWARD_8Q_B512_TRAFFIC = (24042.5 * 2.0 + 2048.0) * 1024
-# the same for the fused-block kernel: profiles/k_fused_cost_8q_b512.md
-FUSED_COST_8Q_B512_TRAFFIC = (10294.0 * 2.0 + 2048.0) * 1024
+# the same for the fused-block cost chain: profiles/k_fused_trace_8q_b512.md
+FUSED_COST_8Q_B512_TRAFFIC = (10323.0 * 2.0 + 2048.0) * 1024
 FP32_MFMA_PEAK_TFLOPS = 157.3
 
 
