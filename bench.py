@@ -106,14 +106,24 @@ def fused_forward_leg(gates, U, params, costs, steps, n, d):
                       "dot product per column)"}
 
 
-def fused_grad_flops(gates, n, m):
-    """Algorithmic flops of one cost + gradient evaluation through the fused blocks: nb
-    block products forward, then per block X <- F^dagger X, the correlation Y X^T and
-    (except for the first block) Y <- F^T Y -- 4 nb - 1 products of 128 flop per
-    element."""
+def fused_grad_flops(gates, n, m, chain=True):
+    """Algorithmic flops of one cost + gradient evaluation through the fused blocks.
+
+    chain (the library's default where the blocks' qubits cover the register): the
+    adjoint vectors from the transposed chain (the cost pass's flops, fused_cost_flops),
+    then a sweep up from the target: per block the correlation Z X^T, X <- F X (all but
+    the last) and Z <- conj(F) Z (all but the last two) -- 3 nb - 3 products of 128 flop
+    per element.
+
+    backward sweep: nb block products forward, then per block X <- F^dagger X, the
+    correlation Y X^T and (except for the first block) Y <- F^T Y -- 4 nb - 1 products."""
     from squander_hip import _native as nat
-    nb = len(nat.inspect_fusion(n, gates))
-    return (4 * nb - 1) * 128.0 * (1 << n) * m
+    plan = nat.inspect_fusion(n, gates)
+    nb = len(plan)
+    covered = set().union(*(set(qs) for qs, _ in plan)) == set(range(n))
+    if not chain or nb < 2 or not covered:
+        return (4 * nb - 1) * 128.0 * (1 << n) * m
+    return fused_cost_flops(gates, n, m, chain=True) + (3 * nb - 3) * 128.0 * (1 << n) * m
 
 
 def log(*a):
@@ -292,8 +302,9 @@ def ten_qubit_leg(steps, o):
                      "algorithmic_flops_per_eval": bfl,
                      "err_vs_fp64_oracle": float(np.abs(costs[:2].cpu().numpy() - ref).max())}
         log(f"10q {name}: {res[name]['evals_per_s']:.0f} evals/s, {tf:.1f} TFLOP/s")
-    # cost + gradient over the batch through the fused blocks (4 nb - 1 block
-    # products), complex128 and complex64
+    # cost + gradient over the batch through the fused blocks (the transposed chain for
+    # the adjoint vectors, then 3 nb - 3 block products up from the target), complex128
+    # and complex64
     grad = torch.empty((B, P), dtype=torch.float64, device="cuda")
     _, gr = o.optimization_problem_combined(gates, U, n, params[0].cpu().numpy())
     for name, prec, peak in (("c128_gradient_fused", PREC_F64, FP64_PEAK_TFLOPS),
@@ -370,7 +381,8 @@ def gradient_leg(ctx, params, gates, U, o, reps=5, fused=False):
     if fused:
         gflops = fused_grad_flops(gates, n, d) * B
         peak = FP64_PEAK_TFLOPS
-        kernel = "k_fused_grad<double,8,T> (v_mfma_f64_16x16x4f64)"
+        kernel = ("k_fused_grad_chain<double,8,T> (v_mfma_f64_16x16x4f64: the transposed chain for the "
+                  "adjoint vectors, then the sweep up from the target)")
     else:
         # adjoint sweep: forward pass, G^dagger on the state, G^T on the adjoint and the
         # pair correlations (32 flop per pair against 28 for a gate): (3 + 8/7) x forward

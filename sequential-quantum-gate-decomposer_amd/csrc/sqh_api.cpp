@@ -304,6 +304,18 @@ bool fused_grad_applies(const sqh_ctx* c, const TraceSpec& ts, bool f32, int B, 
 // target (m columns, row stride m), or with bi.count the optimiser's per-run targets
 // (element e's at U + run_target[elem_run[e]] * tgt_stride, *count live elements).
 // The buffers hold the plan's sizes; nothing is allocated.
+// Whether the fused gradient takes its adjoint vectors from the transposed chain
+// (k_fused_grad_chain): the chain switch on, at least two blocks, and the blocks'
+// qubits covering the register (the chain then ends in the forward layout).
+bool fused_grad_uses_chain(const sqh_ctx* c) {
+    const FusionPlan& fp = c->fplan;
+    if (!c->fused_chain || fp.qubits.size() < 2) return false;
+    unsigned S = 0;
+    for (const auto& q : fp.qubits)
+        for (int k = 0; k < 4; ++k) S |= 1u << q[k];
+    return S == (1u << c->prog.N) - 1u;
+}
+
 int fused_grad_run(sqh_ctx* c, bool f32, const FusedGradPlan& p, int B, int m, const void* kern, const void* U,
                    double* diag, int trace_off, void* fused, double* mpart, double* corr, const BatchIndex& bi,
                    hipStream_t st, bool timed) {
@@ -325,6 +337,7 @@ int fused_grad_run(sqh_ctx* c, bool f32, const FusedGradPlan& p, int B, int m, c
     a.elem_run = bi.elem_run;
     a.run_target = bi.run_target;
     a.tgt_stride = bi.tgt_stride;
+    if (fused_grad_uses_chain(c)) a.tmaps = c->d_tmaps.as<int>();
     if (timed) {
         TimedLaunch tl(c);
         HIPTRY(c, launch_fused_grad(f32, N, a, p.T, p.waves, st));

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include "sqh_device.h"
+#include "sqh_fused_chain.h"
 #include "sqh_kernels.h"
 
 namespace sqh {
@@ -283,6 +284,242 @@ __global__ __launch_bounds__(512) void k_fused_grad(FusedGradArgs a) {
     }
 }
 
+// Cost + gradient with the adjoint vectors from the transposed chain (sqh_fused_chain.h).
+// In the notation above, the adjoint after block b is Z_{b+1} = F_{b+1}^T ... F_{nb-1}^T
+// e_rho (Z_nb = e_rho): exactly the chain's vectors, which touch only the rows inside the
+// support of the blocks seen so far.  Phase 1 runs the chain down to Z_1 (block 0's
+// operand), then Z_0 = F_0^T Z_1 gives the trace entries (dotted with the target's
+// columns).  Phase 2 sweeps up: X_{-1} = the target's columns,
+//   M_b += Z_{b+1} X_{b-1}^T,   X_b = F_b X_{b-1},   Z_{b+2} = conj(F_{b+1}) Z_{b+1}
+// (F unitary: (F^T)^-1 = conj(F)), Z_nb set to the selector itself.  Block products:
+// the chain's, then 3 nb - 3 (the 8-qubit 40-layer ansatz: 4 + 15 = 19 against the
+// backward sweep's 23).  The correlations M_b and their partials are those of
+// k_fused_grad.  Needs nb >= 2 and block 0's support to cover every qubit (the chain
+// then ends in the forward layout).
+template <typename R, int N, int T>
+__global__ __launch_bounds__(512) void k_fused_grad_chain(FusedGradArgs a) {
+    typedef FgMfma<R> MF;
+    constexpr int D = 1 << N;
+    constexpr int O = D >> 4;
+    constexpr int W = 16 * T / O;
+    static_assert(W >= 1, "a wavefront holds whole columns");
+    constexpr int RG = 16 * T * 16;
+    static_assert(RG >= 512 && RG * sizeof(cpx<R>) >= 512 * sizeof(double), "the region holds the tiles");
+    extern __shared__ __align__(16) unsigned char fg_smem[];
+    const int nwv = blockDim.x >> 6;
+    int* smap = reinterpret_cast<int*>(fg_smem);
+    int* tmap = smap + a.nb * kFusedMapStride;
+    const int nmap = a.nb * (kFusedMapStride + kTraceMapStride);
+    cpx<R>* region = reinterpret_cast<cpx<R>*>(fg_smem + (((size_t)nmap * sizeof(int) + 15) & ~(size_t)15));
+    for (int i = threadIdx.x; i < a.nb * kFusedMapStride; i += blockDim.x) smap[i] = a.maps[i];
+    for (int i = threadIdx.x; i < a.nb * kTraceMapStride; i += blockDim.x) tmap[i] = a.tmaps[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    int B = a.B;
+    long long lwg = blockIdx.x;
+    if (a.count) {
+        B = min(*a.count, a.B);
+    } else {
+        const long long per_xcd = gridDim.x >> 3;
+        lwg = (long long)(blockIdx.x & 7) * per_xcd + (blockIdx.x >> 3);
+    }
+    const long long wave = lwg * nwv + wv;
+    if (lwg * nwv >= (long long)B * a.groups) return;  // whole workgroups
+    const int e = (int)(wave / a.groups);
+    const int cg = (int)(wave % a.groups);
+    const int c0 = cg * W;
+    const int wgi = cg / nwv;
+    cpx<R>* lds = region + (size_t)wv * RG;
+    cpx<R>* ys = lds;
+    cpx<R>* xs = lds + 256;
+    const int nl = lane & 15, g = lane >> 4;
+    int w[T], o[T];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        w[t] = (16 * t + nl) / O;
+        o[t] = (16 * t + nl) % O;
+    }
+    const cpx<R>* U = static_cast<const cpx<R>*>(a.U);
+    if (a.count) U += (long long)a.run_target[a.elem_run[e]] * a.tgt_stride;
+    const int m = a.m;
+    const cpx<R>* Fe = static_cast<const cpx<R>*>(a.F) + (long long)e * a.nb * 256;
+    // phase 1: Z_1 in block 0's grouping (B layout)
+    R zr[T][4], zi[T][4], far[4], fai[4];
+    fused_chain_to_block0<R, N, T, false>(tmap, a.nb, lds, Fe, c0, m, a.trace_off, lane, zr, zi, far, fai);
+    typename MF::V er[T], ei[T];
+    {
+        // Z_0 = F_0^T Z_1, times the target's columns: the trace entries
+        R ar[4], ai[4], ain[4];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> f = Fe[(4 * s + g) * 16 + nl];
+            ar[s] = f.re;
+            ai[s] = f.im;
+            ain[s] = -f.im;
+        }
+        fg_product<R, T>(ar, ai, ain, zr, zi, er, ei);
+        constexpr int seg = O < 16 ? O : 16;
+        double sr = 0, si = 0;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int col = c0 + w[t];
+            if (col < m) {
+                const int base = smap[16 + o[t]];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const cpx<R> u = U[(long long)(base + smap[MF::row(r, g)]) * m + col];
+                    sr += (double)er[t][r] * (double)u.re - (double)ei[t][r] * (double)u.im;
+                    si += (double)er[t][r] * (double)u.im + (double)ei[t][r] * (double)u.re;
+                }
+            }
+            if (O < 16 || ((t + 1) * 16) % O == 0) {
+                sr += __shfl_xor(sr, 16, 64);
+                si += __shfl_xor(si, 16, 64);
+                sr += __shfl_xor(sr, 32, 64);
+                si += __shfl_xor(si, 32, 64);
+#pragma unroll
+                for (int d = 1; d < seg; d <<= 1) {
+                    sr += __shfl_xor(sr, d, 64);
+                    si += __shfl_xor(si, d, 64);
+                }
+                if (g == 0 && (nl & (seg - 1)) == 0 && col < m) {
+                    double* dg = a.diag + 2 * ((long long)e * m + col);
+                    dg[0] = sr;
+                    dg[1] = si;
+                }
+                sr = 0;
+                si = 0;
+            }
+        }
+    }
+    // phase 2: X_{-1} = the target's columns in block 0's grouping (B layout)
+    R xr[T][4], xi[T][4];
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
+        const int col = c0 + w[t];
+        const int base = smap[16 + o[t]];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> v = col < m ? U[(long long)(base + smap[4 * s + g]) * m + col] : mk<R>(0, 0);
+            xr[t][s] = v.re;
+            xi[t][s] = v.im;
+        }
+    }
+    // M_bb += Y X^T over the wavefront's vector columns, Y in the B layout (ZB) or the D
+    // layout (er / ei), X in the B layout; then the workgroup's partial of block bb
+    auto correlate = [&](int bb, bool zb) __attribute__((always_inline)) {
+        typename MF::V mr = typename MF::V{0, 0, 0, 0}, mi = typename MF::V{0, 0, 0, 0};
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            if (zb) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) ys[fg_tile(4 * s + g, nl)] = mk<R>(zr[t][s], zi[t][s]);
+            } else {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ys[fg_tile(MF::row(r, g), nl)] = mk<R>(er[t][r], ei[t][r]);
+            }
+#pragma unroll
+            for (int s = 0; s < 4; ++s) xs[fg_tile(4 * s + g, nl)] = mk<R>(xr[t][s], xi[t][s]);
+            fg_wave_fence();
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> yv = ys[fg_tile(nl, 4 * s + g)];
+                const cpx<R> xv = xs[fg_tile(nl, 4 * s + g)];
+                mr = MF::op(yv.re, xv.re, mr);
+                mi = MF::op(yv.re, xv.im, mi);
+                mr = MF::op(-yv.im, xv.im, mr);
+                mi = MF::op(yv.im, xv.re, mi);
+            }
+            fg_wave_fence();
+        }
+        double* red = reinterpret_cast<double*>(lds);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int idx = MF::row(r, g) * 16 + nl;
+            red[idx * 2] = (double)mr[r];
+            red[idx * 2 + 1] = (double)mi[r];
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < 512; i += blockDim.x) {
+            double sum = 0.0;
+            for (int v = 0; v < nwv; ++v) sum += reinterpret_cast<const double*>(region + (size_t)v * RG)[i];
+            a.mpart[(((long long)e * a.wgs + wgi) * a.nb + bb) * 512 + i] = sum;
+        }
+        __syncthreads();
+    };
+    correlate(0, true);
+    typename MF::V dr[T], di[T];
+    // one step up: X_b into block b + 1's grouping, Z_{b+2} (D layout, block b + 1's
+    // grouping) from Z_{b+1} -- in the B layout of block 0 for the first step (zb), in
+    // the D layout of block b after it -- then M_{b+1}.  The first step runs on its own
+    // so the B-layout Z is not live through the loop.
+    auto step = [&](int b, bool zb) __attribute__((always_inline)) {
+        const int* mp = smap + b * kFusedMapStride;
+        const int* mn = mp + kFusedMapStride;
+        {
+            R ar[4], ai[4], ain[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> f = Fe[(long long)b * 256 + nl * 16 + 4 * s + g];
+                ar[s] = f.re;
+                ai[s] = f.im;
+                ain[s] = -f.im;
+            }
+            fg_product<R, T>(ar, ai, ain, xr, xi, dr, di);
+            fg_remap<R, N, T>(dr, di, xr, xi, mp, mn, w, o, g, lds);
+        }
+        if (b + 2 < a.nb) {
+            if (zb) {
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const int base = mp[16 + o[t]];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        lds[w[t] * D + fg_slot<N>(base + mp[4 * s + g])] = mk<R>(zr[t][s], zi[t][s]);
+                }
+                fg_wave_fence();
+#pragma unroll
+                for (int t = 0; t < T; ++t) {
+                    const int base = mn[16 + o[t]];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const cpx<R> v = lds[w[t] * D + fg_slot<N>(base + mn[4 * s + g])];
+                        zr[t][s] = v.re;
+                        zi[t][s] = v.im;
+                    }
+                }
+                fg_wave_fence();
+            } else {
+                fg_remap<R, N, T>(er, ei, zr, zi, mp, mn, w, o, g, lds);
+            }
+            R ar[4], ai[4], ain[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> f = Fe[(long long)(b + 1) * 256 + nl * 16 + 4 * s + g];
+                ar[s] = f.re;
+                ai[s] = -f.im;
+                ain[s] = f.im;
+            }
+            fg_product<R, T>(ar, ai, ain, zr, zi, er, ei);
+        } else {
+            // Z_nb: the selector, row rho = c + trace_off of column c (D layout)
+#pragma unroll
+            for (int t = 0; t < T; ++t) {
+                const int col = c0 + w[t];
+                const int base = mn[16 + o[t]];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    er[t][r] = (col < m && base + mn[MF::row(r, g)] == col + a.trace_off) ? (R)1 : (R)0;
+                    ei[t][r] = (R)0;
+                }
+            }
+        }
+        correlate(b + 1, false);
+    };
+    step(0, true);
+    for (int b = 1; b + 1 < a.nb; ++b) step(b, false);
+}
+
 // The workgroup partials of every (element, block) M summed in a fixed order, in place
 // into the first partial: thread per double, consecutive threads on consecutive
 // entries.
@@ -404,14 +641,22 @@ __global__ __launch_bounds__(16 * kCorrElems) void k_fused_block_corr(const Fuse
 }
 
 size_t fused_grad_lds_bytes(bool f32, int nb, int T, int waves) {
+    // both row tables (the chain kernel's too), then one region per wavefront
     const size_t esz = f32 ? 8 : 16;
-    return (((size_t)nb * kFusedMapStride * sizeof(int) + 15) & ~(size_t)15) + esz * (size_t)(16 * T * 16) * waves;
+    return (((size_t)nb * (kFusedMapStride + kTraceMapStride) * sizeof(int) + 15) & ~(size_t)15) +
+           esz * (size_t)(16 * T * 16) * waves;
 }
 
 template <typename R>
 static hipError_t launch_fg(int N, const FusedGradArgs& a, int T, int grid, int threads, size_t shm, hipStream_t s) {
-#define SQH_FG_CASE(n, t) \
-    if (N == n && T == t) { hipLaunchKernelGGL((k_fused_grad<R, n, t>), dim3(grid), dim3(threads), shm, s, a); return hipGetLastError(); }
+#define SQH_FG_CASE(n, t)                                                                                    \
+    if (N == n && T == t) {                                                                                  \
+        if (a.tmaps)                                                                                         \
+            hipLaunchKernelGGL((k_fused_grad_chain<R, n, t>), dim3(grid), dim3(threads), shm, s, a);         \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_fused_grad<R, n, t>), dim3(grid), dim3(threads), shm, s, a);               \
+        return hipGetLastError();                                                                            \
+    }
     SQH_FG_CASE(4, 4)
     SQH_FG_CASE(5, 4)
     SQH_FG_CASE(6, 4)

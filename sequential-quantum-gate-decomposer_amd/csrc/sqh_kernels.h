@@ -140,6 +140,10 @@ struct FusedGradArgs {
     const int* elem_run = nullptr;
     const int* run_target = nullptr;
     long long tgt_stride = 0;
+    // non-null: the adjoint vectors from the transposed chain (k_fused_grad_chain), with
+    // these chain tables ([nb][kTraceMapStride]); needs nb >= 2 and the blocks' qubits
+    // covering the register
+    const int* tmaps = nullptr;
 };
 // Launch shape: T 16-column tiles per wavefront (2 or 4; 4 at 10 qubits), waves 1, 2, 4
 // or 8 per workgroup dividing the element's column groups; wgs = workgroups per element

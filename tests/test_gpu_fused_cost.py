@@ -230,15 +230,20 @@ def test_fused_gradient_matches_oracle(oracle, n):
         ctx.set_fused_cost(True)
         for nv, ov in VARIANTS:
             ctx.set_cost_function(nv)
-            f64, g64 = _combined(ctx, ps, nat.PREC_F64)
-            f32, g32 = _combined(ctx, ps, nat.PREC_F32)
+            # adjoint vectors from the transposed chain (default) and from the backward sweep
+            got = {}
+            for chain in (True, False):
+                ctx.set_fused_cost_chain(chain)
+                got[chain] = _combined(ctx, ps, nat.PREC_F64) + _combined(ctx, ps, nat.PREC_F32)
+            ctx.set_fused_cost_chain(True)
             toff = off if ov in (orc.COST_FROBENIUS, orc.COST_ABS_TRACE) else 0
             for b in range(B):
                 fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b], ov, toff)
-                assert abs(f64[b] - fo) < 1e-12, (nv, b, f64[b], fo)
-                assert np.abs(g64[b] - go).max() < 1e-12, (nv, b, np.abs(g64[b] - go).max())
-                assert abs(f32[b] - fo) < 1e-5, (nv, b, f32[b], fo)
-                assert np.abs(g32[b] - go).max() < 1e-5, (nv, b, np.abs(g32[b] - go).max())
+                for chain, (f64, g64, f32, g32) in got.items():
+                    assert abs(f64[b] - fo) < 1e-12, (nv, chain, b, f64[b], fo)
+                    assert np.abs(g64[b] - go).max() < 1e-12, (nv, chain, b, np.abs(g64[b] - go).max())
+                    assert abs(f32[b] - fo) < 1e-5, (nv, chain, b, f32[b], fo)
+                    assert np.abs(g32[b] - go).max() < 1e-5, (nv, chain, b, np.abs(g32[b] - go).max())
 
 
 @pytest.mark.parametrize("which", ["8q_40", "10q_60"])
@@ -259,9 +264,12 @@ def test_fused_gradient_benchmark_circuits(oracle, which):
     ctx.set_fused_cost(True)
     f64, g64 = _combined(ctx, ps, nat.PREC_F64)
     f32, g32 = _combined(ctx, ps, nat.PREC_F32)
+    ctx.set_fused_cost_chain(False)
+    fb64, gb64 = _combined(ctx, ps, nat.PREC_F64)
     for b in range(B):
         fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b])
         assert abs(f64[b] - fo) < 1e-12 and np.abs(g64[b] - go).max() < 1e-12
+        assert abs(fb64[b] - fo) < 1e-12 and np.abs(gb64[b] - go).max() < 1e-12
         assert abs(f32[b] - fo) < 1e-5 and np.abs(g32[b] - go).max() < 1e-5
     say(f"  max |fused - tile| gradient c128 {np.abs(g64 - tg).max():.2e}, c64 vs c128 {np.abs(g32 - g64).max():.2e}")
 
