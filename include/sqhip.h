@@ -355,8 +355,11 @@ int sqh_get_fused_cost_config(sqh_ctx* ctx, int is_f32, int B, int* tiles_per_wa
  * 5).  on = 0: the target's columns run forward through the blocks (the store pass
  * always does).  The two agree to rounding; the chain takes one exchange group
  * (sqh_set_fused_cost_config's groups = 1), and circuits of one block always run
- * forward.  No counterpart in the reference (the same trace as
- * Optimization_Interface.cpp:940-1033 computes). */
+ * forward.  The same switch selects where the fused gradient's adjoint vectors come
+ * from: the chain, followed by one sweep up from the target (8-qubit ansatz: 19 block
+ * products instead of 23; needs blocks whose qubits cover the register), or the
+ * backward sweep.  No counterpart in the reference (the same trace and gradient as
+ * Optimization_Interface.cpp:940-1033 and 1401-1450 compute). */
 int sqh_set_fused_cost_chain(sqh_ctx* ctx, int on);
 /* Launch shape of the fused-block cost + gradient kernel: 16-column tiles per
  * wavefront (2 or 4; 10 qubits need 4) and wavefronts per workgroup (1, 2, 4 or 8,
