@@ -289,13 +289,159 @@ __global__ __launch_bounds__(256) void k_fused_trace(FusedCostArgs a) {
     // nothing; trace_off + m <= 2^N)
     auto rho_of = [&](int w) { return (c0 + w < m ? c0 + w : 0) + a.trace_off; };
     const cpx<R>* Fe = static_cast<const cpx<R>*>(a.F) + (long long)e * a.F_bstride;
-    R xr[T][4], xi[T][4], far[4], fai[4];
-    fused_chain_to_block0<R, N, T, PF>(smap, a.nb, lds, Fe, c0, m, a.trace_off, lane, xr, xi, far, fai);
+    // X of block bn (B operand of step s, tile t: X[4 s + g][16 t + nl] in its grouping)
+    // from the LDS region, for the tiles holding its support's groups; vector columns
+    // past the wavefront's columns read column 0 (their products are never stored)
+    R xr[T][4], xi[T][4];
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int s = 0; s < 4; ++s) xr[t][s] = xi[t][s] = 0;
+    auto load_x = [&](int bn) {
+        const int* mn = smap + bn * kTraceMapStride;
+        const int lgn = mn[80], nSn = mn[81];
+        const int tsn = ((W << lgn) + 15) >> 4;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            if (t < tsn) {
+                const int n = 16 * t + nl;
+                const int w = n >> lgn < W ? n >> lgn : 0;
+                const int base = mn[16 + (n & ((1 << lgn) - 1))] | (rho_of(w) & nSn);
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const cpx<R> v = lds[w * D + fc_slot<N>(base + mn[4 * s + g])];
+                    xr[t][s] = v.re;
+                    xi[t][s] = v.im;
+                }
+            }
+        }
+        fc_wave_fence();
+    };
+    // The region starts at zero (rows outside a block's support are never stored and
+    // read as zero; the supports only grow), then holds F_{nb-1}^T e_rho per column:
+    // row loc(rho) of F_{nb-1} on the rows that agree with rho outside its qubits.
+    for (int i = lane; i < W * D; i += 64) lds[i] = mk<R>(0, 0);
+    fc_wave_fence();
+    {
+        const int* mq = smap + (a.nb - 1) * kTraceMapStride;
+        const cpx<R>* Fq = Fe + (long long)(a.nb - 1) * 256;
+        for (int p = lane; p < 16 * W; p += 64) {
+            const int w = p >> 4, i = p & 15;
+            const int rho = rho_of(w);
+            lds[w * D + fc_slot<N>((rho & mq[81]) | mq[i])] = Fq[ft_local(mq, rho) * 16 + i];
+        }
+    }
+    fc_wave_fence();
+    load_x(a.nb - 2);
+    // A operand of step s: F_b^T[nl][4 s + g] = F_b[4 s + g][nl] (with PF, block b - 1's
+    // is loaded while block b runs)
+    R far[4], fai[4];
+    if constexpr (PF) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const cpx<R> f = Fe[(long long)(a.nb - 2) * 256 + (4 * s + g) * 16 + nl];
+            far[s] = f.re;
+            fai[s] = f.im;
+        }
+    }
+    // A operand of block b into ar / ai / ain (with PF from far / fai, block b - 1's then
+    // loaded into them)
+    auto a_operand = [&](int b, R (&ar)[4], R (&ai)[4], R (&ain)[4]) {
+        if constexpr (PF) {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                ar[s] = far[s];
+                ai[s] = fai[s];
+                ain[s] = -fai[s];
+            }
+            if (b > 0) {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const cpx<R> f = Fe[(long long)(b - 1) * 256 + (4 * s + g) * 16 + nl];
+                    far[s] = f.re;
+                    fai[s] = f.im;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> f = Fe[(long long)b * 256 + (4 * s + g) * 16 + nl];
+                ar[s] = f.re;
+                ai[s] = f.im;
+                ain[s] = -f.im;
+            }
+        }
+    };
+    // block b's output Y (its support's tiles) through LDS in natural row order, back in
+    // block b - 1's grouping
+    auto exchange = [&](int b, int lg, int nS, int ts, const typename MF::V (&yr)[T], const typename MF::V (&yi)[T]) {
+        const int* mp = smap + b * kTraceMapStride;
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int n = 16 * t + nl;
+            const int w = n >> lg;
+            if (t < ts && w < W) {
+                const int base = mp[16 + (n & ((1 << lg) - 1))] | (rho_of(w) & nS);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int j = base + mp[MF::row(r, g)];
+                    lds[w * D + fc_slot<N>(j)] = mk<R>(yr[t][r], yi[t][r]);
+                }
+            }
+        }
+        fc_wave_fence();
+        load_x(b - 1);
+    };
+    // The supports only grow along the chain: first the blocks whose support fills
+    // fewer than T tiles (1, or 2 of 4), then the dense ones.  (Separate loops keep the
+    // tile-count variants' registers apart.)
+    int b = a.nb - 2;
+    for (; b > 0; --b) {
+        const int* mp = smap + b * kTraceMapStride;
+        const int lg = mp[80], nS = mp[81];
+        const int ts = ((W << lg) + 15) >> 4;
+        if (ts >= T) break;
+        R ar[4], ai[4], ain[4];
+        a_operand(b, ar, ai, ain);
+        typename MF::V yr[T], yi[T];
+        if (T >= 4 && ts >= 2) ft_mma<R, T, (T >= 4 ? 2 : 1)>(ar, ai, ain, xr, xi, yr, yi);
+        else ft_mma<R, T, 1>(ar, ai, ain, xr, xi, yr, yi);
+        exchange(b, lg, nS, ts, yr, yi);
+    }
+    // Blocks of full support (no row outside it): vector column 16 t + nl is column
+    // wd[t], outer group od[t], as in the forward pass -- no predicates, no row of V.
     int wd[T], od[T];
 #pragma unroll
     for (int t = 0; t < T; ++t) {
         wd[t] = (16 * t + nl) / O;
         od[t] = (16 * t + nl) % O;
+    }
+    for (; b > 0; --b) {
+        const int* mp = smap + b * kTraceMapStride;
+        const int* mn = mp - kTraceMapStride;
+        R ar[4], ai[4], ain[4];
+        a_operand(b, ar, ai, ain);
+        typename MF::V yr[T], yi[T];
+        ft_mma<R, T, T>(ar, ai, ain, xr, xi, yr, yi);
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int base = mp[16 + od[t]];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                lds[wd[t] * D + fc_slot<N>(base + mp[MF::row(r, g)])] = mk<R>(yr[t][r], yi[t][r]);
+        }
+        fc_wave_fence();
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const int base = mn[16 + od[t]];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const cpx<R> v = lds[wd[t] * D + fc_slot<N>(base + mn[4 * s + g])];
+                xr[t][s] = v.re;
+                xi[t][s] = v.im;
+            }
+        }
+        fc_wave_fence();
     }
     // block 0, then row rho of V times column c of U: per lane its rows of the column,
     // then the lanes of the column (the four lane groups and, below 16 support groups,
@@ -305,20 +451,9 @@ __global__ __launch_bounds__(256) void k_fused_trace(FusedCostArgs a) {
     const int lg0 = mp[80], nS = mp[81];
     typename MF::V yr[T], yi[T];
     {
-        // all T tiles (those past the support's are never read)
+        // all T tiles (those past the support's hold zeros when it stays short)
         R ar[4], ai[4], ain[4];
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if constexpr (PF) {
-                ar[s] = far[s];
-                ai[s] = fai[s];
-            } else {
-                const cpx<R> f = Fe[(4 * s + g) * 16 + nl];
-                ar[s] = f.re;
-                ai[s] = f.im;
-            }
-            ain[s] = -ai[s];
-        }
+        a_operand(0, ar, ai, ain);
         ft_mma<R, T, T>(ar, ai, ain, xr, xi, yr, yi);
     }
     const cpx<R>* U =
