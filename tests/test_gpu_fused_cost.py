@@ -209,6 +209,41 @@ def _combined(ctx, ps, prec):
     return f, g
 
 
+@pytest.mark.parametrize("n,sub", [(8, 6), (10, 7)])
+def test_fused_chain_with_qubits_outside_every_block(oracle, n, sub):
+    """A circuit on the first `sub` qubits of an n-qubit register: the blocks never cover
+    the register, so the transposed chain's support stays short up to its last block
+    (the generic row path of its final product) and the gradient falls back to the
+    backward sweep.  Costs (chain on and off) and gradients against the oracle, square
+    and rectangular targets with offsets."""
+    gates = W.mixed_gate_circuit(sub, 6 * sub, 4300 + n)
+    P = W.param_count(gates)
+    d = 1 << n
+    B = 3
+    ps = W.random_params(P, 4400 + n, batch=B)
+    for m, off in [(d, 0), (d // 2 + 5, 3)]:
+        say(f"fused chain, {sub} of {n} qubits, m={m} offset={off}")
+        U = W.haar_unitary(d, 4500 + n)[:, :m].copy()
+        ctx = nat.Context(0)
+        ctx.set_circuit(n, gates)
+        ctx.set_target(U)
+        ctx.set_trace_offset(off)
+        ctx.set_fused_cost(True)
+        ctx.set_cost_function(nat.COST_FROBENIUS)
+        got = {}
+        for chain in (True, False):
+            ctx.set_fused_cost_chain(chain)
+            got[chain] = (_costs(ctx, ps, nat.PREC_F64), _costs(ctx, ps, nat.PREC_F32))
+        ctx.set_fused_cost_chain(True)
+        f64, g64 = _combined(ctx, ps, nat.PREC_F64)
+        for b in range(B):
+            fo, go = oracle.optimization_problem_combined(gates, U, n, ps[b], orc.COST_FROBENIUS, off)
+            for chain, (c64, c32) in got.items():
+                assert abs(c64[b] - fo) < 1e-12, (chain, b, c64[b], fo)
+                assert abs(c32[b] - fo) < 1e-5, (chain, b, c32[b], fo)
+            assert abs(f64[b] - fo) < 1e-12 and np.abs(g64[b] - go).max() < 1e-12
+
+
 @pytest.mark.parametrize("n", [4, 5, 6, 7, 8, 9, 10])
 def test_fused_gradient_matches_oracle(oracle, n):
     """Costs and gradients through the fused blocks (forward on the matrix cores, the
