@@ -64,8 +64,9 @@ class N_Qubit_Decomposition_custom:
         self._ctx = nat.Context(0)
         self._ctx.set_target(U)
         # Circuit-specialised kernels (compiled on first use per circuit: about 0.6 s
-        # for the 8-qubit cost kernel, 12 s for its gradient kernel); a config key of
-        # this build, off by default so single calls keep their latency.
+        # for the 8-qubit cost kernel, 12 s for its gradient kernel); a config key this
+        # package adds (the reference has none), off by default so single calls keep
+        # their latency.
         self._ctx.set_jit(bool(self.config.get("specialised_kernels", False)))
 
     # -- configuration -------------------------------------------------------
