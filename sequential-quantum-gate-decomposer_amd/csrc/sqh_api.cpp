@@ -221,18 +221,22 @@ bool fused_cost_applies(const sqh_ctx* c, bool grad, const TraceSpec& ts) {
     return !grad && ts.nch == 0 && fused_pass_applies(c);
 }
 
-// Costs of B parameter vectors by the tile-resident fused-block kernel: kernel tables
-// (built by the caller into kern) -> block matrices -> one launch over every column ->
-// trace entries into diag.  U: the target (rows x m, row stride m), or with in_index
-// element b's target at U + in_index[b] * in_bstride.
-int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, double* diag, int trace_off, const void* U, int m,
-                    const int* in_index, long long in_bstride) {
+// Costs of B parameter vectors by the tile-resident fused-block kernels: block
+// matrices (from the kernel table kern, or each gate's 2x2 kernel computed from d_params
+// while staging) -> one launch over every column -> trace entries into diag.  U: the
+// target (rows x m, row stride m), or with in_index element b's target at
+// U + in_index[b] * in_bstride.
+int fused_cost_eval(sqh_ctx* c, bool f32, int B, const void* kern, const double* d_params, double* diag, int trace_off,
+                    const void* U, int m, const int* in_index, long long in_bstride) {
     const FusionPlan& fp = c->fplan;
     const int nb = (int)fp.qubits.size(), ns = (int)c->prog.slots.size();
     const size_t esz = f32 ? 8 : 16;
     HIPTRY(c, c->d_fused.ensure((size_t)B * nb * 256 * esz));
+    // the block matrices from the kernel table kern, or with d_params straight from the
+    // parameters (no kernel table)
     HIPTRY(c, launch_fuse_blocks(f32, c->d_fgates.p, c->d_ffirst.as<int>(), nb, kern, (long long)ns * 4, B,
-                                 c->d_fused.p, c->stream));
+                                 c->d_fused.p, c->stream, nullptr, nullptr, d_params, c->prog.P,
+                                 c->d_slots.as<Slot>()));
     FusedCostArgs a;
     a.U = U;
     a.in_index = in_index;
@@ -404,12 +408,14 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
     const bool hs_corr = ts.nch > 0 && !uses_offset(c->variant);
     void* kern = nullptr;
     double* diag = nullptr;
+    // cost-only batches on the fused blocks build their 2x2 kernels inside the block build
+    const bool fused_costs = !fixed && fused_cost_applies(c, d_grad != nullptr, ts);
     if (fixed) {
         kern = fixed->kern;
         diag = fixed->diag;
         HIPTRY(c, launch_build_kernels(f32, d_params, c->prog.P, B, c->d_slots.as<Slot>(), ns, kern, c->stream));
     } else {
-        if ((rc = build_kernels(c, d_params, B, f32))) return rc;
+        if (!fused_costs && (rc = build_kernels(c, d_params, B, f32))) return rc;
         HIPTRY(c, c->d_diag.ensure((size_t)(ts.nch + 1) * B * m * 2 * sizeof(double)));
         kern = c->d_kern.p;
         diag = c->d_diag.as<double>();
@@ -435,8 +441,8 @@ int eval_batch(sqh_ctx* c, const double* d_params, int B, int precision, double*
                                 d_params, c->prog.P, d_grad, c->stream, BatchIndex(), ts));
         return SQH_OK;
     }
-    if (fused_cost_applies(c, d_grad != nullptr, ts) && !fixed) {
-        if ((rc = fused_cost_eval(c, f32, B, kern, diag, off, U, m, nullptr, 0))) return rc;
+    if (fused_costs) {
+        if ((rc = fused_cost_eval(c, f32, B, nullptr, d_params, diag, off, U, m, nullptr, 0))) return rc;
         HIPTRY(c, launch_finish(diag, m, B, c->variant, d_costs, nullptr, 0, ns, c->d_slots.as<Slot>(), d_params,
                                 c->prog.P, nullptr, c->stream, BatchIndex(), ts));
         return SQH_OK;
@@ -526,6 +532,7 @@ int eval_targets(sqh_ctx* c, const double* d_params, int B, const void* d_target
     if (rows != (1 << N) || cols <= 0 || cols + off > rows)
         return fail(c, SQH_ERR_ARG, "targets must be %d x m with m + trace offset <= %d", 1 << N, 1 << N);
     const TraceSpec ts = trace_spec(c, B, m);
+    const bool fused_costs = fused_cost_applies(c, false, ts);
     if ((rc = build_kernels(c, d_params, B, false))) return rc;
     HIPTRY(c, c->d_diag.ensure((size_t)(ts.nch + 1) * B * m * 2 * sizeof(double)));
     TileLaunch t = base_launch(c, false);
@@ -541,8 +548,9 @@ int eval_targets(sqh_ctx* c, const double* d_params, int B, const void* d_target
     t.diag = c->d_diag.as<double>();
     t.ts = ts;
     t.mode = 1;
-    if (fused_cost_applies(c, false, ts)) {
-        if ((rc = fused_cost_eval(c, false, B, c->d_kern.p, t.diag, off, d_targets, m, d_index, t.in_bstride)))
+    if (fused_costs) {
+        if ((rc = fused_cost_eval(c, false, B, c->d_kern.p, nullptr, t.diag, off, d_targets, m, d_index,
+                                  t.in_bstride)))
             return rc;
     } else {
         TimedLaunch tl(c);

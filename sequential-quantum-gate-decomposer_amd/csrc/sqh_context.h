@@ -208,9 +208,12 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 // the tuned one for this shape, or (tuning on, not under graph capture) tuned now by
 // timing every candidate on a scratch batch of the same shape (sqh_tune.cpp).
 int block_config(sqh_ctx* c, bool f32, int rows, int m, int B, BlockCfg* out);
+// With params (and slots), each element's 2x2 kernels are computed from its P
+// parameters while staging (kern unused): the cost pass needs no kernel table.
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
                               long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count = nullptr,
-                              const int* zslot = nullptr);
+                              const int* zslot = nullptr, const double* params = nullptr, int P = 0,
+                              const Slot* slots = nullptr);
 size_t fused_gate_bytes();
 // Fused-block cost + gradient (sqh_api.cpp): launch shape and scratch sizes, the
 // applicability rule, and the enqueue (no allocation) on a stream.

@@ -181,7 +181,8 @@ __global__ __launch_bounds__(16 * kFuseElems) void k_fuse_blocks(const FusedGate
                                                      const int* __restrict__ block_first, int nblocks,
                                                      const cpx<R>* __restrict__ kern, long long kern_bstride, int Bk,
                                                      cpx<R>* __restrict__ out, const int* __restrict__ count,
-                                                     const int* __restrict__ zslot) {
+                                                     const int* __restrict__ zslot, const double* __restrict__ params,
+                                                     int P, const Slot* __restrict__ slots) {
     __shared__ FusedGate srec[kFuseChunk];
     __shared__ cpx<R> skern[kFuseElems][kFuseChunk][4];
     const int blk = blockIdx.x % nblocks;
@@ -201,10 +202,25 @@ __global__ __launch_bounds__(16 * kFuseElems) void k_fuse_blocks(const FusedGate
         __syncthreads();  // the previous chunk is consumed
         for (int i = threadIdx.x; i < cnt; i += blockDim.x) srec[i] = fg[p0 + i];
         __syncthreads();
-        for (int i = threadIdx.x; i < kFuseElems * cnt * 4; i += blockDim.x) {
-            const int e = i / (cnt * 4), gi = (i / 4) % cnt, k = i % 4;
-            const int slot = srec[gi].slot;
-            if (slot >= 0 && b0 + e < Bk) skern[e][gi][k] = kern[(long long)(b0 + e) * kern_bstride + 4 * slot + k];
+        if (params) {
+            // the 2x2 kernels from the elements' parameters (as k_build_kernels computes them)
+            for (int i = threadIdx.x; i < kFuseElems * cnt; i += blockDim.x) {
+                const int e = i / cnt, gi = i % cnt;
+                const int slot = srec[gi].slot;
+                if (slot >= 0 && b0 + e < Bk) {
+                    const Slot sl = slots[slot];
+                    cpx<double> k4[4];
+                    dev_gate_kernel(sl.type, params + (long long)(b0 + e) * P + sl.param_start, k4);
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) skern[e][gi][k] = cvt<R>(k4[k]);
+                }
+            }
+        } else {
+            for (int i = threadIdx.x; i < kFuseElems * cnt * 4; i += blockDim.x) {
+                const int e = i / (cnt * 4), gi = (i / 4) % cnt, k = i % 4;
+                const int slot = srec[gi].slot;
+                if (slot >= 0 && b0 + e < Bk) skern[e][gi][k] = kern[(long long)(b0 + e) * kern_bstride + 4 * slot + k];
+            }
         }
         __syncthreads();
         for (int gi = 0; gi < cnt; ++gi) {
@@ -307,16 +323,17 @@ hipError_t launch_block_apply(bool f32, bool use_mfma, void* mats, long long bst
 
 hipError_t launch_fuse_blocks(bool f32, const void* fg, const int* block_first, int nblocks, const void* kern,
                               long long kern_bstride, int Bk, void* out, hipStream_t s, const int* count,
-                              const int* zslot) {
+                              const int* zslot, const double* params, int P, const Slot* slots) {
+    if (params && !slots) return hipErrorInvalidValue;
     if ((long long)Bk * nblocks == 0) return hipSuccess;
     const long long grid = (long long)nblocks * ((Bk + kFuseElems - 1) / kFuseElems);
     const int threads = 16 * kFuseElems;
     if (f32)
         hipLaunchKernelGGL(k_fuse_blocks<float>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
-                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out, count, zslot);
+                           block_first, nblocks, (const cpx<float>*)kern, kern_bstride, Bk, (cpx<float>*)out, count, zslot, params, P, slots);
     else
         hipLaunchKernelGGL(k_fuse_blocks<double>, dim3((unsigned)grid), dim3(threads), 0, s, (const FusedGate*)fg,
-                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out, count, zslot);
+                           block_first, nblocks, (const cpx<double>*)kern, kern_bstride, Bk, (cpx<double>*)out, count, zslot, params, P, slots);
     return hipGetLastError();
 }
 
