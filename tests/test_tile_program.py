@@ -181,3 +181,43 @@ def test_fusion_plan_in_list_order_above_sixteen_qubits(n):
         for i in src:
             t, tg, c, _ = (int(v) for v in gates[i])
             assert tg in qubits and (t not in (W.GATE_CNOT, W.GATE_CZ, W.GATE_CRY) or c in qubits)
+
+
+@pytest.mark.parametrize("which", ["8q_40", "10q_60", "6q_mixed"])
+def test_chain_flop_accounting_counts_the_rows_the_chain_touches(which):
+    """bench.py's flops for the transposed chain (fused_cost_flops) against a brute-force
+    count: a unit row pushed through random dense 16x16 blocks of the fusion plan, last
+    block first, transposed, counting the rows of every outer group whose input is not
+    all zero (the last block's row is loaded, so it costs nothing), then 8 flop per
+    nonzero row for the product with the target's column."""
+    import bench
+    n, gates = {"8q_40": (8, W.ansatz_8q_40()), "10q_60": (10, W.ansatz_10q_60()),
+                "6q_mixed": (6, W.mixed_gate_circuit(6, 30, 5))}[which]
+    qs = [q for q, _ in nat.inspect_fusion(n, gates)]
+    rng = np.random.default_rng(7)
+    D = 1 << n
+    v = np.zeros(D, dtype=complex)
+    v[3 % D] = 1.0
+    rows = []
+    for b in reversed(range(len(qs))):
+        q = qs[b]
+        off = [sum(1 << q[k] for k in range(4) if (i >> k) & 1) for i in range(16)]
+        Q = sum(1 << x for x in q)
+        F = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+        out = np.zeros_like(v)
+        touched = 0
+        for base in range(D):
+            if base & Q:
+                continue
+            idx = [base | o for o in off]
+            x = v[idx]
+            if np.any(x != 0):
+                touched += 16
+                out[idx] = F.T @ x
+        if b < len(qs) - 1:
+            rows.append(touched)
+        v = out
+    expected = 128.0 * sum(rows) + 8.0 * np.count_nonzero(v)
+    assert bench.fused_cost_flops(gates, n, 1) == expected, (rows, np.count_nonzero(v))
+    if which == "8q_40":
+        assert rows == [128, 128, 256, 256, 256]
